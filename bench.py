@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: block-Jacobi PCG solves on one MI355X.
+
+Default workload (one "step" = one full solve): plain FP32 block-Jacobi PCG
+on the 50^3 beam lattice (750,000 unknowns, 30,352,200 nnz, 6x6 blocks,
+tol 1e-6, max 5000 iterations), matrix, right-hand side and x0 resident in
+HBM before the timed region.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 50] [--no-cpu]
+
+Other workloads (extra lines, not the headline): --workload ir | batch | serve.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s + p99 latency at 1e-8 rel. residual; SpMV HBM GB/s vs peak"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(n, nnz, nb):
+    """Algorithmic HBM bytes per launch of the three PCG kernels."""
+    spmv = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * n          # CSR + row_ptr + p (once) + q
+    update = 4 * n * 7 + 144 * nb + 4 * (nb + 1)         # x rw, p, r_in, q, r_out, z + LU + starts
+    pupdate = 12 * n                                     # p rw, z
+    return spmv, update, pupdate
+
+
+def traffic_from_profiles(workload_key):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(L, iters_gpu, sample_iters):
+    """The oracle (FP32 restatement of the reference loop) on the host cores,
+    on a bounded sample: `sample_iters` PCG iterations of the same system,
+    scaled to solves/s with the GPU's iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib
+    oracle_lib.build()
+    threads = min(16, os.cpu_count() or 1)
+    oracle_lib.set_threads(threads)
+    st, lu = oracle_lib.build_lu_blocks_6(L["n"], L["row_ptr"], L["col_idx"], L["values"],
+                                          L["block_starts"])
+    x0 = np.zeros(L["n"], np.float32)
+    t0 = time.perf_counter()
+    st, it, _, _ = oracle_lib.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"],
+                                  L["rhs"], x0, sample_iters, 1e-6, 0.0)
+    dt = time.perf_counter() - t0
+    per_iter = dt / max(it, 1)
+    solve_s = per_iter * (iters_gpu + 1)
+    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{it} PCG iterations of the same 50^3 system in {dt:.1f} s "
+                      f"({per_iter * 1e3:.2f} ms/iteration), scaled to {iters_gpu} iterations"}
+
+
+def bench_plain(args):
+    import wgpu_solver_backend_amd as H
+    N = args.N
+    log(f"generating {N}^3 lattice")
+    L = H.lattice(N, seed=0)
+    n, nnz, nb = L["n"], L["nnz"], L["n"] // 6
+    ctx = H.GpuContext.create(0)
+    log(ctx.describe())
+    system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    system.set_rhs(L["rhs"], np.zeros(n, np.float32))
+    max_iter, tol = 5000, 1e-6
+    for i in range(args.warmup):
+        res = system.solve_resident(max_iter, tol, 0.0)
+        log(f"warmup {i}: {res.iterations} its, {res.solve_ms:.1f} ms")
+    system.set_kernel_timing(args.sample_every)
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        res = system.solve_resident(max_iter, tol, 0.0)  # returns after the device finished
+        lat.append((time.perf_counter() - ts) * 1e3)
+    elapsed = time.perf_counter() - t0
+    avg_us, cnt = system.kernel_timing()
+    system.set_kernel_timing(0)
+    log(f"timed: {args.steps} solves in {elapsed:.3f} s; {res.iterations} its; "
+        f"kernel avg us {avg_us.round(2).tolist()} ({cnt.tolist()} samples)")
+    b1, b2, b3 = kernel_bytes(n, nnz, nb)
+    spmv_gbps = b1 / (avg_us[0] * 1e-6) / 1e9
+    iter_us = float(avg_us.sum())
+    iter_bytes = b1 + b2 + b3
+    key = f"plain_N{N}"
+    traffic = traffic_from_profiles(key)
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP32 block-Jacobi PCG, {N}^3 beam lattice ({n:,} unknowns, "
+                               f"{nnz:,} nnz, 6x6 blocks), tol {tol:g}, max {max_iter}, x0 = 0",
+                   "n": n, "nnz": nnz, "iterations": int(res.iterations), "seed": 0},
+        "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
+                       "max": max(lat)},
+        "true_rel_residual_fp64": res.rel_res_true_fp64,
+        "spmv_gbps": spmv_gbps,
+        "iteration": {"us": iter_us, "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
+                      "kernel_us": {"spmv": avg_us[0], "update_precond": avg_us[1], "p_update": avg_us[2]},
+                      "kernel_gbps": {"spmv": spmv_gbps,
+                                      "update_precond": b2 / (avg_us[1] * 1e-6) / 1e9,
+                                      "p_update": b3 / (avg_us[2] * 1e-6) / 1e9}},
+        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": spmv_gbps,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": spmv_gbps / HBM_PEAK_GBPS,
+                     "traffic": (traffic or {}).get("k_pcg_spmv_bytes"),
+                     "algorithmic_bytes": b1},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle on host cores)")
+        out["cpu_baseline"] = cpu_baseline(L, int(res.iterations), args.cpu_iters)
+    else:
+        out["cpu_baseline"] = None
+    system.close()
+    ctx.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--N", type=int, default=50)
+    ap.add_argument("--sample-every", type=int, default=8,
+                    help="HIP-event timing of every k-th PCG iteration's kernels")
+    ap.add_argument("--cpu-iters", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="plain", choices=["plain"])
+    args = ap.parse_args()
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or ws != 1:
+        print(json.dumps({"error": "this build measures one GPU: independent systems shard by "
+                                   "system across GPUs with no exchange, so N>1 is not run here",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    out = bench_plain(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

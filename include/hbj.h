@@ -1,0 +1,225 @@
+/*
+ * hbj.h -- C ABI of the MI355X-native block-Jacobi PCG solver.
+ *
+ * Drop-in for the solver API of RomanShushakov/wgpu_solver_backend
+ * (crates/wgpu_solver_backend, Rust + WGSL).  Each entry point names the
+ * reference interface it replaces (paths relative to the reference checkout).
+ * All arguments are plain pointers and sizes; host buffers unless stated.
+ *
+ * Conventions (same as the reference, compute.rs:189-495):
+ *   - inputs are borrowed and copied; the solution buffer is written only
+ *     when the call returns HBJ_OK (x keeps x0 on every error);
+ *   - errors are status codes plus hbj_last_error(), a thread-local text
+ *     with the reference's own wording ("PCG(BlockJacobiGpu): dot(p,Ap) is
+ *     zero (breakdown)", "LU zero pivot at k = 2", ...);
+ *   - one in-flight solve per hbj_system / hbj_batch; a context may be
+ *     shared by threads.
+ */
+#ifndef HBJ_H
+#define HBJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hbj_status {
+  HBJ_OK = 0,
+  HBJ_E_INVALID = 1,        /* bad sizes / CSR / block_starts (loaders.rs:48-184, compute.rs:94-131) */
+  HBJ_E_ZERO_PIVOT = 2,     /* compute.rs:50-52 */
+  HBJ_E_BREAKDOWN_PAP = 3,  /* compute.rs:473-475 */
+  HBJ_E_BREAKDOWN_RZ = 4,   /* compute.rs:476-478 */
+  HBJ_E_NOT_CONVERGED = 5,  /* compute.rs:491-494 */
+  HBJ_E_DEVICE = 6,         /* HIP runtime error (the reference panics in readback.rs:42-46) */
+  HBJ_E_OOM = 7,            /* device allocation failed / over the HBM budget */
+  HBJ_E_NO_DEVICE = 8,      /* no HIP device (context.rs:66-73 request_adapter failure) */
+  HBJ_E_REJECTED = 9        /* serving: request larger than the admission budget */
+} hbj_status;
+
+typedef struct hbj_ctx hbj_ctx;
+typedef struct hbj_system hbj_system;
+typedef struct hbj_batch hbj_batch;
+typedef struct hbj_server hbj_server;
+
+/* Result of one solve.  iterations follows the reference: Ok(k+1) when the
+ * stop test passes after iteration k, Ok(0) when ||b||^2 == 0. */
+typedef struct hbj_result {
+  int32_t status;              /* hbj_status of this system */
+  uint32_t iterations;         /* PCG iterations (IR: inner iterations summed) */
+  uint32_t outer_iterations;   /* IR refinement steps (0 for plain FP32 PCG) */
+  float rr;                    /* last recurrence r.r in FP32 */
+  float b_norm2;               /* ||b||^2 in FP32 as the reference computes it */
+  float reserved0;
+  double rel_res_recurrence;   /* sqrt(rr / b_norm2) */
+  double rel_res_true_fp64;    /* ||b - A x||_2 / ||b||_2 evaluated in FP64 (NaN if not evaluated) */
+  double solve_ms;             /* host wall time of the call's device work */
+} hbj_result;
+
+/* Mixed-precision (iterative refinement) parameters. */
+typedef struct hbj_ir_params {
+  uint32_t max_outer;          /* refinement steps, e.g. 20 */
+  uint32_t max_inner;          /* FP32 PCG iterations per step, e.g. 5000 */
+  float inner_rel_tol;         /* FP32 PCG relative tolerance per step, e.g. 1e-4 */
+  float reserved0;
+  double target_rel_res;       /* stop when ||b - A x||/||b|| (FP64) <= this, e.g. 1e-10 */
+} hbj_ir_params;
+
+const char* hbj_version(void);
+/* Thread-local text of the last error on this thread ("" if none). */
+const char* hbj_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* GpuContext::create (src/gpu/context.rs:60-109): select HIP device `device`. */
+hbj_status hbj_ctx_create(int device, hbj_ctx** out);
+void hbj_ctx_destroy(hbj_ctx* ctx);
+/* GpuContext::describe (context.rs:111-120) / the `info` subcommand. */
+hbj_status hbj_ctx_describe(hbj_ctx* ctx, char* buf, size_t buf_len);
+
+/* ---------------------------------------------------- CPU setup (host) */
+
+/* lu_factor_inplace_6 (compute.rs:37-71): no-pivot LU of the leading n x n of a
+ * row-major 6x6 buffer; strict lower = L (unit diagonal implied), rest = U. */
+hbj_status hbj_lu_factor_inplace_6(float* mat36, uint32_t n);
+
+/* build_lu_blocks_from_csr_block_starts_6 (compute.rs:84-181).  lu_out holds
+ * 36*(n_starts-1) floats.  Slice lengths are passed as the Rust slices carry them. */
+hbj_status hbj_build_lu_blocks_6(uint32_t n, const uint32_t* row_ptr, uint32_t row_ptr_len,
+                                 const uint32_t* col_idx, uint32_t col_idx_len,
+                                 const float* values, uint32_t values_len,
+                                 const uint32_t* block_starts, uint32_t n_starts, float* lu_out);
+
+/* ------------------------------------- single-op executors (device) */
+/* Each runs the same device code as the fused solver kernels on host buffers. */
+
+/* VecOpsExecutor::encode_axpy_inplace (vec_ops_exec.rs:130-160): y = y + alpha*x */
+hbj_status hbj_op_axpy(hbj_ctx* ctx, uint32_t n, float alpha, const float* x, float* y);
+/* VecOpsExecutor::encode_scale_inplace_from_scalar_results (vec_ops_exec.rs:200-232): x = x*beta */
+hbj_status hbj_op_scale(hbj_ctx* ctx, uint32_t n, float beta, float* x);
+/* DotScalarExecutor::encode_dot_scalar_into (dot_scalar_exec.rs:151-247): *out = a.b */
+hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, float* out);
+/* SpmvExecutor::create + encode_copy_x_from + encode_spmv (spmv_exec.rs:38-147): y = A x */
+hbj_status hbj_op_spmv(hbj_ctx* ctx, uint32_t n_rows, uint32_t nnz, const uint32_t* row_ptr,
+                       const uint32_t* col_idx, const float* values, const float* x, float* y);
+/* BlockJacobiExecutor::create + encode_apply (block_jacobi_exec.rs:48-124): z = M^-1 r.
+ * z is in/out: rows no block writes (past the 6th row of an oversize block) keep their value. */
+hbj_status hbj_op_block_jacobi(hbj_ctx* ctx, uint32_t n, const float* lu,
+                               const uint32_t* block_starts, uint32_t n_starts, const float* r,
+                               float* z);
+/* PcgUpdateScalarsExecutor::encode_update_scalars (pcg_update_scalars_exec.rs:72-118)
+ * on a slot array of n_slots floats (in/out). */
+hbj_status hbj_op_update_scalars(hbj_ctx* ctx, float* slots, uint32_t n_slots, uint32_t i_pap,
+                                 uint32_t i_rz_new, uint32_t i_rz_old, uint32_t i_alpha,
+                                 uint32_t i_minus_alpha, uint32_t i_beta);
+
+/* ------------------------------------------- device-resident system */
+
+/* SpmvExecutor::create + build_lu_blocks_from_csr_block_starts_6 +
+ * BlockJacobiExecutor::create (spmv_exec.rs:38-118, compute.rs:84-181,
+ * block_jacobi_exec.rs:48-91): validates like loaders.rs:84-105,164-184,
+ * builds the LU blocks, and keeps CSR, LU and all PCG vectors in HBM. */
+hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                             const uint32_t* col_idx, const float* values,
+                             const uint32_t* block_starts, uint32_t n_starts, hbj_system** out);
+void hbj_system_destroy(hbj_system* sys);
+/* Device bytes held by the system (CSR + LU + vectors). */
+uint64_t hbj_system_device_bytes(const hbj_system* sys);
+
+/* pcg_block_jacobi_csr_wgpu (compute.rs:189-495): the drop-in solve.
+ * b: n floats; x_inout: x0 in, solution out (only on HBJ_OK). */
+hbj_status hbj_solve(hbj_system* sys, const float* b, float* x_inout, uint32_t max_iter,
+                     float rel_tol, float abs_tol, hbj_result* result);
+
+/* Same solve with b and x0 already resident in HBM (uploaded once by
+ * hbj_system_set_rhs); the solution stays on the device until hbj_system_get_x. */
+hbj_status hbj_system_set_rhs(hbj_system* sys, const float* b, const float* x0);
+hbj_status hbj_system_solve_resident(hbj_system* sys, uint32_t max_iter, float rel_tol,
+                                     float abs_tol, hbj_result* result);
+hbj_status hbj_system_get_x(hbj_system* sys, float* x);
+
+/* Sample HIP-event timings of the three per-iteration kernels during
+ * subsequent solves of this system: every `every`-th iteration (0 = off). */
+hbj_status hbj_system_set_kernel_timing(hbj_system* sys, uint32_t every);
+/* avg_us[3]: mean duration of SpMV+p.Ap, update+precond+dots, p-update;
+ * counts[3]: samples.  Clears the accumulators. */
+hbj_status hbj_system_kernel_timing(hbj_system* sys, double* avg_us, uint64_t* counts);
+
+/* ------------------------------------------------ mixed precision (IR) */
+
+/* FP64 outer iterative refinement around the FP32 PCG: r = b - A x in FP64
+ * on the device, FP32 PCG on the scaled residual, x += correction in FP64.
+ * No reference counterpart (extends compute.rs:189 for 1e-10 residuals). */
+hbj_status hbj_solve_ir(hbj_system* sys, const double* b, double* x_inout,
+                        const hbj_ir_params* params, hbj_result* result);
+hbj_status hbj_system_set_rhs64(hbj_system* sys, const double* b, const double* x0);
+hbj_status hbj_system_get_x64(hbj_system* sys, double* x);
+
+/* ----------------------------------------------------------- batching */
+
+/* Many independent systems advanced together: every PCG iteration of every
+ * system in one set of kernel launches (graph-replayed), per-system
+ * alpha/beta/status on the device.  Systems keep their own resident b/x0
+ * (hbj_system_set_rhs / _set_rhs64).  Results are identical to solving each
+ * system alone.  No reference counterpart (the reference solves one system). */
+hbj_status hbj_batch_create(hbj_ctx* ctx, hbj_system* const* systems, uint32_t count,
+                            hbj_batch** out);
+void hbj_batch_destroy(hbj_batch* batch);
+hbj_status hbj_batch_solve(hbj_batch* batch, uint32_t max_iter, float rel_tol, float abs_tol,
+                           hbj_result* results /* count */);
+hbj_status hbj_batch_solve_ir(hbj_batch* batch, const hbj_ir_params* params,
+                              hbj_result* results /* count */);
+
+/* ------------------------------------------------------------ serving */
+
+typedef struct hbj_server_config {
+  uint64_t hbm_budget_bytes;   /* admission limit on device bytes of resident requests */
+  uint32_t max_active;         /* most systems advanced together */
+  uint32_t chunk_iters;        /* iterations between host status polls */
+} hbj_server_config;
+
+typedef struct hbj_server_stats {
+  uint64_t submitted, completed, rejected, active, queued;
+  uint64_t device_bytes_in_use;
+  uint64_t peak_device_bytes;
+  uint64_t peak_active;
+} hbj_server_stats;
+
+/* Continuous-batching request loop: a host thread admits queued requests as
+ * HBM budget and slots free up, uploads them while others iterate, advances
+ * all resident requests together, and retires each as it converges. */
+hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_server** out);
+void hbj_server_destroy(hbj_server* srv);
+/* Copies the request; returns a ticket.  b64 in FP64; solved with IR (params). */
+hbj_status hbj_server_submit(hbj_server* srv, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                             const uint32_t* col_idx, const float* values,
+                             const uint32_t* block_starts, uint32_t n_starts, const double* b64,
+                             const hbj_ir_params* params, uint64_t* ticket);
+/* Blocks until the request is done; x (n doubles, may be NULL) receives the
+ * solution; latency_ms = completion - submission. */
+hbj_status hbj_server_wait(hbj_server* srv, uint64_t ticket, double* x, hbj_result* result,
+                           double* latency_ms);
+hbj_status hbj_server_stats_get(hbj_server* srv, hbj_server_stats* stats);
+
+/* -------------------------------------------- synthetic workload (host) */
+
+/* 3D space-frame lattice of N^3 nodes, 6 DOF/node, Euler-Bernoulli beams
+ * between axis neighbours (spacing 1 m, E=2.1e11, G=8.1e10, A~U[1e-3,1e-2],
+ * Iy,Iz,J~U[1e-6,1e-5]); bottom layer clamped as identity rows; dense 6x6
+ * node blocks kept; RHS ~U[-1e3,1e3] on free DOFs; block_starts = 0,6,..,n.
+ * e_log10_half_range > 0 scales E and G per element by 10^U[-h,h].
+ * Query sizes with hbj_lattice_size, then fill caller-allocated arrays.
+ * No reference counterpart: the benchmark's input generator. */
+hbj_status hbj_lattice_size(uint32_t N, uint32_t* n, uint64_t* nnz);
+hbj_status hbj_lattice_generate(uint32_t N, uint64_t seed, double e_log10_half_range,
+                                uint32_t* row_ptr, uint32_t* col_idx, float* values, float* rhs,
+                                uint32_t* block_starts);
+/* 12x12 global stiffness of one beam element along axis dir (0=x,1=y,2=z). */
+hbj_status hbj_beam_element_stiffness(double E, double G, double A, double Iy, double Iz,
+                                      double J, double L, int dir, double* k144);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HBJ_H */

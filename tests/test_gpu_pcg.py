@@ -1,0 +1,174 @@
+"""The drop-in solve on the GPU against the oracle: same status, same
+iteration count, bit-identical x (the FP32 path follows the reference's
+operation order), the reference's golden x_ref tolerances, error paths."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def oracle_solve(oracle, rp, ci, va, bs, b, x0, max_iter, rel, ab):
+    n = len(rp) - 1
+    st, lu = oracle.build_lu_blocks_6(n, rp, ci, va, bs)
+    assert st == 0
+    return oracle.pcg(rp, ci, va, lu, bs, b, x0, max_iter, rel, ab)
+
+
+DATASETS = [(1, 1e-2, 42), (2, 1e0, 370), (3, 1e1, 1273)]
+
+
+@pytest.mark.parametrize("k,atol,its", DATASETS)
+def test_datasets_bitwise_and_x_ref(ctx, hbj, oracle, k, atol, its):
+    print(f"dataset {k}", flush=True)
+    d = oracle.load_dataset(k)
+    sys_ = hbj.System(ctx, d["row_ptr"], d["col_idx"], d["values"], d["block_starts"])
+    x, res = sys_.solve(d["b"], d["x0"], 2000, 1e-4, 1e-7)
+    st, it, xo, _ = oracle_solve(oracle, d["row_ptr"], d["col_idx"], d["values"], d["block_starts"],
+                                 d["b"], d["x0"], 2000, 1e-4, 1e-7)
+    assert res.status == st == 0
+    assert res.iterations == it == its
+    assert np.array_equal(bits(x), bits(xo))
+    xr = d["x_ref"].astype(np.float64)
+    err = x.astype(np.float64) - xr
+    rel_l2 = np.linalg.norm(err) / np.linalg.norm(xr)
+    assert rel_l2 <= 1e-3 or np.sqrt(np.mean(err ** 2)) <= atol  # commands.txt:55-78
+    # the FP64 residual the library reports for x
+    n = int(d["n"])
+    A = sp.csr_matrix((d["values"].astype(np.float64), d["col_idx"], d["row_ptr"]), shape=(n, n))
+    b64 = d["b"].astype(np.float64)
+    tr = np.linalg.norm(b64 - A @ x.astype(np.float64)) / np.linalg.norm(b64)
+    assert abs(res.rel_res_true_fp64 - tr) <= 1e-9 * max(tr, 1e-12)
+    sys_.close()
+
+
+def test_reference_named_entry_point(ctx, hbj, oracle):
+    d = oracle.load_dataset(2)
+    n = int(d["n"])
+    lu = hbj.build_lu_blocks_from_csr_block_starts_6(n, d["row_ptr"], d["col_idx"], d["values"],
+                                                     d["block_starts"])
+    spmv = hbj.SpmvExecutor.create(ctx, n, d["row_ptr"], d["col_idx"], d["values"])
+    vec = hbj.VecOpsExecutor.create(ctx)
+    dot = hbj.DotScalarExecutor.create(ctx, n, 7)
+    bj = hbj.BlockJacobiExecutor.create(ctx, n, lu, d["block_starts"])
+    upd = hbj.PcgUpdateScalarsExecutor.create(ctx)
+    x = d["x0"].copy()
+    it = hbj.pcg_block_jacobi_csr_wgpu(n, d["b"], x, 2000, 1e-4, 1e-7, ctx, spmv, vec, dot, bj, upd)
+    st, ito, xo, _ = oracle_solve(oracle, d["row_ptr"], d["col_idx"], d["values"], d["block_starts"],
+                                  d["b"], d["x0"], 2000, 1e-4, 1e-7)
+    assert it == ito and np.array_equal(bits(x), bits(xo))
+    with pytest.raises(hbj.SolverError, match="dimension mismatch"):
+        hbj.pcg_block_jacobi_csr_wgpu(n, d["b"][:-1], x, 10, 1e-4, 0, ctx, spmv, vec, dot, bj, upd)
+
+
+@pytest.mark.parametrize("N,h,seed", [(10, 0.0, 0), (6, 1.5, 4), (13, 0.0, 2)])
+def test_lattice_bitwise(ctx, hbj, oracle, N, h, seed):
+    print(f"lattice N={N} h={h}", flush=True)
+    L = hbj.lattice(N, seed=seed, e_log10_half_range=h)
+    sys_ = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    x0 = np.zeros(L["n"], F)
+    x, res = sys_.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+    st, it, xo, _ = oracle_solve(oracle, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                 L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert res.status == st == 0 and res.iterations == it
+    assert np.array_equal(bits(x), bits(xo))
+    # a second solve of the same resident system is bit-identical (no state leaks)
+    x2, res2 = sys_.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert res2.iterations == it and np.array_equal(bits(x2), bits(x))
+    sys_.close()
+
+
+def test_resident_api_and_warm_start(ctx, hbj, oracle):
+    L = hbj.lattice(8, seed=3)
+    n = L["n"]
+    sys_ = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    x0 = (np.random.default_rng(1).standard_normal(n) * 1e-4).astype(F)
+    x0[: 6 * 64] = 0
+    sys_.set_rhs(L["rhs"], x0)
+    res = sys_.solve_resident(5000, 1e-6, 0.0)
+    x = sys_.get_x()
+    st, it, xo, _ = oracle_solve(oracle, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                 L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert res.iterations == it and np.array_equal(bits(x), bits(xo))
+    # kernel timing mode changes nothing numerically
+    sys_.set_kernel_timing(4)
+    res_t = sys_.solve_resident(5000, 1e-6, 0.0)
+    avg, cnt = sys_.kernel_timing()
+    assert res_t.iterations == it and np.array_equal(bits(sys_.get_x()), bits(xo))
+    assert np.all(cnt > 0) and np.all(avg > 0)
+    sys_.set_kernel_timing(0)
+
+
+def test_zero_rhs_ok0_x_untouched(ctx, hbj, oracle):
+    d = oracle.load_dataset(1)
+    sys_ = hbj.System(ctx, d["row_ptr"], d["col_idx"], d["values"], d["block_starts"])
+    x0 = np.arange(int(d["n"]), dtype=F)
+    x, res = sys_.solve(np.zeros(int(d["n"]), F), x0, 100, 1e-4, 0.0)
+    assert res.status == 0 and res.iterations == 0 and np.array_equal(x, x0)
+
+
+def test_breakdown_pap(ctx, hbj):
+    n = 12
+    rp = np.arange(n + 1, dtype=np.uint32)
+    ci = np.arange(n, dtype=np.uint32)
+    va = np.ones(n, F)
+    sys_ = hbj.System(ctx, rp, ci, va, np.arange(0, n + 1, 6, dtype=np.uint32))
+    b = np.arange(1, n + 1, dtype=F)
+    with pytest.raises(hbj.SolverError) as e:
+        sys_.solve(b, b, 100, 1e-6, 0.0)
+    assert str(e.value) == "PCG(BlockJacobiGpu): dot(p,Ap) is zero (breakdown)"
+    assert e.value.result.iterations == 1
+
+
+def test_breakdown_rz(ctx, hbj):
+    rp = np.array([0, 2, 4, 5], np.uint32)
+    ci = np.array([0, 1, 0, 1, 2], np.uint32)
+    va = np.array([1, 0.5, 0.5, -1, 1], F)
+    sys_ = hbj.System(ctx, rp, ci, va, np.array([0, 1, 2, 3], np.uint32))
+    x0 = np.zeros(3, F)
+    x, res = sys_.solve(np.array([1, 1, 0], F), x0, 10, 1e-6, 0.0, raise_on_error=False)
+    assert res.status == 4 and res.iterations == 1 and np.array_equal(x, x0)
+    assert hbj._lib.last_error() == "PCG(BlockJacobiGpu): rz_old is zero (breakdown)"
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 7])
+def test_not_converged_keeps_x0(ctx, hbj, max_iter):
+    L = hbj.lattice(5, seed=1)
+    sys_ = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    x0 = np.full(L["n"], 0.5, F)
+    x, res = sys_.solve(L["rhs"], x0, max_iter, 1e-6, 0.0, raise_on_error=False)
+    assert res.status == 5 and np.array_equal(x, x0)
+    assert hbj._lib.last_error() == f"PCG(BlockJacobiGpu): did not converge in {max_iter} iterations"
+
+
+def test_invalid_inputs_rejected_on_host(ctx, hbj):
+    rp = np.array([0, 1, 2], np.uint32)
+    with pytest.raises(hbj.SolverError, match="block_starts must be strictly increasing"):
+        hbj.System(ctx, rp, np.array([0, 1], np.uint32), np.ones(2, F), np.array([0, 1, 1, 2], np.uint32))
+    with pytest.raises(hbj.SolverError, match="out-of-range column 5"):
+        hbj.System(ctx, rp, np.array([0, 5], np.uint32), np.ones(2, F), np.array([0, 2], np.uint32))
+    with pytest.raises(hbj.SolverError, match="LU zero pivot at k = 0"):
+        hbj.System(ctx, rp, np.array([0, 1], np.uint32), np.array([0, 1], F), np.array([0, 2], np.uint32))
+
+
+@pytest.mark.slow
+def test_full_size_lattice_50_bitwise(ctx, hbj, oracle):
+    """The 50^3 configuration (750,000 unknowns, 30,352,200 nnz) against the
+    oracle on the host cores: same iteration count and bit-identical x."""
+    print("lattice N=50 full size", flush=True)
+    L = hbj.lattice(50, seed=0)
+    sys_ = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    x0 = np.zeros(L["n"], F)
+    x, res = sys_.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+    print(f"gpu: {res.iterations} its, {res.solve_ms:.1f} ms, true res {res.rel_res_true_fp64:.3e}",
+          flush=True)
+    st, it, xo, _ = oracle_solve(oracle, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                 L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert res.status == st == 0 and res.iterations == it
+    assert np.array_equal(bits(x), bits(xo))
+    sys_.close()

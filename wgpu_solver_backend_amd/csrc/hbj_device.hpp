@@ -1,0 +1,86 @@
+// hbj_device.hpp -- layouts shared by the HIP kernels and the host runtime.
+//
+// A "system" is one SPD CSR matrix with its block-Jacobi factors and PCG
+// vectors, all resident in HBM.  Kernels never see a system directly: they
+// see an array of SysDesc (read-only pointers and sizes) plus a tile table
+// that maps every workgroup to (system, 256-row tile).  One system solved
+// alone is a table of one system; a batch or the serving loop is a table of
+// many.  Per-system mutable scalars (alpha, beta, r.z, status, ...) live in
+// SysState on the device, so no host round trip sits inside an iteration.
+#pragma once
+#include <stdint.h>
+
+namespace hbj {
+
+// Rows per tile.  Equals the reference's dot-product workgroup width
+// (dot_partials.wgsl:37 @workgroup_size(256)): each tile's dot partial is the
+// reference's partial for the same 256 rows, bit for bit.
+constexpr int kTile = 256;
+// block_jacobi.wgsl:60-61
+constexpr int kBlock = 6;
+constexpr int kLuStride = 36;
+// CSR entries staged per pass of the SpMV tile loop (kPer per thread).
+constexpr int kChunk = 2048;
+constexpr int kPer = kChunk / kTile;
+// Blocks whose factors a tile stages in LDS (more than this: read from HBM).
+constexpr int kLuStage = 64;
+
+// SysState.status while a system is still iterating; any other value is a
+// final hbj_status.
+constexpr int32_t kRunning = -1;
+
+enum Phase : int32_t {
+  kPhInit = 0,      // r0 = b - A x0, z0 = M^-1 r0, p0 = z0, rz_old = r0.z0
+  kPhIter = 1,      // PCG iteration
+  kPhIrResid = 2,   // IR: r64 = b64 - A x64 (FP64), decide, scale
+  kPhIrReinit = 3,  // IR: b32 = r64 * scale, restart inner PCG at x32 = 0
+};
+
+// SysDesc.flags
+constexpr uint32_t kFlagOversize = 1u;  // some block has more than 6 rows
+constexpr uint32_t kFlagIr = 2u;        // iterative refinement enabled
+constexpr uint32_t kFlagDot64 = 4u;     // inner dots accumulated in FP64
+
+struct alignas(128) SysState {
+  // FP32 PCG scalars, the reference's mailbox slots (compute.rs:271-277)
+  float rz_old, p_ap, alpha, neg_alpha;
+  float rr, rz_new, beta, b_norm2;
+  float rel_tol2, abs_tol2;
+  int32_t status;      // kRunning or final hbj_status
+  int32_t phase;       // Phase
+  uint32_t iter;       // iterations of the current (inner) PCG
+  uint32_t max_iter;   // per inner PCG
+  uint32_t cnt1, cnt2; // last-tile counters of the SpMV and update kernels
+  float* r_in;         // r of the previous iteration (double-buffered)
+  float* r_out;        // r written this iteration
+  // iterative refinement
+  uint32_t outer, max_outer;
+  uint32_t total_iter, pad0;
+  double target2;      // (target relative residual)^2
+  double rr64, bb64;   // ||r64||^2, ||b64||^2 of the last residual
+  double scale;        // b32 = r64 * scale
+  double inv_scale;    // x64 += x32 * inv_scale
+  double rz64_old, p_ap64;  // FP64-accumulated dots (kFlagDot64)
+};
+
+struct SysDesc {
+  const uint32_t* row_ptr;   // n+1
+  const uint32_t* col;       // nnz
+  const float* val;          // nnz
+  const uint32_t* bstart;    // nb+1
+  const float* lu;           // 36*nb, row-major packed LU per block
+  float* x;                  // solution (FP32 PCG) / correction (IR inner)
+  float* p;
+  float* q;                  // A p
+  float* z;
+  float* b;                  // right-hand side of the FP32 PCG
+  float* part;               // 3 * n_tiles per-tile dot partials
+  double* part64;            // 3 * n_tiles FP64 partials (IR / dot64)
+  double* x64;               // IR solution
+  const double* b64;         // IR right-hand side
+  double* r64;               // IR residual
+  SysState* st;
+  uint32_t n, nb, n_tiles, flags;
+};
+
+}  // namespace hbj

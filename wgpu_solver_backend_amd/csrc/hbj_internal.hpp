@@ -1,0 +1,48 @@
+// hbj_internal.hpp -- declarations shared by the host-side translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "hbj.h"
+#include "hbj_device.hpp"
+
+namespace hbj {
+
+// What one launch of a PCG kernel covers: a descriptor array, a tile table
+// and whether to apply the XCD-contiguous tile remap.
+struct LaunchSet {
+  const SysDesc* descs;
+  const uint2* tiles;
+  uint32_t n_tiles;
+  uint32_t remap;
+};
+
+void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
+void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
+void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s);
+void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t n_tiles,
+                       hipStream_t s);
+
+void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s);
+void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s);
+void launch_op_dot(uint32_t n, const float* a, const float* b, float* part, uint32_t* cnt,
+                   float* out, hipStream_t s);
+void launch_op_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, const float* val,
+                    const float* x, float* y, hipStream_t s);
+void launch_op_block_jacobi(uint32_t n, uint32_t nb, const float* lu, const uint32_t* bs,
+                            const float* r, float* z, hipStream_t s);
+void launch_op_update_scalars(float* slots, uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                              uint32_t e, uint32_t f, hipStream_t s);
+
+// thread-local error text (hbj_last_error)
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+// CSR + block_starts validation with the reference's messages
+// (loaders.rs:84-105, 164-184).
+hbj_status validate_case(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                         const uint32_t* col_idx, const uint32_t* block_starts,
+                         uint32_t n_starts);
+
+}  // namespace hbj

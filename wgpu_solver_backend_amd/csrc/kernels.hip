@@ -1,0 +1,700 @@
+// kernels.hip -- gfx950 kernels of the block-Jacobi PCG.
+//
+// One PCG iteration of every active system is three launches over a tile
+// table (workgroup -> (system, 256-row tile)):
+//
+//   k_pcg_spmv    q = A p (CSR rows streamed coalesced through LDS), the
+//                 tile's p.q partial; the system's last tile reduces the
+//                 partials and sets alpha = rz_old / p.q on the device.
+//   k_pcg_update  x += alpha p, r -= alpha q, z = M^-1 r (6x6 LU solves,
+//                 factors staged in LDS), r.r and r.z partials; the last
+//                 tile computes beta and the stop test and writes status.
+//   k_pcg_pupdate p = z + beta p.
+//
+// The same three kernels run the init step (r0 = b - A x0 ...) and, for
+// iterative refinement, the FP64 residual / restart / fold steps, chosen
+// per system by SysState.phase.
+//
+// Arithmetic follows the reference operation for operation (the file is
+// compiled with -ffp-contract=off, IEEE division):
+//   spmv.wgsl:55-61        q_i = ((0 + v_k0 x_c0) + v_k1 x_c1) + ...  in row order
+//   dot_partials.wgsl:46-69  256-slot stride-halving tree per 256 rows
+//   dot_reduce.wgsl:47-68    repeated 256-way trees over the partials
+//   axpy*.wgsl:41          y + a*x, one rounding each
+//   scale + axpy (compute.rs:442-457)  p = (p*beta) + z, two roundings
+//   block_jacobi.wgsl:93-122  forward then backward substitution
+//   pcg_update_scalars.wgsl:56-67  alpha, beta with zero guards
+// so an FP32 solve here matches the CPU restatement bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "hbj_device.hpp"
+#include "hbj_internal.hpp"
+
+namespace hbj {
+
+// ------------------------------------------------------------------ helpers
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remap so that XCD k
+// runs a contiguous 1/8 of the tiles: neighbouring rows gather neighbouring x
+// entries, which then stay in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t g) {
+  const uint32_t per = g / 8u, rem = g % 8u;
+  const uint32_t xcd = b % 8u, k = b / 8u;
+  return xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
+}
+
+// dot_partials.wgsl:52-69: s[t] = s[t] + s[t+off] for off = 128..1.  Offsets
+// 128 and 64 go through LDS, 32..1 through lane shuffles inside wave 0 (same
+// pairs, same order).  Result valid in thread 0.
+template <typename T>
+__device__ __forceinline__ T tree256(T v, T* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  if (t < 128) sh[t] = sh[t] + sh[t + 128];
+  __syncthreads();
+  T w = T(0);
+  if (t < 64) {
+    w = sh[t] + sh[t + 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) w = w + __shfl_down(w, off, 64);
+  }
+  __syncthreads();
+  return w;
+}
+
+// Two independent trees in one pass (same pairing for each component).
+template <typename T>
+__device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
+  const int t = threadIdx.x;
+  sha[t] = a;
+  shb[t] = b;
+  __syncthreads();
+  if (t < 128) {
+    sha[t] = sha[t] + sha[t + 128];
+    shb[t] = shb[t] + shb[t + 128];
+  }
+  __syncthreads();
+  T wa = T(0), wb = T(0);
+  if (t < 64) {
+    wa = sha[t] + sha[t + 64];
+    wb = shb[t] + shb[t + 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      wa = wa + __shfl_down(wa, off, 64);
+      wb = wb + __shfl_down(wb, off, 64);
+    }
+  }
+  __syncthreads();
+  a = wa;
+  b = wb;
+}
+
+// dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
+// chunks of 256 with the same tree.  Done in place by the last tile (chunk g
+// is written to slot g <= 256 g, which is already consumed).  Returns the
+// final value in thread 0.
+template <typename T>
+__device__ T reduce_levels(T* part, uint32_t len, T* sh) {
+  while (len > 1u) {
+    const uint32_t out_len = (len + 255u) / 256u;
+    for (uint32_t g = 0; g < out_len; ++g) {
+      const uint32_t i = g * 256u + threadIdx.x;
+      T v = i < len ? part[i] : T(0);
+      T s = tree256<T>(v, sh);
+      if (threadIdx.x == 0) part[g] = s;
+    }
+    __syncthreads();
+    len = out_len;
+  }
+  T r = part[0];
+  return r;
+}
+
+// Last-tile election: thread 0 publishes the tile's partial(s) before this
+// call; returns true in every thread of the system's last tile to finish.
+__device__ __forceinline__ bool last_tile(uint32_t* cnt, uint32_t n_tiles, int* s_flag) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(cnt, 1u);
+    *s_flag = (prev == n_tiles - 1u);
+  }
+  __syncthreads();
+  const bool last = *s_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+struct TileRef {
+  SysDesc d;
+  uint32_t tile, row0, nrows;
+};
+
+__device__ __forceinline__ bool load_tile(const SysDesc* __restrict__ descs,
+                                          const uint2* __restrict__ tiles, uint32_t n_tiles,
+                                          uint32_t remap, TileRef& tr) {
+  const uint32_t b = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (b >= n_tiles) return false;
+  const uint2 td = tiles[b];
+  tr.d = descs[td.x];
+  tr.tile = td.y;
+  tr.row0 = td.y * (uint32_t)kTile;
+  const uint32_t left = tr.d.n - tr.row0;
+  tr.nrows = left < (uint32_t)kTile ? left : (uint32_t)kTile;
+  return true;
+}
+
+// ------------------------------------------------------------ SpMV tile body
+
+// spmv.wgsl:52-63 for rows [row0, row0+nrows): thread t owns row row0+t and
+// sums its products sequentially from 0.0.  The tile's CSR entries are
+// streamed in coalesced chunks; each product v_k * x[c_k] is rounded once
+// (as the WGSL sum + values[k]*x[j] without contraction) and parked in LDS.
+template <typename TV, typename TA>
+__device__ __forceinline__ TA spmv_tile(const uint32_t* __restrict__ row_ptr,
+                                        const uint32_t* __restrict__ col,
+                                        const float* __restrict__ val,
+                                        const TV* __restrict__ v, uint32_t row0, uint32_t nrows,
+                                        uint32_t* s_rp, TA* s_prod) {
+  constexpr int per = (kChunk * (int)sizeof(float)) / ((int)sizeof(TA) * kTile);
+  constexpr uint32_t chunk = per * kTile;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i <= nrows; i += kTile) s_rp[i] = row_ptr[row0 + i];
+  __syncthreads();
+  const uint32_t e0 = s_rp[0], e1 = s_rp[nrows];
+  uint32_t mb = 0, me = 0;
+  if (t < nrows) {
+    mb = s_rp[t];
+    me = s_rp[t + 1];
+  }
+  TA sum = TA(0);
+  for (uint32_t cs = e0; cs < e1; cs += chunk) {
+    const uint32_t ce = (e1 - cs) < chunk ? e1 : cs + chunk;
+    uint32_t c[per];
+    float a[per];
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      const uint32_t e = cs + (uint32_t)k * kTile + t;
+      const bool ok = e < ce;
+      c[k] = ok ? __builtin_nontemporal_load(col + e) : 0u;
+      a[k] = ok ? __builtin_nontemporal_load(val + e) : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      const uint32_t e = cs + (uint32_t)k * kTile + t;
+      if (e < ce) s_prod[k * kTile + t] = (TA)a[k] * (TA)v[c[k]];
+    }
+    __syncthreads();
+    const uint32_t lo = mb > cs ? mb : cs;
+    const uint32_t hi = me < ce ? me : ce;
+#pragma unroll 4
+    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod[e - cs];
+    __syncthreads();
+  }
+  return sum;
+}
+
+// --------------------------------------------------- block-Jacobi tile body
+
+// z for the rows [row0, row0+nrows) of one tile (block_jacobi.wgsl:63-127).
+// Thread j solves block bfirst+j, i.e. every block whose rows meet the tile;
+// factors are staged through LDS kLuStage blocks at a time (stride 37 words
+// to spread LDS banks).  rnew(row) gives the r entry the solve reads; rows
+// inside the tile come from s_r, rows of straddling blocks outside it are
+// recomputed by the same formula (bit-identical to the owner's value).
+template <typename RNew>
+__device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const float* __restrict__ lu,
+                                        uint32_t n, uint32_t nb, uint32_t row0, uint32_t nrows,
+                                        const float* s_r, float* s_z, float* s_lu,
+                                        uint32_t* s_blk, RNew rnew) {
+  const uint32_t t = threadIdx.x;
+  if (t == 0) {
+    // largest b with bs[b] <= row: first and last block touching the tile
+    uint32_t lo = 0, hi = nb;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bs[mid] <= row0) lo = mid; else hi = mid;
+    }
+    const uint32_t bfirst = lo;
+    const uint32_t last_row = row0 + nrows - 1u;
+    hi = nb;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bs[mid] <= last_row) lo = mid; else hi = mid;
+    }
+    s_blk[0] = bfirst;
+    s_blk[1] = lo - bfirst + 1u;
+  }
+  __syncthreads();
+  const uint32_t bfirst = s_blk[0], nblk = s_blk[1];
+  for (uint32_t g = 0; g < nblk; g += kLuStage) {
+    const uint32_t cnt = (nblk - g) < (uint32_t)kLuStage ? nblk - g : (uint32_t)kLuStage;
+    const float* src = lu + (size_t)(bfirst + g) * kLuStride;
+    for (uint32_t i = t; i < cnt * kLuStride; i += kTile) {
+      const uint32_t jb = i / kLuStride, k = i - jb * kLuStride;
+      s_lu[jb * 37u + k] = src[i];
+    }
+    __syncthreads();
+    const uint32_t j = t - g;
+    if (t >= g && j < cnt) {
+      const uint32_t b = bfirst + g + j;
+      const uint32_t s = bs[b], e = bs[b + 1];
+      if (s < n && e > s) {  // block_jacobi.wgsl:74-76
+        const uint32_t m = (e - s) < (uint32_t)kBlock ? e - s : (uint32_t)kBlock;
+        if (s + m > row0) {
+          const float* L = s_lu + j * 37u;
+          float y[kBlock], xx[kBlock];
+#pragma unroll
+          for (int i = 0; i < kBlock; ++i) {
+            if ((uint32_t)i < m) {
+              const uint32_t row = s + i;
+              float sum = (row >= row0 && row < row0 + nrows) ? s_r[row - row0] : rnew(row);
+#pragma unroll
+              for (int jj = 0; jj < i; ++jj) sum = sum - L[i * 6 + jj] * y[jj];
+              y[i] = sum;
+            }
+          }
+#pragma unroll
+          for (int i = kBlock - 1; i >= 0; --i) {
+            if ((uint32_t)i < m) {
+              float sum = y[i];
+#pragma unroll
+              for (int jj = i + 1; jj < kBlock; ++jj)
+                if ((uint32_t)jj < m) sum = sum - L[i * 6 + jj] * xx[jj];
+              xx[i] = sum / L[i * 6 + i];
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < kBlock; ++i) {
+            const uint32_t row = s + i;
+            if ((uint32_t)i < m && row >= row0 && row < row0 + nrows) s_z[row - row0] = xx[i];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ PCG kernels
+
+__global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ descs,
+                                                    const uint2* __restrict__ tiles,
+                                                    uint32_t n_tiles, uint32_t remap) {
+  __shared__ uint32_t s_rp[kTile + 1];
+  __shared__ double s_prod_d[kChunk / 2];
+  __shared__ double s_ra[kTile];
+  __shared__ double s_rb[kTile];
+  __shared__ int s_flag;
+  TileRef tr;
+  if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
+  const SysDesc& d = tr.d;
+  SysState* st = d.st;
+  const int32_t status = st->status;
+  const int32_t phase = st->phase;
+  if (status != kRunning) return;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t;
+  const bool own = t < tr.nrows;
+  float* s_red = reinterpret_cast<float*>(s_ra);
+
+  if (phase == kPhInit || phase == kPhIter) {
+    const bool init = phase == kPhInit;
+    const float* v = init ? d.x : d.p;
+    const float q = spmv_tile<float, float>(d.row_ptr, d.col, d.val, v, tr.row0, tr.nrows, s_rp,
+                                            reinterpret_cast<float*>(s_prod_d));
+    // dot(b,b) at init (compute.rs:235-242), dot(p,Ap) per iteration (:353-360)
+    float w = 0.0f;
+    double w64 = 0.0;
+    if (own) {
+      d.q[row] = q;
+      const float a = init ? d.b[row] : v[row];
+      const float c = init ? a : q;
+      w = a * c;
+      w64 = (double)a * (double)c;
+    }
+    const bool dot64 = (d.flags & kFlagDot64) && !init;
+    if (dot64) {
+      const double pd = tree256<double>(w64, s_rb);
+      if (t == 0) d.part64[tr.tile] = pd;
+    } else {
+      const float pd = tree256<float>(w, s_red);
+      if (t == 0) d.part[tr.tile] = pd;
+    }
+    if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
+    if (dot64) {
+      const double tot = reduce_levels<double>(d.part64, d.n_tiles, s_rb);
+      if (t == 0) {
+        st->cnt1 = 0;
+        const float pap = (float)tot;
+        const float alpha = tot != 0.0 ? (float)(st->rz64_old / tot) : 0.0f;
+        st->p_ap = pap;
+        st->p_ap64 = tot;
+        st->alpha = alpha;
+        st->neg_alpha = -alpha;
+      }
+    } else {
+      const float tot = reduce_levels<float>(d.part, d.n_tiles, s_red);
+      if (t == 0) {
+        st->cnt1 = 0;
+        if (init) {
+          st->b_norm2 = tot;
+          if (tot == 0.0f) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
+        } else {
+          // pcg_update_scalars.wgsl:59-62, 70-71
+          const float alpha = tot != 0.0f ? st->rz_old / tot : 0.0f;
+          st->p_ap = tot;
+          st->alpha = alpha;
+          st->neg_alpha = -alpha;
+        }
+      }
+    }
+    return;
+  }
+
+  if (phase == kPhIrResid) {
+    // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
+    const double ax = spmv_tile<double, double>(d.row_ptr, d.col, d.val, d.x64, tr.row0,
+                                                tr.nrows, s_rp, s_prod_d);
+    double rr = 0.0, bb = 0.0;
+    if (own) {
+      const double bi = d.b64[row];
+      const double ri = bi - ax;
+      d.r64[row] = ri;
+      rr = ri * ri;
+      bb = bi * bi;
+    }
+    tree256x2<double>(rr, bb, s_ra, s_rb);
+    if (t == 0) {
+      d.part64[tr.tile] = rr;
+      d.part64[d.n_tiles + tr.tile] = bb;
+    }
+    if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
+    const double rr_t = reduce_levels<double>(d.part64, d.n_tiles, s_ra);
+    const double bb_t = reduce_levels<double>(d.part64 + d.n_tiles, d.n_tiles, s_rb);
+    if (t == 0) {
+      st->cnt1 = 0;
+      st->rr64 = rr_t;
+      st->bb64 = bb_t;
+      if (bb_t == 0.0 || rr_t <= st->target2 * bb_t) {
+        st->status = 0;
+      } else if (st->outer >= st->max_outer) {
+        st->status = 5;  // HBJ_E_NOT_CONVERGED
+      } else {
+        const double nr = sqrt(rr_t);
+        st->scale = 1.0 / nr;
+        st->inv_scale = nr;
+        st->outer += 1u;
+        st->phase = kPhIrReinit;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict__ descs,
+                                                      const uint2* __restrict__ tiles,
+                                                      uint32_t n_tiles, uint32_t remap) {
+  __shared__ float s_r[kTile];
+  __shared__ float s_z[kTile];
+  __shared__ float s_lu[kLuStage * 37];
+  __shared__ double s_ra[kTile];
+  __shared__ double s_rb[kTile];
+  __shared__ uint32_t s_blk[2];
+  __shared__ int s_flag;
+  TileRef tr;
+  if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
+  const SysDesc& d = tr.d;
+  SysState* st = d.st;
+  const int32_t status = st->status;
+  const int32_t phase = st->phase;
+  if (status != kRunning) return;
+  if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t;
+  const bool own = t < tr.nrows;
+  const float* __restrict__ r_in = st->r_in;
+  float* __restrict__ r_out = st->r_out;
+  const float na = st->neg_alpha;
+  const float alpha = st->alpha;
+  const double scale = st->scale;
+
+  // r entry of this iteration for any row (used for straddling blocks)
+  auto rnew = [&](uint32_t i) -> float {
+    if (phase == kPhIter) return r_in[i] + na * d.q[i];      // compute.rs:395-403
+    if (phase == kPhInit) return d.b[i] + (-1.0f) * d.q[i];  // compute.rs:294,301-308
+    return (float)(d.r64[i] * scale);                        // IR restart: b32 = r64*scale
+  };
+
+  float rn = 0.0f;
+  if (own) {
+    if (phase == kPhIter) {
+      const float xi = d.x[row], pi = d.p[row];
+      d.x[row] = xi + alpha * pi;  // compute.rs:386-394
+      rn = r_in[row] + na * d.q[row];
+    } else if (phase == kPhInit) {
+      rn = d.b[row] + (-1.0f) * d.q[row];
+    } else {
+      rn = (float)(d.r64[row] * scale);
+      d.b[row] = rn;
+      d.x[row] = 0.0f;
+    }
+    r_out[row] = rn;
+    s_r[t] = rn;
+    if (d.flags & kFlagOversize) s_z[t] = d.z[row];  // rows no block writes keep z
+  }
+  __syncthreads();
+  bj_tile(d.bstart, d.lu, d.n, d.nb, tr.row0, tr.nrows, s_r, s_z, s_lu, s_blk, rnew);
+  float zi = 0.0f;
+  if (own) {
+    zi = s_z[t];
+    d.z[row] = zi;
+  }
+  // r.r (compute.rs:406-413) and r.z (:419-426)
+  const bool dot64 = (d.flags & kFlagDot64) != 0;
+  if (dot64) {
+    double a = own ? (double)rn * (double)rn : 0.0;
+    double b = own ? (double)rn * (double)zi : 0.0;
+    tree256x2<double>(a, b, s_ra, s_rb);
+    if (t == 0) {
+      d.part64[d.n_tiles + tr.tile] = a;
+      d.part64[2u * d.n_tiles + tr.tile] = b;
+    }
+  } else {
+    float a = own ? rn * rn : 0.0f;
+    float b = own ? rn * zi : 0.0f;
+    tree256x2<float>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
+    if (t == 0) {
+      d.part[d.n_tiles + tr.tile] = a;
+      d.part[2u * d.n_tiles + tr.tile] = b;
+    }
+  }
+  if (!last_tile(&st->cnt2, d.n_tiles, &s_flag)) return;
+  float rr, rz;
+  double rz64 = 0.0;
+  if (dot64) {
+    const double a = reduce_levels<double>(d.part64 + d.n_tiles, d.n_tiles, s_ra);
+    rz64 = reduce_levels<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, s_rb);
+    rr = (float)a;
+    rz = (float)rz64;
+  } else {
+    rr = reduce_levels<float>(d.part + d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_ra));
+    rz = reduce_levels<float>(d.part + 2u * d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_rb));
+  }
+  if (t != 0) return;
+  st->cnt2 = 0;
+  st->r_in = r_out;  // r of this iteration becomes the input of the next
+  st->r_out = const_cast<float*>(r_in);
+  if (phase == kPhInit) {
+    st->rz_old = rz;  // compute.rs:316-330
+    st->rz64_old = rz64;
+    st->phase = kPhIter;
+    st->iter = 0;
+    if (st->max_iter == 0) st->status = 5;  // loop body never runs (compute.rs:336,491)
+    return;
+  }
+  if (phase == kPhIrReinit) {
+    st->rz_old = rz;
+    st->rz64_old = rz64;
+    st->b_norm2 = rr;  // ||b32||^2 of the restarted inner solve
+    st->phase = kPhIter;
+    st->iter = 0;
+    return;
+  }
+  // pcg_update_scalars.wgsl:64-67 (late update), then compute.rs:468-488
+  const float rz_old = st->rz_old, p_ap = st->p_ap;
+  float beta = rz_old != 0.0f ? rz / rz_old : 0.0f;
+  if (dot64) beta = st->rz64_old != 0.0 ? (float)(rz64 / st->rz64_old) : 0.0f;
+  st->beta = beta;
+  st->rz_new = rz;
+  st->rr = rr;
+  const uint32_t it = st->iter + 1u;
+  st->iter = it;
+  st->total_iter += 1u;
+  int32_t fin = kRunning;
+  if (p_ap == 0.0f) fin = 3;            // HBJ_E_BREAKDOWN_PAP
+  else if (rz_old == 0.0f) fin = 4;     // HBJ_E_BREAKDOWN_RZ
+  else if (rr <= st->abs_tol2 || rr <= st->rel_tol2 * st->b_norm2) fin = 0;
+  else if (it >= st->max_iter) fin = 5;  // HBJ_E_NOT_CONVERGED
+  if (fin == kRunning) {
+    st->rz_old = rz;
+    st->rz64_old = rz64;
+  } else if (d.flags & kFlagIr) {
+    st->phase = kPhIrResid;  // fold the correction, then a new FP64 residual
+  } else {
+    st->status = fin;
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_pcg_pupdate(const SysDesc* __restrict__ descs,
+                                                       const uint2* __restrict__ tiles,
+                                                       uint32_t n_tiles, uint32_t remap) {
+  TileRef tr;
+  if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
+  const SysDesc& d = tr.d;
+  const SysState* st = d.st;
+  if (st->status != kRunning) return;
+  const int32_t phase = st->phase;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t;
+  if (t >= tr.nrows) return;
+  if (phase == kPhIter) {
+    if (st->iter == 0u) {
+      d.p[row] = d.z[row];  // p0 = z0 (compute.rs:313-314)
+    } else {
+      const float beta = st->beta;
+      const float pb = d.p[row] * beta;  // scale_from_scalar_results.wgsl:37
+      d.p[row] = pb + d.z[row];          // axpy.wgsl:41 with alpha = 1
+    }
+  } else if (phase == kPhIrResid) {
+    d.x64[row] = d.x64[row] + (double)d.x[row] * st->inv_scale;
+  }
+}
+
+// FP64 check of an FP32 solution: per tile ||b - A x||^2 and ||b||^2 with
+// FP32 inputs promoted exactly to FP64 (host sums the n_tiles partials).
+__global__ __launch_bounds__(kTile) void k_true_resid(const SysDesc* __restrict__ dp,
+                                                      const float* __restrict__ x,
+                                                      double* __restrict__ part2,
+                                                      uint32_t n_tiles) {
+  __shared__ uint32_t s_rp[kTile + 1];
+  __shared__ double s_prod_d[kChunk / 2];
+  __shared__ double s_ra[kTile];
+  __shared__ double s_rb[kTile];
+  const SysDesc d = *dp;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t row0 = tile * kTile;
+  const uint32_t nrows = (d.n - row0) < (uint32_t)kTile ? d.n - row0 : (uint32_t)kTile;
+  const double ax = spmv_tile<float, double>(d.row_ptr, d.col, d.val, x, row0, nrows, s_rp,
+                                             s_prod_d);
+  double rr = 0.0, bb = 0.0;
+  if (threadIdx.x < nrows) {
+    const double bi = (double)d.b[row0 + threadIdx.x];
+    const double ri = bi - ax;
+    rr = ri * ri;
+    bb = bi * bi;
+  }
+  tree256x2<double>(rr, bb, s_ra, s_rb);
+  if (threadIdx.x == 0) {
+    part2[tile] = rr;
+    part2[n_tiles + tile] = bb;
+  }
+}
+
+void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t n_tiles,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_true_resid, dim3(n_tiles), dim3(kTile), 0, s, d, x, part2, n_tiles);
+}
+
+// ------------------------------------------------- single-op kernels (KATs)
+
+__global__ void k_op_axpy(uint32_t n, float alpha, const float* __restrict__ x, float* __restrict__ y) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] + alpha * x[i];
+}
+
+__global__ void k_op_scale(uint32_t n, float beta, float* __restrict__ x) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] * beta;
+}
+
+__global__ __launch_bounds__(kTile) void k_op_dot(uint32_t n, const float* __restrict__ a,
+                                                  const float* __restrict__ b, float* part,
+                                                  uint32_t* cnt, float* out) {
+  __shared__ float sh[kTile];
+  __shared__ int s_flag;
+  const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+  const float v = i < n ? a[i] * b[i] : 0.0f;
+  const float p = tree256<float>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = p;
+  if (!last_tile(cnt, gridDim.x, &s_flag)) return;
+  const float r = reduce_levels<float>(part, gridDim.x, sh);
+  if (threadIdx.x == 0) {
+    *out = r;
+    *cnt = 0;
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_op_spmv(uint32_t n, const uint32_t* __restrict__ rp,
+                                                   const uint32_t* __restrict__ col,
+                                                   const float* __restrict__ val,
+                                                   const float* __restrict__ x, float* __restrict__ y) {
+  __shared__ uint32_t s_rp[kTile + 1];
+  __shared__ float s_prod[kChunk];
+  const uint32_t row0 = blockIdx.x * kTile;
+  const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
+  const float q = spmv_tile<float, float>(rp, col, val, x, row0, nrows, s_rp, s_prod);
+  if (threadIdx.x < nrows) y[row0 + threadIdx.x] = q;
+}
+
+__global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t nb,
+                                                           const float* __restrict__ lu,
+                                                           const uint32_t* __restrict__ bs,
+                                                           const float* __restrict__ r,
+                                                           float* __restrict__ z) {
+  __shared__ float s_r[kTile];
+  __shared__ float s_z[kTile];
+  __shared__ float s_lu[kLuStage * 37];
+  __shared__ uint32_t s_blk[2];
+  const uint32_t row0 = blockIdx.x * kTile;
+  const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
+  const uint32_t t = threadIdx.x;
+  if (t < nrows) {
+    s_r[t] = r[row0 + t];
+    s_z[t] = z[row0 + t];
+  }
+  __syncthreads();
+  bj_tile(bs, lu, n, nb, row0, nrows, s_r, s_z, s_lu, s_blk, [&](uint32_t i) { return r[i]; });
+  if (t < nrows) z[row0 + t] = s_z[t];
+}
+
+__global__ void k_op_update_scalars(float* s, uint32_t i_pap, uint32_t i_rz_new, uint32_t i_rz_old,
+                                    uint32_t i_alpha, uint32_t i_malpha, uint32_t i_beta) {
+  // pcg_update_scalars.wgsl:47-73
+  const float p_ap = s[i_pap], rz_new = s[i_rz_new], rz_old = s[i_rz_old];
+  float alpha = 0.0f, beta = 0.0f;
+  if (p_ap != 0.0f) alpha = rz_old / p_ap;
+  if (rz_old != 0.0f) beta = rz_new / rz_old;
+  s[i_alpha] = alpha;
+  s[i_malpha] = -alpha;
+  s[i_beta] = beta;
+}
+
+// ------------------------------------------------------- launch wrappers
+
+void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_spmv, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
+                     ls.n_tiles, ls.remap);
+}
+void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
+                     ls.n_tiles, ls.remap);
+}
+void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_pupdate, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
+                     ls.n_tiles, ls.remap);
+}
+
+void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_op_axpy, dim3((n + 255) / 256), dim3(256), 0, s, n, alpha, x, y);
+}
+void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_op_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, beta, x);
+}
+void launch_op_dot(uint32_t n, const float* a, const float* b, float* part, uint32_t* cnt,
+                   float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_op_dot, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, n, a, b, part,
+                     cnt, out);
+}
+void launch_op_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, const float* val,
+                    const float* x, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_op_spmv, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, n, rp, col,
+                     val, x, y);
+}
+void launch_op_block_jacobi(uint32_t n, uint32_t nb, const float* lu, const uint32_t* bs,
+                            const float* r, float* z, hipStream_t s) {
+  hipLaunchKernelGGL(k_op_block_jacobi, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, n, nb,
+                     lu, bs, r, z);
+}
+void launch_op_update_scalars(float* slots, uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                              uint32_t e, uint32_t f, hipStream_t s) {
+  hipLaunchKernelGGL(k_op_update_scalars, dim3(1), dim3(1), 0, s, slots, a, b, c, d, e, f);
+}
+
+}  // namespace hbj

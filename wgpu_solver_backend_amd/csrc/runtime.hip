@@ -1,0 +1,1104 @@
+// runtime.hip -- host runtime behind the C ABI: context, device-resident
+// systems, the iteration engine that drives the PCG kernels, single-op
+// executors, iterative refinement and batches.
+//
+// The reference drives one PCG iteration per submit and blocks on a scalar
+// read-back every iteration (compute.rs:336-489).  Here every decision of an
+// iteration (alpha, beta, breakdown, stop test) is taken on the device, and
+// the host only enqueues chunks of iterations and polls a status word.  The
+// poll of chunk c-1 overlaps the execution of chunk c, so the GPU never
+// waits on the host; iterations enqueued after a system stops exit at once.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hbj_internal.hpp"
+
+using namespace hbj;
+
+#define HIPCHK(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_error("%s failed: %s", #expr, hipGetErrorString(e_));              \
+      return HBJ_E_DEVICE;                                                    \
+    }                                                                         \
+  } while (0)
+
+#define CHK(expr)                    \
+  do {                               \
+    hbj_status s_ = (expr);          \
+    if (s_ != HBJ_OK) return s_;     \
+  } while (0)
+
+namespace {
+
+template <typename T>
+hbj_status dalloc(T** p, size_t count, uint64_t* acct = nullptr) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    set_error("hipMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
+    return HBJ_E_OOM;
+  }
+  if (acct) *acct += count * sizeof(T);
+  return HBJ_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+template <typename T>
+hbj_status halloc(T** p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error("hipHostMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
+    return HBJ_E_OOM;
+  }
+  return HBJ_OK;
+}
+
+template <typename T>
+void hfree(T*& p) {
+  if (p) (void)hipHostFree((void*)p);
+  p = nullptr;
+}
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+uint32_t env_u32(const char* name, uint32_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return (uint32_t)std::strtoul(v, nullptr, 10);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+
+struct hbj_ctx {
+  int device = 0;
+  hipDeviceProp_t prop{};
+  hipStream_t stream = nullptr;  // single-op executors
+  std::mutex mu;
+};
+
+// ------------------------------------------------------------------ engine
+
+namespace hbj {
+
+// Drives the three PCG kernels over a tile table until every system in it
+// has a final status.
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n_sys = 0, n_tiles = 0;
+  SysDesc* d_descs = nullptr;
+  uint2* d_tiles = nullptr;
+  SysState* d_states = nullptr;
+  SysState* h_init = nullptr;
+  SysState* h_poll[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec = nullptr;
+  uint32_t gchunk = 0;
+  uint32_t chunk = 16;
+  bool use_graph = true;
+  uint32_t remap = 1;
+  // kernel timing (HIP events around sampled iterations)
+  uint32_t timing_every = 0;
+  uint64_t trip = 0;  // iterations enqueued so far (for sampling)
+  std::vector<hipEvent_t> tev[2];
+  std::vector<int> tev_used;  // per pool: number of 4-event groups recorded
+  double t_sum[3] = {0, 0, 0};
+  uint64_t t_cnt[3] = {0, 0, 0};
+
+  hbj_status init(int dev, uint32_t nsys, uint32_t ntiles) {
+    device = dev;
+    n_sys = nsys;
+    n_tiles = ntiles;
+    chunk = env_u32("HBJ_CHUNK", 16);
+    if (chunk < 1) chunk = 1;
+    use_graph = env_u32("HBJ_GRAPH", 1) != 0;
+    remap = env_u32("HBJ_XCD_REMAP", 1);
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    CHK(dalloc(&d_descs, nsys));
+    CHK(dalloc(&d_tiles, ntiles));
+    CHK(dalloc(&d_states, nsys));
+    CHK(halloc(&h_init, nsys));
+    CHK(halloc(&h_poll[0], nsys));
+    CHK(halloc(&h_poll[1], nsys));
+    tev_used.assign(2, 0);
+    return HBJ_OK;
+  }
+
+  // descs[i].st is pointed at this engine's state slot i.
+  hbj_status set_table(std::vector<SysDesc> descs, const std::vector<uint2>& tiles) {
+    for (uint32_t i = 0; i < descs.size(); ++i) descs[i].st = d_states + i;
+    HIPCHK(hipMemcpy(d_descs, descs.data(), sizeof(SysDesc) * descs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
+    n_sys = (uint32_t)descs.size();
+    n_tiles = (uint32_t)tiles.size();
+    drop_graph();
+    return HBJ_OK;
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gchunk = 0;
+  }
+
+  LaunchSet ls() const { return LaunchSet{d_descs, d_tiles, n_tiles, remap}; }
+
+  void launch_iter() {
+    const LaunchSet l = ls();
+    launch_pcg_spmv(l, stream);
+    launch_pcg_update(l, stream);
+    launch_pcg_pupdate(l, stream);
+  }
+
+  hbj_status capture(uint32_t T) {
+    drop_graph();
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t i = 0; i < T; ++i) launch_iter();
+    HIPCHK(hipStreamEndCapture(stream, &g));
+    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      gexec = nullptr;
+      set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+      return HBJ_E_DEVICE;
+    }
+    gchunk = T;
+    return HBJ_OK;
+  }
+
+  hbj_status ensure_timing_events(uint32_t T) {
+    const size_t need = 4u * ((T + timing_every - 1) / timing_every + 1);
+    for (int b = 0; b < 2; ++b)
+      while (tev[b].size() < need) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        tev[b].push_back(e);
+      }
+    return HBJ_OK;
+  }
+
+  hbj_status enqueue_chunk(uint32_t T, int pool) {
+    if (timing_every) {
+      CHK(ensure_timing_events(T));
+      int used = 0;
+      const LaunchSet l = ls();
+      for (uint32_t i = 0; i < T; ++i, ++trip) {
+        if (trip % timing_every == 0) {
+          hipEvent_t* e = &tev[pool][4 * used];
+          HIPCHK(hipEventRecord(e[0], stream));
+          launch_pcg_spmv(l, stream);
+          HIPCHK(hipEventRecord(e[1], stream));
+          launch_pcg_update(l, stream);
+          HIPCHK(hipEventRecord(e[2], stream));
+          launch_pcg_pupdate(l, stream);
+          HIPCHK(hipEventRecord(e[3], stream));
+          ++used;
+        } else {
+          launch_iter();
+        }
+      }
+      tev_used[pool] = used;
+      HIPCHK(hipGetLastError());
+      return HBJ_OK;
+    }
+    if (use_graph) {
+      if (!gexec || gchunk != T) CHK(capture(T));
+      HIPCHK(hipGraphLaunch(gexec, stream));
+      return HBJ_OK;
+    }
+    for (uint32_t i = 0; i < T; ++i) launch_iter();
+    HIPCHK(hipGetLastError());
+    return HBJ_OK;
+  }
+
+  hbj_status harvest_timing(int pool) {
+    for (int u = 0; u < tev_used[pool]; ++u) {
+      hipEvent_t* e = &tev[pool][4 * u];
+      for (int k = 0; k < 3; ++k) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+        t_sum[k] += ms * 1000.0;
+        t_cnt[k] += 1;
+      }
+    }
+    tev_used[pool] = 0;
+    return HBJ_OK;
+  }
+
+  static bool all_done(const SysState* s, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i)
+      if (s[i].status == kRunning) return false;
+    return true;
+  }
+
+  // Runs until every system has a final status.  `init` holds n_sys states;
+  // final states are returned in `out`.
+  hbj_status run(const SysState* init, SysState* out, double* ms) {
+    HIPCHK(hipSetDevice(device));
+    if (n_tiles == 0 || all_done(init, n_sys)) {
+      std::memcpy(out, init, sizeof(SysState) * n_sys);
+      if (ms) *ms = 0.0;
+      return HBJ_OK;
+    }
+    // Upper bound on the iterations any system can run (each launch of the
+    // three kernels advances every running system by one step): the loop
+    // below never polls past it, whatever the device reports.
+    uint64_t max_trips = 0;
+    for (uint32_t i = 0; i < n_sys; ++i) {
+      const uint64_t t = init[i].phase == kPhIrResid
+                             ? (uint64_t)(init[i].max_outer + 1) * (init[i].max_iter + 3) + 4
+                             : (uint64_t)init[i].max_iter + 3;
+      max_trips = t > max_trips ? t : max_trips;
+    }
+    const uint64_t max_chunks = max_trips / chunk + 3;
+    if (use_graph && !timing_every && (!gexec || gchunk != chunk)) {
+      HIPCHK(hipStreamSynchronize(stream));
+      CHK(capture(chunk));
+    }
+    std::memcpy(h_init, init, sizeof(SysState) * n_sys);
+    HIPCHK(hipMemcpyAsync(d_states, h_init, sizeof(SysState) * n_sys, hipMemcpyHostToDevice, stream));
+    const double t0 = now_ms();
+    uint32_t c = 0;
+    int final_pool = -1;
+    for (;;) {
+      if (c >= max_chunks) {
+        HIPCHK(hipStreamSynchronize(stream));
+        set_error("engine: systems still running after %llu iterations (status not advancing)",
+                  (unsigned long long)(c * (uint64_t)chunk));
+        return HBJ_E_DEVICE;
+      }
+      const int pool = (int)(c & 1u);
+      CHK(enqueue_chunk(chunk, pool));
+      HIPCHK(hipMemcpyAsync(h_poll[pool], d_states, sizeof(SysState) * n_sys,
+                            hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipEventRecord(ev[pool], stream));
+      if (c > 0) {
+        const int prev = pool ^ 1;
+        HIPCHK(hipEventSynchronize(ev[prev]));
+        if (timing_every) CHK(harvest_timing(prev));
+        if (all_done(h_poll[prev], n_sys)) {
+          final_pool = pool;
+          break;
+        }
+      }
+      ++c;
+    }
+    HIPCHK(hipEventSynchronize(ev[final_pool]));
+    if (timing_every) CHK(harvest_timing(final_pool));
+    if (ms) *ms = now_ms() - t0;
+    std::memcpy(out, h_poll[final_pool], sizeof(SysState) * n_sys);
+    return HBJ_OK;
+  }
+
+  void destroy() {
+    drop_graph();
+    for (int b = 0; b < 2; ++b) {
+      for (auto e : tev[b]) (void)hipEventDestroy(e);
+      tev[b].clear();
+      if (ev[b]) (void)hipEventDestroy(ev[b]);
+      ev[b] = nullptr;
+      hfree(h_poll[b]);
+    }
+    hfree(h_init);
+    dfree(d_descs);
+    dfree(d_tiles);
+    dfree(d_states);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+}  // namespace hbj
+
+// ------------------------------------------------------------------ system
+
+struct hbj_system {
+  hbj_ctx* ctx = nullptr;
+  uint32_t n = 0, nnz = 0, nb = 0, n_tiles = 0;
+  uint32_t flags = 0;
+  uint32_t *rp = nullptr, *col = nullptr, *bs = nullptr;
+  float *val = nullptr, *lu = nullptr;
+  float *x = nullptr, *x0 = nullptr, *b = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+  float *ra = nullptr, *rb = nullptr, *part = nullptr;
+  double* part64 = nullptr;
+  double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
+  float* b32 = nullptr;
+  double* h_part2 = nullptr;  // pinned, 2 * n_tiles
+  uint64_t bytes = 0;
+  Engine eng;
+  bool eng_ready = false;
+  int eng_mode = -1;  // 0 plain FP32, 1 IR
+  SysDesc* d_desc1 = nullptr;  // plain descriptor for the FP64 check kernel
+
+  SysDesc desc(bool ir) const {
+    SysDesc d{};
+    d.row_ptr = rp;
+    d.col = col;
+    d.val = val;
+    d.bstart = bs;
+    d.lu = lu;
+    d.x = x;
+    d.p = p;
+    d.q = q;
+    d.z = z;
+    d.b = ir ? b32 : b;
+    d.part = part;
+    d.part64 = part64;
+    d.x64 = x64;
+    d.b64 = b64;
+    d.r64 = r64;
+    d.st = nullptr;
+    d.n = n;
+    d.nb = nb;
+    d.n_tiles = n_tiles;
+    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u);
+    return d;
+  }
+
+  SysState init_state(bool ir, uint32_t max_iter, float rel_tol, float abs_tol,
+                      const hbj_ir_params* ip) const {
+    SysState s;
+    std::memset(&s, 0, sizeof s);
+    s.status = kRunning;
+    s.phase = ir ? kPhIrResid : kPhInit;
+    s.max_iter = ir ? ip->max_inner : max_iter;
+    s.rel_tol2 = ir ? ip->inner_rel_tol * ip->inner_rel_tol : rel_tol * rel_tol;  // compute.rs:256
+    s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257
+    s.r_in = ra;
+    s.r_out = rb;
+    if (ir) {
+      s.max_outer = ip->max_outer;
+      s.target2 = ip->target_rel_res * ip->target_rel_res;
+    }
+    return s;
+  }
+
+  hbj_status ensure_ir_buffers() {
+    if (x64) return HBJ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(dalloc(&x64, n, &bytes));
+    CHK(dalloc(&x0_64, n, &bytes));
+    CHK(dalloc(&b64, n, &bytes));
+    CHK(dalloc(&r64, n, &bytes));
+    CHK(dalloc(&b32, n, &bytes));
+    HIPCHK(hipMemset(x0_64, 0, sizeof(double) * n));
+    HIPCHK(hipMemset(b64, 0, sizeof(double) * n));
+    return HBJ_OK;
+  }
+
+  hbj_status ensure_engine(int mode) {
+    if (!eng_ready) {
+      CHK(eng.init(ctx->device, 1, n_tiles));
+      eng_ready = true;
+    }
+    if (eng_mode != mode) {
+      std::vector<uint2> tiles(n_tiles);
+      for (uint32_t t = 0; t < n_tiles; ++t) tiles[t] = make_uint2(0u, t);
+      CHK(eng.set_table({desc(mode == 1)}, tiles));
+      eng_mode = mode;
+    }
+    return HBJ_OK;
+  }
+
+  // ||b - A x|| / ||b|| in FP64 for the FP32 solution in `x` (plain mode).
+  hbj_status true_residual(hipStream_t s, double* out) {
+    launch_true_resid(d_desc1, x, part64, n_tiles, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h_part2, part64, sizeof(double) * 2 * n_tiles, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double rr = 0, bb = 0;
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+      rr += h_part2[t];
+      bb += h_part2[n_tiles + t];
+    }
+    *out = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    return HBJ_OK;
+  }
+
+  void release() {
+    if (eng_ready) eng.destroy();
+    eng_ready = false;
+    dfree(rp); dfree(col); dfree(bs); dfree(val); dfree(lu);
+    dfree(x); dfree(x0); dfree(b); dfree(p); dfree(q); dfree(z);
+    dfree(ra); dfree(rb); dfree(part); dfree(part64);
+    dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
+    dfree(d_desc1);
+    hfree(h_part2);
+  }
+};
+
+namespace {
+
+void status_message(int32_t st, uint32_t max_iter) {
+  switch (st) {
+    case HBJ_E_BREAKDOWN_PAP:
+      set_error("PCG(BlockJacobiGpu): dot(p,Ap) is zero (breakdown)");  // compute.rs:474
+      break;
+    case HBJ_E_BREAKDOWN_RZ:
+      set_error("PCG(BlockJacobiGpu): rz_old is zero (breakdown)");  // compute.rs:477
+      break;
+    case HBJ_E_NOT_CONVERGED:
+      set_error("PCG(BlockJacobiGpu): did not converge in %u iterations", max_iter);  // :492
+      break;
+    default:
+      break;
+  }
+}
+
+void fill_result(const SysState& s, bool ir, double ms, hbj_result* r) {
+  if (!r) return;
+  std::memset(r, 0, sizeof *r);
+  r->status = s.status;
+  r->iterations = ir ? s.total_iter : s.iter;
+  r->outer_iterations = ir ? s.outer : 0;
+  r->rr = s.rr;
+  r->b_norm2 = s.b_norm2;
+  r->rel_res_recurrence = s.b_norm2 > 0 ? std::sqrt((double)s.rr / (double)s.b_norm2) : 0.0;
+  r->rel_res_true_fp64 = ir ? (s.bb64 > 0 ? std::sqrt(s.rr64 / s.bb64) : 0.0) : NAN;
+  r->solve_ms = ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hbj_version(void) { return "hbj 0.1 (gfx950)"; }
+
+hbj_status hbj_ctx_create(int device, hbj_ctx** out) {
+  clear_error();
+  if (!out) {
+    set_error("hbj_ctx_create: out is null");
+    return HBJ_E_INVALID;
+  }
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    set_error("no HIP device available (hipGetDeviceCount: %s, count %d)", hipGetErrorString(e),
+              count);
+    return HBJ_E_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    set_error("device %d out of range (%d devices)", device, count);
+    return HBJ_E_INVALID;
+  }
+  auto* c = new hbj_ctx();
+  c->device = device;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&c->prop, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error("device init failed: %s", hipGetErrorString(e));
+    delete c;
+    return HBJ_E_DEVICE;
+  }
+  *out = c;
+  return HBJ_OK;
+}
+
+void hbj_ctx_destroy(hbj_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+hbj_status hbj_ctx_describe(hbj_ctx* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || len == 0) {
+    set_error("hbj_ctx_describe: bad argument");
+    return HBJ_E_INVALID;
+  }
+  const hipDeviceProp_t& p = ctx->prop;
+  snprintf(buf, len,
+           "{\"adapter_name\": \"%s\", \"backend\": \"HIP\", \"device_type\": \"DiscreteGpu\", "
+           "\"arch\": \"%s\", \"device\": %d, \"compute_units\": %d, \"hbm_bytes\": %zu, "
+           "\"l2_bytes\": %d, \"clock_khz\": %d}",
+           p.name, p.gcnArchName, ctx->device, p.multiProcessorCount, (size_t)p.totalGlobalMem,
+           p.l2CacheSize, p.clockRate);
+  return HBJ_OK;
+}
+
+// ----------------------------------------------------------- single-op API
+
+hbj_status hbj_op_axpy(hbj_ctx* ctx, uint32_t n, float alpha, const float* x, float* y) {
+  clear_error();
+  if (!ctx || (n && (!x || !y))) { set_error("hbj_op_axpy: bad argument"); return HBJ_E_INVALID; }
+  if (n == 0) return HBJ_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  float *dx = nullptr, *dy = nullptr;
+  hbj_status st = dalloc(&dx, n);
+  if (st == HBJ_OK) st = dalloc(&dy, n);
+  if (st == HBJ_OK) {
+    HIPCHK(hipMemcpy(dx, x, 4ull * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dy, y, 4ull * n, hipMemcpyHostToDevice));
+    launch_op_axpy(n, alpha, dx, dy, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(y, dy, 4ull * n, hipMemcpyDeviceToHost));
+  }
+  dfree(dx);
+  dfree(dy);
+  return st;
+}
+
+hbj_status hbj_op_scale(hbj_ctx* ctx, uint32_t n, float beta, float* x) {
+  clear_error();
+  if (!ctx || (n && !x)) { set_error("hbj_op_scale: bad argument"); return HBJ_E_INVALID; }
+  if (n == 0) return HBJ_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  float* dx = nullptr;
+  CHK(dalloc(&dx, n));
+  HIPCHK(hipMemcpy(dx, x, 4ull * n, hipMemcpyHostToDevice));
+  launch_op_scale(n, beta, dx, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(x, dx, 4ull * n, hipMemcpyDeviceToHost));
+  dfree(dx);
+  return HBJ_OK;
+}
+
+hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, float* out) {
+  clear_error();
+  if (!ctx || !out || (n && (!a || !b))) { set_error("hbj_op_dot: bad argument"); return HBJ_E_INVALID; }
+  if (n == 0) {  // dot_scalar_exec.rs:164-174
+    *out = 0.0f;
+    return HBJ_OK;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const uint32_t groups = (n + kTile - 1) / kTile;
+  float *da = nullptr, *db = nullptr, *part = nullptr, *dout = nullptr;
+  uint32_t* cnt = nullptr;
+  hbj_status st = dalloc(&da, n);
+  if (st == HBJ_OK) st = dalloc(&db, n);
+  if (st == HBJ_OK) st = dalloc(&part, groups);
+  if (st == HBJ_OK) st = dalloc(&dout, 1);
+  if (st == HBJ_OK) st = dalloc(&cnt, 1);
+  if (st == HBJ_OK) {
+    HIPCHK(hipMemcpy(da, a, 4ull * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, b, 4ull * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(cnt, 0, 4));
+    launch_op_dot(n, da, db, part, cnt, dout, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, dout, 4, hipMemcpyDeviceToHost));
+  }
+  dfree(da); dfree(db); dfree(part); dfree(dout); dfree(cnt);
+  return st;
+}
+
+hbj_status hbj_op_spmv(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                       const uint32_t* col_idx, const float* values, const float* x, float* y) {
+  clear_error();
+  if (!ctx || !row_ptr || (n && (!x || !y)) || (nnz && (!col_idx || !values))) {
+    set_error("hbj_op_spmv: bad argument");
+    return HBJ_E_INVALID;
+  }
+  if (row_ptr[n] != nnz) {
+    set_error("CSR row_ptr must end with nnz (%u), got Some(%u)", nnz, row_ptr[n]);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t k = 0; k < nnz; ++k)
+    if (col_idx[k] >= n) {
+      set_error("CSR col_idx contains out-of-range column %u for n_cols=%u", col_idx[k], n);
+      return HBJ_E_INVALID;
+    }
+  if (n == 0) return HBJ_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  uint32_t *drp = nullptr, *dcol = nullptr;
+  float *dval = nullptr, *dx = nullptr, *dy = nullptr;
+  hbj_status st = dalloc(&drp, n + 1);
+  if (st == HBJ_OK) st = dalloc(&dcol, nnz);
+  if (st == HBJ_OK) st = dalloc(&dval, nnz);
+  if (st == HBJ_OK) st = dalloc(&dx, n);
+  if (st == HBJ_OK) st = dalloc(&dy, n);
+  if (st == HBJ_OK) {
+    HIPCHK(hipMemcpy(drp, row_ptr, 4ull * (n + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+      HIPCHK(hipMemcpy(dcol, col_idx, 4ull * nnz, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(dval, values, 4ull * nnz, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(dx, x, 4ull * n, hipMemcpyHostToDevice));
+    launch_op_spmv(n, drp, dcol, dval, dx, dy, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(y, dy, 4ull * n, hipMemcpyDeviceToHost));
+  }
+  dfree(drp); dfree(dcol); dfree(dval); dfree(dx); dfree(dy);
+  return st;
+}
+
+hbj_status hbj_op_block_jacobi(hbj_ctx* ctx, uint32_t n, const float* lu, const uint32_t* bs,
+                               uint32_t n_starts, const float* r, float* z) {
+  clear_error();
+  if (!ctx || !bs || (n && (!r || !z)) || (n_starts > 1 && !lu)) {
+    set_error("hbj_op_block_jacobi: bad argument");
+    return HBJ_E_INVALID;
+  }
+  if (n == 0 || n_starts < 2) return HBJ_OK;
+  // the tile search needs a partition of [0, n]
+  if (bs[0] != 0 || bs[n_starts - 1] != n) {
+    set_error("block_starts must start with 0 and end with n (%u)", n);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t w = 0; w + 1 < n_starts; ++w)
+    if (bs[w + 1] <= bs[w]) {
+      set_error("block_starts must be strictly increasing");
+      return HBJ_E_INVALID;
+    }
+  const uint32_t nb = n_starts - 1;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  float *dlu = nullptr, *dr = nullptr, *dz = nullptr;
+  uint32_t* dbs = nullptr;
+  hbj_status st = dalloc(&dlu, 36ull * nb);
+  if (st == HBJ_OK) st = dalloc(&dbs, n_starts);
+  if (st == HBJ_OK) st = dalloc(&dr, n);
+  if (st == HBJ_OK) st = dalloc(&dz, n);
+  if (st == HBJ_OK) {
+    HIPCHK(hipMemcpy(dlu, lu, 144ull * nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dbs, bs, 4ull * n_starts, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dr, r, 4ull * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dz, z, 4ull * n, hipMemcpyHostToDevice));
+    launch_op_block_jacobi(n, nb, dlu, dbs, dr, dz, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(z, dz, 4ull * n, hipMemcpyDeviceToHost));
+  }
+  dfree(dlu); dfree(dbs); dfree(dr); dfree(dz);
+  return st;
+}
+
+hbj_status hbj_op_update_scalars(hbj_ctx* ctx, float* slots, uint32_t n_slots, uint32_t a,
+                                 uint32_t b, uint32_t c, uint32_t d, uint32_t e, uint32_t f) {
+  clear_error();
+  if (!ctx || !slots) { set_error("hbj_op_update_scalars: bad argument"); return HBJ_E_INVALID; }
+  if (a >= n_slots || b >= n_slots || c >= n_slots || d >= n_slots || e >= n_slots || f >= n_slots) {
+    set_error("PcgUpdateScalarsExecutor: slot index out of range");
+    return HBJ_E_INVALID;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  float* ds = nullptr;
+  CHK(dalloc(&ds, n_slots));
+  HIPCHK(hipMemcpy(ds, slots, 4ull * n_slots, hipMemcpyHostToDevice));
+  launch_op_update_scalars(ds, a, b, c, d, e, f, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(slots, ds, 4ull * n_slots, hipMemcpyDeviceToHost));
+  dfree(ds);
+  return HBJ_OK;
+}
+
+// --------------------------------------------------------------- systems
+
+hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                             const uint32_t* col_idx, const float* values,
+                             const uint32_t* block_starts, uint32_t n_starts, hbj_system** out) {
+  clear_error();
+  if (!ctx || !out || (nnz && !values)) {
+    set_error("hbj_system_create: bad argument");
+    return HBJ_E_INVALID;
+  }
+  *out = nullptr;
+  CHK(validate_case(n, nnz, row_ptr, col_idx, block_starts, n_starts));
+  const uint32_t nb = n_starts - 1;
+  std::vector<float> lu_h(36ull * (nb ? nb : 1));
+  CHK(hbj_build_lu_blocks_6(n, row_ptr, n + 1, col_idx, nnz, values, nnz, block_starts, n_starts,
+                            lu_h.data()));
+  auto* s = new hbj_system();
+  s->ctx = ctx;
+  s->n = n;
+  s->nnz = nnz;
+  s->nb = nb;
+  s->n_tiles = (n + kTile - 1) / kTile;
+  for (uint32_t b = 0; b < nb; ++b)
+    if (block_starts[b + 1] - block_starts[b] > (uint32_t)kBlock) s->flags |= kFlagOversize;
+  auto fail = [&](hbj_status st) {
+    s->release();
+    delete s;
+    return st;
+  };
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) {
+    set_error("hipSetDevice failed: %s", hipGetErrorString(e));
+    return fail(HBJ_E_DEVICE);
+  }
+  hbj_status st = HBJ_OK;
+  const uint32_t nt = s->n_tiles ? s->n_tiles : 1;
+#define ALLOC(ptr, cnt) \
+  if (st == HBJ_OK) st = dalloc(&s->ptr, (cnt), &s->bytes)
+  ALLOC(rp, n + 1);
+  ALLOC(col, nnz);
+  ALLOC(val, nnz);
+  ALLOC(bs, n_starts);
+  ALLOC(lu, 36ull * (nb ? nb : 1));
+  ALLOC(x, n);
+  ALLOC(x0, n);
+  ALLOC(b, n);
+  ALLOC(p, n);
+  ALLOC(q, n);
+  ALLOC(z, n);
+  ALLOC(ra, n);
+  ALLOC(rb, n);
+  ALLOC(part, 3ull * nt);
+  ALLOC(part64, 3ull * nt);
+  ALLOC(d_desc1, 1);
+#undef ALLOC
+  if (st != HBJ_OK) return fail(st);
+  st = halloc(&s->h_part2, 2ull * nt);
+  if (st != HBJ_OK) return fail(st);
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes == 0 || e != hipSuccess) return;
+    e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  up(s->rp, row_ptr, 4ull * (n + 1));
+  up(s->col, col_idx, 4ull * nnz);
+  up(s->val, values, 4ull * nnz);
+  up(s->bs, block_starts, 4ull * n_starts);
+  up(s->lu, lu_h.data(), 144ull * nb);
+  if (e == hipSuccess && n) e = hipMemset(s->z, 0, 4ull * n);  // wgpu zero-initialises buffers
+  if (e == hipSuccess && n) e = hipMemset(s->x0, 0, 4ull * n);
+  if (e == hipSuccess && n) e = hipMemset(s->b, 0, 4ull * n);
+  if (e == hipSuccess) {
+    SysDesc d1 = s->desc(false);
+    e = hipMemcpy(s->d_desc1, &d1, sizeof d1, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    set_error("upload failed: %s", hipGetErrorString(e));
+    return fail(HBJ_E_DEVICE);
+  }
+  *out = s;
+  return HBJ_OK;
+}
+
+void hbj_system_destroy(hbj_system* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->ctx->device);
+  s->release();
+  delete s;
+}
+
+uint64_t hbj_system_device_bytes(const hbj_system* s) { return s ? s->bytes : 0; }
+
+hbj_status hbj_system_set_rhs(hbj_system* s, const float* b, const float* x0) {
+  clear_error();
+  if (!s || (s->n && !b)) { set_error("hbj_system_set_rhs: bad argument"); return HBJ_E_INVALID; }
+  if (s->n == 0) return HBJ_OK;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipMemcpy(s->b, b, 4ull * s->n, hipMemcpyHostToDevice));
+  if (x0) HIPCHK(hipMemcpy(s->x0, x0, 4ull * s->n, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(s->x0, 0, 4ull * s->n));
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_get_x(hbj_system* s, float* x) {
+  clear_error();
+  if (!s || (s->n && !x)) { set_error("hbj_system_get_x: bad argument"); return HBJ_E_INVALID; }
+  if (s->n == 0) return HBJ_OK;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipMemcpy(x, s->x, 4ull * s->n, hipMemcpyDeviceToHost));
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_set_kernel_timing(hbj_system* s, uint32_t every) {
+  if (!s) { set_error("hbj_system_set_kernel_timing: null system"); return HBJ_E_INVALID; }
+  CHK(s->ensure_engine(s->eng_mode < 0 ? 0 : s->eng_mode));
+  s->eng.timing_every = every;
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_kernel_timing(hbj_system* s, double* avg_us, uint64_t* counts) {
+  if (!s || !avg_us || !counts) { set_error("hbj_system_kernel_timing: bad argument"); return HBJ_E_INVALID; }
+  for (int k = 0; k < 3; ++k) {
+    counts[k] = s->eng_ready ? s->eng.t_cnt[k] : 0;
+    avg_us[k] = counts[k] ? s->eng.t_sum[k] / (double)counts[k] : 0.0;
+    if (s->eng_ready) {
+      s->eng.t_cnt[k] = 0;
+      s->eng.t_sum[k] = 0;
+    }
+  }
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_solve_resident(hbj_system* s, uint32_t max_iter, float rel_tol,
+                                     float abs_tol, hbj_result* res) {
+  clear_error();
+  if (!s) { set_error("hbj_system_solve_resident: null system"); return HBJ_E_INVALID; }
+  if (s->n == 0) {  // ||b||^2 of an empty vector is 0 -> Ok(0)
+    if (res) {
+      std::memset(res, 0, sizeof *res);
+      res->rel_res_true_fp64 = 0.0;
+    }
+    return HBJ_OK;
+  }
+  HIPCHK(hipSetDevice(s->ctx->device));
+  CHK(s->ensure_engine(0));
+  hipStream_t st = s->eng.stream;
+  HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, st));
+  if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
+  SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr), fin;
+  double ms = 0;
+  CHK(s->eng.run(&init, &fin, &ms));
+  fill_result(fin, false, ms, res);
+  if (fin.status == HBJ_OK && res) {
+    if (fin.b_norm2 == 0.0f) res->rel_res_true_fp64 = 0.0;
+    else CHK(s->true_residual(st, &res->rel_res_true_fp64));
+  }
+  if (fin.status != HBJ_OK) status_message(fin.status, max_iter);
+  return (hbj_status)fin.status;
+}
+
+hbj_status hbj_solve(hbj_system* s, const float* b, float* x_inout, uint32_t max_iter,
+                     float rel_tol, float abs_tol, hbj_result* res) {
+  clear_error();
+  if (!s || (s->n && (!b || !x_inout))) {
+    set_error("hbj_solve: bad argument");
+    return HBJ_E_INVALID;
+  }
+  CHK(hbj_system_set_rhs(s, b, x_inout));
+  hbj_status st = hbj_system_solve_resident(s, max_iter, rel_tol, abs_tol, res);
+  if (st == HBJ_OK) {
+    CHK(hbj_system_get_x(s, x_inout));
+  }
+  return st;
+}
+
+// ----------------------------------------------------------- mixed precision
+
+hbj_status hbj_system_set_rhs64(hbj_system* s, const double* b, const double* x0) {
+  clear_error();
+  if (!s || (s->n && !b)) { set_error("hbj_system_set_rhs64: bad argument"); return HBJ_E_INVALID; }
+  CHK(s->ensure_ir_buffers());
+  if (s->n == 0) return HBJ_OK;
+  HIPCHK(hipMemcpy(s->b64, b, 8ull * s->n, hipMemcpyHostToDevice));
+  if (x0) HIPCHK(hipMemcpy(s->x0_64, x0, 8ull * s->n, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(s->x0_64, 0, 8ull * s->n));
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_get_x64(hbj_system* s, double* x) {
+  clear_error();
+  if (!s || (s->n && !x)) { set_error("hbj_system_get_x64: bad argument"); return HBJ_E_INVALID; }
+  CHK(s->ensure_ir_buffers());
+  if (s->n == 0) return HBJ_OK;
+  HIPCHK(hipMemcpy(x, s->x64, 8ull * s->n, hipMemcpyDeviceToHost));
+  return HBJ_OK;
+}
+
+static hbj_status check_ir(const hbj_ir_params* p) {
+  if (!p || p->max_inner == 0 || !(p->inner_rel_tol > 0.0f) || !(p->target_rel_res > 0.0)) {
+    set_error("hbj_ir_params: max_inner > 0, inner_rel_tol > 0 and target_rel_res > 0 required");
+    return HBJ_E_INVALID;
+  }
+  return HBJ_OK;
+}
+
+static hbj_status solve_ir_resident(hbj_system* s, const hbj_ir_params* p, hbj_result* res) {
+  CHK(check_ir(p));
+  CHK(s->ensure_ir_buffers());
+  if (s->n == 0) {
+    if (res) std::memset(res, 0, sizeof *res);
+    return HBJ_OK;
+  }
+  HIPCHK(hipSetDevice(s->ctx->device));
+  CHK(s->ensure_engine(1));
+  hipStream_t st = s->eng.stream;
+  HIPCHK(hipMemcpyAsync(s->x64, s->x0_64, 8ull * s->n, hipMemcpyDeviceToDevice, st));
+  if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
+  SysState init = s->init_state(true, 0, 0, 0, p), fin;
+  double ms = 0;
+  CHK(s->eng.run(&init, &fin, &ms));
+  fill_result(fin, true, ms, res);
+  if (fin.status != HBJ_OK) {
+    if (fin.status == HBJ_E_NOT_CONVERGED)
+      set_error("IR: relative residual %.3e above target %.3e after %u refinement steps",
+                fin.bb64 > 0 ? std::sqrt(fin.rr64 / fin.bb64) : 0.0, p->target_rel_res, fin.outer);
+    else
+      status_message(fin.status, p->max_inner);
+  }
+  return (hbj_status)fin.status;
+}
+
+hbj_status hbj_solve_ir(hbj_system* s, const double* b, double* x_inout, const hbj_ir_params* p,
+                        hbj_result* res) {
+  clear_error();
+  if (!s || (s->n && (!b || !x_inout))) {
+    set_error("hbj_solve_ir: bad argument");
+    return HBJ_E_INVALID;
+  }
+  CHK(check_ir(p));
+  CHK(hbj_system_set_rhs64(s, b, x_inout));
+  hbj_status st = solve_ir_resident(s, p, res);
+  if (st == HBJ_OK) CHK(hbj_system_get_x64(s, x_inout));
+  return st;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+
+struct hbj_batch {
+  hbj_ctx* ctx = nullptr;
+  std::vector<hbj_system*> sys;
+  Engine eng;
+  bool ready = false;
+  int mode = -1;
+  uint32_t n_tiles = 0;
+
+  hbj_status ensure(int m) {
+    if (!ready) {
+      CHK(eng.init(ctx->device, (uint32_t)sys.size(), n_tiles ? n_tiles : 1));
+      ready = true;
+    }
+    if (mode != m) {
+      std::vector<SysDesc> descs;
+      std::vector<uint2> tiles;
+      tiles.reserve(n_tiles);
+      for (uint32_t i = 0; i < sys.size(); ++i) {
+        descs.push_back(sys[i]->desc(m == 1));
+        for (uint32_t t = 0; t < sys[i]->n_tiles; ++t) tiles.push_back(make_uint2(i, t));
+      }
+      if (tiles.empty()) tiles.push_back(make_uint2(0, 0));
+      CHK(eng.set_table(descs, tiles));
+      if (n_tiles == 0) eng.n_tiles = 0;
+      mode = m;
+    }
+    return HBJ_OK;
+  }
+};
+
+extern "C" {
+
+hbj_status hbj_batch_create(hbj_ctx* ctx, hbj_system* const* systems, uint32_t count,
+                            hbj_batch** out) {
+  clear_error();
+  if (!ctx || !systems || !out || count == 0) {
+    set_error("hbj_batch_create: bad argument");
+    return HBJ_E_INVALID;
+  }
+  auto* b = new hbj_batch();
+  b->ctx = ctx;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!systems[i] || systems[i]->ctx != ctx) {
+      delete b;
+      set_error("hbj_batch_create: system %u is null or on another context", i);
+      return HBJ_E_INVALID;
+    }
+    b->sys.push_back(systems[i]);
+    b->n_tiles += systems[i]->n_tiles;
+  }
+  *out = b;
+  return HBJ_OK;
+}
+
+void hbj_batch_destroy(hbj_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->ready) b->eng.destroy();
+  delete b;
+}
+
+hbj_status hbj_batch_solve(hbj_batch* b, uint32_t max_iter, float rel_tol, float abs_tol,
+                           hbj_result* results) {
+  clear_error();
+  if (!b || !results) { set_error("hbj_batch_solve: bad argument"); return HBJ_E_INVALID; }
+  HIPCHK(hipSetDevice(b->ctx->device));
+  CHK(b->ensure(0));
+  hipStream_t st = b->eng.stream;
+  const uint32_t S = (uint32_t)b->sys.size();
+  std::vector<SysState> init(S), fin(S);
+  for (uint32_t i = 0; i < S; ++i) {
+    hbj_system* s = b->sys[i];
+    init[i] = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr);
+    if (s->n == 0) {
+      init[i].status = HBJ_OK;
+      continue;
+    }
+    HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, st));
+    if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
+  }
+  double ms = 0;
+  CHK(b->eng.run(init.data(), fin.data(), &ms));
+  hbj_status worst = HBJ_OK;
+  for (uint32_t i = 0; i < S; ++i) {
+    fill_result(fin[i], false, ms, &results[i]);
+    hbj_system* s = b->sys[i];
+    if (fin[i].status == HBJ_OK && s->n) {
+      if (fin[i].b_norm2 == 0.0f) results[i].rel_res_true_fp64 = 0.0;
+      else CHK(s->true_residual(st, &results[i].rel_res_true_fp64));
+    }
+    if (fin[i].status != HBJ_OK && worst == HBJ_OK) {
+      worst = (hbj_status)fin[i].status;
+      status_message(fin[i].status, max_iter);
+    }
+  }
+  return worst;
+}
+
+hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* results) {
+  clear_error();
+  if (!b || !results) { set_error("hbj_batch_solve_ir: bad argument"); return HBJ_E_INVALID; }
+  CHK(check_ir(p));
+  HIPCHK(hipSetDevice(b->ctx->device));
+  for (auto* s : b->sys) CHK(s->ensure_ir_buffers());
+  if (b->mode != 1 && b->ready) b->mode = -1;
+  CHK(b->ensure(1));
+  hipStream_t st = b->eng.stream;
+  const uint32_t S = (uint32_t)b->sys.size();
+  std::vector<SysState> init(S), fin(S);
+  for (uint32_t i = 0; i < S; ++i) {
+    hbj_system* s = b->sys[i];
+    init[i] = s->init_state(true, 0, 0, 0, p);
+    if (s->n == 0) {
+      init[i].status = HBJ_OK;
+      continue;
+    }
+    HIPCHK(hipMemcpyAsync(s->x64, s->x0_64, 8ull * s->n, hipMemcpyDeviceToDevice, st));
+    if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
+  }
+  double ms = 0;
+  CHK(b->eng.run(init.data(), fin.data(), &ms));
+  hbj_status worst = HBJ_OK;
+  for (uint32_t i = 0; i < S; ++i) {
+    fill_result(fin[i], true, ms, &results[i]);
+    if (fin[i].status != HBJ_OK && worst == HBJ_OK) {
+      worst = (hbj_status)fin[i].status;
+      status_message(fin[i].status, p->max_inner);
+    }
+  }
+  return worst;
+}
+
+}  // extern "C"

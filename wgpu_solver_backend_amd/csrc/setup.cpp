@@ -1,0 +1,203 @@
+// setup.cpp -- host-side system setup: CSR / block_starts validation and the
+// 6x6 no-pivot LU block builder, with the reference's error texts.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hbj_internal.hpp"
+
+namespace hbj {
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+}
+void clear_error() { g_error.clear(); }
+
+// loaders.rs:84-105 (CSR) and :164-184 (block_starts)
+hbj_status validate_case(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                         const uint32_t* col_idx, const uint32_t* bs, uint32_t n_starts) {
+  if (!row_ptr || (nnz && !col_idx) || !bs) {
+    set_error("null pointer argument");
+    return HBJ_E_INVALID;
+  }
+  if (row_ptr[0] != 0) {
+    set_error("CSR row_ptr must start with 0");
+    return HBJ_E_INVALID;
+  }
+  if (row_ptr[n] != nnz) {
+    set_error("CSR row_ptr must end with nnz (%u), got Some(%u)", nnz, row_ptr[n]);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    if (row_ptr[i + 1] < row_ptr[i]) {
+      set_error("CSR row_ptr must be non-decreasing");
+      return HBJ_E_INVALID;
+    }
+  }
+  uint32_t max_col = 0;
+  for (uint32_t k = 0; k < nnz; ++k) max_col = col_idx[k] > max_col ? col_idx[k] : max_col;
+  if (n > 0 && nnz > 0 && max_col >= n) {
+    set_error("CSR col_idx contains out-of-range column %u for n_cols=%u", max_col, n);
+    return HBJ_E_INVALID;
+  }
+  if (n_starts == 0) {
+    set_error("block_starts must not be empty");
+    return HBJ_E_INVALID;
+  }
+  if (bs[0] != 0) {
+    set_error("block_starts must start with 0, got %u", bs[0]);
+    return HBJ_E_INVALID;
+  }
+  if (bs[n_starts - 1] != n) {
+    set_error("block_starts must end with n (%u), got %u", n, bs[n_starts - 1]);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t w = 0; w + 1 < n_starts; ++w) {
+    if (bs[w + 1] <= bs[w]) {
+      set_error("block_starts must be strictly increasing");
+      return HBJ_E_INVALID;
+    }
+  }
+  return HBJ_OK;
+}
+
+}  // namespace hbj
+
+using namespace hbj;
+
+extern "C" {
+
+const char* hbj_last_error(void) { return g_error.c_str(); }
+
+// compute.rs:37-71
+hbj_status hbj_lu_factor_inplace_6(float* mat, uint32_t n) {
+  if (!mat) {
+    set_error("lu_factor_inplace_6: mat must have len >= 36");
+    return HBJ_E_INVALID;
+  }
+  if (n > 6) {
+    set_error("lu_factor_inplace_6: n must be <= 6, got %u", n);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    const float pivot = mat[k * 6 + k];
+    if (pivot == 0.0f) {
+      set_error("LU zero pivot at k = %u", k);
+      return HBJ_E_ZERO_PIVOT;
+    }
+    for (uint32_t i = k + 1; i < n; ++i) mat[i * 6 + k] /= pivot;
+    for (uint32_t i = k + 1; i < n; ++i) {
+      const float l = mat[i * 6 + k];
+      if (l == 0.0f) continue;  // compute.rs:62
+      for (uint32_t j = k + 1; j < n; ++j) mat[i * 6 + j] -= l * mat[k * 6 + j];
+    }
+  }
+  return HBJ_OK;
+}
+
+// compute.rs:84-181.  Blocks are independent, so large systems are split
+// over host threads; the first failing block (lowest index) reports.
+hbj_status hbj_build_lu_blocks_6(uint32_t n, const uint32_t* row_ptr, uint32_t row_ptr_len,
+                                 const uint32_t* col_idx, uint32_t col_idx_len,
+                                 const float* values, uint32_t values_len,
+                                 const uint32_t* bs, uint32_t n_starts, float* lu_out) {
+  clear_error();
+  if (row_ptr_len != n + 1) {
+    set_error("build_lu_blocks_from_csr_block_starts_6: row_ptr len must be n+1 (%u), got %u",
+              n + 1, row_ptr_len);
+    return HBJ_E_INVALID;
+  }
+  const uint32_t nnz = row_ptr[n];
+  if (col_idx_len != nnz || values_len != nnz) {
+    set_error("build_lu_blocks_from_csr_block_starts_6: nnz mismatch: row_ptr says %u, col_idx %u, values %u",
+              nnz, col_idx_len, values_len);
+    return HBJ_E_INVALID;
+  }
+  if (n_starts < 2) return HBJ_OK;
+  if (bs[0] != 0) {
+    set_error("build_lu_blocks_from_csr_block_starts_6: block_starts must start at 0");
+    return HBJ_E_INVALID;
+  }
+  if (bs[n_starts - 1] != n) {
+    set_error("build_lu_blocks_from_csr_block_starts_6: block_starts last must equal n (%u), got %u",
+              n, bs[n_starts - 1]);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t w = 0; w + 1 < n_starts; ++w) {
+    if (bs[w + 1] <= bs[w]) {
+      set_error("build_lu_blocks_from_csr_block_starts_6: block_starts must be strictly increasing");
+      return HBJ_E_INVALID;
+    }
+  }
+  const uint32_t nb = n_starts - 1;
+  std::memset(lu_out, 0, sizeof(float) * 36ull * nb);
+  // first block index whose factorisation failed, and where
+  auto work = [&](uint32_t b0, uint32_t b1, int64_t* fail_blk, uint32_t* fail_k) {
+    for (uint32_t blk = b0; blk < b1; ++blk) {
+      const uint32_t off = bs[blk], end = bs[blk + 1];
+      if (end > n || off >= end) continue;  // compute.rs:141-143
+      const uint32_t m = end - off < 6 ? end - off : 6;
+      float mat[36] = {0};
+      for (int i = 0; i < 6; ++i) mat[i * 6 + i] = 1.0f;  // identity fill (:151-154)
+      for (uint32_t il = 0; il < m; ++il) {
+        const uint32_t i = off + il;
+        for (uint32_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+          const uint32_t j = col_idx[k];
+          if (j >= off && j < off + m) mat[il * 6 + (j - off)] = values[k];  // last write wins
+        }
+      }
+      // lu_factor_inplace_6, inlined to keep the failing pivot index
+      for (uint32_t k = 0; k < m; ++k) {
+        const float pivot = mat[k * 6 + k];
+        if (pivot == 0.0f) {
+          *fail_blk = blk;
+          *fail_k = k;
+          return;
+        }
+        for (uint32_t i = k + 1; i < m; ++i) mat[i * 6 + k] /= pivot;
+        for (uint32_t i = k + 1; i < m; ++i) {
+          const float l = mat[i * 6 + k];
+          if (l == 0.0f) continue;
+          for (uint32_t j = k + 1; j < m; ++j) mat[i * 6 + j] -= l * mat[k * 6 + j];
+        }
+      }
+      std::memcpy(lu_out + 36ull * blk, mat, sizeof mat);
+    }
+  };
+  unsigned nt = nb >= 65536 ? std::thread::hardware_concurrency() : 1u;
+  if (nt == 0) nt = 1;
+  if (nt > 16) nt = 16;
+  std::vector<int64_t> fb(nt, -1);
+  std::vector<uint32_t> fk(nt, 0);
+  if (nt == 1) {
+    work(0, nb, &fb[0], &fk[0]);
+  } else {
+    std::vector<std::thread> th;
+    const uint32_t per = (nb + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+      const uint32_t b0 = t * per, b1 = (b0 + per) < nb ? b0 + per : nb;
+      if (b0 >= b1) continue;
+      th.emplace_back(work, b0, b1, &fb[t], &fk[t]);
+    }
+    for (auto& x : th) x.join();
+  }
+  for (unsigned t = 0; t < nt; ++t) {
+    if (fb[t] >= 0) {
+      set_error("LU zero pivot at k = %u", fk[t]);
+      return HBJ_E_ZERO_PIVOT;
+    }
+  }
+  return HBJ_OK;
+}
+
+}  // extern "C"

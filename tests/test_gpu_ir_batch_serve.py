@@ -1,0 +1,189 @@
+"""Mixed-precision refinement, batching and serving on the GPU.
+
+Tolerances: the refined FP64 solution must reach the requested relative
+residual ||b - A x||_2 / ||b||_2 (checked here on the host in FP64 from the
+returned x); batched and served results must be bit-identical to the same
+system solved alone (per-system reductions never mix systems)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32 if np.asarray(a).dtype == np.float32 else np.uint64)
+
+
+def host_rel_res(L, x):
+    n = L["n"]
+    A = sp.csr_matrix((L["values"].astype(np.float64), L["col_idx"], L["row_ptr"]), shape=(n, n))
+    b = L["rhs"].astype(np.float64)
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b), A, b
+
+
+@pytest.mark.parametrize("N,h", [(10, 1.5), (8, 0.0), (14, 1.5)])
+def test_ir_reaches_1e10(ctx, hbj, N, h):
+    print(f"IR N={N} h={h}", flush=True)
+    L = hbj.lattice(N, seed=2, e_log10_half_range=h)
+    s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    p = hbj.IrParams(max_outer=30, max_inner=5000, inner_rel_tol=1e-4, target_rel_res=1e-10)
+    x, res = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+    rr, A, b = host_rel_res(L, x)
+    print(f"outer={res.outer_iterations} inner={res.iterations} dev_res={res.rel_res_true_fp64:.3e} "
+          f"host_res={rr:.3e}", flush=True)
+    assert res.status == 0 and res.rel_res_true_fp64 <= 1e-10
+    assert rr <= 1.5e-10
+    assert abs(rr - res.rel_res_true_fp64) <= 1e-3 * rr + 1e-14
+    # error against an FP64 direct solve of the same FP32-stored matrix
+    xd = spla.spsolve(A.tocsc(), b)
+    err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    print(f"rel error vs FP64 direct: {err:.3e}", flush=True)
+    assert err < 1e-6
+    # plain FP32 PCG cannot get there on its own
+    x32, r32 = s.solve(L["rhs"], np.zeros(L["n"], F), 5000, 1e-6, 0.0)
+    assert host_rel_res(L, x32.astype(np.float64))[0] > 1e-8
+
+
+def test_ir_warm_start_and_edges(ctx, hbj):
+    L = hbj.lattice(6, seed=4)
+    s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    b = L["rhs"].astype(np.float64)
+    p = hbj.IrParams(target_rel_res=1e-10)
+    x, res = s.solve_ir(b, None, p)
+    assert res.status == 0
+    x2, res2 = s.solve_ir(b, x, p)  # already converged: no refinement step
+    assert res2.status == 0 and res2.outer_iterations == 0 and np.array_equal(x2, x)
+    xz, resz = s.solve_ir(np.zeros_like(b), None, p)  # b = 0 -> Ok, x = x0 = 0
+    assert resz.status == 0 and np.all(xz == 0)
+    with pytest.raises(hbj.SolverError) as e:
+        s.solve_ir(b, None, hbj.IrParams(max_outer=1, target_rel_res=1e-15))
+    assert e.value.status_name == "NOT_CONVERGED"
+    # repeated solves are bit-identical
+    x3, _ = s.solve_ir(b, None, p)
+    assert np.array_equal(bits(x3), bits(x))
+
+
+def _systems(ctx, hbj, sides, seed0=10, h=0.0):
+    out = []
+    for i, N in enumerate(sides):
+        L = hbj.lattice(N, seed=seed0 + i, e_log10_half_range=h)
+        out.append((L, hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])))
+    return out
+
+
+def test_batch_fp32_identical_to_alone(ctx, hbj, oracle):
+    sides = [4, 9, 5, 12, 7, 4, 10]
+    sysl = _systems(ctx, hbj, sides)
+    alone = []
+    for L, s in sysl:
+        x, res = s.solve(L["rhs"], np.zeros(L["n"], F), 5000, 1e-6, 0.0)
+        alone.append((x, res.iterations))
+        s.set_rhs(L["rhs"])
+    batch = hbj.Batch(ctx, [s for _, s in sysl])
+    st, results = batch.solve(5000, 1e-6, 0.0)
+    assert st == 0
+    for (L, s), (xa, ita), r in zip(sysl, alone, results):
+        assert r.status == 0 and r.iterations == ita
+        assert np.array_equal(bits(s.get_x()), bits(xa))
+    # and against the oracle for one of them
+    L, s = sysl[3]
+    stl, lu = oracle.build_lu_blocks_6(L["n"], L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    sto, ito, xo, _ = oracle.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"],
+                                 L["rhs"], np.zeros(L["n"], F), 5000, 1e-6, 0.0)
+    assert results[3].iterations == ito and np.array_equal(bits(s.get_x()), bits(xo))
+    batch.close()
+
+
+def test_batch_isolates_failures(ctx, hbj):
+    sysl = _systems(ctx, hbj, [5, 6])
+    n = 12
+    ident = hbj.System(ctx, np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32),
+                       np.ones(n, F), np.arange(0, n + 1, 6, dtype=np.uint32))
+    b = np.arange(1, n + 1, dtype=F)
+    ident.set_rhs(b, b)  # x0 = b: p.Ap = 0 breakdown
+    for L, s in sysl:
+        s.set_rhs(L["rhs"])
+    batch = hbj.Batch(ctx, [sysl[0][1], ident, sysl[1][1]])
+    st, results = batch.solve(5000, 1e-6, 0.0)
+    assert st == 3
+    assert results[0].status == 0 and results[2].status == 0 and results[1].status == 3
+    for (L, s), r in zip(sysl, (results[0], results[2])):
+        x, ra = s.solve(L["rhs"], np.zeros(L["n"], F), 5000, 1e-6, 0.0)
+        assert ra.iterations == r.iterations
+
+
+def test_batch_ir_identical_to_alone(ctx, hbj):
+    sides = [4, 8, 5, 11, 6]
+    sysl = _systems(ctx, hbj, sides, seed0=30, h=1.5)
+    p = hbj.IrParams(target_rel_res=1e-10)
+    alone = []
+    for L, s in sysl:
+        x, res = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+        alone.append((x, res))
+        s.set_rhs64(L["rhs"].astype(np.float64))
+    batch = hbj.Batch(ctx, [s for _, s in sysl])
+    st, results = batch.solve_ir(p)
+    assert st == 0
+    for (L, s), (xa, ra), r in zip(sysl, alone, results):
+        assert r.status == 0 and r.iterations == ra.iterations and r.outer_iterations == ra.outer_iterations
+        xb = s.get_x64()
+        assert np.array_equal(bits(xb), bits(xa))
+        assert host_rel_res(L, xb)[0] <= 1.5e-10
+    batch.close()
+
+
+def test_server_mixed_stream_identical_to_alone(ctx, hbj):
+    rng = np.random.default_rng(0)
+    sides = [int(round(np.exp(rng.uniform(np.log(4), np.log(12))))) for _ in range(24)]
+    p = hbj.IrParams(target_rel_res=1e-8)
+    systems = [hbj.lattice(N, seed=100 + i) for i, N in enumerate(sides)]
+    # reference: each alone
+    alone = []
+    for L in systems:
+        s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+        x, res = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+        alone.append((x, res))
+        s.close()
+    budget = 64 << 20  # forces queueing: not all requests fit at once
+    srv = hbj.Server(ctx, budget, max_active=6, chunk_iters=8)
+    tickets = [srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                          L["rhs"].astype(np.float64), p) for L in systems]
+    lat = []
+    for t, L, (xa, ra) in zip(tickets, systems, alone):
+        st, x, res, ms = srv.wait(t, n=L["n"])
+        assert st == 0 and res.status == 0
+        assert res.iterations == ra.iterations
+        assert np.array_equal(bits(x), bits(xa))
+        assert host_rel_res(L, x)[0] <= 1.5e-8
+        lat.append(ms)
+    stats = srv.stats()
+    assert stats["completed"] == len(systems) and stats["submitted"] == len(systems)
+    assert stats["peak_device_bytes"] <= budget
+    assert 1 < stats["peak_active"] <= 6
+    assert min(lat) > 0
+    srv.close()
+
+
+def test_server_rejects_and_reports_failures(ctx, hbj):
+    srv = hbj.Server(ctx, 8 << 20, max_active=4)
+    big = hbj.lattice(20)
+    with pytest.raises(hbj.SolverError) as e:
+        srv.submit(big["row_ptr"], big["col_idx"], big["values"], big["block_starts"],
+                   big["rhs"].astype(np.float64))
+    assert e.value.status_name == "REJECTED"
+    ok = hbj.lattice(4)
+    t1 = srv.submit(ok["row_ptr"], ok["col_idx"], ok["values"], ok["block_starts"],
+                    ok["rhs"].astype(np.float64))
+    rp = np.array([0, 1, 2], np.uint32)
+    t2 = srv.submit(rp, np.array([0, 1], np.uint32), np.array([0, 1], F), np.array([0, 2], np.uint32),
+                    np.ones(2))
+    st2, _, res2, _ = srv.wait(t2, n=2)
+    assert st2 == 2 and "LU zero pivot at k = 0" in hbj._lib.last_error()
+    st1, x1, res1, _ = srv.wait(t1, n=ok["n"])
+    assert st1 == 0
+    s = srv.stats()
+    assert s["rejected"] == 1 and s["completed"] == 2
+    srv.close()

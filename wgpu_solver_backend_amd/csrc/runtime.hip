@@ -21,438 +21,9 @@
 
 #include "hbj_internal.hpp"
 
+#include "engine.hpp"
+
 using namespace hbj;
-
-#define HIPCHK(expr)                                                          \
-  do {                                                                        \
-    hipError_t e_ = (expr);                                                   \
-    if (e_ != hipSuccess) {                                                   \
-      set_error("%s failed: %s", #expr, hipGetErrorString(e_));              \
-      return HBJ_E_DEVICE;                                                    \
-    }                                                                         \
-  } while (0)
-
-#define CHK(expr)                    \
-  do {                               \
-    hbj_status s_ = (expr);          \
-    if (s_ != HBJ_OK) return s_;     \
-  } while (0)
-
-namespace {
-
-template <typename T>
-hbj_status dalloc(T** p, size_t count, uint64_t* acct = nullptr) {
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    (void)hipGetLastError();
-    set_error("hipMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
-    return HBJ_E_OOM;
-  }
-  if (acct) *acct += count * sizeof(T);
-  return HBJ_OK;
-}
-
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
-
-template <typename T>
-hbj_status halloc(T** p, size_t count) {
-  if (count == 0) count = 1;
-  hipError_t e = hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocDefault);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    set_error("hipHostMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
-    return HBJ_E_OOM;
-  }
-  return HBJ_OK;
-}
-
-template <typename T>
-void hfree(T*& p) {
-  if (p) (void)hipHostFree((void*)p);
-  p = nullptr;
-}
-
-double now_ms() {
-  using namespace std::chrono;
-  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
-}
-
-uint32_t env_u32(const char* name, uint32_t dflt) {
-  const char* v = std::getenv(name);
-  if (!v || !*v) return dflt;
-  return (uint32_t)std::strtoul(v, nullptr, 10);
-}
-
-}  // namespace
-
-// ------------------------------------------------------------------ context
-
-struct hbj_ctx {
-  int device = 0;
-  hipDeviceProp_t prop{};
-  hipStream_t stream = nullptr;  // single-op executors
-  std::mutex mu;
-};
-
-// ------------------------------------------------------------------ engine
-
-namespace hbj {
-
-// Drives the three PCG kernels over a tile table until every system in it
-// has a final status.
-struct Engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint32_t n_sys = 0, n_tiles = 0;
-  SysDesc* d_descs = nullptr;
-  uint2* d_tiles = nullptr;
-  SysState* d_states = nullptr;
-  SysState* h_init = nullptr;
-  SysState* h_poll[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipGraphExec_t gexec = nullptr;
-  uint32_t gchunk = 0;
-  uint32_t chunk = 16;
-  bool use_graph = true;
-  uint32_t remap = 1;
-  // kernel timing (HIP events around sampled iterations)
-  uint32_t timing_every = 0;
-  uint64_t trip = 0;  // iterations enqueued so far (for sampling)
-  std::vector<hipEvent_t> tev[2];
-  std::vector<int> tev_used;  // per pool: number of 4-event groups recorded
-  double t_sum[3] = {0, 0, 0};
-  uint64_t t_cnt[3] = {0, 0, 0};
-
-  hbj_status init(int dev, uint32_t nsys, uint32_t ntiles) {
-    device = dev;
-    n_sys = nsys;
-    n_tiles = ntiles;
-    chunk = env_u32("HBJ_CHUNK", 16);
-    if (chunk < 1) chunk = 1;
-    use_graph = env_u32("HBJ_GRAPH", 1) != 0;
-    remap = env_u32("HBJ_XCD_REMAP", 1);
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    CHK(dalloc(&d_descs, nsys));
-    CHK(dalloc(&d_tiles, ntiles));
-    CHK(dalloc(&d_states, nsys));
-    CHK(halloc(&h_init, nsys));
-    CHK(halloc(&h_poll[0], nsys));
-    CHK(halloc(&h_poll[1], nsys));
-    tev_used.assign(2, 0);
-    return HBJ_OK;
-  }
-
-  // descs[i].st is pointed at this engine's state slot i.
-  hbj_status set_table(std::vector<SysDesc> descs, const std::vector<uint2>& tiles) {
-    for (uint32_t i = 0; i < descs.size(); ++i) descs[i].st = d_states + i;
-    HIPCHK(hipMemcpy(d_descs, descs.data(), sizeof(SysDesc) * descs.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
-    n_sys = (uint32_t)descs.size();
-    n_tiles = (uint32_t)tiles.size();
-    drop_graph();
-    return HBJ_OK;
-  }
-
-  void drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-    gchunk = 0;
-  }
-
-  LaunchSet ls() const { return LaunchSet{d_descs, d_tiles, n_tiles, remap}; }
-
-  void launch_iter() {
-    const LaunchSet l = ls();
-    launch_pcg_spmv(l, stream);
-    launch_pcg_update(l, stream);
-    launch_pcg_pupdate(l, stream);
-  }
-
-  hbj_status capture(uint32_t T) {
-    drop_graph();
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t i = 0; i < T; ++i) launch_iter();
-    HIPCHK(hipStreamEndCapture(stream, &g));
-    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      gexec = nullptr;
-      set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
-      return HBJ_E_DEVICE;
-    }
-    gchunk = T;
-    return HBJ_OK;
-  }
-
-  hbj_status ensure_timing_events(uint32_t T) {
-    const size_t need = 4u * ((T + timing_every - 1) / timing_every + 1);
-    for (int b = 0; b < 2; ++b)
-      while (tev[b].size() < need) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        tev[b].push_back(e);
-      }
-    return HBJ_OK;
-  }
-
-  hbj_status enqueue_chunk(uint32_t T, int pool) {
-    if (timing_every) {
-      CHK(ensure_timing_events(T));
-      int used = 0;
-      const LaunchSet l = ls();
-      for (uint32_t i = 0; i < T; ++i, ++trip) {
-        if (trip % timing_every == 0) {
-          hipEvent_t* e = &tev[pool][4 * used];
-          HIPCHK(hipEventRecord(e[0], stream));
-          launch_pcg_spmv(l, stream);
-          HIPCHK(hipEventRecord(e[1], stream));
-          launch_pcg_update(l, stream);
-          HIPCHK(hipEventRecord(e[2], stream));
-          launch_pcg_pupdate(l, stream);
-          HIPCHK(hipEventRecord(e[3], stream));
-          ++used;
-        } else {
-          launch_iter();
-        }
-      }
-      tev_used[pool] = used;
-      HIPCHK(hipGetLastError());
-      return HBJ_OK;
-    }
-    if (use_graph) {
-      if (!gexec || gchunk != T) CHK(capture(T));
-      HIPCHK(hipGraphLaunch(gexec, stream));
-      return HBJ_OK;
-    }
-    for (uint32_t i = 0; i < T; ++i) launch_iter();
-    HIPCHK(hipGetLastError());
-    return HBJ_OK;
-  }
-
-  hbj_status harvest_timing(int pool) {
-    for (int u = 0; u < tev_used[pool]; ++u) {
-      hipEvent_t* e = &tev[pool][4 * u];
-      for (int k = 0; k < 3; ++k) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
-        t_sum[k] += ms * 1000.0;
-        t_cnt[k] += 1;
-      }
-    }
-    tev_used[pool] = 0;
-    return HBJ_OK;
-  }
-
-  static bool all_done(const SysState* s, uint32_t n) {
-    for (uint32_t i = 0; i < n; ++i)
-      if (s[i].status == kRunning) return false;
-    return true;
-  }
-
-  // Runs until every system has a final status.  `init` holds n_sys states;
-  // final states are returned in `out`.
-  hbj_status run(const SysState* init, SysState* out, double* ms) {
-    HIPCHK(hipSetDevice(device));
-    if (n_tiles == 0 || all_done(init, n_sys)) {
-      std::memcpy(out, init, sizeof(SysState) * n_sys);
-      if (ms) *ms = 0.0;
-      return HBJ_OK;
-    }
-    // Upper bound on the iterations any system can run (each launch of the
-    // three kernels advances every running system by one step): the loop
-    // below never polls past it, whatever the device reports.
-    uint64_t max_trips = 0;
-    for (uint32_t i = 0; i < n_sys; ++i) {
-      const uint64_t t = init[i].phase == kPhIrResid
-                             ? (uint64_t)(init[i].max_outer + 1) * (init[i].max_iter + 3) + 4
-                             : (uint64_t)init[i].max_iter + 3;
-      max_trips = t > max_trips ? t : max_trips;
-    }
-    const uint64_t max_chunks = max_trips / chunk + 3;
-    if (use_graph && !timing_every && (!gexec || gchunk != chunk)) {
-      HIPCHK(hipStreamSynchronize(stream));
-      CHK(capture(chunk));
-    }
-    std::memcpy(h_init, init, sizeof(SysState) * n_sys);
-    HIPCHK(hipMemcpyAsync(d_states, h_init, sizeof(SysState) * n_sys, hipMemcpyHostToDevice, stream));
-    const double t0 = now_ms();
-    uint32_t c = 0;
-    int final_pool = -1;
-    for (;;) {
-      if (c >= max_chunks) {
-        HIPCHK(hipStreamSynchronize(stream));
-        set_error("engine: systems still running after %llu iterations (status not advancing)",
-                  (unsigned long long)(c * (uint64_t)chunk));
-        return HBJ_E_DEVICE;
-      }
-      const int pool = (int)(c & 1u);
-      CHK(enqueue_chunk(chunk, pool));
-      HIPCHK(hipMemcpyAsync(h_poll[pool], d_states, sizeof(SysState) * n_sys,
-                            hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipEventRecord(ev[pool], stream));
-      if (c > 0) {
-        const int prev = pool ^ 1;
-        HIPCHK(hipEventSynchronize(ev[prev]));
-        if (timing_every) CHK(harvest_timing(prev));
-        if (all_done(h_poll[prev], n_sys)) {
-          final_pool = pool;
-          break;
-        }
-      }
-      ++c;
-    }
-    HIPCHK(hipEventSynchronize(ev[final_pool]));
-    if (timing_every) CHK(harvest_timing(final_pool));
-    if (ms) *ms = now_ms() - t0;
-    std::memcpy(out, h_poll[final_pool], sizeof(SysState) * n_sys);
-    return HBJ_OK;
-  }
-
-  void destroy() {
-    drop_graph();
-    for (int b = 0; b < 2; ++b) {
-      for (auto e : tev[b]) (void)hipEventDestroy(e);
-      tev[b].clear();
-      if (ev[b]) (void)hipEventDestroy(ev[b]);
-      ev[b] = nullptr;
-      hfree(h_poll[b]);
-    }
-    hfree(h_init);
-    dfree(d_descs);
-    dfree(d_tiles);
-    dfree(d_states);
-    if (stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
-  }
-};
-
-}  // namespace hbj
-
-// ------------------------------------------------------------------ system
-
-struct hbj_system {
-  hbj_ctx* ctx = nullptr;
-  uint32_t n = 0, nnz = 0, nb = 0, n_tiles = 0;
-  uint32_t flags = 0;
-  uint32_t *rp = nullptr, *col = nullptr, *bs = nullptr;
-  float *val = nullptr, *lu = nullptr;
-  float *x = nullptr, *x0 = nullptr, *b = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
-  float *ra = nullptr, *rb = nullptr, *part = nullptr;
-  double* part64 = nullptr;
-  double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
-  float* b32 = nullptr;
-  double* h_part2 = nullptr;  // pinned, 2 * n_tiles
-  uint64_t bytes = 0;
-  Engine eng;
-  bool eng_ready = false;
-  int eng_mode = -1;  // 0 plain FP32, 1 IR
-  SysDesc* d_desc1 = nullptr;  // plain descriptor for the FP64 check kernel
-
-  SysDesc desc(bool ir) const {
-    SysDesc d{};
-    d.row_ptr = rp;
-    d.col = col;
-    d.val = val;
-    d.bstart = bs;
-    d.lu = lu;
-    d.x = x;
-    d.p = p;
-    d.q = q;
-    d.z = z;
-    d.b = ir ? b32 : b;
-    d.part = part;
-    d.part64 = part64;
-    d.x64 = x64;
-    d.b64 = b64;
-    d.r64 = r64;
-    d.st = nullptr;
-    d.n = n;
-    d.nb = nb;
-    d.n_tiles = n_tiles;
-    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u);
-    return d;
-  }
-
-  SysState init_state(bool ir, uint32_t max_iter, float rel_tol, float abs_tol,
-                      const hbj_ir_params* ip) const {
-    SysState s;
-    std::memset(&s, 0, sizeof s);
-    s.status = kRunning;
-    s.phase = ir ? kPhIrResid : kPhInit;
-    s.max_iter = ir ? ip->max_inner : max_iter;
-    s.rel_tol2 = ir ? ip->inner_rel_tol * ip->inner_rel_tol : rel_tol * rel_tol;  // compute.rs:256
-    s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257
-    s.r_in = ra;
-    s.r_out = rb;
-    if (ir) {
-      s.max_outer = ip->max_outer;
-      s.target2 = ip->target_rel_res * ip->target_rel_res;
-    }
-    return s;
-  }
-
-  hbj_status ensure_ir_buffers() {
-    if (x64) return HBJ_OK;
-    HIPCHK(hipSetDevice(ctx->device));
-    CHK(dalloc(&x64, n, &bytes));
-    CHK(dalloc(&x0_64, n, &bytes));
-    CHK(dalloc(&b64, n, &bytes));
-    CHK(dalloc(&r64, n, &bytes));
-    CHK(dalloc(&b32, n, &bytes));
-    HIPCHK(hipMemset(x0_64, 0, sizeof(double) * n));
-    HIPCHK(hipMemset(b64, 0, sizeof(double) * n));
-    return HBJ_OK;
-  }
-
-  hbj_status ensure_engine(int mode) {
-    if (!eng_ready) {
-      CHK(eng.init(ctx->device, 1, n_tiles));
-      eng_ready = true;
-    }
-    if (eng_mode != mode) {
-      std::vector<uint2> tiles(n_tiles);
-      for (uint32_t t = 0; t < n_tiles; ++t) tiles[t] = make_uint2(0u, t);
-      CHK(eng.set_table({desc(mode == 1)}, tiles));
-      eng_mode = mode;
-    }
-    return HBJ_OK;
-  }
-
-  // ||b - A x|| / ||b|| in FP64 for the FP32 solution in `x` (plain mode).
-  hbj_status true_residual(hipStream_t s, double* out) {
-    launch_true_resid(d_desc1, x, part64, n_tiles, s);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h_part2, part64, sizeof(double) * 2 * n_tiles, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    double rr = 0, bb = 0;
-    for (uint32_t t = 0; t < n_tiles; ++t) {
-      rr += h_part2[t];
-      bb += h_part2[n_tiles + t];
-    }
-    *out = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-    return HBJ_OK;
-  }
-
-  void release() {
-    if (eng_ready) eng.destroy();
-    eng_ready = false;
-    dfree(rp); dfree(col); dfree(bs); dfree(val); dfree(lu);
-    dfree(x); dfree(x0); dfree(b); dfree(p); dfree(q); dfree(z);
-    dfree(ra); dfree(rb); dfree(part); dfree(part64);
-    dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
-    dfree(d_desc1);
-    hfree(h_part2);
-  }
-};
 
 namespace {
 

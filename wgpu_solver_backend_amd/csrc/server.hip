@@ -1,0 +1,407 @@
+// server.hip -- continuous-batching request loop (hbj_server_*).
+//
+// Two host threads share one device:
+//   admitter  takes queued requests in arrival order while the HBM budget
+//             and the slot count allow, and turns each into a resident
+//             system (validation, LU blocks, uploads, FP64 b) -- this host
+//             work overlaps the iterations already running on the GPU;
+//   worker    owns the slot table.  It enqueues chunks of PCG iterations
+//             (one launch of each kernel covers every resident request),
+//             polls the per-slot status words of the previous chunk while the
+//             next one runs, retires requests whose status is final (x copied
+//             out, latency stamped) and folds newly prepared requests into
+//             the table between chunks.
+// Each request's arithmetic only depends on its own system (tile partials and
+// last-tile reductions are per system), so its result is the same alone or
+// in any mix.
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+namespace {
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t n = 0, nnz = 0, n_starts = 0;
+  std::vector<uint32_t> rp, col, bs;
+  std::vector<float> val;
+  std::vector<double> b64;
+  hbj_ir_params params{};
+  uint64_t est_bytes = 0;
+  double t_submit = 0, t_done = 0;
+  hbj_system* sys = nullptr;
+  int slot = -1;
+  bool done = false;
+  int32_t status = HBJ_OK;
+  std::string error;
+  hbj_result res{};
+  std::vector<double> x;
+};
+
+}  // namespace
+
+struct hbj_server {
+  hbj_ctx* ctx = nullptr;
+  hbj_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_admit, cv_work, cv_done;
+  std::deque<Request*> queue;     // submitted, not yet admitted
+  std::deque<Request*> ready;     // resident on the device, not yet in the table
+  std::unordered_map<uint64_t, Request*> reqs;
+  uint64_t next_ticket = 1;
+  uint64_t bytes_in_use = 0;      // reserved by admitted requests
+  uint32_t resident = 0;          // admitted and not yet released
+  hbj_server_stats st{};
+  bool stop = false;
+  std::thread admitter, worker;
+  Engine eng;
+};
+
+namespace {
+
+void finish(hbj_server* S, Request* r, int32_t status, const std::string& err) {
+  r->status = status;
+  r->error = err;
+  r->t_done = now_ms();
+  r->done = true;
+  S->st.completed += 1;
+  S->cv_done.notify_all();
+}
+
+void admitter_main(hbj_server* S) {
+  (void)hipSetDevice(S->ctx->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    S->cv_admit.wait(lk, [&] {
+      if (S->stop && S->queue.empty()) return true;
+      if (S->queue.empty()) return false;
+      const Request* h = S->queue.front();
+      return S->resident < S->cfg.max_active && S->bytes_in_use + h->est_bytes <= S->cfg.hbm_budget_bytes;
+    });
+    if (S->queue.empty()) break;  // stop requested and nothing left
+    Request* r = S->queue.front();
+    S->queue.pop_front();
+    S->bytes_in_use += r->est_bytes;
+    S->resident += 1;
+    lk.unlock();
+    hbj_system* sys = nullptr;
+    hbj_status s = hbj_system_create(S->ctx, r->n, r->nnz, r->rp.data(), r->col.data(), r->val.data(),
+                                     r->bs.data(), r->n_starts, &sys);
+    if (s == HBJ_OK) s = hbj_system_set_rhs64(sys, r->b64.data(), nullptr);
+    std::string err = s == HBJ_OK ? "" : hbj_last_error();
+    lk.lock();
+    if (s != HBJ_OK) {
+      if (sys) hbj_system_destroy(sys);
+      S->bytes_in_use -= r->est_bytes;
+      S->resident -= 1;
+      finish(S, r, s, err);
+      S->cv_admit.notify_all();
+      continue;
+    }
+    // host copies are no longer needed once the system is resident
+    r->sys = sys;
+    std::vector<uint32_t>().swap(r->rp);
+    std::vector<uint32_t>().swap(r->col);
+    std::vector<uint32_t>().swap(r->bs);
+    std::vector<float>().swap(r->val);
+    std::vector<double>().swap(r->b64);
+    S->ready.push_back(r);
+    if (S->bytes_in_use > S->st.peak_device_bytes) S->st.peak_device_bytes = S->bytes_in_use;
+    S->cv_work.notify_all();
+  }
+}
+
+// Worker: returns a status only on a device failure (all requests then fail).
+hbj_status worker_loop(hbj_server* S) {
+  HIPCHK(hipSetDevice(S->ctx->device));
+  Engine& E = S->eng;
+  const uint32_t cap = S->cfg.max_active;
+  std::vector<Request*> slots(cap, nullptr);   // in the table (running or retiring)
+  std::vector<Request*> retiring;              // final, still referenced by an in-flight chunk
+  int inflight = -1;                           // pool of the chunk whose poll is pending
+  int cur = 0;
+  uint32_t active = 0;
+  const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 16;
+
+  auto drain = [&]() -> hbj_status {
+    if (inflight >= 0) {
+      CHK(E.wait_poll(inflight));
+      inflight = -1;
+    }
+    return HBJ_OK;
+  };
+  // retire every slot whose polled status is final
+  auto harvest = [&](const SysState* polled) -> hbj_status {
+    for (uint32_t i = 0; i < cap; ++i) {
+      Request* r = slots[i];
+      if (!r || r->done || polled[i].status == kRunning) continue;
+      hbj_result res{};
+      const SysState& s = polled[i];
+      res.status = s.status;
+      res.iterations = s.total_iter;
+      res.outer_iterations = s.outer;
+      res.rr = s.rr;
+      res.b_norm2 = s.b_norm2;
+      res.rel_res_recurrence = s.b_norm2 > 0 ? std::sqrt((double)s.rr / (double)s.b_norm2) : 0.0;
+      res.rel_res_true_fp64 = s.bb64 > 0 ? std::sqrt(s.rr64 / s.bb64) : 0.0;
+      r->x.assign(r->n, 0.0);
+      // the in-flight chunk no longer writes this system (its status is final)
+      CHK(hbj_system_get_x64(r->sys, r->x.data()));
+      std::lock_guard<std::mutex> g(S->mu);
+      r->res = res;
+      r->res.solve_ms = now_ms() - r->t_submit;
+      std::string err;
+      if (s.status == HBJ_E_NOT_CONVERGED)
+        err = "IR: relative residual above target after the refinement limit";
+      else if (s.status == HBJ_E_BREAKDOWN_PAP)
+        err = "PCG(BlockJacobiGpu): dot(p,Ap) is zero (breakdown)";
+      else if (s.status == HBJ_E_BREAKDOWN_RZ)
+        err = "PCG(BlockJacobiGpu): rz_old is zero (breakdown)";
+      finish(S, r, s.status, err);
+      retiring.push_back(r);
+      --active;
+    }
+    return HBJ_OK;
+  };
+
+  for (;;) {
+    std::vector<Request*> incoming;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (active == 0 && retiring.empty() && inflight < 0) {
+        S->cv_work.wait(lk, [&] { return !S->ready.empty() || (S->stop && S->queue.empty() && S->resident == 0); });
+        if (S->ready.empty() && S->stop && S->queue.empty() && S->resident == 0) break;
+      }
+      while (!S->ready.empty()) {
+        incoming.push_back(S->ready.front());
+        S->ready.pop_front();
+      }
+    }
+    if (!incoming.empty() || !retiring.empty()) {
+      // table change: let the in-flight chunk finish first
+      if (inflight >= 0) {
+        const int p = inflight;
+        CHK(drain());
+        CHK(harvest(E.h_poll[p]));
+      }
+      for (Request* r : retiring) {
+        slots[r->slot] = nullptr;
+        SysState idle;
+        std::memset(&idle, 0, sizeof idle);  // status 0: never iterated again
+        CHK(E.write_slot((uint32_t)r->slot, SysDesc{}, idle));
+        const uint64_t b = r->est_bytes;
+        hbj_system_destroy(r->sys);
+        r->sys = nullptr;
+        std::lock_guard<std::mutex> g(S->mu);
+        S->bytes_in_use -= b;
+        S->resident -= 1;
+        S->cv_admit.notify_all();
+      }
+      retiring.clear();
+      for (Request* r : incoming) {
+        uint32_t slot = 0;
+        while (slots[slot]) ++slot;  // resident <= cap guarantees a free slot
+        slots[slot] = r;
+        r->slot = (int)slot;
+        CHK(r->sys->ensure_ir_buffers());
+        if (r->sys->flags & kFlagOversize) HIPCHK(hipMemset(r->sys->z, 0, 4ull * r->n));
+        HIPCHK(hipMemset(r->sys->x64, 0, 8ull * r->n));
+        CHK(E.write_slot(slot, r->sys->desc(true), r->sys->init_state(true, 0, 0, 0, &r->params)));
+        ++active;
+      }
+      std::vector<uint2> tiles;
+      for (uint32_t i = 0; i < cap; ++i)
+        if (slots[i])
+          for (uint32_t t = 0; t < slots[i]->sys->n_tiles; ++t) tiles.push_back(make_uint2(i, t));
+      CHK(E.write_tiles(tiles));
+      {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->st.active = active;
+        if (active > S->st.peak_active) S->st.peak_active = active;
+      }
+    }
+    if (active == 0) continue;
+    CHK(E.enqueue_poll(T, cur));
+    if (inflight >= 0) {
+      const int p = inflight;
+      CHK(E.wait_poll(p));
+      CHK(harvest(E.h_poll[p]));
+    }
+    inflight = cur;
+    cur ^= 1;
+    std::lock_guard<std::mutex> g(S->mu);
+    S->st.active = active;
+  }
+  return HBJ_OK;
+}
+
+void worker_main(hbj_server* S) {
+  hbj_status s = worker_loop(S);
+  if (s != HBJ_OK) {
+    // device failure: fail everything that is still outstanding
+    const std::string err = hbj_last_error();
+    std::lock_guard<std::mutex> g(S->mu);
+    S->stop = true;
+    for (auto& kv : S->reqs)
+      if (!kv.second->done) finish(S, kv.second, s, err);
+    S->cv_admit.notify_all();
+  }
+}
+
+uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
+  auto c = [](uint64_t k) { return k ? k : 1; };
+  const uint64_t nt = c((n + kTile - 1) / kTile), nb = n_starts ? n_starts - 1 : 0;
+  return 4 * c(n + 1) + 8 * c(nnz) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
+         4 * 3 * nt + 8 * 3 * nt + sizeof(SysDesc) + 4 * 8 * c(n) + 4 * c(n);
+}
+
+}  // namespace
+
+extern "C" {
+
+hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_server** out) {
+  clear_error();
+  if (!ctx || !cfg || !out || cfg->max_active == 0 || cfg->hbm_budget_bytes == 0) {
+    set_error("hbj_server_create: need a context, max_active > 0 and hbm_budget_bytes > 0");
+    return HBJ_E_INVALID;
+  }
+  auto* S = new hbj_server();
+  S->ctx = ctx;
+  S->cfg = *cfg;
+  if (S->cfg.chunk_iters == 0) S->cfg.chunk_iters = 16;
+  hipError_t e = hipSetDevice(ctx->device);
+  hbj_status s = e == hipSuccess ? S->eng.init(ctx->device, cfg->max_active, 1) : HBJ_E_DEVICE;
+  if (s == HBJ_OK) {
+    S->eng.use_graph = false;  // the table changes between chunks
+    S->eng.n_tiles = 0;
+    e = hipMemset(S->eng.d_states, 0, sizeof(SysState) * cfg->max_active);
+    if (e != hipSuccess) {
+      set_error("hipMemset failed: %s", hipGetErrorString(e));
+      s = HBJ_E_DEVICE;
+    }
+  }
+  if (s != HBJ_OK) {
+    S->eng.destroy();
+    delete S;
+    return s;
+  }
+  S->admitter = std::thread(admitter_main, S);
+  S->worker = std::thread(worker_main, S);
+  *out = S;
+  return HBJ_OK;
+}
+
+void hbj_server_destroy(hbj_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    S->stop = true;  // outstanding requests still run to completion
+  }
+  S->cv_admit.notify_all();
+  S->cv_work.notify_all();
+  if (S->admitter.joinable()) S->admitter.join();
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    S->cv_work.notify_all();
+  }
+  if (S->worker.joinable()) S->worker.join();
+  (void)hipSetDevice(S->ctx->device);
+  for (auto& kv : S->reqs) {
+    if (kv.second->sys) hbj_system_destroy(kv.second->sys);
+    delete kv.second;
+  }
+  S->eng.destroy();
+  delete S;
+}
+
+hbj_status hbj_server_submit(hbj_server* S, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                             const uint32_t* col_idx, const float* values,
+                             const uint32_t* block_starts, uint32_t n_starts, const double* b64,
+                             const hbj_ir_params* params, uint64_t* ticket) {
+  clear_error();
+  if (!S || !row_ptr || !block_starts || !params || !ticket || (n && !b64) ||
+      (nnz && (!col_idx || !values))) {
+    set_error("hbj_server_submit: bad argument");
+    return HBJ_E_INVALID;
+  }
+  if (params->max_inner == 0 || !(params->inner_rel_tol > 0.0f) || !(params->target_rel_res > 0.0)) {
+    set_error("hbj_ir_params: max_inner > 0, inner_rel_tol > 0 and target_rel_res > 0 required");
+    return HBJ_E_INVALID;
+  }
+  CHK(validate_case(n, nnz, row_ptr, col_idx, block_starts, n_starts));
+  auto* r = new Request();
+  r->n = n;
+  r->nnz = nnz;
+  r->n_starts = n_starts;
+  r->est_bytes = estimate_bytes(n, nnz, n_starts);
+  std::lock_guard<std::mutex> g(S->mu);
+  if (S->stop) {
+    delete r;
+    set_error("hbj_server_submit: server is shutting down");
+    return HBJ_E_INVALID;
+  }
+  if (r->est_bytes > S->cfg.hbm_budget_bytes) {
+    S->st.rejected += 1;
+    set_error("request needs %llu device bytes, above the admission budget of %llu",
+              (unsigned long long)r->est_bytes, (unsigned long long)S->cfg.hbm_budget_bytes);
+    delete r;
+    return HBJ_E_REJECTED;
+  }
+  r->rp.assign(row_ptr, row_ptr + n + 1);
+  r->col.assign(col_idx, col_idx + nnz);
+  r->val.assign(values, values + nnz);
+  r->bs.assign(block_starts, block_starts + n_starts);
+  r->b64.assign(b64, b64 + n);
+  r->params = *params;
+  r->ticket = S->next_ticket++;
+  r->t_submit = now_ms();
+  S->reqs[r->ticket] = r;
+  S->queue.push_back(r);
+  S->st.submitted += 1;
+  *ticket = r->ticket;
+  S->cv_admit.notify_all();
+  return HBJ_OK;
+}
+
+hbj_status hbj_server_wait(hbj_server* S, uint64_t ticket, double* x, hbj_result* result,
+                           double* latency_ms) {
+  clear_error();
+  if (!S) {
+    set_error("hbj_server_wait: null server");
+    return HBJ_E_INVALID;
+  }
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(ticket);
+  if (it == S->reqs.end()) {
+    set_error("hbj_server_wait: unknown ticket %llu", (unsigned long long)ticket);
+    return HBJ_E_INVALID;
+  }
+  Request* r = it->second;
+  S->cv_done.wait(lk, [&] { return r->done; });
+  if (x && r->status == HBJ_OK) std::memcpy(x, r->x.data(), sizeof(double) * r->n);
+  if (result) *result = r->res;
+  if (result) result->status = r->status;
+  if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+  const int32_t st = r->status;
+  if (st != HBJ_OK) set_error("%s", r->error.c_str());
+  S->reqs.erase(it);
+  delete r;
+  return (hbj_status)st;
+}
+
+hbj_status hbj_server_stats_get(hbj_server* S, hbj_server_stats* out) {
+  if (!S || !out) {
+    set_error("hbj_server_stats_get: bad argument");
+    return HBJ_E_INVALID;
+  }
+  std::lock_guard<std::mutex> g(S->mu);
+  *out = S->st;
+  out->queued = S->queue.size() + S->ready.size();
+  out->device_bytes_in_use = S->bytes_in_use;
+  return HBJ_OK;
+}
+
+}  // extern "C"

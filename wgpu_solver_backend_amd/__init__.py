@@ -28,8 +28,13 @@ __all__ = [
     "SolverError", "GpuContext", "lu_factor_inplace_6", "build_lu_blocks_from_csr_block_starts_6",
     "SpmvExecutor", "DotScalarExecutor", "VecOpsExecutor", "BlockJacobiExecutor",
     "PcgUpdateScalarsExecutor", "pcg_block_jacobi_csr_wgpu", "System", "Batch", "Server",
-    "IrParams", "lattice", "lattice_size", "beam_element_stiffness",
+    "IrParams", "lattice", "lattice_size", "beam_element_stiffness", "text_to_bin",
+    "load_case_dir", "write_x_bin", "read_x_bin", "compare_x", "CLI_PATH",
 ]
+
+import os as _os
+
+CLI_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "hbj_cli")
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "ZERO_PIVOT", 3: "BREAKDOWN_PAP", 4: "BREAKDOWN_RZ",
                 5: "NOT_CONVERGED", 6: "DEVICE", 7: "OOM", 8: "NO_DEVICE", 9: "REJECTED"}
@@ -455,3 +460,64 @@ def beam_element_stiffness(E, G, A, Iy, Iz, J, L, direction):
     k = np.zeros(144, np.float64)
     _check(_lib.lib().hbj_beam_element_stiffness(E, G, A, Iy, Iz, J, L, int(direction), k))
     return k.reshape(12, 12)
+
+
+# ------------------------------------------------------- case-directory IO
+
+def text_to_bin(input_txt, out_dir, force=False):
+    """convert main.rs:104-494 (text-to-bin)."""
+    _check(_lib.lib().hbj_text_to_bin(str(input_txt).encode(), str(out_dir).encode(), int(force)))
+
+
+def load_case_dir(case_dir):
+    """loaders.rs:8-42: dict of row_ptr, col_idx, values, b, x0, block_starts (copies)."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    _check(L.hbj_case_load(str(case_dir).encode(), ctypes.byref(h)))
+    try:
+        n, nnz, ns = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        ptrs = [ctypes.c_void_p() for _ in range(6)]
+        _check(L.hbj_case_view(h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(ns),
+                               *[ctypes.byref(p) for p in ptrs]))
+
+        def arr(p, count, dt):
+            if count == 0:
+                return np.zeros(0, dt)
+            ct = ctypes.c_uint32 if dt == np.uint32 else ctypes.c_float
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(count,)).astype(dt).copy()
+
+        N, NNZ, NS = n.value, nnz.value, ns.value
+        return {"n": N, "nnz": NNZ, "row_ptr": arr(ptrs[0], N + 1, np.uint32),
+                "col_idx": arr(ptrs[1], NNZ, np.uint32), "values": arr(ptrs[2], NNZ, np.float32),
+                "b": arr(ptrs[3], N, np.float32), "x0": arr(ptrs[4], N, np.float32),
+                "block_starts": arr(ptrs[5], NS, np.uint32)}
+    finally:
+        L.hbj_case_free(h)
+
+
+def write_x_bin(path, x):
+    x = _f32(x)
+    _check(_lib.lib().hbj_vec_bin_write(str(path).encode(), _nz(x, np.float32), x.size))
+
+
+def read_x_bin(path):
+    L = _lib.lib()
+    n = ctypes.c_uint32()
+    _check(L.hbj_vec_bin_read(str(path).encode(), None, 0, ctypes.byref(n)))
+    x = np.zeros(max(n.value, 1), np.float32)
+    _check(L.hbj_vec_bin_read(str(path).encode(), x.ctypes.data, x.size, ctypes.byref(n)))
+    return x[: n.value]
+
+
+def compare_x(x_ref, x, top_k=10):
+    """cli main.rs:603-662: dict with n, l2_ref, l2_err, rel_l2, rmse, max_abs_err, worst."""
+    xr, xx = _f32(x_ref), _f32(x)
+    if xr.size != xx.size:
+        raise SolverError(1, f"Length mismatch: x_ref len {xr.size} vs x len {xx.size}")
+    st = _lib.hbj_compare_stats()
+    worst = np.zeros(max(top_k, 1), np.uint32)
+    _check(_lib.lib().hbj_compare_x(_nz(xr, np.float32), _nz(xx, np.float32), xr.size, int(top_k),
+                                    ctypes.byref(st), worst.ctypes.data))
+    d = {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("reserved")}
+    d["worst"] = worst[: st.n_worst].tolist()
+    return d

@@ -43,6 +43,11 @@ class hbj_server_stats(ctypes.Structure):
                 ("peak_active", c_u64)]
 
 
+class hbj_compare_stats(ctypes.Structure):
+    _fields_ = [("n", c_u32), ("n_worst", c_u32), ("l2_ref", c_f64), ("l2_err", c_f64),
+                ("rel_l2", c_f64), ("rmse", c_f64), ("max_abs_err", c_f32), ("reserved0", c_f32)]
+
+
 P = ctypes.POINTER
 _SIGS = {
     "hbj_version": (c_char_p, []),
@@ -80,6 +85,13 @@ _SIGS = {
                                   P(hbj_ir_params), P(c_u64)]),
     "hbj_server_wait": (c_i32, [c_void_p, c_u64, c_void_p, P(hbj_result), P(c_f64)]),
     "hbj_server_stats_get": (c_i32, [c_void_p, P(hbj_server_stats)]),
+    "hbj_case_load": (c_i32, [c_char_p, P(c_void_p)]),
+    "hbj_case_free": (None, [c_void_p]),
+    "hbj_case_view": (c_i32, [c_void_p, P(c_u32), P(c_u32), P(c_u32)] + [P(c_void_p)] * 6),
+    "hbj_text_to_bin": (c_i32, [c_char_p, c_char_p, ctypes.c_int]),
+    "hbj_vec_bin_write": (c_i32, [c_char_p, f32p, c_u32]),
+    "hbj_vec_bin_read": (c_i32, [c_char_p, c_void_p, c_u32, P(c_u32)]),
+    "hbj_compare_x": (c_i32, [f32p, f32p, c_u32, c_u32, P(hbj_compare_stats), c_void_p]),
     "hbj_lattice_size": (c_i32, [c_u32, P(c_u32), P(c_u64)]),
     "hbj_lattice_generate": (c_i32, [c_u32, c_u64, c_f64, u32p, u32p, f32p, f32p, u32p]),
     "hbj_beam_element_stiffness": (c_i32, [c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, ctypes.c_int, f64p]),

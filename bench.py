@@ -141,6 +141,156 @@ def bench_plain(args):
     return out
 
 
+def _lattice_sides(rng, count, lo, hi):
+    return [int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(count)]
+
+
+def _host_rel_res(L, x):
+    import scipy.sparse as sp
+    n = L["n"]
+    A = sp.csr_matrix((L["values"].astype(np.float64), L["col_idx"], L["row_ptr"]), shape=(n, n))
+    b = L["rhs"].astype(np.float64)
+    return float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+
+
+def bench_ir(args):
+    """Mixed precision at 50^3 with E varied over 3 decades: FP32 PCG inside FP64
+    refinement to a 1e-10 relative residual (verified on the host in FP64)."""
+    import wgpu_solver_backend_amd as H
+    N = args.N
+    L = H.lattice(N, seed=0, e_log10_half_range=1.5)
+    n, nnz = L["n"], L["nnz"]
+    ctx = H.GpuContext.create(0)
+    system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    b64 = L["rhs"].astype(np.float64)
+    system.set_rhs64(b64)
+    params = H.IrParams(max_outer=40, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-10)
+    for _ in range(args.warmup):
+        x, res = system.solve_ir(b64, None, params)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        x, res = system.solve_ir(b64, None, params)
+        lat.append((time.perf_counter() - ts) * 1e3)
+    elapsed = time.perf_counter() - t0
+    host_res = _host_rel_res(L, x)
+    # plain FP32 PCG on the same system for comparison (its FP64 true residual)
+    system.set_rhs(L["rhs"])
+    r32 = system.solve_resident(20000, 1e-6, 0.0, raise_on_error=False)
+    return {"metric": METRIC, "value": args.steps / elapsed, "unit": "solves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": f"mixed precision: {N}^3 beam lattice, E over 3 decades, FP32 PCG "
+                                   f"(inner rel tol {args.inner_tol:g}) in FP64 refinement to 1e-10",
+                       "n": n, "nnz": nnz, "outer": int(res.outer_iterations),
+                       "inner_iterations": int(res.iterations)},
+            "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99))},
+            "true_rel_residual_fp64_device": res.rel_res_true_fp64,
+            "true_rel_residual_fp64_host": host_res,
+            "plain_fp32_true_rel_residual_fp64": r32.rel_res_true_fp64,
+            "plain_fp32_iterations": int(r32.iterations)}
+
+
+def bench_batch(args):
+    """256 independent lattices (sides log-uniform in [4,24]) solved together,
+    FP32 inner + FP64 refinement to 1e-8, against the same systems solved one
+    at a time through the per-system path."""
+    import wgpu_solver_backend_amd as H
+    rng = np.random.default_rng(0)
+    sides = _lattice_sides(rng, args.batch, 4, 24)
+    ctx = H.GpuContext.create(0)
+    Ls = [H.lattice(N, seed=1000 + i) for i, N in enumerate(sides)]
+    systems = [H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"]) for L in Ls]
+    for L, s in zip(Ls, systems):
+        s.set_rhs64(L["rhs"].astype(np.float64))
+    params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
+    batch = H.Batch(ctx, systems)
+    for _ in range(args.warmup):
+        batch.solve_ir(params)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st, results = batch.solve_ir(params)
+    elapsed = time.perf_counter() - t0
+    batched_sps = args.steps * len(systems) / elapsed
+    # per-system path on the same systems (one solve each)
+    t1 = time.perf_counter()
+    alone = []
+    for L, s in zip(Ls, systems):
+        x, r = s.solve_ir(L["rhs"].astype(np.float64), None, params)
+        alone.append(r)
+    t_alone = time.perf_counter() - t1
+    alone_sps = len(systems) / t_alone
+    worst = max(r.rel_res_true_fp64 for r in results)
+    same = all(a.iterations == r.iterations for a, r in zip(alone, results))
+    ns = [L["n"] for L in Ls]
+    return {"metric": METRIC, "value": batched_sps, "unit": "solves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": f"batched: {len(systems)} beam lattices, sides log-uniform [4,24] "
+                                   f"({min(ns):,}..{max(ns):,} unknowns, sum {sum(ns):,}), FP32 inner + "
+                                   f"FP64 refinement to 1e-8, one tile table, graph-replayed chunks",
+                       "systems": len(systems)},
+            "per_system_path_solves_per_s": alone_sps, "speedup_vs_per_system": batched_sps / alone_sps,
+            "worst_true_rel_residual_fp64": worst, "iterations_identical_alone_vs_batched": same,
+            "status_all_ok": st == 0}
+
+
+def bench_serve(args):
+    """Poisson request stream (seeded, target 200 req/s) of lattices with sides
+    log-uniform in [6,40], IR to 1e-8, continuous batching under an HBM budget."""
+    import wgpu_solver_backend_amd as H
+    import threading
+    rng = np.random.default_rng(0)
+    count = args.requests
+    sides = _lattice_sides(rng, count, 6, 40)
+    gaps = rng.exponential(1.0 / args.rate, count)
+    log(f"serve: generating {count} requests")
+    reqs = [H.lattice(N, seed=2000 + i) for i, N in enumerate(sides)]
+    ctx = H.GpuContext.create(0)
+    params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
+    srv = H.Server(ctx, int(args.budget_gb * (1 << 30)), max_active=args.max_active, chunk_iters=8)
+    tickets = [None] * count
+    lat = [None] * count
+    status = [None] * count
+
+    def waiter():
+        for i in range(count):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            st, x, res, ms = srv.wait(tickets[i])
+            lat[i], status[i] = ms, st
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, L in enumerate(reqs):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets[i] = srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                L["rhs"].astype(np.float64), params)
+    th.join()
+    elapsed = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    ok = sum(1 for s in status if s == 0)
+    return {"metric": METRIC, "value": ok / elapsed, "unit": "solves/s", "n_gpus": 1,
+            "steps": count, "warmup": 0, "ms_per_step": elapsed / count * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": f"serving: Poisson arrivals at {args.rate:g} req/s (seed 0), {count} "
+                                   f"requests, lattice sides log-uniform [6,40], FP64-verified 1e-8, "
+                                   f"HBM budget {args.budget_gb:g} GiB, <= {args.max_active} resident"},
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(max(lat))},
+            "offered_rate": count / (t_next - t0), "completed_ok": ok, "server": stats}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +301,13 @@ def main():
                     help="HIP-event timing of every k-th PCG iteration's kernels")
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--max-active", type=int, default=256)
     args = ap.parse_args()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -159,7 +315,7 @@ def main():
                                    "system across GPUs with no exchange, so N>1 is not run here",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    out = bench_plain(args)
+    out = {"plain": bench_plain, "ir": bench_ir, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

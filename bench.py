@@ -80,6 +80,18 @@ def bench_plain(args):
     system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
     system.set_rhs(L["rhs"], np.zeros(n, np.float32))
     max_iter, tol = 5000, 1e-6
+    copy_bytes = 256 << 20
+    copy_us = ctx.stream_copy_us(copy_bytes, 10)
+    copy_gbps = 2 * copy_bytes / (copy_us * 1e-6) / 1e9
+    log(f"stream copy {copy_bytes >> 20} MiB: {copy_us:.1f} us, {copy_gbps:.0f} GB/s")
+    if args.profile_iters:
+        # short counter-collection run: a few full-size iterations, no JSON headline
+        for _ in range(max(args.steps, 1)):
+            r = system.solve_resident(args.profile_iters, tol, 0.0, raise_on_error=False)
+            log(f"profile run: status {r.status}, {r.iterations} its")
+        print(json.dumps({"profile_run": True, "iterations_per_solve": args.profile_iters,
+                          "copy_bytes": copy_bytes, "copy_us": copy_us}), flush=True)
+        return None
     for i in range(args.warmup):
         res = system.solve_resident(max_iter, tol, 0.0)
         log(f"warmup {i}: {res.iterations} its, {res.solve_ms:.1f} ms")
@@ -121,6 +133,7 @@ def bench_plain(args):
                        "max": max(lat)},
         "true_rel_residual_fp64": res.rel_res_true_fp64,
         "spmv_gbps": spmv_gbps,
+        "stream_copy_gbps": copy_gbps,
         "iteration": {"us": iter_us, "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
                       "kernel_us": {"spmv": avg_us[0], "update_precond": avg_us[1], "p_update": avg_us[2]},
                       "kernel_gbps": {"spmv": spmv_gbps,
@@ -303,6 +316,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--profile-iters", type=int, default=0,
+                    help="counter runs: solves capped at this many iterations, no headline line")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate", type=float, default=200.0)
@@ -316,7 +331,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     out = {"plain": bench_plain, "ir": bench_ir, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
-    print(json.dumps(out), flush=True)
+    if out is not None:
+        print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -202,6 +202,12 @@ hbj_status hbj_server_wait(hbj_server* srv, uint64_t ticket, double* x, hbj_resu
                            double* latency_ms);
 hbj_status hbj_server_stats_get(hbj_server* srv, hbj_server_stats* stats);
 
+/* Calibration: `reps` launches of a plain streaming copy of `bytes` (read
+ * bytes + write bytes per launch); *avg_us = mean launch duration (HIP events).
+ * No reference counterpart: the known-size kernel against which the HBM
+ * traffic counters are scaled, and a measured bandwidth ceiling. */
+hbj_status hbj_stream_copy(hbj_ctx* ctx, uint64_t bytes, uint32_t reps, double* avg_us);
+
 /* -------------------------------------------- synthetic workload (host) */
 
 /* 3D space-frame lattice of N^3 nodes, 6 DOF/node, Euler-Bernoulli beams

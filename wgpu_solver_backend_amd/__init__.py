@@ -92,6 +92,12 @@ class GpuContext:
         _check(_lib.lib().hbj_ctx_describe(self._h, buf, len(buf)))
         return buf.value.decode()
 
+    def stream_copy_us(self, nbytes, reps=20):
+        """Mean duration (us) of a plain copy kernel moving nbytes in and nbytes out."""
+        us = ctypes.c_double()
+        _check(_lib.lib().hbj_stream_copy(self._h, int(nbytes), int(reps), ctypes.byref(us)))
+        return us.value
+
     def close(self):
         if self._h:
             _lib.lib().hbj_ctx_destroy(self._h)

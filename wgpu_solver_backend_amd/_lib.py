@@ -92,6 +92,7 @@ _SIGS = {
     "hbj_vec_bin_write": (c_i32, [c_char_p, f32p, c_u32]),
     "hbj_vec_bin_read": (c_i32, [c_char_p, c_void_p, c_u32, P(c_u32)]),
     "hbj_compare_x": (c_i32, [f32p, f32p, c_u32, c_u32, P(hbj_compare_stats), c_void_p]),
+    "hbj_stream_copy": (c_i32, [c_void_p, c_u64, c_u32, P(c_f64)]),
     "hbj_lattice_size": (c_i32, [c_u32, P(c_u32), P(c_u64)]),
     "hbj_lattice_generate": (c_i32, [c_u32, c_u64, c_f64, u32p, u32p, f32p, f32p, u32p]),
     "hbj_beam_element_stiffness": (c_i32, [c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, ctypes.c_int, f64p]),

@@ -582,6 +582,23 @@ void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t
   hipLaunchKernelGGL(k_true_resid, dim3(n_tiles), dim3(kTile), 0, s, d, x, part2, n_tiles);
 }
 
+// Plain streaming copy (16-byte loads and stores): the calibration kernel for
+// the HBM traffic counters and a measured ceiling for the bandwidth figures.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_copy(const f32x4* __restrict__ src,
+                                                     f32x4* __restrict__ dst, uint64_t n4) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+    dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+void launch_stream_copy(const void* src, void* dst, uint64_t n4, hipStream_t s) {
+  const uint64_t want = (n4 + 255) / 256;
+  const uint32_t grid = (uint32_t)(want < 16384 ? (want ? want : 1) : 16384);
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const f32x4*)src, (f32x4*)dst, n4);
+}
+
 // ------------------------------------------------- single-op kernels (KATs)
 
 __global__ void k_op_axpy(uint32_t n, float alpha, const float* __restrict__ x, float* __restrict__ y) {

@@ -117,6 +117,41 @@ hbj_status hbj_ctx_describe(hbj_ctx* ctx, char* buf, size_t len) {
   return HBJ_OK;
 }
 
+// ------------------------------------------------------------ calibration
+
+hbj_status hbj_stream_copy(hbj_ctx* ctx, uint64_t bytes, uint32_t reps, double* avg_us) {
+  clear_error();
+  if (!ctx || !avg_us || bytes < 16 || reps == 0) {
+    set_error("hbj_stream_copy: bad argument");
+    return HBJ_E_INVALID;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const uint64_t n4 = bytes / 16;
+  float *a = nullptr, *b = nullptr;
+  hbj_status st = dalloc(&a, n4 * 4);
+  if (st == HBJ_OK) st = dalloc(&b, n4 * 4);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == HBJ_OK) {
+    HIPCHK(hipMemset(a, 0, n4 * 16));
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch_stream_copy(a, b, n4, ctx->stream);  // warm
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (uint32_t r = 0; r < reps; ++r) launch_stream_copy(a, b, n4, ctx->stream);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_us = ms * 1000.0 / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  dfree(a);
+  dfree(b);
+  return st;
+}
+
 // ----------------------------------------------------------- single-op API
 
 hbj_status hbj_op_axpy(hbj_ctx* ctx, uint32_t n, float alpha, const float* x, float* y) {

@@ -153,7 +153,51 @@ struct Engine {
     HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
     n_sys = (uint32_t)descs.size();
     n_tiles = (uint32_t)tiles.size();
+    full_tiles = tiles;
+    table_full = true;
     drop_graph();
+    return HBJ_OK;
+  }
+
+  // Tile table compaction for many-system runs: once fewer than 3/4 of the
+  // table's tiles belong to running systems, the table is rewritten with only
+  // those (finished systems' tiles would otherwise still be dispatched and
+  // exit every launch).  `full_tiles` is the table as set; each run starts
+  // from it again.
+  std::vector<uint2> full_tiles;
+  bool table_full = true;
+
+  hbj_status restore_full_table() {
+    if (table_full) return HBJ_OK;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(d_tiles, full_tiles.data(), sizeof(uint2) * full_tiles.size(),
+                     hipMemcpyHostToDevice));
+    n_tiles = (uint32_t)full_tiles.size();
+    table_full = true;
+    drop_graph();
+    return HBJ_OK;
+  }
+
+  // Returns true when the table was compacted (the stream is idle afterwards
+  // and `latest` holds the states of the last enqueued chunk).
+  hbj_status maybe_compact(const SysState* polled, const SysState* latest, bool* compacted) {
+    *compacted = false;
+    if (n_sys < 2 || full_tiles.empty()) return HBJ_OK;
+    uint64_t alive = 0;
+    for (const uint2& t : full_tiles) alive += polled[t.x].status == kRunning ? 1 : 0;
+    if (alive * 4 >= (uint64_t)n_tiles * 3) return HBJ_OK;
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<uint2> live;
+    live.reserve(alive);
+    for (const uint2& t : full_tiles)
+      if (latest[t.x].status == kRunning) live.push_back(t);
+    if (live.empty()) return HBJ_OK;  // everything finished in the last chunk
+    HIPCHK(hipMemcpy(d_tiles, live.data(), sizeof(uint2) * live.size(), hipMemcpyHostToDevice));
+    n_tiles = (uint32_t)live.size();
+    table_full = false;
+    drop_graph();
+    if (use_graph && !timing_every) CHK(capture(chunk));
+    *compacted = true;
     return HBJ_OK;
   }
 
@@ -274,6 +318,7 @@ struct Engine {
       max_trips = t > max_trips ? t : max_trips;
     }
     const uint64_t max_chunks = max_trips / chunk + 3;
+    CHK(restore_full_table());
     if (use_graph && !timing_every && (!gexec || gchunk != chunk)) {
       HIPCHK(hipStreamSynchronize(stream));
       CHK(capture(chunk));
@@ -300,6 +345,12 @@ struct Engine {
         HIPCHK(hipEventSynchronize(ev[prev]));
         if (timing_every) CHK(harvest_timing(prev));
         if (all_done(h_poll[prev], n_sys)) {
+          final_pool = pool;
+          break;
+        }
+        bool compacted = false;
+        CHK(maybe_compact(h_poll[prev], h_poll[pool], &compacted));
+        if (compacted && all_done(h_poll[pool], n_sys)) {
           final_pool = pool;
           break;
         }
@@ -391,6 +442,7 @@ struct hbj_system {
   double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
   float* b32 = nullptr;
   double* h_part2 = nullptr;  // pinned, 2 * n_tiles
+  uint2* tile_blk = nullptr;  // per tile: first block touching it, count
   uint64_t bytes = 0;
   Engine eng;
   bool eng_ready = false;
@@ -414,6 +466,7 @@ struct hbj_system {
     d.x64 = x64;
     d.b64 = b64;
     d.r64 = r64;
+    d.tile_blk = tile_blk;
     d.st = nullptr;
     d.n = n;
     d.nb = nb;
@@ -490,6 +543,7 @@ struct hbj_system {
     dfree(ra); dfree(rb); dfree(part); dfree(part64);
     dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
     dfree(d_desc1);
+    dfree(tile_blk);
     hfree(h_part2);
   }
 };

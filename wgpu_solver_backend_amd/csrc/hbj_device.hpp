@@ -19,9 +19,10 @@ constexpr int kTile = 256;
 // block_jacobi.wgsl:60-61
 constexpr int kBlock = 6;
 constexpr int kLuStride = 36;
-// CSR entries staged per pass of the SpMV tile loop (kPer per thread).
-constexpr int kChunk = 2048;
-constexpr int kPer = kChunk / kTile;
+// CSR entries each lane stages per pass of the SpMV wave loop (FP32 path;
+// the FP64 residual path stages half as many 8-byte products).
+constexpr int kPer = 8;
+constexpr int kWaves = kTile / 64;
 // Blocks whose factors a tile stages in LDS (more than this: read from HBM).
 constexpr int kLuStage = 64;
 
@@ -79,6 +80,7 @@ struct SysDesc {
   double* x64;               // IR solution
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual
+  const uint2* tile_blk;     // per tile: first block touching it, number of such blocks
   SysState* st;
   uint32_t n, nb, n_tiles, flags;
 };

@@ -3,9 +3,11 @@
 // One PCG iteration of every active system is three launches over a tile
 // table (workgroup -> (system, 256-row tile)):
 //
-//   k_pcg_spmv    q = A p (CSR rows streamed coalesced through LDS), the
-//                 tile's p.q partial; the system's last tile reduces the
-//                 partials and sets alpha = rz_old / p.q on the device.
+//   k_pcg_spmv    q = A p (each wave streams its 64 rows' CSR entries
+//                 coalesced, products parked in the wave's LDS slice, each
+//                 row summed by its owner lane), the tile's p.q partial; the
+//                 system's last tile reduces the partials and sets
+//                 alpha = rz_old / p.q on the device.
 //   k_pcg_update  x += alpha p, r -= alpha q, z = M^-1 r (6x6 LU solves,
 //                 factors staged in LDS), r.r and r.z partials; the last
 //                 tile computes beta and the stop test and writes status.
@@ -17,12 +19,12 @@
 //
 // Arithmetic follows the reference operation for operation (the file is
 // compiled with -ffp-contract=off, IEEE division):
-//   spmv.wgsl:55-61        q_i = ((0 + v_k0 x_c0) + v_k1 x_c1) + ...  in row order
+//   spmv.wgsl:55-61          q_i = ((0 + v_k0 x_c0) + v_k1 x_c1) + ...  in row order
 //   dot_partials.wgsl:46-69  256-slot stride-halving tree per 256 rows
 //   dot_reduce.wgsl:47-68    repeated 256-way trees over the partials
-//   axpy*.wgsl:41          y + a*x, one rounding each
+//   axpy*.wgsl:41            y + a*x, one rounding each
 //   scale + axpy (compute.rs:442-457)  p = (p*beta) + z, two roundings
-//   block_jacobi.wgsl:93-122  forward then backward substitution
+//   block_jacobi.wgsl:93-122 forward then backward substitution
 //   pcg_update_scalars.wgsl:56-67  alpha, beta with zero guards
 // so an FP32 solve here matches the CPU restatement bit for bit.
 #include <hip/hip_runtime.h>
@@ -43,9 +45,27 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t g) {
   return xcd < rem ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
 }
 
-// dot_partials.wgsl:52-69: s[t] = s[t] + s[t+off] for off = 128..1.  Offsets
-// 128 and 64 go through LDS, 32..1 through lane shuffles inside wave 0 (same
-// pairs, same order).  Result valid in thread 0.
+// Orders LDS writes before LDS reads of other lanes of the same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The reference's 256-slot tree (dot_partials.wgsl:57-69) computed by one
+// wave: lane l holds slots l, l+64, l+128, l+192 as v0..v3.  Offset 128 pairs
+// (l, l+128) and (l+64, l+192); offset 64 pairs the two results; offsets
+// 32..1 are lane shuffles.  Same pairs, same order.  Result in lane 0.
+template <typename T>
+__device__ __forceinline__ T wave_tree256(T v0, T v1, T v2, T v3) {
+  T w = (v0 + v2) + (v1 + v3);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) w = w + __shfl_down(w, off, 64);
+  return w;
+}
+
+// The same tree over a workgroup of 256 threads (slot t = thread t): offsets
+// 128 and 64 through LDS, 32..1 inside wave 0.  Result valid in thread 0.
 template <typename T>
 __device__ __forceinline__ T tree256(T v, T* sh) {
   const int t = threadIdx.x;
@@ -91,24 +111,70 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
 }
 
 // dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
-// chunks of 256 with the same tree.  Done in place by the last tile (chunk g
-// is written to slot g <= 256 g, which is already consumed).  Returns the
-// final value in thread 0.
+// chunks of 256 with the tree.  Run by all 256 threads of the last tile:
+// chunk g is reduced by wave g % 4, each wave loading all of its chunks
+// before reducing any (one global latency instead of one per chunk).  Level
+// results live in lds (256 entries).  Systems with more than 65,536 tiles
+// (over 16.7M rows) first take in-place passes through one wave.  Returns the
+// final value in every thread.
 template <typename T>
-__device__ T reduce_levels(T* part, uint32_t len, T* sh) {
-  while (len > 1u) {
-    const uint32_t out_len = (len + 255u) / 256u;
-    for (uint32_t g = 0; g < out_len; ++g) {
-      const uint32_t i = g * 256u + threadIdx.x;
-      T v = i < len ? part[i] : T(0);
-      T s = tree256<T>(v, sh);
-      if (threadIdx.x == 0) part[g] = s;
+__device__ T reduce_partials(T* part, uint32_t len, T* lds) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (len <= 1u) return part[0];  // one partial: no reduce pass (:210)
+  uint32_t out_len = (len + 255u) / 256u;
+  while (out_len > 256u) {  // every thread runs the loop control; wave 0 does the pass
+    if (wave == 0) {
+      for (uint32_t g = 0; g < out_len; ++g) {  // chunk g >= slot g: in place is safe
+        T v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t i = g * 256u + lane + 64u * j;
+          v[j] = i < len ? part[i] : T(0);
+        }
+        const T w = wave_tree256(v[0], v[1], v[2], v[3]);
+        if (lane == 0) part[g] = w;
+      }
     }
+    __threadfence_block();
     __syncthreads();
     len = out_len;
+    out_len = (len + 255u) / 256u;
   }
-  T r = part[0];
-  return r;
+  for (uint32_t gb = 0; gb < out_len; gb += 16u) {
+    T v[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint32_t g = gb + wave + 4u * m;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t i = g * 256u + lane + 64u * j;
+        v[m][j] = (g < out_len && i < len) ? part[i] : T(0);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint32_t g = gb + wave + 4u * m;
+      if (g < out_len) {
+        const T w = wave_tree256(v[m][0], v[m][1], v[m][2], v[m][3]);
+        if (lane == 0) lds[g] = w;
+      }
+    }
+  }
+  __syncthreads();
+  if (out_len > 1u) {
+    if (wave == 0) {
+      T v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t i = lane + 64u * j;
+        v[j] = i < out_len ? lds[i] : T(0);
+      }
+      const T w = wave_tree256(v[0], v[1], v[2], v[3]);
+      if (lane == 0) lds[0] = w;
+    }
+    __syncthreads();
+  }
+  return lds[0];
 }
 
 // Last-tile election: thread 0 publishes the tile's partial(s) before this
@@ -144,52 +210,66 @@ __device__ __forceinline__ bool load_tile(const SysDesc* __restrict__ descs,
   return true;
 }
 
-// ------------------------------------------------------------ SpMV tile body
+// ------------------------------------------------------------ SpMV wave body
 
-// spmv.wgsl:52-63 for rows [row0, row0+nrows): thread t owns row row0+t and
-// sums its products sequentially from 0.0.  The tile's CSR entries are
-// streamed in coalesced chunks; each product v_k * x[c_k] is rounded once
-// (as the WGSL sum + values[k]*x[j] without contraction) and parked in LDS.
-template <typename TV, typename TA>
-__device__ __forceinline__ TA spmv_tile(const uint32_t* __restrict__ row_ptr,
+// spmv.wgsl:52-63 for the 64 rows of this wave inside the tile
+// [row0, row0+nrows): lane l owns row row0 + 64*wave + l and sums its
+// products sequentially from 0.0.  The wave streams its rows' CSR entries in
+// coalesced chunks of 64*PER (the next chunk's loads are issued before the
+// current chunk is summed); each product v_k * x[c_k] is rounded once (the
+// WGSL sum + values[k]*x[j] without contraction) and parked in the wave's
+// LDS slice.  No workgroup barrier: waves run independently.
+template <typename TV, typename TA, int PER>
+__device__ __forceinline__ TA spmv_wave(const uint32_t* __restrict__ row_ptr,
                                         const uint32_t* __restrict__ col,
                                         const float* __restrict__ val,
                                         const TV* __restrict__ v, uint32_t row0, uint32_t nrows,
-                                        uint32_t* s_rp, TA* s_prod) {
-  constexpr int per = (kChunk * (int)sizeof(float)) / ((int)sizeof(TA) * kTile);
-  constexpr uint32_t chunk = per * kTile;
-  const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i <= nrows; i += kTile) s_rp[i] = row_ptr[row0 + i];
-  __syncthreads();
-  const uint32_t e0 = s_rp[0], e1 = s_rp[nrows];
+                                        TA* s_prod_wave) {
+  constexpr uint32_t chunk = 64u * PER;
+  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
+  if (wr0 >= nrows) return TA(0);
+  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
   uint32_t mb = 0, me = 0;
-  if (t < nrows) {
-    mb = s_rp[t];
-    me = s_rp[t + 1];
+  if (lane < wrows) {
+    mb = row_ptr[row0 + wr0 + lane];
+    me = row_ptr[row0 + wr0 + lane + 1u];
+  }
+  const uint32_t wb = __shfl(mb, 0, 64);
+  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
+  uint32_t c[PER];
+  float a[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t e = wb + (uint32_t)k * 64u + lane;
+    const bool ok = e < we;
+    c[k] = ok ? __builtin_nontemporal_load(col + e) : 0u;
+    a[k] = ok ? __builtin_nontemporal_load(val + e) : 0.0f;
   }
   TA sum = TA(0);
-  for (uint32_t cs = e0; cs < e1; cs += chunk) {
-    const uint32_t ce = (e1 - cs) < chunk ? e1 : cs + chunk;
-    uint32_t c[per];
-    float a[per];
+  for (uint32_t cs = wb; cs < we; cs += chunk) {
+    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
+    TA pr[PER];
 #pragma unroll
-    for (int k = 0; k < per; ++k) {
-      const uint32_t e = cs + (uint32_t)k * kTile + t;
-      const bool ok = e < ce;
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t e = cs + (uint32_t)k * 64u + lane;
+      pr[k] = e < ce ? (TA)a[k] * (TA)v[c[k]] : TA(0);
+    }
+    const uint32_t ns = cs + chunk;  // prefetch the next chunk
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t e = ns + (uint32_t)k * 64u + lane;
+      const bool ok = e < we;
       c[k] = ok ? __builtin_nontemporal_load(col + e) : 0u;
       a[k] = ok ? __builtin_nontemporal_load(val + e) : 0.0f;
     }
 #pragma unroll
-    for (int k = 0; k < per; ++k) {
-      const uint32_t e = cs + (uint32_t)k * kTile + t;
-      if (e < ce) s_prod[k * kTile + t] = (TA)a[k] * (TA)v[c[k]];
-    }
-    __syncthreads();
+    for (int k = 0; k < PER; ++k) s_prod_wave[k * 64 + lane] = pr[k];
+    wave_sync();
     const uint32_t lo = mb > cs ? mb : cs;
     const uint32_t hi = me < ce ? me : ce;
 #pragma unroll 4
-    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod[e - cs];
-    __syncthreads();
+    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+    wave_sync();
   }
   return sum;
 }
@@ -197,36 +277,47 @@ __device__ __forceinline__ TA spmv_tile(const uint32_t* __restrict__ row_ptr,
 // --------------------------------------------------- block-Jacobi tile body
 
 // z for the rows [row0, row0+nrows) of one tile (block_jacobi.wgsl:63-127).
-// Thread j solves block bfirst+j, i.e. every block whose rows meet the tile;
-// factors are staged through LDS kLuStage blocks at a time (stride 37 words
-// to spread LDS banks).  rnew(row) gives the r entry the solve reads; rows
-// inside the tile come from s_r, rows of straddling blocks outside it are
-// recomputed by the same formula (bit-identical to the owner's value).
+// Thread j solves block bfirst+j, i.e. every block whose rows meet the tile
+// (bfirst and the count come from the system's per-tile table, or from a
+// binary search when there is none); factors are staged through LDS in groups
+// of kLuStage blocks (stride 37 words to spread LDS banks).  rnew(row) gives
+// the r entry the solve reads; rows inside the tile come from s_r, rows of
+// straddling blocks outside it are recomputed by the same formula
+// (bit-identical to the owner's value).
 template <typename RNew>
 __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const float* __restrict__ lu,
+                                        const uint2* __restrict__ tile_blk, uint32_t tile,
                                         uint32_t n, uint32_t nb, uint32_t row0, uint32_t nrows,
                                         const float* s_r, float* s_z, float* s_lu,
                                         uint32_t* s_blk, RNew rnew) {
   const uint32_t t = threadIdx.x;
-  if (t == 0) {
-    // largest b with bs[b] <= row: first and last block touching the tile
-    uint32_t lo = 0, hi = nb;
-    while (hi - lo > 1u) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (bs[mid] <= row0) lo = mid; else hi = mid;
+  uint32_t bfirst, nblk;
+  if (tile_blk) {
+    const uint2 tb = tile_blk[tile];
+    bfirst = tb.x;
+    nblk = tb.y;
+  } else {
+    if (t == 0) {
+      // largest b with bs[b] <= row: first and last block touching the tile
+      uint32_t lo = 0, hi = nb;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bs[mid] <= row0) lo = mid; else hi = mid;
+      }
+      const uint32_t bf = lo;
+      const uint32_t last_row = row0 + nrows - 1u;
+      hi = nb;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bs[mid] <= last_row) lo = mid; else hi = mid;
+      }
+      s_blk[0] = bf;
+      s_blk[1] = lo - bf + 1u;
     }
-    const uint32_t bfirst = lo;
-    const uint32_t last_row = row0 + nrows - 1u;
-    hi = nb;
-    while (hi - lo > 1u) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (bs[mid] <= last_row) lo = mid; else hi = mid;
-    }
-    s_blk[0] = bfirst;
-    s_blk[1] = lo - bfirst + 1u;
+    __syncthreads();
+    bfirst = s_blk[0];
+    nblk = s_blk[1];
   }
-  __syncthreads();
-  const uint32_t bfirst = s_blk[0], nblk = s_blk[1];
   for (uint32_t g = 0; g < nblk; g += kLuStage) {
     const uint32_t cnt = (nblk - g) < (uint32_t)kLuStage ? nblk - g : (uint32_t)kLuStage;
     const float* src = lu + (size_t)(bfirst + g) * kLuStride;
@@ -281,8 +372,8 @@ __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const f
 __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ descs,
                                                     const uint2* __restrict__ tiles,
                                                     uint32_t n_tiles, uint32_t remap) {
-  __shared__ uint32_t s_rp[kTile + 1];
-  __shared__ double s_prod_d[kChunk / 2];
+  // per-wave product slices: 64*kPer floats, or 64*kPer/2 doubles
+  __shared__ double s_prod[kWaves * 64 * kPer / 2];
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
   __shared__ int s_flag;
@@ -293,15 +384,15 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
   const int32_t status = st->status;
   const int32_t phase = st->phase;
   if (status != kRunning) return;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
   float* s_red = reinterpret_cast<float*>(s_ra);
 
   if (phase == kPhInit || phase == kPhIter) {
     const bool init = phase == kPhInit;
     const float* v = init ? d.x : d.p;
-    const float q = spmv_tile<float, float>(d.row_ptr, d.col, d.val, v, tr.row0, tr.nrows, s_rp,
-                                            reinterpret_cast<float*>(s_prod_d));
+    float* slice = reinterpret_cast<float*>(s_prod) + wave * 64 * kPer;
+    const float q = spmv_wave<float, float, kPer>(d.row_ptr, d.col, d.val, v, tr.row0, tr.nrows, slice);
     // dot(b,b) at init (compute.rs:235-242), dot(p,Ap) per iteration (:353-360)
     float w = 0.0f;
     double w64 = 0.0;
@@ -322,7 +413,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
     }
     if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
     if (dot64) {
-      const double tot = reduce_levels<double>(d.part64, d.n_tiles, s_rb);
+      const double tot = reduce_partials<double>(d.part64, d.n_tiles, s_rb);
       if (t == 0) {
         st->cnt1 = 0;
         const float pap = (float)tot;
@@ -333,7 +424,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
         st->neg_alpha = -alpha;
       }
     } else {
-      const float tot = reduce_levels<float>(d.part, d.n_tiles, s_red);
+      const float tot = reduce_partials<float>(d.part, d.n_tiles, s_red);
       if (t == 0) {
         st->cnt1 = 0;
         if (init) {
@@ -353,8 +444,9 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
 
   if (phase == kPhIrResid) {
     // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
-    const double ax = spmv_tile<double, double>(d.row_ptr, d.col, d.val, d.x64, tr.row0,
-                                                tr.nrows, s_rp, s_prod_d);
+    double* slice = s_prod + wave * 64 * (kPer / 2);
+    const double ax = spmv_wave<double, double, kPer / 2>(d.row_ptr, d.col, d.val, d.x64, tr.row0,
+                                                          tr.nrows, slice);
     double rr = 0.0, bb = 0.0;
     if (own) {
       const double bi = d.b64[row];
@@ -369,8 +461,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
       d.part64[d.n_tiles + tr.tile] = bb;
     }
     if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
-    const double rr_t = reduce_levels<double>(d.part64, d.n_tiles, s_ra);
-    const double bb_t = reduce_levels<double>(d.part64 + d.n_tiles, d.n_tiles, s_rb);
+    const double rr_t = reduce_partials<double>(d.part64, d.n_tiles, s_ra);
+    const double bb_t = reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_rb);
     if (t == 0) {
       st->cnt1 = 0;
       st->rr64 = rr_t;
@@ -441,7 +533,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
     if (d.flags & kFlagOversize) s_z[t] = d.z[row];  // rows no block writes keep z
   }
   __syncthreads();
-  bj_tile(d.bstart, d.lu, d.n, d.nb, tr.row0, tr.nrows, s_r, s_z, s_lu, s_blk, rnew);
+  bj_tile(d.bstart, d.lu, d.tile_blk, tr.tile, d.n, d.nb, tr.row0, tr.nrows, s_r, s_z, s_lu, s_blk,
+          rnew);
   float zi = 0.0f;
   if (own) {
     zi = s_z[t];
@@ -470,13 +563,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
   float rr, rz;
   double rz64 = 0.0;
   if (dot64) {
-    const double a = reduce_levels<double>(d.part64 + d.n_tiles, d.n_tiles, s_ra);
-    rz64 = reduce_levels<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, s_rb);
+    const double a = reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_ra);
+    rz64 = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, s_rb);
     rr = (float)a;
     rz = (float)rz64;
   } else {
-    rr = reduce_levels<float>(d.part + d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_ra));
-    rz = reduce_levels<float>(d.part + 2u * d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_rb));
+    rr = reduce_partials<float>(d.part + d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_ra));
+    rz = reduce_partials<float>(d.part + 2u * d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_rb));
   }
   if (t != 0) return;
   st->cnt2 = 0;
@@ -553,16 +646,15 @@ __global__ __launch_bounds__(kTile) void k_true_resid(const SysDesc* __restrict_
                                                       const float* __restrict__ x,
                                                       double* __restrict__ part2,
                                                       uint32_t n_tiles) {
-  __shared__ uint32_t s_rp[kTile + 1];
-  __shared__ double s_prod_d[kChunk / 2];
+  __shared__ double s_prod[kWaves * 64 * kPer / 2];
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
   const SysDesc d = *dp;
   const uint32_t tile = blockIdx.x;
   const uint32_t row0 = tile * kTile;
   const uint32_t nrows = (d.n - row0) < (uint32_t)kTile ? d.n - row0 : (uint32_t)kTile;
-  const double ax = spmv_tile<float, double>(d.row_ptr, d.col, d.val, x, row0, nrows, s_rp,
-                                             s_prod_d);
+  const double ax = spmv_wave<float, double, kPer / 2>(d.row_ptr, d.col, d.val, x, row0, nrows,
+                                                       s_prod + (threadIdx.x >> 6) * 64 * (kPer / 2));
   double rr = 0.0, bb = 0.0;
   if (threadIdx.x < nrows) {
     const double bi = (double)d.b[row0 + threadIdx.x];
@@ -621,7 +713,7 @@ __global__ __launch_bounds__(kTile) void k_op_dot(uint32_t n, const float* __res
   const float p = tree256<float>(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = p;
   if (!last_tile(cnt, gridDim.x, &s_flag)) return;
-  const float r = reduce_levels<float>(part, gridDim.x, sh);
+  const float r = reduce_partials<float>(part, gridDim.x, sh);
   if (threadIdx.x == 0) {
     *out = r;
     *cnt = 0;
@@ -632,11 +724,11 @@ __global__ __launch_bounds__(kTile) void k_op_spmv(uint32_t n, const uint32_t* _
                                                    const uint32_t* __restrict__ col,
                                                    const float* __restrict__ val,
                                                    const float* __restrict__ x, float* __restrict__ y) {
-  __shared__ uint32_t s_rp[kTile + 1];
-  __shared__ float s_prod[kChunk];
+  __shared__ float s_prod[kWaves * 64 * kPer];
   const uint32_t row0 = blockIdx.x * kTile;
   const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
-  const float q = spmv_tile<float, float>(rp, col, val, x, row0, nrows, s_rp, s_prod);
+  const float q = spmv_wave<float, float, kPer>(rp, col, val, x, row0, nrows,
+                                                s_prod + (threadIdx.x >> 6) * 64 * kPer);
   if (threadIdx.x < nrows) y[row0 + threadIdx.x] = q;
 }
 
@@ -657,7 +749,8 @@ __global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t 
     s_z[t] = z[row0 + t];
   }
   __syncthreads();
-  bj_tile(bs, lu, n, nb, row0, nrows, s_r, s_z, s_lu, s_blk, [&](uint32_t i) { return r[i]; });
+  bj_tile(bs, lu, nullptr, blockIdx.x, n, nb, row0, nrows, s_r, s_z, s_lu, s_blk,
+          [&](uint32_t i) { return r[i]; });
   if (t < nrows) z[row0 + t] = s_z[t];
 }
 

@@ -10,6 +10,7 @@
 // waits on the host; iterations enqueued after a system stops exit at once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -381,6 +382,7 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   ALLOC(part, 3ull * nt);
   ALLOC(part64, 3ull * nt);
   ALLOC(d_desc1, 1);
+  ALLOC(tile_blk, nt);
 #undef ALLOC
   if (st != HBJ_OK) return fail(st);
   st = halloc(&s->h_part2, 2ull * nt);
@@ -394,6 +396,17 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   up(s->val, values, 4ull * nnz);
   up(s->bs, block_starts, 4ull * n_starts);
   up(s->lu, lu_h.data(), 144ull * nb);
+  {
+    // per tile: the first block touching rows [256t, 256t+256) and how many do
+    std::vector<uint2> tb(nt, make_uint2(0u, 0u));
+    for (uint32_t t = 0; t < s->n_tiles; ++t) {
+      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n) - 1;
+      const uint32_t bf = (uint32_t)(std::upper_bound(block_starts, block_starts + n_starts, r0) - block_starts) - 1;
+      const uint32_t bl = (uint32_t)(std::upper_bound(block_starts, block_starts + n_starts, r1) - block_starts) - 1;
+      tb[t] = make_uint2(bf, bl - bf + 1);
+    }
+    up(s->tile_blk, tb.data(), sizeof(uint2) * nt);
+  }
   if (e == hipSuccess && n) e = hipMemset(s->z, 0, 4ull * n);  // wgpu zero-initialises buffers
   if (e == hipSuccess && n) e = hipMemset(s->x0, 0, 4ull * n);
   if (e == hipSuccess && n) e = hipMemset(s->b, 0, 4ull * n);

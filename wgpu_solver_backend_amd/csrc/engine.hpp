@@ -437,8 +437,11 @@ struct hbj_system {
   uint32_t *rp = nullptr, *col = nullptr, *bs = nullptr;
   float *val = nullptr, *lu = nullptr;
   float *x = nullptr, *x0 = nullptr, *b = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
-  float *ra = nullptr, *rb = nullptr, *part = nullptr;
-  double* part64 = nullptr;
+  float *ra = nullptr, *rb = nullptr;
+  uint64_t* part = nullptr;    // 3 * n_tiles tagged FP32 partials
+  uint64_t* part64 = nullptr;  // 6 * n_tiles tagged FP64 partial words
+  double* resid2 = nullptr;    // 2 * n_tiles FP64 partials of the residual check
+  uint32_t seq_next = 1;       // tag of this system's next reduction launch
   double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
   float* b32 = nullptr;
   double* h_part2 = nullptr;  // pinned, 2 * n_tiles
@@ -486,6 +489,7 @@ struct hbj_system {
     s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257
     s.r_in = ra;
     s.r_out = rb;
+    s.seq = seq_next;
     if (ir) {
       s.max_outer = ip->max_outer;
       s.target2 = ip->target_rel_res * ip->target_rel_res;
@@ -522,9 +526,9 @@ struct hbj_system {
 
   // ||b - A x|| / ||b|| in FP64 for the FP32 solution in `x` (plain mode).
   hbj_status true_residual(hipStream_t s, double* out) {
-    launch_true_resid(d_desc1, x, part64, n_tiles, s);
+    launch_true_resid(d_desc1, x, resid2, n_tiles, s);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h_part2, part64, sizeof(double) * 2 * n_tiles, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_part2, resid2, sizeof(double) * 2 * n_tiles, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     double rr = 0, bb = 0;
     for (uint32_t t = 0; t < n_tiles; ++t) {
@@ -540,7 +544,7 @@ struct hbj_system {
     eng_ready = false;
     dfree(rp); dfree(col); dfree(bs); dfree(val); dfree(lu);
     dfree(x); dfree(x0); dfree(b); dfree(p); dfree(q); dfree(z);
-    dfree(ra); dfree(rb); dfree(part); dfree(part64);
+    dfree(ra); dfree(rb); dfree(part); dfree(part64); dfree(resid2);
     dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
     dfree(d_desc1);
     dfree(tile_blk);

@@ -29,6 +29,8 @@ constexpr int kLuStage = 64;
 // SysState.status while a system is still iterating; any other value is a
 // final hbj_status.
 constexpr int32_t kRunning = -1;
+// Most tiles one system may have (its partials must fit one LDS level).
+constexpr uint32_t kMaxTiles = 65536;
 
 enum Phase : int32_t {
   kPhInit = 0,      // r0 = b - A x0, z0 = M^-1 r0, p0 = z0, rz_old = r0.z0
@@ -56,7 +58,8 @@ struct alignas(128) SysState {
   float* r_out;        // r written this iteration
   // iterative refinement
   uint32_t outer, max_outer;
-  uint32_t total_iter, pad0;
+  uint32_t total_iter;
+  uint32_t seq;        // tag of the current reduction launch (unique per system)
   double target2;      // (target relative residual)^2
   double rr64, bb64;   // ||r64||^2, ||b64||^2 of the last residual
   double scale;        // b32 = r64 * scale
@@ -75,8 +78,8 @@ struct SysDesc {
   float* q;                  // A p
   float* z;
   float* b;                  // right-hand side of the FP32 PCG
-  float* part;               // 3 * n_tiles per-tile dot partials
-  double* part64;            // 3 * n_tiles FP64 partials (IR / dot64)
+  uint64_t* part;            // 3 * n_tiles tagged FP32 partials {tag, bits}
+  uint64_t* part64;          // 3 * 2 * n_tiles tagged FP64 partials (lo, hi words)
   double* x64;               // IR solution
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual

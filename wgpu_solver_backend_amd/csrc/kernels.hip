@@ -110,54 +110,111 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
   b = wb;
 }
 
+// ---- tagged per-tile partials
+//
+// A tile publishes its dot partial as one 64-bit word {launch tag, value
+// bits} with an agent-scope atomic store, then counts itself done.  The last
+// tile to count reads every partial with agent-scope atomic loads and takes a
+// word only when it carries the current tag, re-polling (bounded) otherwise.
+// No tile issues a device-wide release fence, which on this GPU writes back
+// the whole L2 of its XCD.  FP64 partials use two tagged words (low, high).
+constexpr uint32_t kSpinLimit = 1u << 18;
+
+__device__ __forceinline__ void put_tagged(uint64_t* slot, uint32_t tag, uint32_t bits) {
+  __hip_atomic_store(slot, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t load_word(const uint64_t* slot) {
+  return __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// the value bits of a tagged word, waiting for the current tag if needed
+__device__ __forceinline__ uint32_t settle(uint64_t w, const uint64_t* slot, uint32_t tag,
+                                           volatile int* bad) {
+  uint32_t spin = 0;
+  while ((uint32_t)(w >> 32) != tag) {
+    if (*bad || ++spin > kSpinLimit) {  // after one timeout nothing waits again
+      *bad = 1;
+      return 0u;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    w = load_word(slot);
+  }
+  return (uint32_t)w;
+}
+
+template <typename T> struct Tagged;
+template <> struct Tagged<float> {
+  static constexpr int kWords = 1;
+  __device__ static void put(uint64_t* part, uint32_t i, uint32_t tag, float v) {
+    put_tagged(part + i, tag, __float_as_uint(v));
+  }
+  __device__ static float get(const uint64_t* part, uint32_t i, uint32_t tag, volatile int* bad) {
+    const uint64_t* p = part + i;
+    return __uint_as_float(settle(load_word(p), p, tag, bad));
+  }
+};
+template <> struct Tagged<double> {
+  static constexpr int kWords = 2;
+  __device__ static void put(uint64_t* part, uint32_t i, uint32_t tag, double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    put_tagged(part + 2u * i, tag, (uint32_t)b);
+    put_tagged(part + 2u * i + 1u, tag, (uint32_t)(b >> 32));
+  }
+  __device__ static double get(const uint64_t* part, uint32_t i, uint32_t tag, volatile int* bad) {
+    const uint64_t* p = part + 2u * i;
+    const uint64_t w0 = load_word(p), w1 = load_word(p + 1);
+    const uint32_t lo = settle(w0, p, tag, bad), hi = settle(w1, p + 1, tag, bad);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+  }
+};
+
 // dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
 // chunks of 256 with the tree.  Run by all 256 threads of the last tile:
-// chunk g is reduced by wave g % 4, each wave loading all of its chunks
-// before reducing any (one global latency instead of one per chunk).  Level
-// results live in lds (256 entries).  Systems with more than 65,536 tiles
-// (over 16.7M rows) first take in-place passes through one wave.  Returns the
-// final value in every thread.
+// chunk g is reduced by wave g % 4; each wave issues the loads of all its
+// chunks before reducing any (one memory latency, not one per chunk).  The
+// first level's results (at most kMaxTiles / 256 = 256) live in lds, the
+// second level is one wave tree.  Returns the final value in every thread;
+// *bad is set if a partial never showed the current tag.
 template <typename T>
-__device__ T reduce_partials(T* part, uint32_t len, T* lds) {
+__device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T* lds,
+                             volatile int* bad) {
+  using TG = Tagged<T>;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  if (len <= 1u) return part[0];  // one partial: no reduce pass (:210)
-  uint32_t out_len = (len + 255u) / 256u;
-  while (out_len > 256u) {  // every thread runs the loop control; wave 0 does the pass
-    if (wave == 0) {
-      for (uint32_t g = 0; g < out_len; ++g) {  // chunk g >= slot g: in place is safe
-        T v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const uint32_t i = g * 256u + lane + 64u * j;
-          v[j] = i < len ? part[i] : T(0);
-        }
-        const T w = wave_tree256(v[0], v[1], v[2], v[3]);
-        if (lane == 0) part[g] = w;
-      }
-    }
-    __threadfence_block();
-    __syncthreads();
-    len = out_len;
-    out_len = (len + 255u) / 256u;
-  }
+  if (len <= 1u) return TG::get(part, 0, tag, bad);  // one partial: no reduce pass (:210)
+  const uint32_t out_len = (len + 255u) / 256u;
   for (uint32_t gb = 0; gb < out_len; gb += 16u) {
-    T v[4][4];
+    uint64_t w[4][4][TG::kWords];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const uint32_t g = gb + wave + 4u * m;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t i = g * 256u + lane + 64u * j;
-        v[m][j] = (g < out_len && i < len) ? part[i] : T(0);
+#pragma unroll
+        for (int h = 0; h < TG::kWords; ++h)
+          w[m][j][h] = (g < out_len && i < len) ? load_word(part + TG::kWords * i + h) : 0ull;
       }
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const uint32_t g = gb + wave + 4u * m;
-      if (g < out_len) {
-        const T w = wave_tree256(v[m][0], v[m][1], v[m][2], v[m][3]);
-        if (lane == 0) lds[g] = w;
+      if (g >= out_len) continue;
+      T v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t i = g * 256u + lane + 64u * j;
+        if (i < len) {
+          uint32_t bits[TG::kWords];
+#pragma unroll
+          for (int h = 0; h < TG::kWords; ++h)
+            bits[h] = settle(w[m][j][h], part + TG::kWords * i + h, tag, bad);
+          if constexpr (TG::kWords == 1) v[j] = __uint_as_float(bits[0]);
+          else v[j] = __longlong_as_double((long long)(((uint64_t)bits[1] << 32) | bits[0]));
+        } else {
+          v[j] = T(0);
+        }
       }
+      const T r = wave_tree256(v[0], v[1], v[2], v[3]);
+      if (lane == 0) lds[g] = r;
     }
   }
   __syncthreads();
@@ -169,26 +226,28 @@ __device__ T reduce_partials(T* part, uint32_t len, T* lds) {
         const uint32_t i = lane + 64u * j;
         v[j] = i < out_len ? lds[i] : T(0);
       }
-      const T w = wave_tree256(v[0], v[1], v[2], v[3]);
-      if (lane == 0) lds[0] = w;
+      const T r = wave_tree256(v[0], v[1], v[2], v[3]);
+      if (lane == 0) lds[0] = r;
     }
     __syncthreads();
   }
   return lds[0];
 }
 
-// Last-tile election: thread 0 publishes the tile's partial(s) before this
-// call; returns true in every thread of the system's last tile to finish.
+// Last-tile election: thread 0 has published the tile's partial(s) before
+// this call; returns true in every thread of the system's last tile to count.
 __device__ __forceinline__ bool last_tile(uint32_t* cnt, uint32_t n_tiles, int* s_flag) {
   if (threadIdx.x == 0) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);  // partial stores acknowledged before counting
     const uint32_t prev = atomicAdd(cnt, 1u);
     *s_flag = (prev == n_tiles - 1u);
   }
   __syncthreads();
-  const bool last = *s_flag != 0;
-  if (last) __threadfence();
-  return last;
+  return *s_flag != 0;
+}
+
+__device__ __forceinline__ void reset_counter(uint32_t* cnt) {
+  __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct TileRef {
@@ -377,13 +436,16 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
   __shared__ int s_flag;
+  __shared__ int s_bad;
   TileRef tr;
   if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
   const SysDesc& d = tr.d;
   SysState* st = d.st;
   const int32_t status = st->status;
   const int32_t phase = st->phase;
+  const uint32_t tag = st->seq;
   if (status != kRunning) return;
+  if (threadIdx.x == 0) s_bad = 0;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
   float* s_red = reinterpret_cast<float*>(s_ra);
@@ -406,16 +468,19 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
     const bool dot64 = (d.flags & kFlagDot64) && !init;
     if (dot64) {
       const double pd = tree256<double>(w64, s_rb);
-      if (t == 0) d.part64[tr.tile] = pd;
+      if (t == 0) Tagged<double>::put(d.part64, tr.tile, tag, pd);
     } else {
       const float pd = tree256<float>(w, s_red);
-      if (t == 0) d.part[tr.tile] = pd;
+      if (t == 0) Tagged<float>::put(d.part, tr.tile, tag, pd);
     }
     if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
     if (dot64) {
-      const double tot = reduce_partials<double>(d.part64, d.n_tiles, s_rb);
+      const double tot = reduce_partials<double>(d.part64, d.n_tiles, tag, s_rb, &s_bad);
+      __syncthreads();
       if (t == 0) {
-        st->cnt1 = 0;
+        reset_counter(&st->cnt1);
+        st->seq = tag + 1u;
+        if (s_bad) st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
         const float pap = (float)tot;
         const float alpha = tot != 0.0 ? (float)(st->rz64_old / tot) : 0.0f;
         st->p_ap = pap;
@@ -424,10 +489,14 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
         st->neg_alpha = -alpha;
       }
     } else {
-      const float tot = reduce_partials<float>(d.part, d.n_tiles, s_red);
+      const float tot = reduce_partials<float>(d.part, d.n_tiles, tag, s_red, &s_bad);
+      __syncthreads();
       if (t == 0) {
-        st->cnt1 = 0;
-        if (init) {
+        reset_counter(&st->cnt1);
+        st->seq = tag + 1u;
+        if (s_bad) {
+          st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
+        } else if (init) {
           st->b_norm2 = tot;
           if (tot == 0.0f) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
         } else {
@@ -457,14 +526,20 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
     }
     tree256x2<double>(rr, bb, s_ra, s_rb);
     if (t == 0) {
-      d.part64[tr.tile] = rr;
-      d.part64[d.n_tiles + tr.tile] = bb;
+      Tagged<double>::put(d.part64, tr.tile, tag, rr);
+      Tagged<double>::put(d.part64 + 2u * d.n_tiles, tr.tile, tag, bb);
     }
     if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
-    const double rr_t = reduce_partials<double>(d.part64, d.n_tiles, s_ra);
-    const double bb_t = reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_rb);
+    const double rr_t = reduce_partials<double>(d.part64, d.n_tiles, tag, s_ra, &s_bad);
+    const double bb_t = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, tag, s_rb, &s_bad);
+    __syncthreads();
     if (t == 0) {
-      st->cnt1 = 0;
+      reset_counter(&st->cnt1);
+      st->seq = tag + 1u;
+      if (s_bad) {
+        st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
+        return;
+      }
       st->rr64 = rr_t;
       st->bb64 = bb_t;
       if (bb_t == 0.0 || rr_t <= st->target2 * bb_t) {
@@ -492,14 +567,17 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
   __shared__ double s_rb[kTile];
   __shared__ uint32_t s_blk[2];
   __shared__ int s_flag;
+  __shared__ int s_bad;
   TileRef tr;
   if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
   const SysDesc& d = tr.d;
   SysState* st = d.st;
   const int32_t status = st->status;
   const int32_t phase = st->phase;
+  const uint32_t tag = st->seq;
   if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
+  if (threadIdx.x == 0) s_bad = 0;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
   const float* __restrict__ r_in = st->r_in;
@@ -547,32 +625,39 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
     double b = own ? (double)rn * (double)zi : 0.0;
     tree256x2<double>(a, b, s_ra, s_rb);
     if (t == 0) {
-      d.part64[d.n_tiles + tr.tile] = a;
-      d.part64[2u * d.n_tiles + tr.tile] = b;
+      Tagged<double>::put(d.part64 + 2u * d.n_tiles, tr.tile, tag, a);
+      Tagged<double>::put(d.part64 + 4u * d.n_tiles, tr.tile, tag, b);
     }
   } else {
     float a = own ? rn * rn : 0.0f;
     float b = own ? rn * zi : 0.0f;
     tree256x2<float>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
     if (t == 0) {
-      d.part[d.n_tiles + tr.tile] = a;
-      d.part[2u * d.n_tiles + tr.tile] = b;
+      Tagged<float>::put(d.part + d.n_tiles, tr.tile, tag, a);
+      Tagged<float>::put(d.part + 2u * d.n_tiles, tr.tile, tag, b);
     }
   }
   if (!last_tile(&st->cnt2, d.n_tiles, &s_flag)) return;
   float rr, rz;
   double rz64 = 0.0;
   if (dot64) {
-    const double a = reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_ra);
-    rz64 = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, s_rb);
+    const double a = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, tag, s_ra, &s_bad);
+    rz64 = reduce_partials<double>(d.part64 + 4u * d.n_tiles, d.n_tiles, tag, s_rb, &s_bad);
     rr = (float)a;
     rz = (float)rz64;
   } else {
-    rr = reduce_partials<float>(d.part + d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_ra));
-    rz = reduce_partials<float>(d.part + 2u * d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_rb));
+    rr = reduce_partials<float>(d.part + d.n_tiles, d.n_tiles, tag, reinterpret_cast<float*>(s_ra), &s_bad);
+    rz = reduce_partials<float>(d.part + 2u * d.n_tiles, d.n_tiles, tag, reinterpret_cast<float*>(s_rb),
+                                &s_bad);
   }
+  __syncthreads();
   if (t != 0) return;
-  st->cnt2 = 0;
+  reset_counter(&st->cnt2);
+  st->seq = tag + 1u;
+  if (s_bad) {
+    st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
+    return;
+  }
   st->r_in = r_out;  // r of this iteration becomes the input of the next
   st->r_out = const_cast<float*>(r_in);
   if (phase == kPhInit) {
@@ -704,19 +789,22 @@ __global__ void k_op_scale(uint32_t n, float beta, float* __restrict__ x) {
 }
 
 __global__ __launch_bounds__(kTile) void k_op_dot(uint32_t n, const float* __restrict__ a,
-                                                  const float* __restrict__ b, float* part,
+                                                  const float* __restrict__ b, uint64_t* part,
                                                   uint32_t* cnt, float* out) {
   __shared__ float sh[kTile];
   __shared__ int s_flag;
+  __shared__ int s_bad;
+  if (threadIdx.x == 0) s_bad = 0;
   const uint32_t i = blockIdx.x * kTile + threadIdx.x;
   const float v = i < n ? a[i] * b[i] : 0.0f;
   const float p = tree256<float>(v, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = p;
+  if (threadIdx.x == 0) Tagged<float>::put(part, blockIdx.x, 1u, p);
   if (!last_tile(cnt, gridDim.x, &s_flag)) return;
-  const float r = reduce_partials<float>(part, gridDim.x, sh);
+  const float r = reduce_partials<float>(part, gridDim.x, 1u, sh, &s_bad);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    *out = r;
-    *cnt = 0;
+    *out = s_bad ? __int_as_float(0x7fc00000) : r;  // NaN if a partial never arrived
+    reset_counter(cnt);
   }
 }
 
@@ -787,7 +875,7 @@ void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream
 void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s) {
   hipLaunchKernelGGL(k_op_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, beta, x);
 }
-void launch_op_dot(uint32_t n, const float* a, const float* b, float* part, uint32_t* cnt,
+void launch_op_dot(uint32_t n, const float* a, const float* b, uint64_t* part, uint32_t* cnt,
                    float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_op_dot, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, n, a, b, part,
                      cnt, out);

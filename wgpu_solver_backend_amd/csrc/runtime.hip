@@ -39,6 +39,9 @@ void status_message(int32_t st, uint32_t max_iter) {
     case HBJ_E_NOT_CONVERGED:
       set_error("PCG(BlockJacobiGpu): did not converge in %u iterations", max_iter);  // :492
       break;
+    case HBJ_E_DEVICE:
+      set_error("device: a tile's dot-product partial never became visible to the reducing tile");
+      break;
     default:
       break;
   }
@@ -204,7 +207,8 @@ hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, 
   std::lock_guard<std::mutex> g(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   const uint32_t groups = (n + kTile - 1) / kTile;
-  float *da = nullptr, *db = nullptr, *part = nullptr, *dout = nullptr;
+  float *da = nullptr, *db = nullptr, *dout = nullptr;
+  uint64_t* part = nullptr;
   uint32_t* cnt = nullptr;
   hbj_status st = dalloc(&da, n);
   if (st == HBJ_OK) st = dalloc(&db, n);
@@ -215,6 +219,7 @@ hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, 
     HIPCHK(hipMemcpy(da, a, 4ull * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(db, b, 4ull * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(cnt, 0, 4));
+    HIPCHK(hipMemset(part, 0, 8ull * groups));
     launch_op_dot(n, da, db, part, cnt, dout, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -340,6 +345,11 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   }
   *out = nullptr;
   CHK(validate_case(n, nnz, row_ptr, col_idx, block_starts, n_starts));
+  if ((uint64_t)n > (uint64_t)kMaxTiles * kTile) {
+    set_error("hbj_system_create: n = %u exceeds the per-system limit of %llu rows", n,
+              (unsigned long long)kMaxTiles * kTile);
+    return HBJ_E_INVALID;
+  }
   const uint32_t nb = n_starts - 1;
   std::vector<float> lu_h(36ull * (nb ? nb : 1));
   CHK(hbj_build_lu_blocks_6(n, row_ptr, n + 1, col_idx, nnz, values, nnz, block_starts, n_starts,
@@ -380,7 +390,8 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   ALLOC(ra, n);
   ALLOC(rb, n);
   ALLOC(part, 3ull * nt);
-  ALLOC(part64, 3ull * nt);
+  ALLOC(part64, 6ull * nt);
+  ALLOC(resid2, 2ull * nt);
   ALLOC(d_desc1, 1);
   ALLOC(tile_blk, nt);
 #undef ALLOC
@@ -408,6 +419,8 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
     up(s->tile_blk, tb.data(), sizeof(uint2) * nt);
   }
   if (e == hipSuccess && n) e = hipMemset(s->z, 0, 4ull * n);  // wgpu zero-initialises buffers
+  if (e == hipSuccess) e = hipMemset(s->part, 0, 8ull * 3 * nt);     // partial tags start at 0
+  if (e == hipSuccess) e = hipMemset(s->part64, 0, 8ull * 6 * nt);
   if (e == hipSuccess && n) e = hipMemset(s->x0, 0, 4ull * n);
   if (e == hipSuccess && n) e = hipMemset(s->b, 0, 4ull * n);
   if (e == hipSuccess) {
@@ -490,6 +503,7 @@ hbj_status hbj_system_solve_resident(hbj_system* s, uint32_t max_iter, float rel
   SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
+  s->seq_next = fin.seq;
   fill_result(fin, false, ms, res);
   if (fin.status == HBJ_OK && res) {
     if (fin.b_norm2 == 0.0f) res->rel_res_true_fp64 = 0.0;
@@ -559,6 +573,7 @@ static hbj_status solve_ir_resident(hbj_system* s, const hbj_ir_params* p, hbj_r
   SysState init = s->init_state(true, 0, 0, 0, p), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
+  s->seq_next = fin.seq;
   fill_result(fin, true, ms, res);
   if (fin.status != HBJ_OK) {
     if (fin.status == HBJ_E_NOT_CONVERGED)
@@ -672,6 +687,7 @@ hbj_status hbj_batch_solve(hbj_batch* b, uint32_t max_iter, float rel_tol, float
   CHK(b->eng.run(init.data(), fin.data(), &ms));
   hbj_status worst = HBJ_OK;
   for (uint32_t i = 0; i < S; ++i) {
+    b->sys[i]->seq_next = fin[i].seq;
     fill_result(fin[i], false, ms, &results[i]);
     hbj_system* s = b->sys[i];
     if (fin[i].status == HBJ_OK && s->n) {
@@ -711,6 +727,7 @@ hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* 
   CHK(b->eng.run(init.data(), fin.data(), &ms));
   hbj_status worst = HBJ_OK;
   for (uint32_t i = 0; i < S; ++i) {
+    b->sys[i]->seq_next = fin[i].seq;
     fill_result(fin[i], true, ms, &results[i]);
     if (fin[i].status != HBJ_OK && worst == HBJ_OK) {
       worst = (hbj_status)fin[i].status;

@@ -167,22 +167,18 @@ template <> struct Tagged<double> {
   }
 };
 
-// dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
-// chunks of 256 with the tree.  Run by all 256 threads of the last tile:
-// chunk g is reduced by wave g % 4; each wave issues the loads of all its
-// chunks before reducing any (one memory latency, not one per chunk).  The
-// first level's results (at most kMaxTiles / 256 = 256) live in lds, the
-// second level is one wave tree.  Returns the final value in every thread;
-// *bad is set if a partial never showed the current tag.
-template <typename T>
-__device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T* lds,
-                             volatile int* bad) {
-  using TG = Tagged<T>;
+// One level of dot_scalar_exec.rs:204-237: dst[g] = tree of chunk g (256
+// values, zero padded) for g < ceil(len/256).  Chunk g is reduced by wave
+// g % 4; each wave issues the loads of up to four of its chunks before
+// reducing any.  get(i, word) returns the raw word i of the source (a tagged
+// partial word, or the bit pattern of a plain value); value(i, words) turns
+// the loaded words into a T.
+template <typename T, int WORDS, typename Get, typename Value>
+__device__ __forceinline__ void level_pass(uint32_t len, T* dst, Get get, Value value) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  if (len <= 1u) return TG::get(part, 0, tag, bad);  // one partial: no reduce pass (:210)
   const uint32_t out_len = (len + 255u) / 256u;
   for (uint32_t gb = 0; gb < out_len; gb += 16u) {
-    uint64_t w[4][4][TG::kWords];
+    uint64_t w[4][4][WORDS];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const uint32_t g = gb + wave + 4u * m;
@@ -190,8 +186,7 @@ __device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T
       for (int j = 0; j < 4; ++j) {
         const uint32_t i = g * 256u + lane + 64u * j;
 #pragma unroll
-        for (int h = 0; h < TG::kWords; ++h)
-          w[m][j][h] = (g < out_len && i < len) ? load_word(part + TG::kWords * i + h) : 0ull;
+        for (int h = 0; h < WORDS; ++h) w[m][j][h] = (g < out_len && i < len) ? get(i, h) : 0ull;
       }
     }
 #pragma unroll
@@ -202,24 +197,58 @@ __device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t i = g * 256u + lane + 64u * j;
-        if (i < len) {
-          uint32_t bits[TG::kWords];
-#pragma unroll
-          for (int h = 0; h < TG::kWords; ++h)
-            bits[h] = settle(w[m][j][h], part + TG::kWords * i + h, tag, bad);
-          if constexpr (TG::kWords == 1) v[j] = __uint_as_float(bits[0]);
-          else v[j] = __longlong_as_double((long long)(((uint64_t)bits[1] << 32) | bits[0]));
-        } else {
-          v[j] = T(0);
-        }
+        v[j] = i < len ? value(i, w[m][j]) : T(0);
       }
       const T r = wave_tree256(v[0], v[1], v[2], v[3]);
-      if (lane == 0) lds[g] = r;
+      if (lane == 0) dst[g] = r;
     }
   }
+}
+
+// dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
+// chunks of 256 with the tree.  Run by all 256 threads of the last tile.  The
+// first level reads the tagged partials; its results go to lds when they fit
+// (at most 256, i.e. up to 65,536 tiles -- every PCG system) and otherwise
+// to `scratch` (global, ceil(len/256) values), which a second level then
+// reduces into lds.  Returns the final value in every thread; *bad is set if
+// a partial never showed the current tag.
+template <typename T>
+__device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T* lds,
+                             volatile int* bad, T* scratch = nullptr) {
+  using TG = Tagged<T>;
+  constexpr int W = TG::kWords;
+  if (len <= 1u) return TG::get(part, 0, tag, bad);  // one partial: no reduce pass (:210)
+  auto get_tagged = [&](uint32_t i, int h) { return load_word(part + W * i + h); };
+  auto val_tagged = [&](uint32_t i, const uint64_t* w) -> T {
+    uint32_t bits[W];
+#pragma unroll
+    for (int h = 0; h < W; ++h) bits[h] = settle(w[h], part + W * i + h, tag, bad);
+    if constexpr (W == 1) return __uint_as_float(bits[0]);
+    else return __longlong_as_double((long long)(((uint64_t)bits[1] << 32) | bits[0]));
+  };
+  uint32_t out_len = (len + 255u) / 256u;
+  if (out_len > 256u) {  // more than 65,536 partials: one level through global scratch
+    level_pass<T, W>(len, scratch, get_tagged, val_tagged);
+    __threadfence_block();
+    __syncthreads();
+    len = out_len;
+    out_len = (len + 255u) / 256u;
+    auto get_plain = [&](uint32_t i, int) -> uint64_t {
+      if constexpr (W == 1) return (uint64_t)__float_as_uint(scratch[i]);
+      else return (uint64_t)__double_as_longlong(scratch[i]);
+    };
+    auto val_plain = [&](uint32_t, const uint64_t* w) -> T {
+      if constexpr (W == 1) return __uint_as_float((uint32_t)w[0]);
+      else return __longlong_as_double((long long)w[0]);
+    };
+    level_pass<T, 1>(len, lds, get_plain, val_plain);
+  } else {
+    level_pass<T, W>(len, lds, get_tagged, val_tagged);
+  }
   __syncthreads();
-  if (out_len > 1u) {
-    if (wave == 0) {
+  if (out_len > 1u) {  // out_len <= 256: one more level, one wave
+    if ((threadIdx.x >> 6) == 0) {
+      const uint32_t lane = threadIdx.x & 63u;
       T v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -790,7 +819,7 @@ __global__ void k_op_scale(uint32_t n, float beta, float* __restrict__ x) {
 
 __global__ __launch_bounds__(kTile) void k_op_dot(uint32_t n, const float* __restrict__ a,
                                                   const float* __restrict__ b, uint64_t* part,
-                                                  uint32_t* cnt, float* out) {
+                                                  float* scratch, uint32_t* cnt, float* out) {
   __shared__ float sh[kTile];
   __shared__ int s_flag;
   __shared__ int s_bad;
@@ -800,7 +829,7 @@ __global__ __launch_bounds__(kTile) void k_op_dot(uint32_t n, const float* __res
   const float p = tree256<float>(v, sh);
   if (threadIdx.x == 0) Tagged<float>::put(part, blockIdx.x, 1u, p);
   if (!last_tile(cnt, gridDim.x, &s_flag)) return;
-  const float r = reduce_partials<float>(part, gridDim.x, 1u, sh, &s_bad);
+  const float r = reduce_partials<float>(part, gridDim.x, 1u, sh, &s_bad, scratch);
   __syncthreads();
   if (threadIdx.x == 0) {
     *out = s_bad ? __int_as_float(0x7fc00000) : r;  // NaN if a partial never arrived
@@ -875,10 +904,10 @@ void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream
 void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s) {
   hipLaunchKernelGGL(k_op_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, beta, x);
 }
-void launch_op_dot(uint32_t n, const float* a, const float* b, uint64_t* part, uint32_t* cnt,
-                   float* out, hipStream_t s) {
+void launch_op_dot(uint32_t n, const float* a, const float* b, uint64_t* part, float* scratch,
+                   uint32_t* cnt, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_op_dot, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, n, a, b, part,
-                     cnt, out);
+                     scratch, cnt, out);
 }
 void launch_op_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, const float* val,
                     const float* x, float* y, hipStream_t s) {

@@ -207,12 +207,13 @@ hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, 
   std::lock_guard<std::mutex> g(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   const uint32_t groups = (n + kTile - 1) / kTile;
-  float *da = nullptr, *db = nullptr, *dout = nullptr;
+  float *da = nullptr, *db = nullptr, *dout = nullptr, *scratch = nullptr;
   uint64_t* part = nullptr;
   uint32_t* cnt = nullptr;
   hbj_status st = dalloc(&da, n);
   if (st == HBJ_OK) st = dalloc(&db, n);
   if (st == HBJ_OK) st = dalloc(&part, groups);
+  if (st == HBJ_OK) st = dalloc(&scratch, (groups + 255) / 256);
   if (st == HBJ_OK) st = dalloc(&dout, 1);
   if (st == HBJ_OK) st = dalloc(&cnt, 1);
   if (st == HBJ_OK) {
@@ -220,12 +221,12 @@ hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, 
     HIPCHK(hipMemcpy(db, b, 4ull * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(cnt, 0, 4));
     HIPCHK(hipMemset(part, 0, 8ull * groups));
-    launch_op_dot(n, da, db, part, cnt, dout, ctx->stream);
+    launch_op_dot(n, da, db, part, scratch, cnt, dout, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, dout, 4, hipMemcpyDeviceToHost));
   }
-  dfree(da); dfree(db); dfree(part); dfree(dout); dfree(cnt);
+  dfree(da); dfree(db); dfree(part); dfree(scratch); dfree(dout); dfree(cnt);
   return st;
 }
 

@@ -113,18 +113,31 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
 // ---- tagged per-tile partials
 //
 // A tile publishes its dot partial as one 64-bit word {launch tag, value
-// bits} with an agent-scope atomic store, then counts itself done.  The last
-// tile to count reads every partial with agent-scope atomic loads and takes a
-// word only when it carries the current tag, re-polling (bounded) otherwise.
+// bits} with a device-scope atomic exchange, then counts itself done.  The
+// last tile to count reads every partial with a device-scope compare-exchange
+// that never matches
+// and takes a word only when it carries the current tag, re-polling (bounded)
+// otherwise.
 // No tile issues a device-wide release fence, which on this GPU writes back
 // the whole L2 of its XCD.  FP64 partials use two tagged words (low, high).
 constexpr uint32_t kSpinLimit = 1u << 18;
 
+// Both sides are device-scope read-modify-write atomics: they execute at the
+// point of coherence shared by all XCDs (plain or atomic loads may be served
+// by a stale copy in the reader's XCD L2).
 __device__ __forceinline__ void put_tagged(uint64_t* slot, uint32_t tag, uint32_t bits) {
-  __hip_atomic_store(slot, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  (void)__hip_atomic_exchange(slot, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t load_word(const uint64_t* slot) {
-  return __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // A compare-exchange that always fails (tag 0xffffffff is never issued)
+  // returns the current word from the coherence point without writing it;
+  // idempotent or/add/max with 0 would be folded back into a plain load.
+  uint64_t expected = ~0ull;
+  (void)__hip_atomic_compare_exchange_strong(const_cast<uint64_t*>(slot), &expected, ~0ull - 1ull,
+                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
 }
 // the value bits of a tagged word, waiting for the current tag if needed
 __device__ __forceinline__ uint32_t settle(uint64_t w, const uint64_t* slot, uint32_t tag,
@@ -169,19 +182,20 @@ template <> struct Tagged<double> {
 
 // One level of dot_scalar_exec.rs:204-237: dst[g] = tree of chunk g (256
 // values, zero padded) for g < ceil(len/256).  Chunk g is reduced by wave
-// g % 4; each wave issues the loads of up to four of its chunks before
-// reducing any.  get(i, word) returns the raw word i of the source (a tagged
-// partial word, or the bit pattern of a plain value); value(i, words) turns
-// the loaded words into a T.
-template <typename T, int WORDS, typename Get, typename Value>
+// g % 4; each wave issues the loads of B of its chunks before reducing them
+// (B is kept small: this code shares the kernel's register budget with the
+// streaming loops).  get(i, word) returns the raw word i of the source (a
+// tagged partial word, or the bit pattern of a plain value); value(i, words)
+// turns the loaded words into a T.
+template <typename T, int WORDS, int B, typename Get, typename Value>
 __device__ __forceinline__ void level_pass(uint32_t len, T* dst, Get get, Value value) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t out_len = (len + 255u) / 256u;
-  for (uint32_t gb = 0; gb < out_len; gb += 16u) {
-    uint64_t w[4][4][WORDS];
+  for (uint32_t g0 = wave; g0 < out_len; g0 += 4u * B) {
+    uint64_t w[B][4][WORDS];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const uint32_t g = gb + wave + 4u * m;
+    for (int m = 0; m < B; ++m) {
+      const uint32_t g = g0 + 4u * m;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t i = g * 256u + lane + 64u * j;
@@ -190,8 +204,8 @@ __device__ __forceinline__ void level_pass(uint32_t len, T* dst, Get get, Value 
       }
     }
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const uint32_t g = gb + wave + 4u * m;
+    for (int m = 0; m < B; ++m) {
+      const uint32_t g = g0 + 4u * m;
       if (g >= out_len) continue;
       T v[4];
 #pragma unroll
@@ -228,7 +242,7 @@ __device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T
   };
   uint32_t out_len = (len + 255u) / 256u;
   if (out_len > 256u) {  // more than 65,536 partials: one level through global scratch
-    level_pass<T, W>(len, scratch, get_tagged, val_tagged);
+    level_pass<T, W, 1>(len, scratch, get_tagged, val_tagged);
     __threadfence_block();
     __syncthreads();
     len = out_len;
@@ -241,9 +255,9 @@ __device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T
       if constexpr (W == 1) return __uint_as_float((uint32_t)w[0]);
       else return __longlong_as_double((long long)w[0]);
     };
-    level_pass<T, 1>(len, lds, get_plain, val_plain);
+    level_pass<T, 1, 1>(len, lds, get_plain, val_plain);
   } else {
-    level_pass<T, W>(len, lds, get_tagged, val_tagged);
+    level_pass<T, W, 3 - W>(len, lds, get_tagged, val_tagged);
   }
   __syncthreads();
   if (out_len > 1u) {  // out_len <= 256: one more level, one wave

@@ -135,10 +135,12 @@ def bench_plain(args):
         "spmv_gbps": spmv_gbps,
         "stream_copy_gbps": copy_gbps,
         "iteration": {"us": iter_us, "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
-                      "kernel_us": {"spmv": avg_us[0], "update_precond": avg_us[1], "p_update": avg_us[2]},
+                      "kernel_us": {"spmv": avg_us[0], "reduce_alpha": avg_us[1],
+                                    "update_precond": avg_us[2], "reduce_beta": avg_us[3],
+                                    "p_update": avg_us[4]},
                       "kernel_gbps": {"spmv": spmv_gbps,
-                                      "update_precond": b2 / (avg_us[1] * 1e-6) / 1e9,
-                                      "p_update": b3 / (avg_us[2] * 1e-6) / 1e9}},
+                                      "update_precond": b2 / (avg_us[2] * 1e-6) / 1e9,
+                                      "p_update": b3 / (avg_us[4] * 1e-6) / 1e9}},
         "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": spmv_gbps,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": spmv_gbps / HBM_PEAK_GBPS,
                      "traffic": (traffic or {}).get("k_pcg_spmv_bytes"),

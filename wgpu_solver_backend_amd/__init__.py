@@ -323,8 +323,8 @@ class System:
         _check(_lib.lib().hbj_system_set_kernel_timing(self._h, int(every)))
 
     def kernel_timing(self):
-        avg = np.zeros(3, np.float64)
-        cnt = np.zeros(3, np.uint64)
+        avg = np.zeros(5, np.float64)
+        cnt = np.zeros(5, np.uint64)
         _check(_lib.lib().hbj_system_kernel_timing(self._h, avg, cnt))
         return avg, cnt
 

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libhbj.so")
+# HBJ_LIB_PATH selects another build of the same library (e.g. a debug build)
+LIB_PATH = os.environ.get("HBJ_LIB_PATH") or os.path.join(PKG, "libhbj.so")
 
 c_u32, c_u64, c_i32, c_f32, c_f64 = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
                                      ctypes.c_float, ctypes.c_double)

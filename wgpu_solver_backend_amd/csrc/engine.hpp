@@ -121,8 +121,9 @@ struct Engine {
   uint64_t trip = 0;  // iterations enqueued so far (for sampling)
   std::vector<hipEvent_t> tev[2];
   std::vector<int> tev_used;  // per pool: number of 4-event groups recorded
-  double t_sum[3] = {0, 0, 0};
-  uint64_t t_cnt[3] = {0, 0, 0};
+  static constexpr int kKernels = 5;  // spmv, reduce1, update, reduce2, p-update
+  double t_sum[kKernels] = {0, 0, 0, 0, 0};
+  uint64_t t_cnt[kKernels] = {0, 0, 0, 0, 0};
 
   hbj_status init(int dev, uint32_t nsys, uint32_t ntiles) {
     device = dev;
@@ -207,12 +208,16 @@ struct Engine {
     gchunk = 0;
   }
 
-  LaunchSet ls() const { return LaunchSet{d_descs, d_tiles, n_tiles, remap}; }
+  LaunchSet ls() const { return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys}; }
 
+  // one PCG iteration of every system in the table: three tile kernels, each
+  // streaming kernel followed by its per-system reduction
   void launch_iter() {
     const LaunchSet l = ls();
     launch_pcg_spmv(l, stream);
+    launch_pcg_reduce1(l, stream);
     launch_pcg_update(l, stream);
+    launch_pcg_reduce2(l, stream);
     launch_pcg_pupdate(l, stream);
   }
 
@@ -234,7 +239,7 @@ struct Engine {
   }
 
   hbj_status ensure_timing_events(uint32_t T) {
-    const size_t need = 4u * ((T + timing_every - 1) / timing_every + 1);
+    const size_t need = (kKernels + 1u) * ((T + timing_every - 1) / timing_every + 1);
     for (int b = 0; b < 2; ++b)
       while (tev[b].size() < need) {
         hipEvent_t e;
@@ -251,14 +256,18 @@ struct Engine {
       const LaunchSet l = ls();
       for (uint32_t i = 0; i < T; ++i, ++trip) {
         if (trip % timing_every == 0) {
-          hipEvent_t* e = &tev[pool][4 * used];
+          hipEvent_t* e = &tev[pool][(kKernels + 1) * used];
           HIPCHK(hipEventRecord(e[0], stream));
           launch_pcg_spmv(l, stream);
           HIPCHK(hipEventRecord(e[1], stream));
-          launch_pcg_update(l, stream);
+          launch_pcg_reduce1(l, stream);
           HIPCHK(hipEventRecord(e[2], stream));
-          launch_pcg_pupdate(l, stream);
+          launch_pcg_update(l, stream);
           HIPCHK(hipEventRecord(e[3], stream));
+          launch_pcg_reduce2(l, stream);
+          HIPCHK(hipEventRecord(e[4], stream));
+          launch_pcg_pupdate(l, stream);
+          HIPCHK(hipEventRecord(e[5], stream));
           ++used;
         } else {
           launch_iter();
@@ -280,8 +289,8 @@ struct Engine {
 
   hbj_status harvest_timing(int pool) {
     for (int u = 0; u < tev_used[pool]; ++u) {
-      hipEvent_t* e = &tev[pool][4 * u];
-      for (int k = 0; k < 3; ++k) {
+      hipEvent_t* e = &tev[pool][(kKernels + 1) * u];
+      for (int k = 0; k < kKernels; ++k) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
         t_sum[k] += ms * 1000.0;
@@ -438,10 +447,9 @@ struct hbj_system {
   float *val = nullptr, *lu = nullptr;
   float *x = nullptr, *x0 = nullptr, *b = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
   float *ra = nullptr, *rb = nullptr;
-  uint64_t* part = nullptr;    // 3 * n_tiles tagged FP32 partials
-  uint64_t* part64 = nullptr;  // 6 * n_tiles tagged FP64 partial words
-  double* resid2 = nullptr;    // 2 * n_tiles FP64 partials of the residual check
-  uint32_t seq_next = 1;       // tag of this system's next reduction launch
+  float* part = nullptr;      // 3 * n_tiles per-tile FP32 partials
+  double* part64 = nullptr;  // 3 * n_tiles per-tile FP64 partials
+  double* resid2 = nullptr;  // 2 * n_tiles FP64 partials of the residual check
   double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
   float* b32 = nullptr;
   double* h_part2 = nullptr;  // pinned, 2 * n_tiles
@@ -489,7 +497,6 @@ struct hbj_system {
     s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257
     s.r_in = ra;
     s.r_out = rb;
-    s.seq = seq_next;
     if (ir) {
       s.max_outer = ip->max_outer;
       s.target2 = ip->target_rel_res * ip->target_rel_res;

@@ -29,7 +29,7 @@ constexpr int kLuStage = 64;
 // SysState.status while a system is still iterating; any other value is a
 // final hbj_status.
 constexpr int32_t kRunning = -1;
-// Most tiles one system may have (its partials must fit one LDS level).
+// Most tiles one system may have (the first reduction level must fit LDS).
 constexpr uint32_t kMaxTiles = 65536;
 
 enum Phase : int32_t {
@@ -53,13 +53,11 @@ struct alignas(128) SysState {
   int32_t phase;       // Phase
   uint32_t iter;       // iterations of the current (inner) PCG
   uint32_t max_iter;   // per inner PCG
-  uint32_t cnt1, cnt2; // last-tile counters of the SpMV and update kernels
   float* r_in;         // r of the previous iteration (double-buffered)
   float* r_out;        // r written this iteration
   // iterative refinement
   uint32_t outer, max_outer;
-  uint32_t total_iter;
-  uint32_t seq;        // tag of the current reduction launch (unique per system)
+  uint32_t total_iter, pad0;
   double target2;      // (target relative residual)^2
   double rr64, bb64;   // ||r64||^2, ||b64||^2 of the last residual
   double scale;        // b32 = r64 * scale
@@ -78,8 +76,8 @@ struct SysDesc {
   float* q;                  // A p
   float* z;
   float* b;                  // right-hand side of the FP32 PCG
-  uint64_t* part;            // 3 * n_tiles tagged FP32 partials {tag, bits}
-  uint64_t* part64;          // 3 * 2 * n_tiles tagged FP64 partials (lo, hi words)
+  float* part;               // 3 * n_tiles per-tile dot partials
+  double* part64;            // 3 * n_tiles FP64 partials (IR / FP64-accumulated dots)
   double* x64;               // IR solution
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual

@@ -16,10 +16,13 @@ struct LaunchSet {
   const uint2* tiles;
   uint32_t n_tiles;
   uint32_t remap;
+  uint32_t n_sys;
 };
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
+void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
+void launch_pcg_reduce2(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s);
 void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t n_tiles,
                        hipStream_t s);
@@ -27,8 +30,8 @@ void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t
 void launch_stream_copy(const void* src, void* dst, uint64_t n4, hipStream_t s);
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s);
 void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s);
-void launch_op_dot(uint32_t n, const float* a, const float* b, uint64_t* part, float* scratch,
-                   uint32_t* cnt, float* out, hipStream_t s);
+void launch_op_dot(uint32_t n, const float* a, const float* b, float* part, float* scratch,
+                   float* out, hipStream_t s);
 void launch_op_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, const float* val,
                     const float* x, float* y, hipStream_t s);
 void launch_op_block_jacobi(uint32_t n, uint32_t nb, const float* lu, const uint32_t* bs,

@@ -110,154 +110,56 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
   b = wb;
 }
 
-// ---- tagged per-tile partials
-//
-// A tile publishes its dot partial as one 64-bit word {launch tag, value
-// bits} with a device-scope atomic exchange, then counts itself done.  The
-// last tile to count reads every partial with a device-scope compare-exchange
-// that never matches
-// and takes a word only when it carries the current tag, re-polling (bounded)
-// otherwise.
-// No tile issues a device-wide release fence, which on this GPU writes back
-// the whole L2 of its XCD.  FP64 partials use two tagged words (low, high).
-constexpr uint32_t kSpinLimit = 1u << 18;
-
-// Both sides are device-scope read-modify-write atomics: they execute at the
-// point of coherence shared by all XCDs (plain or atomic loads may be served
-// by a stale copy in the reader's XCD L2).
-__device__ __forceinline__ void put_tagged(uint64_t* slot, uint32_t tag, uint32_t bits) {
-  (void)__hip_atomic_exchange(slot, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t load_word(const uint64_t* slot) {
-  // A compare-exchange that always fails (tag 0xffffffff is never issued)
-  // returns the current word from the coherence point without writing it;
-  // idempotent or/add/max with 0 would be folded back into a plain load.
-  uint64_t expected = ~0ull;
-  (void)__hip_atomic_compare_exchange_strong(const_cast<uint64_t*>(slot), &expected, ~0ull - 1ull,
-                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-  return expected;
-}
-// the value bits of a tagged word, waiting for the current tag if needed
-__device__ __forceinline__ uint32_t settle(uint64_t w, const uint64_t* slot, uint32_t tag,
-                                           volatile int* bad) {
-  uint32_t spin = 0;
-  while ((uint32_t)(w >> 32) != tag) {
-    if (*bad || ++spin > kSpinLimit) {  // after one timeout nothing waits again
-      *bad = 1;
-      return 0u;
-    }
-    __builtin_amdgcn_s_sleep(2);
-    w = load_word(slot);
-  }
-  return (uint32_t)w;
-}
-
-template <typename T> struct Tagged;
-template <> struct Tagged<float> {
-  static constexpr int kWords = 1;
-  __device__ static void put(uint64_t* part, uint32_t i, uint32_t tag, float v) {
-    put_tagged(part + i, tag, __float_as_uint(v));
-  }
-  __device__ static float get(const uint64_t* part, uint32_t i, uint32_t tag, volatile int* bad) {
-    const uint64_t* p = part + i;
-    return __uint_as_float(settle(load_word(p), p, tag, bad));
-  }
-};
-template <> struct Tagged<double> {
-  static constexpr int kWords = 2;
-  __device__ static void put(uint64_t* part, uint32_t i, uint32_t tag, double v) {
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    put_tagged(part + 2u * i, tag, (uint32_t)b);
-    put_tagged(part + 2u * i + 1u, tag, (uint32_t)(b >> 32));
-  }
-  __device__ static double get(const uint64_t* part, uint32_t i, uint32_t tag, volatile int* bad) {
-    const uint64_t* p = part + 2u * i;
-    const uint64_t w0 = load_word(p), w1 = load_word(p + 1);
-    const uint32_t lo = settle(w0, p, tag, bad), hi = settle(w1, p + 1, tag, bad);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-  }
-};
-
 // One level of dot_scalar_exec.rs:204-237: dst[g] = tree of chunk g (256
 // values, zero padded) for g < ceil(len/256).  Chunk g is reduced by wave
-// g % 4; each wave issues the loads of B of its chunks before reducing them
-// (B is kept small: this code shares the kernel's register budget with the
-// streaming loops).  get(i, word) returns the raw word i of the source (a
-// tagged partial word, or the bit pattern of a plain value); value(i, words)
-// turns the loaded words into a T.
-template <typename T, int WORDS, int B, typename Get, typename Value>
-__device__ __forceinline__ void level_pass(uint32_t len, T* dst, Get get, Value value) {
+// g % 4; each wave issues the loads of two of its chunks before reducing
+// them.
+template <typename T>
+__device__ __forceinline__ void level_pass(const T* __restrict__ src, uint32_t len, T* dst) {
+  constexpr int B = 2;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t out_len = (len + 255u) / 256u;
   for (uint32_t g0 = wave; g0 < out_len; g0 += 4u * B) {
-    uint64_t w[B][4][WORDS];
+    T v[B][4];
 #pragma unroll
     for (int m = 0; m < B; ++m) {
       const uint32_t g = g0 + 4u * m;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t i = g * 256u + lane + 64u * j;
-#pragma unroll
-        for (int h = 0; h < WORDS; ++h) w[m][j][h] = (g < out_len && i < len) ? get(i, h) : 0ull;
+        v[m][j] = (g < out_len && i < len) ? src[i] : T(0);
       }
     }
 #pragma unroll
     for (int m = 0; m < B; ++m) {
       const uint32_t g = g0 + 4u * m;
       if (g >= out_len) continue;
-      T v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t i = g * 256u + lane + 64u * j;
-        v[j] = i < len ? value(i, w[m][j]) : T(0);
-      }
-      const T r = wave_tree256(v[0], v[1], v[2], v[3]);
+      const T r = wave_tree256(v[m][0], v[m][1], v[m][2], v[m][3]);
       if (lane == 0) dst[g] = r;
     }
   }
 }
 
 // dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
-// chunks of 256 with the tree.  Run by all 256 threads of the last tile.  The
-// first level reads the tagged partials; its results go to lds when they fit
-// (at most 256, i.e. up to 65,536 tiles -- every PCG system) and otherwise
-// to `scratch` (global, ceil(len/256) values), which a second level then
-// reduces into lds.  Returns the final value in every thread; *bad is set if
-// a partial never showed the current tag.
+// chunks of 256 with the tree; with exactly one partial there is no pass
+// (:210).  Run by all 256 threads of a workgroup.  The first level's results
+// go to lds when they fit (at most 256, i.e. up to 65,536 partials -- every
+// PCG system) and otherwise to `scratch` (global, ceil(len/256) values),
+// which a second level reduces into lds.  Returns the value in every thread.
 template <typename T>
-__device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T* lds,
-                             volatile int* bad, T* scratch = nullptr) {
-  using TG = Tagged<T>;
-  constexpr int W = TG::kWords;
-  if (len <= 1u) return TG::get(part, 0, tag, bad);  // one partial: no reduce pass (:210)
-  auto get_tagged = [&](uint32_t i, int h) { return load_word(part + W * i + h); };
-  auto val_tagged = [&](uint32_t i, const uint64_t* w) -> T {
-    uint32_t bits[W];
-#pragma unroll
-    for (int h = 0; h < W; ++h) bits[h] = settle(w[h], part + W * i + h, tag, bad);
-    if constexpr (W == 1) return __uint_as_float(bits[0]);
-    else return __longlong_as_double((long long)(((uint64_t)bits[1] << 32) | bits[0]));
-  };
+__device__ T reduce_partials(const T* __restrict__ part, uint32_t len, T* lds,
+                             T* scratch = nullptr) {
+  if (len <= 1u) return part[0];
   uint32_t out_len = (len + 255u) / 256u;
-  if (out_len > 256u) {  // more than 65,536 partials: one level through global scratch
-    level_pass<T, W, 1>(len, scratch, get_tagged, val_tagged);
+  if (out_len > 256u) {
+    level_pass<T>(part, len, scratch);
     __threadfence_block();
     __syncthreads();
     len = out_len;
     out_len = (len + 255u) / 256u;
-    auto get_plain = [&](uint32_t i, int) -> uint64_t {
-      if constexpr (W == 1) return (uint64_t)__float_as_uint(scratch[i]);
-      else return (uint64_t)__double_as_longlong(scratch[i]);
-    };
-    auto val_plain = [&](uint32_t, const uint64_t* w) -> T {
-      if constexpr (W == 1) return __uint_as_float((uint32_t)w[0]);
-      else return __longlong_as_double((long long)w[0]);
-    };
-    level_pass<T, 1, 1>(len, lds, get_plain, val_plain);
+    level_pass<T>(scratch, len, lds);
   } else {
-    level_pass<T, W, 3 - W>(len, lds, get_tagged, val_tagged);
+    level_pass<T>(part, len, lds);
   }
   __syncthreads();
   if (out_len > 1u) {  // out_len <= 256: one more level, one wave
@@ -275,22 +177,6 @@ __device__ T reduce_partials(const uint64_t* part, uint32_t len, uint32_t tag, T
     __syncthreads();
   }
   return lds[0];
-}
-
-// Last-tile election: thread 0 has published the tile's partial(s) before
-// this call; returns true in every thread of the system's last tile to count.
-__device__ __forceinline__ bool last_tile(uint32_t* cnt, uint32_t n_tiles, int* s_flag) {
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_s_waitcnt(0);  // partial stores acknowledged before counting
-    const uint32_t prev = atomicAdd(cnt, 1u);
-    *s_flag = (prev == n_tiles - 1u);
-  }
-  __syncthreads();
-  return *s_flag != 0;
-}
-
-__device__ __forceinline__ void reset_counter(uint32_t* cnt) {
-  __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct TileRef {
@@ -470,6 +356,13 @@ __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const f
 }
 
 // ------------------------------------------------------------ PCG kernels
+//
+// Per iteration: k_pcg_spmv (tiles) -> k_pcg_reduce1 (one workgroup per
+// system) -> k_pcg_update (tiles) -> k_pcg_reduce2 (systems) ->
+// k_pcg_pupdate (tiles).  Tiles write plain per-tile partials; the reduce
+// kernels read them after the kernel boundary, so no workgroup ever waits on
+// or synchronises with another one.  SysState is written only by the reduce
+// kernels and read by the kernels after them.
 
 __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ descs,
                                                     const uint2* __restrict__ tiles,
@@ -478,20 +371,14 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
   __shared__ double s_prod[kWaves * 64 * kPer / 2];
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
-  __shared__ int s_flag;
-  __shared__ int s_bad;
   TileRef tr;
   if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
   const SysDesc& d = tr.d;
-  SysState* st = d.st;
-  const int32_t status = st->status;
+  const SysState* st = d.st;
+  if (st->status != kRunning) return;
   const int32_t phase = st->phase;
-  const uint32_t tag = st->seq;
-  if (status != kRunning) return;
-  if (threadIdx.x == 0) s_bad = 0;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
-  float* s_red = reinterpret_cast<float*>(s_ra);
 
   if (phase == kPhInit || phase == kPhIter) {
     const bool init = phase == kPhInit;
@@ -508,53 +395,14 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
       w = a * c;
       w64 = (double)a * (double)c;
     }
-    const bool dot64 = (d.flags & kFlagDot64) && !init;
-    if (dot64) {
+    if ((d.flags & kFlagDot64) && !init) {
       const double pd = tree256<double>(w64, s_rb);
-      if (t == 0) Tagged<double>::put(d.part64, tr.tile, tag, pd);
+      if (t == 0) d.part64[tr.tile] = pd;
     } else {
-      const float pd = tree256<float>(w, s_red);
-      if (t == 0) Tagged<float>::put(d.part, tr.tile, tag, pd);
+      const float pd = tree256<float>(w, reinterpret_cast<float*>(s_ra));
+      if (t == 0) d.part[tr.tile] = pd;
     }
-    if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
-    if (dot64) {
-      const double tot = reduce_partials<double>(d.part64, d.n_tiles, tag, s_rb, &s_bad);
-      __syncthreads();
-      if (t == 0) {
-        reset_counter(&st->cnt1);
-        st->seq = tag + 1u;
-        if (s_bad) st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
-        const float pap = (float)tot;
-        const float alpha = tot != 0.0 ? (float)(st->rz64_old / tot) : 0.0f;
-        st->p_ap = pap;
-        st->p_ap64 = tot;
-        st->alpha = alpha;
-        st->neg_alpha = -alpha;
-      }
-    } else {
-      const float tot = reduce_partials<float>(d.part, d.n_tiles, tag, s_red, &s_bad);
-      __syncthreads();
-      if (t == 0) {
-        reset_counter(&st->cnt1);
-        st->seq = tag + 1u;
-        if (s_bad) {
-          st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
-        } else if (init) {
-          st->b_norm2 = tot;
-          if (tot == 0.0f) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
-        } else {
-          // pcg_update_scalars.wgsl:59-62, 70-71
-          const float alpha = tot != 0.0f ? st->rz_old / tot : 0.0f;
-          st->p_ap = tot;
-          st->alpha = alpha;
-          st->neg_alpha = -alpha;
-        }
-      }
-    }
-    return;
-  }
-
-  if (phase == kPhIrResid) {
+  } else if (phase == kPhIrResid) {
     // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
     double* slice = s_prod + wave * 64 * (kPer / 2);
     const double ax = spmv_wave<double, double, kPer / 2>(d.row_ptr, d.col, d.val, d.x64, tr.row0,
@@ -569,28 +417,62 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
     }
     tree256x2<double>(rr, bb, s_ra, s_rb);
     if (t == 0) {
-      Tagged<double>::put(d.part64, tr.tile, tag, rr);
-      Tagged<double>::put(d.part64 + 2u * d.n_tiles, tr.tile, tag, bb);
+      d.part64[tr.tile] = rr;
+      d.part64[d.n_tiles + tr.tile] = bb;
     }
-    if (!last_tile(&st->cnt1, d.n_tiles, &s_flag)) return;
-    const double rr_t = reduce_partials<double>(d.part64, d.n_tiles, tag, s_ra, &s_bad);
-    const double bb_t = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, tag, s_rb, &s_bad);
-    __syncthreads();
+  }
+}
+
+// After k_pcg_spmv: ||b||^2 (init), p.Ap and alpha (iteration), or the FP64
+// residual decision (refinement).  One workgroup per system.
+__global__ __launch_bounds__(kTile) void k_pcg_reduce1(const SysDesc* __restrict__ descs,
+                                                       uint32_t n_sys) {
+  __shared__ double s_a[256];
+  __shared__ double s_b[256];
+  if (blockIdx.x >= n_sys) return;
+  const SysDesc d = descs[blockIdx.x];
+  SysState* st = d.st;
+  if (!st || st->status != kRunning) return;  // empty serving slot / finished system
+  const int32_t phase = st->phase;
+  const uint32_t t = threadIdx.x;
+  if (phase == kPhInit) {
+    const float tot = reduce_partials<float>(d.part, d.n_tiles, reinterpret_cast<float*>(s_a));
     if (t == 0) {
-      reset_counter(&st->cnt1);
-      st->seq = tag + 1u;
-      if (s_bad) {
-        st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
-        return;
+      st->b_norm2 = tot;
+      if (tot == 0.0f) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
+    }
+  } else if (phase == kPhIter) {
+    if (d.flags & kFlagDot64) {
+      const double tot = reduce_partials<double>(d.part64, d.n_tiles, s_a);
+      if (t == 0) {
+        const float alpha = tot != 0.0 ? (float)(st->rz64_old / tot) : 0.0f;
+        st->p_ap = (float)tot;
+        st->p_ap64 = tot;
+        st->alpha = alpha;
+        st->neg_alpha = -alpha;
       }
-      st->rr64 = rr_t;
-      st->bb64 = bb_t;
-      if (bb_t == 0.0 || rr_t <= st->target2 * bb_t) {
+    } else {
+      const float tot = reduce_partials<float>(d.part, d.n_tiles, reinterpret_cast<float*>(s_a));
+      if (t == 0) {
+        // pcg_update_scalars.wgsl:59-62, 70-71
+        const float alpha = tot != 0.0f ? st->rz_old / tot : 0.0f;
+        st->p_ap = tot;
+        st->alpha = alpha;
+        st->neg_alpha = -alpha;
+      }
+    }
+  } else if (phase == kPhIrResid) {
+    const double rr = reduce_partials<double>(d.part64, d.n_tiles, s_a);
+    const double bb = reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_b);
+    if (t == 0) {
+      st->rr64 = rr;
+      st->bb64 = bb;
+      if (bb == 0.0 || rr <= st->target2 * bb) {
         st->status = 0;
       } else if (st->outer >= st->max_outer) {
         st->status = 5;  // HBJ_E_NOT_CONVERGED
       } else {
-        const double nr = sqrt(rr_t);
+        const double nr = sqrt(rr);
         st->scale = 1.0 / nr;
         st->inv_scale = nr;
         st->outer += 1u;
@@ -609,18 +491,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
   __shared__ uint32_t s_blk[2];
-  __shared__ int s_flag;
-  __shared__ int s_bad;
   TileRef tr;
   if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
   const SysDesc& d = tr.d;
-  SysState* st = d.st;
-  const int32_t status = st->status;
+  const SysState* st = d.st;
+  if (st->status != kRunning) return;
   const int32_t phase = st->phase;
-  const uint32_t tag = st->seq;
-  if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
-  if (threadIdx.x == 0) s_bad = 0;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
   const float* __restrict__ r_in = st->r_in;
@@ -662,47 +539,52 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
     d.z[row] = zi;
   }
   // r.r (compute.rs:406-413) and r.z (:419-426)
-  const bool dot64 = (d.flags & kFlagDot64) != 0;
-  if (dot64) {
+  if (d.flags & kFlagDot64) {
     double a = own ? (double)rn * (double)rn : 0.0;
     double b = own ? (double)rn * (double)zi : 0.0;
     tree256x2<double>(a, b, s_ra, s_rb);
     if (t == 0) {
-      Tagged<double>::put(d.part64 + 2u * d.n_tiles, tr.tile, tag, a);
-      Tagged<double>::put(d.part64 + 4u * d.n_tiles, tr.tile, tag, b);
+      d.part64[d.n_tiles + tr.tile] = a;
+      d.part64[2u * d.n_tiles + tr.tile] = b;
     }
   } else {
     float a = own ? rn * rn : 0.0f;
     float b = own ? rn * zi : 0.0f;
     tree256x2<float>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
     if (t == 0) {
-      Tagged<float>::put(d.part + d.n_tiles, tr.tile, tag, a);
-      Tagged<float>::put(d.part + 2u * d.n_tiles, tr.tile, tag, b);
+      d.part[d.n_tiles + tr.tile] = a;
+      d.part[2u * d.n_tiles + tr.tile] = b;
     }
   }
-  if (!last_tile(&st->cnt2, d.n_tiles, &s_flag)) return;
+}
+
+// After k_pcg_update: r.z (and r.r) -> beta, the breakdown checks and the
+// stop test in the reference's order, the restart bookkeeping of iterative
+// refinement, and the swap of the double-buffered r.
+__global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysDesc* __restrict__ descs,
+                                                       uint32_t n_sys) {
+  __shared__ double s_a[256];
+  __shared__ double s_b[256];
+  if (blockIdx.x >= n_sys) return;
+  const SysDesc d = descs[blockIdx.x];
+  SysState* st = d.st;
+  if (!st || st->status != kRunning) return;  // empty serving slot / finished system
+  const int32_t phase = st->phase;
+  if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
   float rr, rz;
   double rz64 = 0.0;
-  if (dot64) {
-    const double a = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, tag, s_ra, &s_bad);
-    rz64 = reduce_partials<double>(d.part64 + 4u * d.n_tiles, d.n_tiles, tag, s_rb, &s_bad);
-    rr = (float)a;
+  if (d.flags & kFlagDot64) {
+    rr = (float)reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_a);
+    rz64 = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, s_b);
     rz = (float)rz64;
   } else {
-    rr = reduce_partials<float>(d.part + d.n_tiles, d.n_tiles, tag, reinterpret_cast<float*>(s_ra), &s_bad);
-    rz = reduce_partials<float>(d.part + 2u * d.n_tiles, d.n_tiles, tag, reinterpret_cast<float*>(s_rb),
-                                &s_bad);
+    rr = reduce_partials<float>(d.part + d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_a));
+    rz = reduce_partials<float>(d.part + 2u * d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_b));
   }
-  __syncthreads();
-  if (t != 0) return;
-  reset_counter(&st->cnt2);
-  st->seq = tag + 1u;
-  if (s_bad) {
-    st->status = 6;  // HBJ_E_DEVICE: a partial never arrived
-    return;
-  }
-  st->r_in = r_out;  // r of this iteration becomes the input of the next
-  st->r_out = const_cast<float*>(r_in);
+  if (threadIdx.x != 0) return;
+  float* r_in = st->r_in;
+  st->r_in = st->r_out;  // r of this iteration becomes the input of the next
+  st->r_out = r_in;
   if (phase == kPhInit) {
     st->rz_old = rz;  // compute.rs:316-330
     st->rz64_old = rz64;
@@ -722,7 +604,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
   // pcg_update_scalars.wgsl:64-67 (late update), then compute.rs:468-488
   const float rz_old = st->rz_old, p_ap = st->p_ap;
   float beta = rz_old != 0.0f ? rz / rz_old : 0.0f;
-  if (dot64) beta = st->rz64_old != 0.0 ? (float)(rz64 / st->rz64_old) : 0.0f;
+  if (d.flags & kFlagDot64) beta = st->rz64_old != 0.0 ? (float)(rz64 / st->rz64_old) : 0.0f;
   st->beta = beta;
   st->rz_new = rz;
   st->rr = rr;
@@ -831,24 +713,20 @@ __global__ void k_op_scale(uint32_t n, float beta, float* __restrict__ x) {
   if (i < n) x[i] = x[i] * beta;
 }
 
-__global__ __launch_bounds__(kTile) void k_op_dot(uint32_t n, const float* __restrict__ a,
-                                                  const float* __restrict__ b, uint64_t* part,
-                                                  float* scratch, uint32_t* cnt, float* out) {
+__global__ __launch_bounds__(kTile) void k_op_dot_partials(uint32_t n, const float* __restrict__ a,
+                                                           const float* __restrict__ b, float* part) {
   __shared__ float sh[kTile];
-  __shared__ int s_flag;
-  __shared__ int s_bad;
-  if (threadIdx.x == 0) s_bad = 0;
   const uint32_t i = blockIdx.x * kTile + threadIdx.x;
-  const float v = i < n ? a[i] * b[i] : 0.0f;
+  const float v = i < n ? a[i] * b[i] : 0.0f;  // dot_partials.wgsl:48-52
   const float p = tree256<float>(v, sh);
-  if (threadIdx.x == 0) Tagged<float>::put(part, blockIdx.x, 1u, p);
-  if (!last_tile(cnt, gridDim.x, &s_flag)) return;
-  const float r = reduce_partials<float>(part, gridDim.x, 1u, sh, &s_bad, scratch);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    *out = s_bad ? __int_as_float(0x7fc00000) : r;  // NaN if a partial never arrived
-    reset_counter(cnt);
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = p;
+}
+
+__global__ __launch_bounds__(kTile) void k_op_dot_reduce(const float* __restrict__ part, uint32_t len,
+                                                         float* scratch, float* out) {
+  __shared__ float sh[kTile];
+  const float r = reduce_partials<float>(part, len, sh, scratch);
+  if (threadIdx.x == 0) *out = r;
 }
 
 __global__ __launch_bounds__(kTile) void k_op_spmv(uint32_t n, const uint32_t* __restrict__ rp,
@@ -903,6 +781,12 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_spmv, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
                      ls.n_tiles, ls.remap);
 }
+void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_reduce1, dim3(ls.n_sys), dim3(kTile), 0, s, ls.descs, ls.n_sys);
+}
+void launch_pcg_reduce2(const LaunchSet& ls, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_reduce2, dim3(ls.n_sys), dim3(kTile), 0, s, ls.descs, ls.n_sys);
+}
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_update, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
                      ls.n_tiles, ls.remap);
@@ -918,10 +802,11 @@ void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream
 void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s) {
   hipLaunchKernelGGL(k_op_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, beta, x);
 }
-void launch_op_dot(uint32_t n, const float* a, const float* b, uint64_t* part, float* scratch,
-                   uint32_t* cnt, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_op_dot, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, n, a, b, part,
-                     scratch, cnt, out);
+void launch_op_dot(uint32_t n, const float* a, const float* b, float* part, float* scratch,
+                   float* out, hipStream_t s) {
+  const uint32_t groups = (n + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_op_dot_partials, dim3(groups), dim3(kTile), 0, s, n, a, b, part);
+  hipLaunchKernelGGL(k_op_dot_reduce, dim3(1), dim3(kTile), 0, s, part, groups, scratch, out);
 }
 void launch_op_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, const float* val,
                     const float* x, float* y, hipStream_t s) {

@@ -39,9 +39,6 @@ void status_message(int32_t st, uint32_t max_iter) {
     case HBJ_E_NOT_CONVERGED:
       set_error("PCG(BlockJacobiGpu): did not converge in %u iterations", max_iter);  // :492
       break;
-    case HBJ_E_DEVICE:
-      set_error("device: a tile's dot-product partial never became visible to the reducing tile");
-      break;
     default:
       break;
   }
@@ -207,26 +204,21 @@ hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, 
   std::lock_guard<std::mutex> g(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   const uint32_t groups = (n + kTile - 1) / kTile;
-  float *da = nullptr, *db = nullptr, *dout = nullptr, *scratch = nullptr;
-  uint64_t* part = nullptr;
-  uint32_t* cnt = nullptr;
+  float *da = nullptr, *db = nullptr, *dout = nullptr, *scratch = nullptr, *part = nullptr;
   hbj_status st = dalloc(&da, n);
   if (st == HBJ_OK) st = dalloc(&db, n);
   if (st == HBJ_OK) st = dalloc(&part, groups);
   if (st == HBJ_OK) st = dalloc(&scratch, (groups + 255) / 256);
   if (st == HBJ_OK) st = dalloc(&dout, 1);
-  if (st == HBJ_OK) st = dalloc(&cnt, 1);
   if (st == HBJ_OK) {
     HIPCHK(hipMemcpy(da, a, 4ull * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(db, b, 4ull * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(cnt, 0, 4));
-    HIPCHK(hipMemset(part, 0, 8ull * groups));
-    launch_op_dot(n, da, db, part, scratch, cnt, dout, ctx->stream);
+    launch_op_dot(n, da, db, part, scratch, dout, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, dout, 4, hipMemcpyDeviceToHost));
   }
-  dfree(da); dfree(db); dfree(part); dfree(scratch); dfree(dout); dfree(cnt);
+  dfree(da); dfree(db); dfree(part); dfree(scratch); dfree(dout);
   return st;
 }
 
@@ -391,7 +383,7 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   ALLOC(ra, n);
   ALLOC(rb, n);
   ALLOC(part, 3ull * nt);
-  ALLOC(part64, 6ull * nt);
+  ALLOC(part64, 3ull * nt);
   ALLOC(resid2, 2ull * nt);
   ALLOC(d_desc1, 1);
   ALLOC(tile_blk, nt);
@@ -420,8 +412,6 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
     up(s->tile_blk, tb.data(), sizeof(uint2) * nt);
   }
   if (e == hipSuccess && n) e = hipMemset(s->z, 0, 4ull * n);  // wgpu zero-initialises buffers
-  if (e == hipSuccess) e = hipMemset(s->part, 0, 8ull * 3 * nt);     // partial tags start at 0
-  if (e == hipSuccess) e = hipMemset(s->part64, 0, 8ull * 6 * nt);
   if (e == hipSuccess && n) e = hipMemset(s->x0, 0, 4ull * n);
   if (e == hipSuccess && n) e = hipMemset(s->b, 0, 4ull * n);
   if (e == hipSuccess) {
@@ -474,7 +464,7 @@ hbj_status hbj_system_set_kernel_timing(hbj_system* s, uint32_t every) {
 
 hbj_status hbj_system_kernel_timing(hbj_system* s, double* avg_us, uint64_t* counts) {
   if (!s || !avg_us || !counts) { set_error("hbj_system_kernel_timing: bad argument"); return HBJ_E_INVALID; }
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < Engine::kKernels; ++k) {
     counts[k] = s->eng_ready ? s->eng.t_cnt[k] : 0;
     avg_us[k] = counts[k] ? s->eng.t_sum[k] / (double)counts[k] : 0.0;
     if (s->eng_ready) {
@@ -504,7 +494,6 @@ hbj_status hbj_system_solve_resident(hbj_system* s, uint32_t max_iter, float rel
   SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
-  s->seq_next = fin.seq;
   fill_result(fin, false, ms, res);
   if (fin.status == HBJ_OK && res) {
     if (fin.b_norm2 == 0.0f) res->rel_res_true_fp64 = 0.0;
@@ -574,7 +563,6 @@ static hbj_status solve_ir_resident(hbj_system* s, const hbj_ir_params* p, hbj_r
   SysState init = s->init_state(true, 0, 0, 0, p), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
-  s->seq_next = fin.seq;
   fill_result(fin, true, ms, res);
   if (fin.status != HBJ_OK) {
     if (fin.status == HBJ_E_NOT_CONVERGED)
@@ -688,7 +676,6 @@ hbj_status hbj_batch_solve(hbj_batch* b, uint32_t max_iter, float rel_tol, float
   CHK(b->eng.run(init.data(), fin.data(), &ms));
   hbj_status worst = HBJ_OK;
   for (uint32_t i = 0; i < S; ++i) {
-    b->sys[i]->seq_next = fin[i].seq;
     fill_result(fin[i], false, ms, &results[i]);
     hbj_system* s = b->sys[i];
     if (fin[i].status == HBJ_OK && s->n) {
@@ -728,7 +715,6 @@ hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* 
   CHK(b->eng.run(init.data(), fin.data(), &ms));
   hbj_status worst = HBJ_OK;
   for (uint32_t i = 0; i < S; ++i) {
-    b->sys[i]->seq_next = fin[i].seq;
     fill_result(fin[i], true, ms, &results[i]);
     if (fin[i].status != HBJ_OK && worst == HBJ_OK) {
       worst = (hbj_status)fin[i].status;

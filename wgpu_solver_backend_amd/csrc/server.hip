@@ -255,7 +255,7 @@ uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
   auto c = [](uint64_t k) { return k ? k : 1; };
   const uint64_t nt = c((n + kTile - 1) / kTile), nb = n_starts ? n_starts - 1 : 0;
   return 4 * c(n + 1) + 8 * c(nnz) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
-         8 * 3 * nt + 8 * 6 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n);
+         4 * 3 * nt + 8 * 3 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n);
 }
 
 }  // namespace
@@ -277,11 +277,12 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   if (s == HBJ_OK) {
     S->eng.use_graph = false;  // the table changes between chunks
     S->eng.n_tiles = 0;
-    e = hipMemset(S->eng.d_states, 0, sizeof(SysState) * cfg->max_active);
-    if (e != hipSuccess) {
-      set_error("hipMemset failed: %s", hipGetErrorString(e));
-      s = HBJ_E_DEVICE;
-    }
+    // every slot starts idle: a descriptor whose state says "finished", so
+    // the per-system kernels (one workgroup per slot) skip it
+    SysState idle;
+    std::memset(&idle, 0, sizeof idle);
+    for (uint32_t i = 0; i < cfg->max_active && s == HBJ_OK; ++i)
+      s = S->eng.write_slot(i, SysDesc{}, idle);
   }
   if (s != HBJ_OK) {
     S->eng.destroy();

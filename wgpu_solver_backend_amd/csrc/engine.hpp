@@ -133,6 +133,7 @@ struct Engine {
     if (chunk < 1) chunk = 1;
     use_graph = env_u32("HBJ_GRAPH", 1) != 0;
     remap = env_u32("HBJ_XCD_REMAP", 1);
+    spmv_variant = env_u32("HBJ_SPMV_VARIANT", kSpmvDefault) % kSpmvVariantCount;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -154,6 +155,8 @@ struct Engine {
     HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
     n_sys = (uint32_t)descs.size();
     n_tiles = (uint32_t)tiles.size();
+    one_mode = n_sys == 1 && env_u32("HBJ_ONE", 1) != 0;
+    if (one_mode) h_one = descs[0];
     full_tiles = tiles;
     table_full = true;
     drop_graph();
@@ -208,7 +211,14 @@ struct Engine {
     gchunk = 0;
   }
 
-  LaunchSet ls() const { return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys}; }
+  uint32_t spmv_variant = kSpmvDefault;
+  // A table of one system launches the single-system kernels, which take the
+  // descriptor as a kernel argument instead of loading it.
+  bool one_mode = false;
+  SysDesc h_one{};
+  LaunchSet ls() const {
+    return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys, spmv_variant, one_mode ? &h_one : nullptr};
+  }
 
   // one PCG iteration of every system in the table: three tile kernels, each
   // streaming kernel followed by its per-system reduction

@@ -17,7 +17,11 @@ struct LaunchSet {
   uint32_t n_tiles;
   uint32_t remap;
   uint32_t n_sys;
+  uint32_t spmv_variant;  // streaming variant of the SpMV kernel (bit-identical results)
+  const SysDesc* one;     // host copy of the only system's descriptor, or null (table)
 };
+constexpr uint32_t kSpmvVariantCount = 8;
+constexpr uint32_t kSpmvDefault = 6;  // 16-byte loads, pipelined gather, 512 entries per pass
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s);

@@ -112,11 +112,11 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
 
 // One level of dot_scalar_exec.rs:204-237: dst[g] = tree of chunk g (256
 // values, zero padded) for g < ceil(len/256).  Chunk g is reduced by wave
-// g % 4; each wave issues the loads of two of its chunks before reducing
+// g % 4; each wave issues the loads of four of its chunks before reducing
 // them.
 template <typename T>
 __device__ __forceinline__ void level_pass(const T* __restrict__ src, uint32_t len, T* dst) {
-  constexpr int B = 2;
+  constexpr int B = 4;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t out_len = (len + 255u) / 256u;
   for (uint32_t g0 = wave; g0 < out_len; g0 += 4u * B) {
@@ -138,6 +138,52 @@ __device__ __forceinline__ void level_pass(const T* __restrict__ src, uint32_t l
       if (lane == 0) dst[g] = r;
     }
   }
+}
+
+// The first level of reduce_partials for 1 < len <= kFetchMax partials in
+// two steps, so that a kernel can issue the loads (level_fetch) before it
+// reads anything else and reduce them afterwards (reduce_fetched).  Same
+// chunks and trees as level_pass; the result is reduce_partials' result.
+constexpr uint32_t kFetchMax = 4096;
+template <typename T>
+__device__ __forceinline__ void level_fetch(const T* __restrict__ src, uint32_t len, T (&v)[4][4]) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = (wave + 4u * m) * 256u + lane + 64u * j;
+      v[m][j] = i < len ? src[i] : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T reduce_fetched(const T (&v)[4][4], uint32_t len, T* lds) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t out_len = (len + 255u) / 256u;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const uint32_t g = wave + 4u * m;
+    if (g < out_len) {
+      const T r = wave_tree256(v[m][0], v[m][1], v[m][2], v[m][3]);
+      if (lane == 0) lds[g] = r;
+    }
+  }
+  __syncthreads();
+  if (out_len > 1u) {
+    if (wave == 0) {
+      T w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t i = lane + 64u * j;
+        w[j] = i < out_len ? lds[i] : T(0);
+      }
+      const T r = wave_tree256(w[0], w[1], w[2], w[3]);
+      if (lane == 0) lds[0] = r;
+    }
+    __syncthreads();
+  }
+  return lds[0];
 }
 
 // dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
@@ -184,21 +230,56 @@ struct TileRef {
   uint32_t tile, row0, nrows;
 };
 
-__device__ __forceinline__ bool load_tile(const SysDesc* __restrict__ descs,
-                                          const uint2* __restrict__ tiles, uint32_t n_tiles,
-                                          uint32_t remap, TileRef& tr) {
-  const uint32_t b = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  if (b >= n_tiles) return false;
-  const uint2 td = tiles[b];
-  tr.d = descs[td.x];
-  tr.tile = td.y;
-  tr.row0 = td.y * (uint32_t)kTile;
+// Arguments of the tile kernels.  A single system travels by value in `one`
+// (kOne kernels: tile = workgroup, no table or descriptor load before the
+// first data load); otherwise `tiles` maps workgroups to (system, tile) and
+// `descs` holds the systems.
+struct TileArgs {
+  SysDesc one;
+  const SysDesc* descs;
+  const uint2* tiles;
+  uint32_t n_tiles, remap;
+};
+
+// Arguments of the per-system reduce kernels (one workgroup per system).
+struct SysArgs {
+  SysDesc one;
+  const SysDesc* descs;
+  uint32_t n_sys;
+};
+
+template <bool kOne>
+__device__ __forceinline__ bool get_tile(const TileArgs& a, TileRef& tr) {
+  const uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (b >= a.n_tiles) return false;
+  if constexpr (kOne) {
+    tr.d = a.one;
+    tr.tile = b;
+  } else {
+    const uint2 td = a.tiles[b];
+    tr.d = a.descs[td.x];
+    tr.tile = td.y;
+  }
+  tr.row0 = tr.tile * (uint32_t)kTile;
   const uint32_t left = tr.d.n - tr.row0;
   tr.nrows = left < (uint32_t)kTile ? left : (uint32_t)kTile;
   return true;
 }
 
 // ------------------------------------------------------------ SpMV wave body
+
+// CSR bounds of this thread's row row0 + threadIdx.x of a tile (0, 0 past
+// the tile's last row).  The wave loops below take them as arguments so a
+// kernel can issue these loads before anything else it waits for.
+__device__ __forceinline__ void lane_bounds(const uint32_t* __restrict__ row_ptr, uint32_t row0,
+                                            uint32_t nrows, uint32_t& mb, uint32_t& me) {
+  mb = 0u;
+  me = 0u;
+  if (threadIdx.x < nrows) {
+    mb = row_ptr[row0 + threadIdx.x];
+    me = row_ptr[row0 + threadIdx.x + 1u];
+  }
+}
 
 // spmv.wgsl:52-63 for the 64 rows of this wave inside the tile
 // [row0, row0+nrows): lane l owns row row0 + 64*wave + l and sums its
@@ -208,20 +289,15 @@ __device__ __forceinline__ bool load_tile(const SysDesc* __restrict__ descs,
 // WGSL sum + values[k]*x[j] without contraction) and parked in the wave's
 // LDS slice.  No workgroup barrier: waves run independently.
 template <typename TV, typename TA, int PER>
-__device__ __forceinline__ TA spmv_wave(const uint32_t* __restrict__ row_ptr,
+__device__ __forceinline__ TA spmv_wave(uint32_t mb, uint32_t me,
                                         const uint32_t* __restrict__ col,
                                         const float* __restrict__ val,
-                                        const TV* __restrict__ v, uint32_t row0, uint32_t nrows,
+                                        const TV* __restrict__ v, uint32_t nrows,
                                         TA* s_prod_wave) {
   constexpr uint32_t chunk = 64u * PER;
   const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
   if (wr0 >= nrows) return TA(0);
   const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  uint32_t mb = 0, me = 0;
-  if (lane < wrows) {
-    mb = row_ptr[row0 + wr0 + lane];
-    me = row_ptr[row0 + wr0 + lane + 1u];
-  }
   const uint32_t wb = __shfl(mb, 0, 64);
   const uint32_t we = __shfl(me, (int)wrows - 1, 64);
   uint32_t c[PER];
@@ -262,6 +338,229 @@ __device__ __forceinline__ TA spmv_wave(const uint32_t* __restrict__ row_ptr,
   return sum;
 }
 
+// The same wave loop with 16-byte loads: each lane streams K groups of four
+// consecutive (col, value) entries per pass (chunk = 256*K entries).  The
+// pass starts at the wave's first entry rounded down to a multiple of four;
+// entries outside [wb, we) are loaded (the CSR arrays carry four entries of
+// padding) but never gathered or summed.  Products land in LDS in entry
+// order, so the owner lanes sum exactly as in spmv_wave.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+template <typename TV, typename TA, int K>
+__device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
+                                            const uint32_t* __restrict__ col,
+                                            const float* __restrict__ val,
+                                            const TV* __restrict__ v, uint32_t nrows,
+                                            TA* s_prod_wave) {
+  constexpr uint32_t chunk = 256u * K;
+  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
+  if (wr0 >= nrows) return TA(0);
+  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
+  const uint32_t wb = __shfl(mb, 0, 64);
+  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
+  const uint32_t cs0 = wb & ~3u;
+  u32x4 c[K];
+  f32x4v a[K];
+  auto load = [&](uint32_t cs) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane);
+      if (e < we) {
+        c[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + e));
+        a[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(val + e));
+      } else {
+        c[k] = u32x4{0u, 0u, 0u, 0u};
+        a[k] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+    }
+  };
+  load(cs0);
+  TA sum = TA(0);
+  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
+    TA pr[K][4];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane) + (uint32_t)j;
+        pr[k][j] = (e >= wb && e < ce) ? (TA)a[k][j] * (TA)v[c[k][j]] : TA(0);
+      }
+    }
+    load(cs + chunk);  // prefetch the next pass
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s_prod_wave[4u * ((uint32_t)k * 64u + lane) + (uint32_t)j] = pr[k][j];
+    wave_sync();
+    const uint32_t lo = mb > cs ? mb : cs;
+    const uint32_t hi = me < ce ? me : ce;
+#pragma unroll 4
+    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+    wave_sync();
+  }
+  return sum;
+}
+
+// Row-walk form: the wave stages (col, value) pairs of 256*K consecutive
+// entries in LDS with 16-byte loads, then every lane walks its own row's
+// entries, gathering v[col] itself.  Lanes of one gather instruction are
+// then neighbouring rows at the same position in their rows; for node-block
+// matrices (several rows with one column pattern) they read the same or
+// adjacent v entries, so an instruction touches few cache lines.  Sums are
+// the same sequential per-row sums as in spmv_wave.
+template <typename TV, typename TA, int K>
+__device__ __forceinline__ TA spmv_wave_walk(uint32_t mb, uint32_t me,
+                                             const uint32_t* __restrict__ col,
+                                             const float* __restrict__ val,
+                                             const TV* __restrict__ v,
+                                             uint32_t nrows, uint2* s_pair) {
+  constexpr uint32_t chunk = 256u * K;
+  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
+  if (wr0 >= nrows) return TA(0);
+  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
+  const uint32_t wb = __shfl(mb, 0, 64);
+  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
+  const uint32_t cs0 = wb & ~3u;
+  u32x4 c[K];
+  u32x4 a[K];
+  auto load = [&](uint32_t cs) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane);
+      if (e < we) {
+        c[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + e));
+        a[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(val + e));
+      } else {
+        c[k] = u32x4{0u, 0u, 0u, 0u};
+        a[k] = u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+  };
+  load(cs0);
+  TA sum = TA(0);
+  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      uint4* dst = reinterpret_cast<uint4*>(s_pair + 4u * ((uint32_t)k * 64u + lane));
+      dst[0] = uint4{c[k][0], a[k][0], c[k][1], a[k][1]};
+      dst[1] = uint4{c[k][2], a[k][2], c[k][3], a[k][3]};
+    }
+    load(cs + chunk);  // prefetch the next pass
+    wave_sync();
+    const uint32_t lo = mb > cs ? mb : cs;
+    const uint32_t hi = me < ce ? me : ce;
+#pragma unroll 8
+    for (uint32_t e = lo; e < hi; ++e) {
+      const uint2 pr = s_pair[e - cs];
+      sum = sum + (TA)__uint_as_float(pr.y) * (TA)v[pr.x];
+    }
+    wave_sync();
+  }
+  return sum;
+}
+
+// spmv_wave_vec with the gather software-pipelined: the gather of pass i+1
+// and the column/value loads of pass i+2 are in flight while pass i is
+// summed, so the dependent load chain (columns, then v[col]) is not waited
+// on once per pass.  Same products and sums as spmv_wave.
+template <typename TV, typename TA, int K>
+__device__ __forceinline__ TA spmv_wave_pipe(uint32_t mb, uint32_t me,
+                                             const uint32_t* __restrict__ col,
+                                             const float* __restrict__ val,
+                                             const TV* __restrict__ v,
+                                             uint32_t nrows, TA* s_prod_wave) {
+  constexpr uint32_t chunk = 256u * K;
+  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
+  if (wr0 >= nrows) return TA(0);
+  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
+  const uint32_t wb = __shfl(mb, 0, 64);
+  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
+  const uint32_t cs0 = wb & ~3u;
+  u32x4 c[K];   // columns of the pass in flight
+  f32x4v a[K];  // values of the pass in flight
+  f32x4v ac[K]; // values of the pass being multiplied
+  TV g[K][4];   // v[col] of the pass being multiplied
+  auto load = [&](uint32_t cs) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane);
+      if (e < we) {
+        c[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + e));
+        a[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(val + e));
+      } else {
+        c[k] = u32x4{0u, 0u, 0u, 0u};
+        a[k] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+    }
+  };
+  auto gather = [&](uint32_t cs) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane) + (uint32_t)j;
+        g[k][j] = e < we ? v[c[k][j]] : TV(0);
+      }
+      ac[k] = a[k];
+    }
+  };
+  load(cs0);
+  gather(cs0);
+  load(cs0 + chunk);
+  TA sum = TA(0);
+  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        s_prod_wave[4u * ((uint32_t)k * 64u + lane) + (uint32_t)j] = (TA)ac[k][j] * (TA)g[k][j];
+    gather(cs + chunk);
+    load(cs + 2u * chunk);
+    wave_sync();
+    const uint32_t lo = mb > cs ? mb : cs;
+    const uint32_t hi = me < ce ? me : ce;
+#pragma unroll 4
+    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+    wave_sync();
+  }
+  return sum;
+}
+
+// Streaming variants of the FP32 SpMV wave loop (all bit-identical; the
+// autotuner picks one per system): 0 = dword loads, 512 entries per pass;
+// 1 = dword, 1024; 2 = 16-byte loads, 512; 3 = 16-byte loads, 1024;
+// 4 = row walk, 512; 5 = row walk, 1024; 6 = 16-byte loads with the gather
+// pipelined, 512; 7 = the same, 1024.  kSlice = LDS floats per wave.
+template <int V> struct SpmvCfg;
+template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
+template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
+template <> struct SpmvCfg<2> { static constexpr int kSlice = 512; };
+template <> struct SpmvCfg<3> { static constexpr int kSlice = 1024; };
+template <> struct SpmvCfg<4> { static constexpr int kSlice = 1024; };
+template <> struct SpmvCfg<5> { static constexpr int kSlice = 2048; };
+template <> struct SpmvCfg<6> { static constexpr int kSlice = 512; };
+template <> struct SpmvCfg<7> { static constexpr int kSlice = 1024; };
+
+template <int V>
+__device__ __forceinline__ float spmv_rows(uint32_t mb, uint32_t me, const uint32_t* col,
+                                           const float* val, const float* v, uint32_t nrows,
+                                           float* slice) {
+  if constexpr (V == 0) return spmv_wave<float, float, 8>(mb, me, col, val, v, nrows, slice);
+  if constexpr (V == 1) return spmv_wave<float, float, 16>(mb, me, col, val, v, nrows, slice);
+  if constexpr (V == 2) return spmv_wave_vec<float, float, 2>(mb, me, col, val, v, nrows, slice);
+  if constexpr (V == 3) return spmv_wave_vec<float, float, 4>(mb, me, col, val, v, nrows, slice);
+  if constexpr (V == 4)
+    return spmv_wave_walk<float, float, 2>(mb, me, col, val, v, nrows, reinterpret_cast<uint2*>(slice));
+  if constexpr (V == 5)
+    return spmv_wave_walk<float, float, 4>(mb, me, col, val, v, nrows, reinterpret_cast<uint2*>(slice));
+  if constexpr (V == 6) return spmv_wave_pipe<float, float, 2>(mb, me, col, val, v, nrows, slice);
+  if constexpr (V == 7) return spmv_wave_pipe<float, float, 4>(mb, me, col, val, v, nrows, slice);
+}
+
 // --------------------------------------------------- block-Jacobi tile body
 
 // z for the rows [row0, row0+nrows) of one tile (block_jacobi.wgsl:63-127).
@@ -272,48 +571,64 @@ __device__ __forceinline__ TA spmv_wave(const uint32_t* __restrict__ row_ptr,
 // the r entry the solve reads; rows inside the tile come from s_r, rows of
 // straddling blocks outside it are recomputed by the same formula
 // (bit-identical to the owner's value).
+// Block range of a tile: from the system's per-tile table, or by binary
+// search over block starts when there is none (thread 0, shared via s_blk).
+__device__ __forceinline__ uint2 tile_blocks(const uint32_t* __restrict__ bs,
+                                             const uint2* __restrict__ tile_blk, uint32_t tile,
+                                             uint32_t nb, uint32_t row0, uint32_t nrows,
+                                             uint32_t* s_blk) {
+  if (tile_blk) return tile_blk[tile];
+  if (threadIdx.x == 0) {
+    // largest b with bs[b] <= row: first and last block touching the tile
+    uint32_t lo = 0, hi = nb;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bs[mid] <= row0) lo = mid; else hi = mid;
+    }
+    const uint32_t bf = lo;
+    const uint32_t last_row = row0 + nrows - 1u;
+    hi = nb;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bs[mid] <= last_row) lo = mid; else hi = mid;
+    }
+    s_blk[0] = bf;
+    s_blk[1] = lo - bf + 1u;
+  }
+  __syncthreads();
+  return make_uint2(s_blk[0], s_blk[1]);
+}
+
+// Copies the factors of blocks [bfirst+g, bfirst+g+cnt) into s_lu (stride 37
+// words per block to spread LDS banks).  No barrier.
+__device__ __forceinline__ void bj_stage(const float* __restrict__ lu, uint32_t bfirst,
+                                         uint32_t g, uint32_t cnt, float* s_lu) {
+  const float* src = lu + (size_t)(bfirst + g) * kLuStride;
+  for (uint32_t i = threadIdx.x; i < cnt * kLuStride; i += kTile) {
+    const uint32_t jb = i / kLuStride, k = i - jb * kLuStride;
+    s_lu[jb * 37u + k] = src[i];
+  }
+}
+
+// z for the rows [row0, row0+nrows) of one tile (block_jacobi.wgsl:63-127).
+// Thread j solves block bfirst+j, i.e. every block whose rows meet the tile;
+// factors are staged through LDS in groups of kLuStage blocks, the first
+// group by the caller (bj_stage(..., 0, ...) before the barrier that
+// publishes s_r).  rnew(row) gives the r entry the solve reads; rows inside
+// the tile come from s_r, rows of straddling blocks outside it are
+// recomputed by the same formula (bit-identical to the owner's value).
 template <typename RNew>
 __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const float* __restrict__ lu,
-                                        const uint2* __restrict__ tile_blk, uint32_t tile,
-                                        uint32_t n, uint32_t nb, uint32_t row0, uint32_t nrows,
-                                        const float* s_r, float* s_z, float* s_lu,
-                                        uint32_t* s_blk, RNew rnew) {
+                                        uint2 blk, uint32_t n, uint32_t row0, uint32_t nrows,
+                                        const float* s_r, float* s_z, float* s_lu, RNew rnew) {
   const uint32_t t = threadIdx.x;
-  uint32_t bfirst, nblk;
-  if (tile_blk) {
-    const uint2 tb = tile_blk[tile];
-    bfirst = tb.x;
-    nblk = tb.y;
-  } else {
-    if (t == 0) {
-      // largest b with bs[b] <= row: first and last block touching the tile
-      uint32_t lo = 0, hi = nb;
-      while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (bs[mid] <= row0) lo = mid; else hi = mid;
-      }
-      const uint32_t bf = lo;
-      const uint32_t last_row = row0 + nrows - 1u;
-      hi = nb;
-      while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (bs[mid] <= last_row) lo = mid; else hi = mid;
-      }
-      s_blk[0] = bf;
-      s_blk[1] = lo - bf + 1u;
-    }
-    __syncthreads();
-    bfirst = s_blk[0];
-    nblk = s_blk[1];
-  }
+  const uint32_t bfirst = blk.x, nblk = blk.y;
   for (uint32_t g = 0; g < nblk; g += kLuStage) {
     const uint32_t cnt = (nblk - g) < (uint32_t)kLuStage ? nblk - g : (uint32_t)kLuStage;
-    const float* src = lu + (size_t)(bfirst + g) * kLuStride;
-    for (uint32_t i = t; i < cnt * kLuStride; i += kTile) {
-      const uint32_t jb = i / kLuStride, k = i - jb * kLuStride;
-      s_lu[jb * 37u + k] = src[i];
+    if (g > 0) {
+      bj_stage(lu, bfirst, g, cnt, s_lu);
+      __syncthreads();
     }
-    __syncthreads();
     const uint32_t j = t - g;
     if (t >= g && j < cnt) {
       const uint32_t b = bfirst + g + j;
@@ -363,28 +678,37 @@ __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const f
 // kernels read them after the kernel boundary, so no workgroup ever waits on
 // or synchronises with another one.  SysState is written only by the reduce
 // kernels and read by the kernels after them.
+//
+// Every kernel issues the loads that do not depend on the system's state
+// (row bounds, own-row vector entries, factors, partials) before it reads
+// SysState, so a workgroup waits for one memory round trip where the state
+// and its data would otherwise be two or three dependent ones.
 
-__global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ descs,
-                                                    const uint2* __restrict__ tiles,
-                                                    uint32_t n_tiles, uint32_t remap) {
-  // per-wave product slices: 64*kPer floats, or 64*kPer/2 doubles
-  __shared__ double s_prod[kWaves * 64 * kPer / 2];
-  __shared__ double s_ra[kTile];
-  __shared__ double s_rb[kTile];
+template <int V, bool kOne>
+__global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
+  // per-wave product slices: SpmvCfg<V>::kSlice floats, or 64*kPer/2 doubles
+  constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
+  static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
+  __shared__ __align__(16) double s_prod[kWaves * kSliceF / 2];
+  double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
+  double* s_rb = s_prod + kTile;
   TileRef tr;
-  if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
+  if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
   const SysState* st = d.st;
-  if (st->status != kRunning) return;
-  const int32_t phase = st->phase;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
+  uint32_t mb, me;
+  lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
+  const int32_t status = st->status, phase = st->phase;
+  if (status != kRunning) return;
 
   if (phase == kPhInit || phase == kPhIter) {
     const bool init = phase == kPhInit;
     const float* v = init ? d.x : d.p;
-    float* slice = reinterpret_cast<float*>(s_prod) + wave * 64 * kPer;
-    const float q = spmv_wave<float, float, kPer>(d.row_ptr, d.col, d.val, v, tr.row0, tr.nrows, slice);
+    float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
+    const float q = spmv_rows<V>(mb, me, d.col, d.val, v, tr.nrows, slice);
+    __syncthreads();  // all waves are done with their slices
     // dot(b,b) at init (compute.rs:235-242), dot(p,Ap) per iteration (:353-360)
     float w = 0.0f;
     double w64 = 0.0;
@@ -404,9 +728,10 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
     }
   } else if (phase == kPhIrResid) {
     // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
-    double* slice = s_prod + wave * 64 * (kPer / 2);
-    const double ax = spmv_wave<double, double, kPer / 2>(d.row_ptr, d.col, d.val, d.x64, tr.row0,
-                                                          tr.nrows, slice);
+    double* slice = s_prod + wave * (kSliceF / 2);
+    const double ax = spmv_wave<double, double, kPer / 2>(mb, me, d.col, d.val, d.x64, tr.nrows,
+                                                          slice);
+    __syncthreads();
     double rr = 0.0, bb = 0.0;
     if (own) {
       const double bi = d.b64[row];
@@ -424,46 +749,60 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const SysDesc* __restrict__ 
 }
 
 // After k_pcg_spmv: ||b||^2 (init), p.Ap and alpha (iteration), or the FP64
-// residual decision (refinement).  One workgroup per system.
-__global__ __launch_bounds__(kTile) void k_pcg_reduce1(const SysDesc* __restrict__ descs,
-                                                       uint32_t n_sys) {
+// residual decision (refinement).  One workgroup per system.  The p.Ap
+// partials of an iteration are fetched together with the state.
+template <bool kOne>
+__global__ __launch_bounds__(kTile) void k_pcg_reduce1(const SysArgs sa) {
   __shared__ double s_a[256];
   __shared__ double s_b[256];
-  if (blockIdx.x >= n_sys) return;
-  const SysDesc d = descs[blockIdx.x];
+  if (blockIdx.x >= sa.n_sys) return;
+  const SysDesc d = kOne ? sa.one : sa.descs[blockIdx.x];
   SysState* st = d.st;
-  if (!st || st->status != kRunning) return;  // empty serving slot / finished system
-  const int32_t phase = st->phase;
-  const uint32_t t = threadIdx.x;
+  if (!st) return;  // empty serving slot
+  const uint32_t t = threadIdx.x, nt = d.n_tiles;
+  const bool dot64 = (d.flags & kFlagDot64) != 0;
+  const bool pre = nt > 1u && nt <= kFetchMax;
+  float vf[4][4];
+  double vd[4][4];
+  if (pre) {
+    if (dot64) level_fetch<double>(d.part64, nt, vd);
+    else level_fetch<float>(d.part, nt, vf);
+  }
+  const int32_t status = st->status, phase = st->phase;
+  const float rz_old = st->rz_old;
+  const double rz64_old = st->rz64_old;
+  if (status != kRunning) return;  // finished system
   if (phase == kPhInit) {
-    const float tot = reduce_partials<float>(d.part, d.n_tiles, reinterpret_cast<float*>(s_a));
+    const float tot = reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_a));
     if (t == 0) {
       st->b_norm2 = tot;
       if (tot == 0.0f) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
     }
   } else if (phase == kPhIter) {
-    if (d.flags & kFlagDot64) {
-      const double tot = reduce_partials<double>(d.part64, d.n_tiles, s_a);
+    if (dot64) {
+      const double tot = pre ? reduce_fetched<double>(vd, nt, s_a)
+                             : reduce_partials<double>(d.part64, nt, s_a);
       if (t == 0) {
-        const float alpha = tot != 0.0 ? (float)(st->rz64_old / tot) : 0.0f;
+        const float alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
         st->p_ap = (float)tot;
         st->p_ap64 = tot;
         st->alpha = alpha;
         st->neg_alpha = -alpha;
       }
     } else {
-      const float tot = reduce_partials<float>(d.part, d.n_tiles, reinterpret_cast<float*>(s_a));
+      const float tot = pre ? reduce_fetched<float>(vf, nt, reinterpret_cast<float*>(s_a))
+                            : reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_a));
       if (t == 0) {
         // pcg_update_scalars.wgsl:59-62, 70-71
-        const float alpha = tot != 0.0f ? st->rz_old / tot : 0.0f;
+        const float alpha = tot != 0.0f ? rz_old / tot : 0.0f;
         st->p_ap = tot;
         st->alpha = alpha;
         st->neg_alpha = -alpha;
       }
     }
   } else if (phase == kPhIrResid) {
-    const double rr = reduce_partials<double>(d.part64, d.n_tiles, s_a);
-    const double bb = reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_b);
+    const double rr = reduce_partials<double>(d.part64, nt, s_a);
+    const double bb = reduce_partials<double>(d.part64 + nt, nt, s_b);
     if (t == 0) {
       st->rr64 = rr;
       st->bb64 = bb;
@@ -482,9 +821,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce1(const SysDesc* __restrict
   }
 }
 
-__global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict__ descs,
-                                                      const uint2* __restrict__ tiles,
-                                                      uint32_t n_tiles, uint32_t remap) {
+template <bool kOne>
+__global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __shared__ float s_r[kTile];
   __shared__ float s_z[kTile];
   __shared__ float s_lu[kLuStage * 37];
@@ -492,19 +830,28 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
   __shared__ double s_rb[kTile];
   __shared__ uint32_t s_blk[2];
   TileRef tr;
-  if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
+  if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
   const SysState* st = d.st;
-  if (st->status != kRunning) return;
-  const int32_t phase = st->phase;
-  if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
+  // state-independent loads first: own-row x, p, q, z and the tile's blocks
+  float xi = 0.0f, pi = 0.0f, qi = 0.0f, zi_old = 0.0f;
+  if (own) {
+    xi = d.x[row];
+    pi = d.p[row];
+    qi = d.q[row];
+    if (d.flags & kFlagOversize) zi_old = d.z[row];  // rows no block writes keep z
+  }
+  const uint2 blk = d.tile_blk ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
+  const int32_t status = st->status, phase = st->phase;
   const float* __restrict__ r_in = st->r_in;
   float* __restrict__ r_out = st->r_out;
   const float na = st->neg_alpha;
   const float alpha = st->alpha;
   const double scale = st->scale;
+  if (status != kRunning) return;
+  if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
 
   // r entry of this iteration for any row (used for straddling blocks)
   auto rnew = [&](uint32_t i) -> float {
@@ -513,26 +860,33 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
     return (float)(d.r64[i] * scale);                        // IR restart: b32 = r64*scale
   };
 
+  const uint2 tb = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
+                                                  tr.nrows, s_blk);
   float rn = 0.0f;
   if (own) {
     if (phase == kPhIter) {
-      const float xi = d.x[row], pi = d.p[row];
-      d.x[row] = xi + alpha * pi;  // compute.rs:386-394
-      rn = r_in[row] + na * d.q[row];
+      rn = r_in[row] + na * qi;
     } else if (phase == kPhInit) {
-      rn = d.b[row] + (-1.0f) * d.q[row];
+      rn = d.b[row] + (-1.0f) * qi;
     } else {
       rn = (float)(d.r64[row] * scale);
+    }
+  }
+  // the first factor group goes to LDS while r arrives
+  bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+  if (own) {
+    if (phase == kPhIter) {
+      d.x[row] = xi + alpha * pi;  // compute.rs:386-394
+    } else if (phase == kPhIrReinit) {
       d.b[row] = rn;
       d.x[row] = 0.0f;
     }
     r_out[row] = rn;
     s_r[t] = rn;
-    if (d.flags & kFlagOversize) s_z[t] = d.z[row];  // rows no block writes keep z
+    s_z[t] = zi_old;
   }
   __syncthreads();
-  bj_tile(d.bstart, d.lu, d.tile_blk, tr.tile, d.n, d.nb, tr.row0, tr.nrows, s_r, s_z, s_lu, s_blk,
-          rnew);
+  bj_tile(d.bstart, d.lu, tb, d.n, tr.row0, tr.nrows, s_r, s_z, s_lu, rnew);
   float zi = 0.0f;
   if (own) {
     zi = s_z[t];
@@ -560,26 +914,45 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const SysDesc* __restrict_
 
 // After k_pcg_update: r.z (and r.r) -> beta, the breakdown checks and the
 // stop test in the reference's order, the restart bookkeeping of iterative
-// refinement, and the swap of the double-buffered r.
-__global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysDesc* __restrict__ descs,
-                                                       uint32_t n_sys) {
+// refinement, and the swap of the double-buffered r.  The r.r and r.z
+// partials are fetched together with the state.
+template <bool kOne>
+__global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysArgs sa) {
   __shared__ double s_a[256];
   __shared__ double s_b[256];
-  if (blockIdx.x >= n_sys) return;
-  const SysDesc d = descs[blockIdx.x];
+  if (blockIdx.x >= sa.n_sys) return;
+  const SysDesc d = kOne ? sa.one : sa.descs[blockIdx.x];
   SysState* st = d.st;
-  if (!st || st->status != kRunning) return;  // empty serving slot / finished system
-  const int32_t phase = st->phase;
+  if (!st) return;  // empty serving slot
+  const uint32_t nt = d.n_tiles;
+  const bool dot64 = (d.flags & kFlagDot64) != 0;
+  const bool pre = nt > 1u && nt <= kFetchMax;
+  float af[4][4], bf[4][4];
+  double ad[4][4], bd[4][4];
+  if (pre) {
+    if (dot64) {
+      level_fetch<double>(d.part64 + nt, nt, ad);
+      level_fetch<double>(d.part64 + 2u * nt, nt, bd);
+    } else {
+      level_fetch<float>(d.part + nt, nt, af);
+      level_fetch<float>(d.part + 2u * nt, nt, bf);
+    }
+  }
+  const int32_t status = st->status, phase = st->phase;
+  if (status != kRunning) return;  // finished system
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
   float rr, rz;
   double rz64 = 0.0;
-  if (d.flags & kFlagDot64) {
-    rr = (float)reduce_partials<double>(d.part64 + d.n_tiles, d.n_tiles, s_a);
-    rz64 = reduce_partials<double>(d.part64 + 2u * d.n_tiles, d.n_tiles, s_b);
+  if (dot64) {
+    rr = (float)(pre ? reduce_fetched<double>(ad, nt, s_a)
+                     : reduce_partials<double>(d.part64 + nt, nt, s_a));
+    rz64 = pre ? reduce_fetched<double>(bd, nt, s_b) : reduce_partials<double>(d.part64 + 2u * nt, nt, s_b);
     rz = (float)rz64;
   } else {
-    rr = reduce_partials<float>(d.part + d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_a));
-    rz = reduce_partials<float>(d.part + 2u * d.n_tiles, d.n_tiles, reinterpret_cast<float*>(s_b));
+    float* fa = reinterpret_cast<float*>(s_a);
+    float* fb = reinterpret_cast<float*>(s_b);
+    rr = pre ? reduce_fetched<float>(af, nt, fa) : reduce_partials<float>(d.part + nt, nt, fa);
+    rz = pre ? reduce_fetched<float>(bf, nt, fb) : reduce_partials<float>(d.part + 2u * nt, nt, fb);
   }
   if (threadIdx.x != 0) return;
   float* r_in = st->r_in;
@@ -604,7 +977,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysDesc* __restrict
   // pcg_update_scalars.wgsl:64-67 (late update), then compute.rs:468-488
   const float rz_old = st->rz_old, p_ap = st->p_ap;
   float beta = rz_old != 0.0f ? rz / rz_old : 0.0f;
-  if (d.flags & kFlagDot64) beta = st->rz64_old != 0.0 ? (float)(rz64 / st->rz64_old) : 0.0f;
+  if (dot64) beta = st->rz64_old != 0.0 ? (float)(rz64 / st->rz64_old) : 0.0f;
   st->beta = beta;
   st->rz_new = rz;
   st->rr = rr;
@@ -626,24 +999,25 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysDesc* __restrict
   }
 }
 
-__global__ __launch_bounds__(kTile) void k_pcg_pupdate(const SysDesc* __restrict__ descs,
-                                                       const uint2* __restrict__ tiles,
-                                                       uint32_t n_tiles, uint32_t remap) {
+template <bool kOne>
+__global__ __launch_bounds__(kTile) void k_pcg_pupdate(const TileArgs ta) {
   TileRef tr;
-  if (!load_tile(descs, tiles, n_tiles, remap, tr)) return;
+  if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
   const SysState* st = d.st;
-  if (st->status != kRunning) return;
-  const int32_t phase = st->phase;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   if (t >= tr.nrows) return;
+  const float zi = d.z[row], pi = d.p[row];
+  const int32_t status = st->status, phase = st->phase;
+  const uint32_t iter = st->iter;
+  const float beta = st->beta;
+  if (status != kRunning) return;
   if (phase == kPhIter) {
-    if (st->iter == 0u) {
-      d.p[row] = d.z[row];  // p0 = z0 (compute.rs:313-314)
+    if (iter == 0u) {
+      d.p[row] = zi;  // p0 = z0 (compute.rs:313-314)
     } else {
-      const float beta = st->beta;
-      const float pb = d.p[row] * beta;  // scale_from_scalar_results.wgsl:37
-      d.p[row] = pb + d.z[row];          // axpy.wgsl:41 with alpha = 1
+      const float pb = pi * beta;  // scale_from_scalar_results.wgsl:37
+      d.p[row] = pb + zi;          // axpy.wgsl:41 with alpha = 1
     }
   } else if (phase == kPhIrResid) {
     d.x64[row] = d.x64[row] + (double)d.x[row] * st->inv_scale;
@@ -663,7 +1037,9 @@ __global__ __launch_bounds__(kTile) void k_true_resid(const SysDesc* __restrict_
   const uint32_t tile = blockIdx.x;
   const uint32_t row0 = tile * kTile;
   const uint32_t nrows = (d.n - row0) < (uint32_t)kTile ? d.n - row0 : (uint32_t)kTile;
-  const double ax = spmv_wave<float, double, kPer / 2>(d.row_ptr, d.col, d.val, x, row0, nrows,
+  uint32_t mb, me;
+  lane_bounds(d.row_ptr, row0, nrows, mb, me);
+  const double ax = spmv_wave<float, double, kPer / 2>(mb, me, d.col, d.val, x, nrows,
                                                        s_prod + (threadIdx.x >> 6) * 64 * (kPer / 2));
   double rr = 0.0, bb = 0.0;
   if (threadIdx.x < nrows) {
@@ -736,7 +1112,9 @@ __global__ __launch_bounds__(kTile) void k_op_spmv(uint32_t n, const uint32_t* _
   __shared__ float s_prod[kWaves * 64 * kPer];
   const uint32_t row0 = blockIdx.x * kTile;
   const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
-  const float q = spmv_wave<float, float, kPer>(rp, col, val, x, row0, nrows,
+  uint32_t mb, me;
+  lane_bounds(rp, row0, nrows, mb, me);
+  const float q = spmv_wave<float, float, kPer>(mb, me, col, val, x, nrows,
                                                 s_prod + (threadIdx.x >> 6) * 64 * kPer);
   if (threadIdx.x < nrows) y[row0 + threadIdx.x] = q;
 }
@@ -757,9 +1135,10 @@ __global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t 
     s_r[t] = r[row0 + t];
     s_z[t] = z[row0 + t];
   }
+  const uint2 blk = tile_blocks(bs, nullptr, blockIdx.x, nb, row0, nrows, s_blk);
+  bj_stage(lu, blk.x, 0, blk.y < (uint32_t)kLuStage ? blk.y : (uint32_t)kLuStage, s_lu);
   __syncthreads();
-  bj_tile(bs, lu, nullptr, blockIdx.x, n, nb, row0, nrows, s_r, s_z, s_lu, s_blk,
-          [&](uint32_t i) { return r[i]; });
+  bj_tile(bs, lu, blk, n, row0, nrows, s_r, s_z, s_lu, [&](uint32_t i) { return r[i]; });
   if (t < nrows) z[row0 + t] = s_z[t];
 }
 
@@ -777,23 +1156,63 @@ __global__ void k_op_update_scalars(float* s, uint32_t i_pap, uint32_t i_rz_new,
 
 // ------------------------------------------------------- launch wrappers
 
+static TileArgs tile_args(const LaunchSet& ls) {
+  TileArgs a{};
+  if (ls.one) a.one = *ls.one;
+  a.descs = ls.descs;
+  a.tiles = ls.tiles;
+  a.n_tiles = ls.n_tiles;
+  a.remap = ls.remap;
+  return a;
+}
+static SysArgs sys_args(const LaunchSet& ls) {
+  SysArgs a{};
+  if (ls.one) a.one = *ls.one;
+  a.descs = ls.descs;
+  a.n_sys = ls.n_sys;
+  return a;
+}
+
+template <int V>
+static void launch_spmv_v(const LaunchSet& ls, hipStream_t s) {
+  const TileArgs a = tile_args(ls);
+  if (ls.one)
+    hipLaunchKernelGGL((k_pcg_spmv<V, true>), dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_pcg_spmv<V, false>), dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+}
+
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_spmv, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
-                     ls.n_tiles, ls.remap);
+  switch (ls.spmv_variant) {
+    case 1: launch_spmv_v<1>(ls, s); break;
+    case 2: launch_spmv_v<2>(ls, s); break;
+    case 3: launch_spmv_v<3>(ls, s); break;
+    case 4: launch_spmv_v<4>(ls, s); break;
+    case 5: launch_spmv_v<5>(ls, s); break;
+    case 6: launch_spmv_v<6>(ls, s); break;
+    case 7: launch_spmv_v<7>(ls, s); break;
+    default: launch_spmv_v<0>(ls, s); break;
+  }
 }
 void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_reduce1, dim3(ls.n_sys), dim3(kTile), 0, s, ls.descs, ls.n_sys);
+  const SysArgs a = sys_args(ls);
+  if (ls.one) hipLaunchKernelGGL(k_pcg_reduce1<true>, dim3(1), dim3(kTile), 0, s, a);
+  else hipLaunchKernelGGL(k_pcg_reduce1<false>, dim3(ls.n_sys), dim3(kTile), 0, s, a);
 }
 void launch_pcg_reduce2(const LaunchSet& ls, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_reduce2, dim3(ls.n_sys), dim3(kTile), 0, s, ls.descs, ls.n_sys);
+  const SysArgs a = sys_args(ls);
+  if (ls.one) hipLaunchKernelGGL(k_pcg_reduce2<true>, dim3(1), dim3(kTile), 0, s, a);
+  else hipLaunchKernelGGL(k_pcg_reduce2<false>, dim3(ls.n_sys), dim3(kTile), 0, s, a);
 }
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_update, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
-                     ls.n_tiles, ls.remap);
+  const TileArgs a = tile_args(ls);
+  if (ls.one) hipLaunchKernelGGL(k_pcg_update<true>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+  else hipLaunchKernelGGL(k_pcg_update<false>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
 }
 void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(ls.n_tiles), dim3(kTile), 0, s, ls.descs, ls.tiles,
-                     ls.n_tiles, ls.remap);
+  const TileArgs a = tile_args(ls);
+  if (ls.one) hipLaunchKernelGGL(k_pcg_pupdate<true>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+  else hipLaunchKernelGGL(k_pcg_pupdate<false>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
 }
 
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {

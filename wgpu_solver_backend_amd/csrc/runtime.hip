@@ -244,8 +244,8 @@ hbj_status hbj_op_spmv(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* r
   uint32_t *drp = nullptr, *dcol = nullptr;
   float *dval = nullptr, *dx = nullptr, *dy = nullptr;
   hbj_status st = dalloc(&drp, n + 1);
-  if (st == HBJ_OK) st = dalloc(&dcol, nnz);
-  if (st == HBJ_OK) st = dalloc(&dval, nnz);
+  if (st == HBJ_OK) st = dalloc(&dcol, nnz + 4ull);
+  if (st == HBJ_OK) st = dalloc(&dval, nnz + 4ull);
   if (st == HBJ_OK) st = dalloc(&dx, n);
   if (st == HBJ_OK) st = dalloc(&dy, n);
   if (st == HBJ_OK) {
@@ -370,8 +370,8 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
 #define ALLOC(ptr, cnt) \
   if (st == HBJ_OK) st = dalloc(&s->ptr, (cnt), &s->bytes)
   ALLOC(rp, n + 1);
-  ALLOC(col, nnz);
-  ALLOC(val, nnz);
+  ALLOC(col, nnz + 4ull);  // 4 entries of padding for 16-byte loads
+  ALLOC(val, nnz + 4ull);
   ALLOC(bs, n_starts);
   ALLOC(lu, 36ull * (nb ? nb : 1));
   ALLOC(x, n);
@@ -411,6 +411,9 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
     }
     up(s->tile_blk, tb.data(), sizeof(uint2) * nt);
   }
+  // the padding past nnz is read by 16-byte loads: keep it zero (column 0)
+  if (e == hipSuccess) e = hipMemset(s->col + nnz, 0, 16);
+  if (e == hipSuccess) e = hipMemset(s->val + nnz, 0, 16);
   if (e == hipSuccess && n) e = hipMemset(s->z, 0, 4ull * n);  // wgpu zero-initialises buffers
   if (e == hipSuccess && n) e = hipMemset(s->x0, 0, 4ull * n);
   if (e == hipSuccess && n) e = hipMemset(s->b, 0, 4ull * n);

@@ -254,7 +254,7 @@ void worker_main(hbj_server* S) {
 uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
   auto c = [](uint64_t k) { return k ? k : 1; };
   const uint64_t nt = c((n + kTile - 1) / kTile), nb = n_starts ? n_starts - 1 : 0;
-  return 4 * c(n + 1) + 8 * c(nnz) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
+  return 4 * c(n + 1) + 8 * (uint64_t)(nnz + 4) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
          4 * 3 * nt + 8 * 3 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n);
 }
 

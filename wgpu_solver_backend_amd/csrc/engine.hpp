@@ -464,6 +464,9 @@ struct hbj_system {
   float* b32 = nullptr;
   double* h_part2 = nullptr;  // pinned, 2 * n_tiles
   uint2* tile_blk = nullptr;  // per tile: first block touching it, count
+  uint16_t* lidx = nullptr;   // tile-local SpMV tables (build_tile_local)
+  uint2* tloc = nullptr;
+  uint32_t* tcols = nullptr;
   uint64_t bytes = 0;
   Engine eng;
   bool eng_ready = false;
@@ -488,6 +491,9 @@ struct hbj_system {
     d.b64 = b64;
     d.r64 = r64;
     d.tile_blk = tile_blk;
+    d.lidx = lidx;
+    d.tloc = tloc;
+    d.tcols = tcols;
     d.st = nullptr;
     d.n = n;
     d.nb = nb;
@@ -565,6 +571,7 @@ struct hbj_system {
     dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
     dfree(d_desc1);
     dfree(tile_blk);
+    dfree(lidx); dfree(tloc); dfree(tcols);
     hfree(h_part2);
   }
 };

@@ -31,6 +31,11 @@ constexpr int kLuStage = 64;
 constexpr int32_t kRunning = -1;
 // Most tiles one system may have (the first reduction level must fit LDS).
 constexpr uint32_t kMaxTiles = 65536;
+// Tile-local SpMV tables: a tile whose entries reference at most kLocalMax
+// distinct columns gathers from an LDS copy of those x entries; others
+// (tloc.y == kLocalOff) gather from HBM.
+constexpr uint32_t kLocalMax = 2048;
+constexpr uint32_t kLocalOff = 0xFFFFFFFFu;
 
 enum Phase : int32_t {
   kPhInit = 0,      // r0 = b - A x0, z0 = M^-1 r0, p0 = z0, rz_old = r0.z0
@@ -82,6 +87,9 @@ struct SysDesc {
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual
   const uint2* tile_blk;     // per tile: first block touching it, number of such blocks
+  const uint16_t* lidx;      // nnz (+8 zero padding): entry's index in its tile's column list
+  const uint2* tloc;         // per tile: offset of its column list in tcols, length (or kLocalOff)
+  const uint32_t* tcols;     // concatenated sorted distinct column lists of the tiles
   SysState* st;
   uint32_t n, nb, n_tiles, flags;
 };

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "hbj.h"
 #include "hbj_device.hpp"
@@ -20,8 +21,8 @@ struct LaunchSet {
   uint32_t spmv_variant;  // streaming variant of the SpMV kernel (bit-identical results)
   const SysDesc* one;     // host copy of the only system's descriptor, or null (table)
 };
-constexpr uint32_t kSpmvVariantCount = 8;
-constexpr uint32_t kSpmvDefault = 6;  // 16-byte loads, pipelined gather, 512 entries per pass
+constexpr uint32_t kSpmvVariantCount = 10;
+constexpr uint32_t kSpmvDefault = 8;  // tile-local columns, 512 entries per pass
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s);
@@ -42,6 +43,15 @@ void launch_op_block_jacobi(uint32_t n, uint32_t nb, const float* lu, const uint
                             const float* r, float* z, hipStream_t s);
 void launch_op_update_scalars(float* slots, uint32_t a, uint32_t b, uint32_t c, uint32_t d,
                               uint32_t e, uint32_t f, hipStream_t s);
+
+// Tile-local column tables of the SpMV for a validated CSR matrix: per
+// 256-row tile the sorted distinct columns of its entries (tcols from
+// tloc[t].x, tloc[t].y of them; kLocalOff and no list when there are more
+// than kLocalMax) and per entry its index in that list (lidx, nnz + 8 with
+// zero padding; 0 for entries of tiles without a list).
+void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                      std::vector<uint16_t>& lidx, std::vector<uint2>& tloc,
+                      std::vector<uint32_t>& tcols);
 
 // thread-local error text (hbj_last_error)
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));

@@ -281,123 +281,176 @@ __device__ __forceinline__ void lane_bounds(const uint32_t* __restrict__ row_ptr
   }
 }
 
-// spmv.wgsl:52-63 for the 64 rows of this wave inside the tile
-// [row0, row0+nrows): lane l owns row row0 + 64*wave + l and sums its
+// spmv.wgsl:52-63 for the 64 rows of one wave of a tile: lane l owns row
+// row0 + 64*wave + l (bounds [mb, me), from lane_bounds) and sums its
 // products sequentially from 0.0.  The wave streams its rows' CSR entries in
-// coalesced chunks of 64*PER (the next chunk's loads are issued before the
-// current chunk is summed); each product v_k * x[c_k] is rounded once (the
-// WGSL sum + values[k]*x[j] without contraction) and parked in the wave's
-// LDS slice.  No workgroup barrier: waves run independently.
+// coalesced chunks (the next chunk's loads are issued before the current
+// chunk is summed); each product v_k * x[c_k] is rounded once (the WGSL
+// sum + values[k]*x[j] without contraction) and parked in the wave's LDS
+// slice.  No workgroup barrier: waves run independently.
+//
+// All loads here are raw buffer loads: 32-bit offsets (no 64-bit address
+// pairs to assemble per lane) and a hardware range check, so a position past
+// the wave's last entry reads 0 instead of faulting and every load can be
+// unconditional.  Such positions produce products that are never summed.
+// (A load under a branch makes the compiler wait for it where the branch
+// joins, which would serialise the loads the loops keep in flight.)  Byte
+// offsets are 32-bit: vectors hold fewer than 2^30 FP32 or 2^29 FP64
+// entries, which the host checks.
+
+// Raw buffer resource over [p, p + bytes) (gfx9 word 3: 32-bit data format,
+// stride 0); loads at offsets >= bytes return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+constexpr int kNt = 2;  // cache policy of the streaming loads: non-temporal
+
+template <typename T>
+__device__ __forceinline__ T buf_gather(__amdgpu_buffer_rsrc_t r, uint32_t i);
+template <>
+__device__ __forceinline__ float buf_gather<float>(__amdgpu_buffer_rsrc_t r, uint32_t i) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i << 2), 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_gather<double>(__amdgpu_buffer_rsrc_t r, uint32_t i) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i << 3), 0, 0));
+}
+
+// The wave's entry range [wb, we); false when it has no rows or no entries.
+// Read with readlane so the values (and the buffer resources built from
+// them) sit in scalar registers: a resource the compiler cannot prove
+// uniform turns every buffer load into a loop over lanes.  Called with the
+// whole wave active.
+__device__ __forceinline__ bool wave_range(uint32_t mb, uint32_t me, uint32_t nrows, uint32_t& wb,
+                                           uint32_t& we) {
+  const uint32_t wr0 = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+  if (wr0 >= nrows) return false;
+  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
+  wb = __builtin_amdgcn_readfirstlane(mb);  // lane 0: the wave's first row
+  we = __builtin_amdgcn_readlane(me, (int)wrows - 1);
+  return wb < we;
+}
+
+// Owner-lane sums of the entries [cs, ce) parked in the slice.
+template <typename TA>
+__device__ __forceinline__ TA sum_slice(TA sum, const TA* s_prod_wave, uint32_t mb, uint32_t me,
+                                        uint32_t cs, uint32_t ce) {
+  const uint32_t lo = mb > cs ? mb : cs;
+  const uint32_t hi = me < ce ? me : ce;
+#pragma unroll 4
+  for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+  return sum;
+}
+
+// dword loads, 64*PER entries per pass (lane l takes entries cs + 64k + l).
+// rv covers the gathered vector.
 template <typename TV, typename TA, int PER>
 __device__ __forceinline__ TA spmv_wave(uint32_t mb, uint32_t me,
                                         const uint32_t* __restrict__ col,
                                         const float* __restrict__ val,
-                                        const TV* __restrict__ v, uint32_t nrows,
+                                        __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
                                         TA* s_prod_wave) {
   constexpr uint32_t chunk = 64u * PER;
-  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
-  if (wr0 >= nrows) return TA(0);
-  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  const uint32_t wb = __shfl(mb, 0, 64);
-  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t wb, we;
+  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
+  const __amdgpu_buffer_rsrc_t rc = buf_rsrc(col + wb, (we - wb) * 4u);
+  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(val + wb, (we - wb) * 4u);
   uint32_t c[PER];
   float a[PER];
+  auto load = [&](uint32_t cs) {
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const uint32_t e = wb + (uint32_t)k * 64u + lane;
-    const bool ok = e < we;
-    c[k] = ok ? __builtin_nontemporal_load(col + e) : 0u;
-    a[k] = ok ? __builtin_nontemporal_load(val + e) : 0.0f;
-  }
+    for (int k = 0; k < PER; ++k) {
+      const int off = (int)((cs - wb + (uint32_t)k * 64u + lane) * 4u);
+      c[k] = __builtin_amdgcn_raw_buffer_load_b32(rc, off, 0, kNt);
+      a[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ra, off, 0, kNt));
+    }
+  };
+  load(wb);
   TA sum = TA(0);
   for (uint32_t cs = wb; cs < we; cs += chunk) {
     const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
     TA pr[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const uint32_t e = cs + (uint32_t)k * 64u + lane;
-      pr[k] = e < ce ? (TA)a[k] * (TA)v[c[k]] : TA(0);
-    }
-    const uint32_t ns = cs + chunk;  // prefetch the next chunk
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const uint32_t e = ns + (uint32_t)k * 64u + lane;
-      const bool ok = e < we;
-      c[k] = ok ? __builtin_nontemporal_load(col + e) : 0u;
-      a[k] = ok ? __builtin_nontemporal_load(val + e) : 0.0f;
-    }
+    for (int k = 0; k < PER; ++k) pr[k] = (TA)a[k] * (TA)buf_gather<TV>(rv, c[k]);
+    load(cs + chunk);  // prefetch the next pass
 #pragma unroll
     for (int k = 0; k < PER; ++k) s_prod_wave[k * 64 + lane] = pr[k];
     wave_sync();
-    const uint32_t lo = mb > cs ? mb : cs;
-    const uint32_t hi = me < ce ? me : ce;
-#pragma unroll 4
-    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+    sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
     wave_sync();
   }
   return sum;
 }
 
 // The same wave loop with 16-byte loads: each lane streams K groups of four
-// consecutive (col, value) entries per pass (chunk = 256*K entries).  The
-// pass starts at the wave's first entry rounded down to a multiple of four;
-// entries outside [wb, we) are loaded (the CSR arrays carry four entries of
-// padding) but never gathered or summed.  Products land in LDS in entry
-// order, so the owner lanes sum exactly as in spmv_wave.
+// consecutive (col, value) entries per pass (chunk = 256*K entries, lane l
+// group k = entries cs + 4(64k + l) + 0..3).  Passes start at the wave's
+// first entry rounded down to a multiple of four, so loads are 16-byte
+// aligned; the range of the wave's resources reaches four entries past its
+// last one (the CSR arrays carry four entries of zero padding), so a group
+// that holds the wave's last entry is read whole.  Products land in LDS in
+// entry order, so the owner lanes sum exactly as in spmv_wave.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+struct GroupSrc {
+  __amdgpu_buffer_rsrc_t rc, ra;  // columns and values from entry cs0 on
+  uint32_t cs0;
+};
+
+__device__ __forceinline__ GroupSrc group_src(const uint32_t* col, const float* val, uint32_t cs0,
+                                              uint32_t we) {
+  const uint32_t bytes = (we - cs0 + 4u) * 4u;
+  return GroupSrc{buf_rsrc(col + cs0, bytes), buf_rsrc(val + cs0, bytes), cs0};
+}
+
+template <int K>
+__device__ __forceinline__ void load_groups(const GroupSrc& g, uint32_t cs, u32x4 (&c)[K],
+                                            f32x4v (&a)[K]) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int off = (int)((cs - g.cs0 + 4u * ((uint32_t)k * 64u + lane)) * 4u);
+    c[k] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(g.rc, off, 0, kNt));
+    a[k] = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(g.ra, off, 0, kNt));
+  }
+}
+
+template <typename TA, int K>
+__device__ __forceinline__ void park_groups(TA* s_prod_wave, const TA (&pr)[K][4]) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s_prod_wave[4u * ((uint32_t)k * 64u + lane) + (uint32_t)j] = pr[k][j];
+}
 
 template <typename TV, typename TA, int K>
 __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
                                             const uint32_t* __restrict__ col,
                                             const float* __restrict__ val,
-                                            const TV* __restrict__ v, uint32_t nrows,
+                                            __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
                                             TA* s_prod_wave) {
   constexpr uint32_t chunk = 256u * K;
-  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
-  if (wr0 >= nrows) return TA(0);
-  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  const uint32_t wb = __shfl(mb, 0, 64);
-  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
-  const uint32_t cs0 = wb & ~3u;
+  uint32_t wb, we;
+  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
+  const GroupSrc gs = group_src(col, val, wb & ~3u, we);
   u32x4 c[K];
   f32x4v a[K];
-  auto load = [&](uint32_t cs) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane);
-      if (e < we) {
-        c[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + e));
-        a[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(val + e));
-      } else {
-        c[k] = u32x4{0u, 0u, 0u, 0u};
-        a[k] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
-      }
-    }
-  };
-  load(cs0);
+  load_groups<K>(gs, gs.cs0, c, a);
   TA sum = TA(0);
-  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+  for (uint32_t cs = gs.cs0; cs < we; cs += chunk) {
     const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
     TA pr[K][4];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane) + (uint32_t)j;
-        pr[k][j] = (e >= wb && e < ce) ? (TA)a[k][j] * (TA)v[c[k][j]] : TA(0);
-      }
-    }
-    load(cs + chunk);  // prefetch the next pass
-#pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) s_prod_wave[4u * ((uint32_t)k * 64u + lane) + (uint32_t)j] = pr[k][j];
+      for (int j = 0; j < 4; ++j) pr[k][j] = (TA)a[k][j] * (TA)buf_gather<TV>(rv, c[k][j]);
+    load_groups<K>(gs, cs + chunk, c, a);  // prefetch the next pass
+    park_groups<TA, K>(s_prod_wave, pr);
     wave_sync();
-    const uint32_t lo = mb > cs ? mb : cs;
-    const uint32_t hi = me < ce ? me : ce;
-#pragma unroll 4
-    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+    sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
     wave_sync();
   }
   return sum;
@@ -406,56 +459,39 @@ __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
 // Row-walk form: the wave stages (col, value) pairs of 256*K consecutive
 // entries in LDS with 16-byte loads, then every lane walks its own row's
 // entries, gathering v[col] itself.  Lanes of one gather instruction are
-// then neighbouring rows at the same position in their rows; for node-block
-// matrices (several rows with one column pattern) they read the same or
-// adjacent v entries, so an instruction touches few cache lines.  Sums are
-// the same sequential per-row sums as in spmv_wave.
+// then neighbouring rows at the same position in their rows.  Sums are the
+// same sequential per-row sums as in spmv_wave.
 template <typename TV, typename TA, int K>
 __device__ __forceinline__ TA spmv_wave_walk(uint32_t mb, uint32_t me,
                                              const uint32_t* __restrict__ col,
                                              const float* __restrict__ val,
-                                             const TV* __restrict__ v,
-                                             uint32_t nrows, uint2* s_pair) {
+                                             __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
+                                             uint2* s_pair) {
   constexpr uint32_t chunk = 256u * K;
-  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
-  if (wr0 >= nrows) return TA(0);
-  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  const uint32_t wb = __shfl(mb, 0, 64);
-  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
-  const uint32_t cs0 = wb & ~3u;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t wb, we;
+  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
+  const GroupSrc gs = group_src(col, val, wb & ~3u, we);
   u32x4 c[K];
-  u32x4 a[K];
-  auto load = [&](uint32_t cs) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane);
-      if (e < we) {
-        c[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + e));
-        a[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(val + e));
-      } else {
-        c[k] = u32x4{0u, 0u, 0u, 0u};
-        a[k] = u32x4{0u, 0u, 0u, 0u};
-      }
-    }
-  };
-  load(cs0);
+  f32x4v a[K];
+  load_groups<K>(gs, gs.cs0, c, a);
   TA sum = TA(0);
-  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+  for (uint32_t cs = gs.cs0; cs < we; cs += chunk) {
     const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       uint4* dst = reinterpret_cast<uint4*>(s_pair + 4u * ((uint32_t)k * 64u + lane));
-      dst[0] = uint4{c[k][0], a[k][0], c[k][1], a[k][1]};
-      dst[1] = uint4{c[k][2], a[k][2], c[k][3], a[k][3]};
+      dst[0] = uint4{c[k][0], __float_as_uint(a[k][0]), c[k][1], __float_as_uint(a[k][1])};
+      dst[1] = uint4{c[k][2], __float_as_uint(a[k][2]), c[k][3], __float_as_uint(a[k][3])};
     }
-    load(cs + chunk);  // prefetch the next pass
+    load_groups<K>(gs, cs + chunk, c, a);  // prefetch the next pass
     wave_sync();
     const uint32_t lo = mb > cs ? mb : cs;
     const uint32_t hi = me < ce ? me : ce;
 #pragma unroll 8
     for (uint32_t e = lo; e < hi; ++e) {
       const uint2 pr = s_pair[e - cs];
-      sum = sum + (TA)__uint_as_float(pr.y) * (TA)v[pr.x];
+      sum = sum + (TA)__uint_as_float(pr.y) * (TA)buf_gather<TV>(rv, pr.x);
     }
     wave_sync();
   }
@@ -465,76 +501,148 @@ __device__ __forceinline__ TA spmv_wave_walk(uint32_t mb, uint32_t me,
 // spmv_wave_vec with the gather software-pipelined: the gather of pass i+1
 // and the column/value loads of pass i+2 are in flight while pass i is
 // summed, so the dependent load chain (columns, then v[col]) is not waited
-// on once per pass.  Same products and sums as spmv_wave.
+// on once per pass.  The loop is unrolled twice over two register sets
+// (set 0 holds even passes, set 1 odd ones), so no loaded value is copied
+// across the back edge: such a copy would wait for the loads just issued.
+// Same products and sums as spmv_wave.
 template <typename TV, typename TA, int K>
 __device__ __forceinline__ TA spmv_wave_pipe(uint32_t mb, uint32_t me,
                                              const uint32_t* __restrict__ col,
                                              const float* __restrict__ val,
-                                             const TV* __restrict__ v,
-                                             uint32_t nrows, TA* s_prod_wave) {
+                                             __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
+                                             TA* s_prod_wave) {
   constexpr uint32_t chunk = 256u * K;
-  const uint32_t lane = threadIdx.x & 63u, wr0 = (threadIdx.x >> 6) * 64u;
-  if (wr0 >= nrows) return TA(0);
-  const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  const uint32_t wb = __shfl(mb, 0, 64);
-  const uint32_t we = __shfl(me, (int)wrows - 1, 64);
-  const uint32_t cs0 = wb & ~3u;
-  u32x4 c[K];   // columns of the pass in flight
-  f32x4v a[K];  // values of the pass in flight
-  f32x4v ac[K]; // values of the pass being multiplied
-  TV g[K][4];   // v[col] of the pass being multiplied
-  auto load = [&](uint32_t cs) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane);
-      if (e < we) {
-        c[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + e));
-        a[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(val + e));
-      } else {
-        c[k] = u32x4{0u, 0u, 0u, 0u};
-        a[k] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
-      }
-    }
-  };
-  auto gather = [&](uint32_t cs) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t e = cs + 4u * ((uint32_t)k * 64u + lane) + (uint32_t)j;
-        g[k][j] = e < we ? v[c[k][j]] : TV(0);
-      }
-      ac[k] = a[k];
-    }
-  };
-  load(cs0);
-  gather(cs0);
-  load(cs0 + chunk);
+  uint32_t wb, we;
+  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
+  const GroupSrc gs = group_src(col, val, wb & ~3u, we);
+  u32x4 c0[K], c1[K];
+  f32x4v a0[K], a1[K];
+  TV g[K][4];
   TA sum = TA(0);
-  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+  // one pass: products of the pass in (a, g), gather of the next pass from
+  // cn, loads of the pass after it into (c, a), sum of this pass
+  auto step = [&](uint32_t cs, f32x4v (&a)[K], u32x4 (&c)[K], const u32x4 (&cn)[K]) {
     const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
+    TA pr[K][4];
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        s_prod_wave[4u * ((uint32_t)k * 64u + lane) + (uint32_t)j] = (TA)ac[k][j] * (TA)g[k][j];
-    gather(cs + chunk);
-    load(cs + 2u * chunk);
+      for (int j = 0; j < 4; ++j) pr[k][j] = (TA)a[k][j] * (TA)g[k][j];
+    park_groups<TA, K>(s_prod_wave, pr);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) g[k][j] = buf_gather<TV>(rv, cn[k][j]);
+    load_groups<K>(gs, cs + 2u * chunk, c, a);
     wave_sync();
-    const uint32_t lo = mb > cs ? mb : cs;
-    const uint32_t hi = me < ce ? me : ce;
-#pragma unroll 4
-    for (uint32_t e = lo; e < hi; ++e) sum = sum + s_prod_wave[e - cs];
+    sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
+    wave_sync();
+  };
+  load_groups<K>(gs, gs.cs0, c0, a0);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) g[k][j] = buf_gather<TV>(rv, c0[k][j]);
+  load_groups<K>(gs, gs.cs0 + chunk, c1, a1);
+  for (uint32_t cs = gs.cs0;; cs += 2u * chunk) {
+    step(cs, a0, c0, c1);  // pass in set 0; next pass's columns in set 1
+    if (cs + chunk >= we) break;
+    step(cs + chunk, a1, c1, c0);
+    if (cs + 2u * chunk >= we) break;
+  }
+  return sum;
+}
+
+// Tile-local form (FP32): the tile's distinct columns are listed at setup
+// (d.tcols, sorted) and every entry carries its 16-bit index into that list
+// (d.lidx).  stage() copies x at the listed columns into LDS (s_v, coalesced:
+// the list is sorted); the entries then gather from LDS, and stream 6 bytes
+// each (value + local index) instead of 8.  The first pass's loads are
+// issued before stage() so they overlap it.  stage() holds workgroup
+// barriers and runs in every wave, including waves without rows.  Products
+// are a_k * x[c_k] with the same x values, summed as in spmv_wave.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int K, typename Stage>
+__device__ __forceinline__ float spmv_wave_local(uint32_t mb, uint32_t me,
+                                                 const uint16_t* __restrict__ lidx,
+                                                 const float* __restrict__ val, uint32_t nrows,
+                                                 const float* s_v, float* s_prod_wave,
+                                                 Stage stage) {
+  constexpr uint32_t chunk = 256u * K;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t wb = 0, we = 0;
+  const bool has = wave_range(mb, me, nrows, wb, we);
+  const uint32_t cs0 = has ? (wb & ~3u) : 0u, wend = has ? we : 0u;
+  // ranges reach four entries past the wave's last one (zero padding)
+  const __amdgpu_buffer_rsrc_t ri = buf_rsrc(lidx + cs0, (wend - cs0 + 4u) * 2u);
+  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(val + cs0, (wend - cs0 + 4u) * 4u);
+  u32x2 ci[K];   // four 16-bit local indices per group
+  f32x4v a[K];
+  auto load = [&](uint32_t cs) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t e = cs - cs0 + 4u * ((uint32_t)k * 64u + lane);
+      ci[k] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(ri, (int)(e * 2u), 0, kNt));
+      a[k] = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(e * 4u), 0, kNt));
+    }
+  };
+  load(cs0);
+  stage();
+  if (!has) return 0.0f;
+  float sum = 0.0f;
+  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
+    float pr[K][4];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      pr[k][0] = a[k][0] * s_v[ci[k][0] & 0xFFFFu];
+      pr[k][1] = a[k][1] * s_v[ci[k][0] >> 16];
+      pr[k][2] = a[k][2] * s_v[ci[k][1] & 0xFFFFu];
+      pr[k][3] = a[k][3] * s_v[ci[k][1] >> 16];
+    }
+    load(cs + chunk);  // prefetch the next pass
+    park_groups<float, K>(s_prod_wave, pr);
+    wave_sync();
+    sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
     wave_sync();
   }
   return sum;
+}
+
+// Copies x[tcols[off + i]] to s_v[i] for i < m (m <= kLocalMax), then a
+// barrier.  Run by the whole workgroup.  Loads are unconditional (positions
+// past m re-read the list's first entry).
+__device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, uint32_t off,
+                                            uint32_t m, __amdgpu_buffer_rsrc_t rv, float* s_v) {
+  constexpr int kSteps = (int)(kLocalMax / kTile);
+  const uint32_t t = threadIdx.x;
+  if (m > 0u) {
+    uint32_t c[kSteps];
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) {
+      const uint32_t i = t + (uint32_t)k * kTile;
+      c[k] = tcols[off + (i < m ? i : 0u)];
+    }
+    float x[kSteps];
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) x[k] = buf_gather<float>(rv, c[k]);
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) {
+      const uint32_t i = t + (uint32_t)k * kTile;
+      if (i < m) s_v[i] = x[k];
+    }
+  }
+  __syncthreads();
 }
 
 // Streaming variants of the FP32 SpMV wave loop (all bit-identical; the
 // autotuner picks one per system): 0 = dword loads, 512 entries per pass;
 // 1 = dword, 1024; 2 = 16-byte loads, 512; 3 = 16-byte loads, 1024;
 // 4 = row walk, 512; 5 = row walk, 1024; 6 = 16-byte loads with the gather
-// pipelined, 512; 7 = the same, 1024.  kSlice = LDS floats per wave.
+// pipelined, 512; 7 = the same, 1024; 8 = tile-local columns, 512; 9 = the
+// same, 1024 (8 and 9 fall back to variant 0 on tiles without a local
+// column list).  kSlice = LDS floats per wave.
 template <int V> struct SpmvCfg;
 template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
@@ -544,33 +652,27 @@ template <> struct SpmvCfg<4> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<5> { static constexpr int kSlice = 2048; };
 template <> struct SpmvCfg<6> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<7> { static constexpr int kSlice = 1024; };
+template <> struct SpmvCfg<8> { static constexpr int kSlice = 512; };
+template <> struct SpmvCfg<9> { static constexpr int kSlice = 1024; };
 
 template <int V>
 __device__ __forceinline__ float spmv_rows(uint32_t mb, uint32_t me, const uint32_t* col,
-                                           const float* val, const float* v, uint32_t nrows,
+                                           const float* val, __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
                                            float* slice) {
-  if constexpr (V == 0) return spmv_wave<float, float, 8>(mb, me, col, val, v, nrows, slice);
-  if constexpr (V == 1) return spmv_wave<float, float, 16>(mb, me, col, val, v, nrows, slice);
-  if constexpr (V == 2) return spmv_wave_vec<float, float, 2>(mb, me, col, val, v, nrows, slice);
-  if constexpr (V == 3) return spmv_wave_vec<float, float, 4>(mb, me, col, val, v, nrows, slice);
+  if constexpr (V == 0) return spmv_wave<float, float, 8>(mb, me, col, val, rv, nrows, slice);
+  if constexpr (V == 1) return spmv_wave<float, float, 16>(mb, me, col, val, rv, nrows, slice);
+  if constexpr (V == 2) return spmv_wave_vec<float, float, 2>(mb, me, col, val, rv, nrows, slice);
+  if constexpr (V == 3) return spmv_wave_vec<float, float, 4>(mb, me, col, val, rv, nrows, slice);
   if constexpr (V == 4)
-    return spmv_wave_walk<float, float, 2>(mb, me, col, val, v, nrows, reinterpret_cast<uint2*>(slice));
+    return spmv_wave_walk<float, float, 2>(mb, me, col, val, rv, nrows, reinterpret_cast<uint2*>(slice));
   if constexpr (V == 5)
-    return spmv_wave_walk<float, float, 4>(mb, me, col, val, v, nrows, reinterpret_cast<uint2*>(slice));
-  if constexpr (V == 6) return spmv_wave_pipe<float, float, 2>(mb, me, col, val, v, nrows, slice);
-  if constexpr (V == 7) return spmv_wave_pipe<float, float, 4>(mb, me, col, val, v, nrows, slice);
+    return spmv_wave_walk<float, float, 4>(mb, me, col, val, rv, nrows, reinterpret_cast<uint2*>(slice));
+  if constexpr (V == 6) return spmv_wave_pipe<float, float, 2>(mb, me, col, val, rv, nrows, slice);
+  if constexpr (V == 7) return spmv_wave_pipe<float, float, 4>(mb, me, col, val, rv, nrows, slice);
 }
 
 // --------------------------------------------------- block-Jacobi tile body
 
-// z for the rows [row0, row0+nrows) of one tile (block_jacobi.wgsl:63-127).
-// Thread j solves block bfirst+j, i.e. every block whose rows meet the tile
-// (bfirst and the count come from the system's per-tile table, or from a
-// binary search when there is none); factors are staged through LDS in groups
-// of kLuStage blocks (stride 37 words to spread LDS banks).  rnew(row) gives
-// the r entry the solve reads; rows inside the tile come from s_r, rows of
-// straddling blocks outside it are recomputed by the same formula
-// (bit-identical to the owner's value).
 // Block range of a tile: from the system's per-tile table, or by binary
 // search over block starts when there is none (thread 0, shared via s_blk).
 __device__ __forceinline__ uint2 tile_blocks(const uint32_t* __restrict__ bs,
@@ -610,17 +712,22 @@ __device__ __forceinline__ void bj_stage(const float* __restrict__ lu, uint32_t 
   }
 }
 
+// Rows on each side of a tile whose r a straddling block's solve reads: the
+// leading (at most 6) rows of a block that meets the tile lie within 5 rows
+// of it.
+constexpr uint32_t kHalo = 8;
+
 // z for the rows [row0, row0+nrows) of one tile (block_jacobi.wgsl:63-127).
-// Thread j solves block bfirst+j, i.e. every block whose rows meet the tile;
-// factors are staged through LDS in groups of kLuStage blocks, the first
-// group by the caller (bj_stage(..., 0, ...) before the barrier that
-// publishes s_r).  rnew(row) gives the r entry the solve reads; rows inside
-// the tile come from s_r, rows of straddling blocks outside it are
-// recomputed by the same formula (bit-identical to the owner's value).
-template <typename RNew>
-__device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const float* __restrict__ lu,
-                                        uint2 blk, uint32_t n, uint32_t row0, uint32_t nrows,
-                                        const float* s_r, float* s_z, float* s_lu, RNew rnew) {
+// Thread t solves block blk.x+t (blk.y blocks meet the tile; [bs_s, bs_e)
+// are that block's start and end, loaded by the caller).  Factors are
+// staged through LDS in groups of kLuStage blocks, the first group by the
+// caller (bj_stage(..., 0, ...) before the barrier that publishes s_rh).
+// s_rh[row + kHalo - row0] holds r for rows [row0-kHalo, row0+nrows+kHalo):
+// the tile's own rows and the halo rows of straddling blocks (recomputed by
+// the owner's formula, bit-identical to the owner's value).
+__device__ __forceinline__ void bj_tile(const float* __restrict__ lu, uint2 blk, uint32_t n,
+                                        uint32_t row0, uint32_t nrows, uint32_t bs_s,
+                                        uint32_t bs_e, const float* s_rh, float* s_z, float* s_lu) {
   const uint32_t t = threadIdx.x;
   const uint32_t bfirst = blk.x, nblk = blk.y;
   for (uint32_t g = 0; g < nblk; g += kLuStage) {
@@ -631,18 +738,17 @@ __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const f
     }
     const uint32_t j = t - g;
     if (t >= g && j < cnt) {
-      const uint32_t b = bfirst + g + j;
-      const uint32_t s = bs[b], e = bs[b + 1];
+      const uint32_t s = bs_s, e = bs_e;
       if (s < n && e > s) {  // block_jacobi.wgsl:74-76
         const uint32_t m = (e - s) < (uint32_t)kBlock ? e - s : (uint32_t)kBlock;
         if (s + m > row0) {
           const float* L = s_lu + j * 37u;
+          const float* R = s_rh + (s + kHalo - row0);
           float y[kBlock], xx[kBlock];
 #pragma unroll
           for (int i = 0; i < kBlock; ++i) {
             if ((uint32_t)i < m) {
-              const uint32_t row = s + i;
-              float sum = (row >= row0 && row < row0 + nrows) ? s_r[row - row0] : rnew(row);
+              float sum = R[i];
 #pragma unroll
               for (int jj = 0; jj < i; ++jj) sum = sum - L[i * 6 + jj] * y[jj];
               y[i] = sum;
@@ -670,6 +776,25 @@ __device__ __forceinline__ void bj_tile(const uint32_t* __restrict__ bs, const f
   }
 }
 
+// Halo row handled by thread t < 2*kHalo (rows just before and just after the
+// tile); false when there is none.
+__device__ __forceinline__ bool halo_row(uint32_t row0, uint32_t nrows, uint32_t n, uint32_t& hrow,
+                                         uint32_t& slot) {
+  const uint32_t t = threadIdx.x;
+  if (t < kHalo) {
+    if (row0 + t < kHalo) return false;
+    hrow = row0 + t - kHalo;
+    slot = t;
+    return true;
+  }
+  if (t < 2u * kHalo) {
+    hrow = row0 + nrows + (t - kHalo);
+    slot = nrows + t;
+    return hrow < n;
+  }
+  return false;
+}
+
 // ------------------------------------------------------------ PCG kernels
 //
 // Per iteration: k_pcg_spmv (tiles) -> k_pcg_reduce1 (one workgroup per
@@ -690,6 +815,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
   __shared__ __align__(16) double s_prod[kWaves * kSliceF / 2];
+  __shared__ float s_v[(V == 8 || V == 9) ? kLocalMax : 1];  // tile-local copy of x
   double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
   double* s_rb = s_prod + kTile;
   TileRef tr;
@@ -707,7 +833,23 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
     const bool init = phase == kPhInit;
     const float* v = init ? d.x : d.p;
     float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
-    const float q = spmv_rows<V>(mb, me, d.col, d.val, v, tr.nrows, slice);
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(v, d.n * 4u);
+    float q;
+    if constexpr (V == 8 || V == 9) {
+      // workgroup-uniform: keep it in scalar registers (uniform branch)
+      const uint2 tl0 = d.tloc[tr.tile];
+      const uint2 tl = make_uint2(__builtin_amdgcn_readfirstlane(tl0.x),
+                                  __builtin_amdgcn_readfirstlane(tl0.y));
+      if (tl.y != kLocalOff) {
+        q = spmv_wave_local<V == 8 ? 2 : 4>(mb, me, d.lidx, d.val, tr.nrows, s_v, slice, [&]() {
+          stage_local(d.tcols, tl.x, tl.y, rv, s_v);
+        });
+      } else {
+        q = spmv_rows<0>(mb, me, d.col, d.val, rv, tr.nrows, slice);
+      }
+    } else {
+      q = spmv_rows<V>(mb, me, d.col, d.val, rv, tr.nrows, slice);
+    }
     __syncthreads();  // all waves are done with their slices
     // dot(b,b) at init (compute.rs:235-242), dot(p,Ap) per iteration (:353-360)
     float w = 0.0f;
@@ -729,8 +871,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   } else if (phase == kPhIrResid) {
     // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
     double* slice = s_prod + wave * (kSliceF / 2);
-    const double ax = spmv_wave<double, double, kPer / 2>(mb, me, d.col, d.val, d.x64, tr.nrows,
-                                                          slice);
+    const double ax = spmv_wave<double, double, kPer / 2>(mb, me, d.col, d.val,
+                                                          buf_rsrc(d.x64, d.n * 8u), tr.nrows, slice);
     __syncthreads();
     double rr = 0.0, bb = 0.0;
     if (own) {
@@ -823,7 +965,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce1(const SysArgs sa) {
 
 template <bool kOne>
 __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
-  __shared__ float s_r[kTile];
+  __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
   __shared__ float s_lu[kLuStage * 37];
   __shared__ double s_ra[kTile];
@@ -835,14 +977,18 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   const SysState* st = d.st;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
-  // state-independent loads first: own-row x, p, q, z and the tile's blocks
-  float xi = 0.0f, pi = 0.0f, qi = 0.0f, zi_old = 0.0f;
+  // state-independent loads first: own-row x, p, q, z, the halo row's q and
+  // the tile's blocks
+  float xi = 0.0f, pi = 0.0f, qi = 0.0f, zi_old = 0.0f, hq = 0.0f;
   if (own) {
     xi = d.x[row];
     pi = d.p[row];
     qi = d.q[row];
     if (d.flags & kFlagOversize) zi_old = d.z[row];  // rows no block writes keep z
   }
+  uint32_t hrow = 0, hslot = 0;
+  const bool halo = halo_row(tr.row0, tr.nrows, d.n, hrow, hslot);
+  if (halo) hq = d.q[hrow];
   const uint2 blk = d.tile_blk ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
   const int32_t status = st->status, phase = st->phase;
   const float* __restrict__ r_in = st->r_in;
@@ -853,24 +999,21 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
 
-  // r entry of this iteration for any row (used for straddling blocks)
-  auto rnew = [&](uint32_t i) -> float {
-    if (phase == kPhIter) return r_in[i] + na * d.q[i];      // compute.rs:395-403
-    if (phase == kPhInit) return d.b[i] + (-1.0f) * d.q[i];  // compute.rs:294,301-308
-    return (float)(d.r64[i] * scale);                        // IR restart: b32 = r64*scale
-  };
-
   const uint2 tb = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
                                                   tr.nrows, s_blk);
-  float rn = 0.0f;
-  if (own) {
-    if (phase == kPhIter) {
-      rn = r_in[row] + na * qi;
-    } else if (phase == kPhInit) {
-      rn = d.b[row] + (-1.0f) * qi;
-    } else {
-      rn = (float)(d.r64[row] * scale);
-    }
+  // r of this iteration: compute.rs:395-403 (iteration), :294,301-308 (init),
+  // b32 = r64*scale (refinement restart)
+  auto rnew = [&](uint32_t i, float q) -> float {
+    if (phase == kPhIter) return r_in[i] + na * q;
+    if (phase == kPhInit) return d.b[i] + (-1.0f) * q;
+    return (float)(d.r64[i] * scale);
+  };
+  const float rn = own ? rnew(row, qi) : 0.0f;
+  const float rh = halo ? rnew(hrow, hq) : 0.0f;
+  uint32_t bs_s = 0, bs_e = 0;
+  if (t < tb.y) {
+    bs_s = d.bstart[tb.x + t];
+    bs_e = d.bstart[tb.x + t + 1u];
   }
   // the first factor group goes to LDS while r arrives
   bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
@@ -882,11 +1025,12 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
       d.x[row] = 0.0f;
     }
     r_out[row] = rn;
-    s_r[t] = rn;
+    s_rh[t + kHalo] = rn;
     s_z[t] = zi_old;
   }
+  if (halo) s_rh[hslot] = rh;
   __syncthreads();
-  bj_tile(d.bstart, d.lu, tb, d.n, tr.row0, tr.nrows, s_r, s_z, s_lu, rnew);
+  bj_tile(d.lu, tb, d.n, tr.row0, tr.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
   float zi = 0.0f;
   if (own) {
     zi = s_z[t];
@@ -1039,7 +1183,7 @@ __global__ __launch_bounds__(kTile) void k_true_resid(const SysDesc* __restrict_
   const uint32_t nrows = (d.n - row0) < (uint32_t)kTile ? d.n - row0 : (uint32_t)kTile;
   uint32_t mb, me;
   lane_bounds(d.row_ptr, row0, nrows, mb, me);
-  const double ax = spmv_wave<float, double, kPer / 2>(mb, me, d.col, d.val, x, nrows,
+  const double ax = spmv_wave<float, double, kPer / 2>(mb, me, d.col, d.val, buf_rsrc(x, d.n * 4u), nrows,
                                                        s_prod + (threadIdx.x >> 6) * 64 * (kPer / 2));
   double rr = 0.0, bb = 0.0;
   if (threadIdx.x < nrows) {
@@ -1114,7 +1258,7 @@ __global__ __launch_bounds__(kTile) void k_op_spmv(uint32_t n, const uint32_t* _
   const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
   uint32_t mb, me;
   lane_bounds(rp, row0, nrows, mb, me);
-  const float q = spmv_wave<float, float, kPer>(mb, me, col, val, x, nrows,
+  const float q = spmv_wave<float, float, kPer>(mb, me, col, val, buf_rsrc(x, n * 4u), nrows,
                                                 s_prod + (threadIdx.x >> 6) * 64 * kPer);
   if (threadIdx.x < nrows) y[row0 + threadIdx.x] = q;
 }
@@ -1124,7 +1268,7 @@ __global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t 
                                                            const uint32_t* __restrict__ bs,
                                                            const float* __restrict__ r,
                                                            float* __restrict__ z) {
-  __shared__ float s_r[kTile];
+  __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
   __shared__ float s_lu[kLuStage * 37];
   __shared__ uint32_t s_blk[2];
@@ -1132,13 +1276,20 @@ __global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t 
   const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
   const uint32_t t = threadIdx.x;
   if (t < nrows) {
-    s_r[t] = r[row0 + t];
-    s_z[t] = z[row0 + t];
+    s_rh[t + kHalo] = r[row0 + t];
+    s_z[t] = z[row0 + t];  // rows no block writes keep z (block_jacobi.wgsl:106-122)
   }
+  uint32_t hrow = 0, hslot = 0;
+  if (halo_row(row0, nrows, n, hrow, hslot)) s_rh[hslot] = r[hrow];
   const uint2 blk = tile_blocks(bs, nullptr, blockIdx.x, nb, row0, nrows, s_blk);
+  uint32_t bs_s = 0, bs_e = 0;
+  if (t < blk.y) {
+    bs_s = bs[blk.x + t];
+    bs_e = bs[blk.x + t + 1u];
+  }
   bj_stage(lu, blk.x, 0, blk.y < (uint32_t)kLuStage ? blk.y : (uint32_t)kLuStage, s_lu);
   __syncthreads();
-  bj_tile(bs, lu, blk, n, row0, nrows, s_r, s_z, s_lu, [&](uint32_t i) { return r[i]; });
+  bj_tile(lu, blk, n, row0, nrows, bs_s, bs_e, s_rh, s_z, s_lu);
   if (t < nrows) z[row0 + t] = s_z[t];
 }
 
@@ -1191,6 +1342,8 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 5: launch_spmv_v<5>(ls, s); break;
     case 6: launch_spmv_v<6>(ls, s); break;
     case 7: launch_spmv_v<7>(ls, s); break;
+    case 8: launch_spmv_v<8>(ls, s); break;
+    case 9: launch_spmv_v<9>(ls, s); break;
     default: launch_spmv_v<0>(ls, s); break;
   }
 }

@@ -347,6 +347,10 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   std::vector<float> lu_h(36ull * (nb ? nb : 1));
   CHK(hbj_build_lu_blocks_6(n, row_ptr, n + 1, col_idx, nnz, values, nnz, block_starts, n_starts,
                             lu_h.data()));
+  std::vector<uint16_t> lidx_h;
+  std::vector<uint2> tloc_h;
+  std::vector<uint32_t> tcols_h;
+  build_tile_local(n, row_ptr, col_idx, lidx_h, tloc_h, tcols_h);
   auto* s = new hbj_system();
   s->ctx = ctx;
   s->n = n;
@@ -387,6 +391,9 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   ALLOC(resid2, 2ull * nt);
   ALLOC(d_desc1, 1);
   ALLOC(tile_blk, nt);
+  ALLOC(lidx, lidx_h.size());
+  ALLOC(tloc, nt);
+  ALLOC(tcols, tcols_h.size());
 #undef ALLOC
   if (st != HBJ_OK) return fail(st);
   st = halloc(&s->h_part2, 2ull * nt);
@@ -400,6 +407,9 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   up(s->val, values, 4ull * nnz);
   up(s->bs, block_starts, 4ull * n_starts);
   up(s->lu, lu_h.data(), 144ull * nb);
+  up(s->lidx, lidx_h.data(), sizeof(uint16_t) * lidx_h.size());
+  up(s->tloc, tloc_h.data(), sizeof(uint2) * tloc_h.size());
+  up(s->tcols, tcols_h.data(), sizeof(uint32_t) * tcols_h.size());
   {
     // per tile: the first block touching rows [256t, 256t+256) and how many do
     std::vector<uint2> tb(nt, make_uint2(0u, 0u));

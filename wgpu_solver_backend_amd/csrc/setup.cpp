@@ -1,5 +1,7 @@
-// setup.cpp -- host-side system setup: CSR / block_starts validation and the
-// 6x6 no-pivot LU block builder, with the reference's error texts.
+// setup.cpp -- host-side system setup: CSR / block_starts validation, the
+// 6x6 no-pivot LU block builder (with the reference's error texts) and the
+// tile-local column tables of the SpMV.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -69,6 +71,68 @@ hbj_status validate_case(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
     }
   }
   return HBJ_OK;
+}
+
+// Runs fn(i0, i1) over [0, count) on up to 16 threads (one when count is
+// below `serial`).
+template <typename Fn>
+static void parallel_ranges(uint32_t count, uint32_t serial, Fn fn) {
+  unsigned nth = count >= serial ? std::thread::hardware_concurrency() : 1u;
+  if (nth == 0) nth = 1;
+  if (nth > 16) nth = 16;
+  if (nth == 1) {
+    fn(0u, count);
+    return;
+  }
+  std::vector<std::thread> th;
+  const uint32_t per = (count + nth - 1) / nth;
+  for (unsigned t = 0; t < nth; ++t) {
+    const uint32_t a = t * per, b = (a + per) < count ? a + per : count;
+    if (a < b) th.emplace_back(fn, a, b);
+  }
+  for (auto& x : th) x.join();
+}
+
+void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                      std::vector<uint16_t>& lidx, std::vector<uint2>& tloc,
+                      std::vector<uint32_t>& tcols) {
+  const uint32_t nt = (n + kTile - 1) / kTile;
+  const uint32_t nnz = n ? row_ptr[n] : 0u;
+  std::vector<std::vector<uint32_t>> lists(nt);
+  tloc.assign(nt, make_uint2(0u, 0u));
+  lidx.assign((size_t)nnz + 8u, 0u);
+  // sorted distinct columns of each tile
+  parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
+    for (uint32_t t = t0; t < t1; ++t) {
+      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
+      std::vector<uint32_t>& u = lists[t];
+      u.assign(col + row_ptr[r0], col + row_ptr[r1]);
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      if (u.size() > kLocalMax) {
+        u.clear();
+        tloc[t].y = kLocalOff;
+      }
+    }
+  });
+  size_t total = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    tloc[t].x = (uint32_t)total;
+    if (tloc[t].y != kLocalOff) tloc[t].y = (uint32_t)lists[t].size();
+    total += lists[t].size();
+  }
+  tcols.resize(total ? total : 1u, 0u);
+  // each entry's position in its tile's list
+  parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
+    for (uint32_t t = t0; t < t1; ++t) {
+      const std::vector<uint32_t>& u = lists[t];
+      if (u.empty()) continue;
+      std::copy(u.begin(), u.end(), tcols.begin() + tloc[t].x);
+      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
+      for (uint32_t e = row_ptr[r0]; e < row_ptr[r1]; ++e)
+        lidx[e] = (uint16_t)(std::lower_bound(u.begin(), u.end(), col[e]) - u.begin());
+    }
+  });
 }
 
 }  // namespace hbj

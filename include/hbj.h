@@ -139,13 +139,16 @@ hbj_status hbj_system_solve_resident(hbj_system* sys, uint32_t max_iter, float r
                                      float abs_tol, hbj_result* result);
 hbj_status hbj_system_get_x(hbj_system* sys, float* x);
 
-/* Sample HIP-event timings of the three per-iteration kernels during
- * subsequent solves of this system: every `every`-th iteration (0 = off). */
+/* Launches per PCG iteration. */
+#define HBJ_ITER_KERNELS 4
+
+/* Sample HIP-event timings of the per-iteration kernels during subsequent
+ * solves of this system: every `every`-th iteration (0 = off). */
 hbj_status hbj_system_set_kernel_timing(hbj_system* sys, uint32_t every);
-/* avg_us[5], counts[5]: mean duration and samples of the five launches of an
- * iteration -- SpMV + p.Ap partials, p.Ap reduction (alpha), update + block
- * solve + r.r/r.z partials, their reduction (beta, stop test), p-update.
- * Clears the accumulators. */
+/* avg_us[HBJ_ITER_KERNELS], counts[HBJ_ITER_KERNELS]: mean duration and
+ * samples of the launches of an iteration -- SpMV + p.Ap partials; alpha +
+ * x/r update + block solve + r.r/r.z partials; their reduction (beta, stop
+ * test); p-update.  Clears the accumulators. */
 hbj_status hbj_system_kernel_timing(hbj_system* sys, double* avg_us, uint64_t* counts);
 
 /* ------------------------------------------------ mixed precision (IR) */

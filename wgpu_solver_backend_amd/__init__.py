@@ -35,6 +35,9 @@ __all__ = [
 import os as _os
 
 CLI_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "hbj_cli")
+# launches per PCG iteration (HBJ_ITER_KERNELS in include/hbj.h)
+ITER_KERNELS = 4
+ITER_KERNEL_NAMES = ("spmv", "update", "reduce", "p_update")
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "ZERO_PIVOT", 3: "BREAKDOWN_PAP", 4: "BREAKDOWN_RZ",
                 5: "NOT_CONVERGED", 6: "DEVICE", 7: "OOM", 8: "NO_DEVICE", 9: "REJECTED"}
@@ -323,8 +326,11 @@ class System:
         _check(_lib.lib().hbj_system_set_kernel_timing(self._h, int(every)))
 
     def kernel_timing(self):
-        avg = np.zeros(5, np.float64)
-        cnt = np.zeros(5, np.uint64)
+        """Mean HIP-event duration (us) and sample count of each launch of an
+        iteration: (spmv, update with alpha and the block solve, reduce,
+        p-update)."""
+        avg = np.zeros(ITER_KERNELS, np.float64)
+        cnt = np.zeros(ITER_KERNELS, np.uint64)
         _check(_lib.lib().hbj_system_kernel_timing(self._h, avg, cnt))
         return avg, cnt
 

@@ -121,9 +121,9 @@ struct Engine {
   uint64_t trip = 0;  // iterations enqueued so far (for sampling)
   std::vector<hipEvent_t> tev[2];
   std::vector<int> tev_used;  // per pool: number of 4-event groups recorded
-  static constexpr int kKernels = 5;  // spmv, reduce1, update, reduce2, p-update
-  double t_sum[kKernels] = {0, 0, 0, 0, 0};
-  uint64_t t_cnt[kKernels] = {0, 0, 0, 0, 0};
+  static constexpr int kKernels = HBJ_ITER_KERNELS;  // spmv, update, reduce, p-update
+  double t_sum[kKernels] = {0, 0, 0, 0};
+  uint64_t t_cnt[kKernels] = {0, 0, 0, 0};
 
   hbj_status init(int dev, uint32_t nsys, uint32_t ntiles) {
     device = dev;
@@ -157,6 +157,12 @@ struct Engine {
     n_tiles = (uint32_t)tiles.size();
     one_mode = n_sys == 1 && env_u32("HBJ_ONE", 1) != 0;
     if (one_mode) h_one = descs[0];
+    dot64 = descs.empty() ? 0u : ((descs[0].flags & kFlagDot64) ? 1u : 0u);
+    for (const SysDesc& d : descs)
+      if (((d.flags & kFlagDot64) ? 1u : 0u) != dot64) {
+        set_error("engine: a table cannot mix FP32- and FP64-accumulated systems");
+        return HBJ_E_INVALID;
+      }
     full_tiles = tiles;
     table_full = true;
     drop_graph();
@@ -216,18 +222,20 @@ struct Engine {
   // descriptor as a kernel argument instead of loading it.
   bool one_mode = false;
   SysDesc h_one{};
+  // FP64-accumulated dots (kFlagDot64): one setting for the whole table
+  uint32_t dot64 = 0;
   LaunchSet ls() const {
-    return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys, spmv_variant, one_mode ? &h_one : nullptr};
+    return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys, spmv_variant,
+                     one_mode ? &h_one : nullptr, dot64};
   }
 
-  // one PCG iteration of every system in the table: three tile kernels, each
-  // streaming kernel followed by its per-system reduction
+  // one PCG iteration of every system in the table: SpMV, update, the
+  // per-system reduction and the p-update
   void launch_iter() {
     const LaunchSet l = ls();
     launch_pcg_spmv(l, stream);
-    launch_pcg_reduce1(l, stream);
     launch_pcg_update(l, stream);
-    launch_pcg_reduce2(l, stream);
+    launch_pcg_reduce(l, stream);
     launch_pcg_pupdate(l, stream);
   }
 
@@ -270,14 +278,12 @@ struct Engine {
           HIPCHK(hipEventRecord(e[0], stream));
           launch_pcg_spmv(l, stream);
           HIPCHK(hipEventRecord(e[1], stream));
-          launch_pcg_reduce1(l, stream);
-          HIPCHK(hipEventRecord(e[2], stream));
           launch_pcg_update(l, stream);
+          HIPCHK(hipEventRecord(e[2], stream));
+          launch_pcg_reduce(l, stream);
           HIPCHK(hipEventRecord(e[3], stream));
-          launch_pcg_reduce2(l, stream);
-          HIPCHK(hipEventRecord(e[4], stream));
           launch_pcg_pupdate(l, stream);
-          HIPCHK(hipEventRecord(e[5], stream));
+          HIPCHK(hipEventRecord(e[4], stream));
           ++used;
         } else {
           launch_iter();
@@ -397,6 +403,13 @@ struct Engine {
   }
 
   hbj_status write_slot(uint32_t slot, SysDesc d, const SysState& s) {
+    if (d.n) {  // a system (not an idle slot): the table's dot accumulation follows it
+      const uint32_t want = (d.flags & kFlagDot64) ? 1u : 0u;
+      if (want != dot64) {
+        dot64 = want;
+        drop_graph();
+      }
+    }
     d.st = d_states + slot;
     HIPCHK(hipMemcpy(d_descs + slot, &d, sizeof d, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_states + slot, &s, sizeof s, hipMemcpyHostToDevice));
@@ -458,7 +471,7 @@ struct hbj_system {
   float *x = nullptr, *x0 = nullptr, *b = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
   float *ra = nullptr, *rb = nullptr;
   float* part = nullptr;      // 3 * n_tiles per-tile FP32 partials
-  double* part64 = nullptr;  // 3 * n_tiles per-tile FP64 partials
+  double* part64 = nullptr;  // 5 * n_tiles per-tile FP64 partials
   double* resid2 = nullptr;  // 2 * n_tiles FP64 partials of the residual check
   double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
   float* b32 = nullptr;
@@ -481,10 +494,10 @@ struct hbj_system {
     d.bstart = bs;
     d.lu = lu;
     d.x = x;
-    d.p = p;
     d.q = q;
     d.z = z;
     d.b = ir ? b32 : b;
+    d.p = p;
     d.part = part;
     d.part64 = part64;
     d.x64 = x64;

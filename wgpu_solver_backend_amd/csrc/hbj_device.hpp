@@ -40,8 +40,9 @@ constexpr uint32_t kLocalOff = 0xFFFFFFFFu;
 enum Phase : int32_t {
   kPhInit = 0,      // r0 = b - A x0, z0 = M^-1 r0, p0 = z0, rz_old = r0.z0
   kPhIter = 1,      // PCG iteration
-  kPhIrResid = 2,   // IR: r64 = b64 - A x64 (FP64), decide, scale
-  kPhIrReinit = 3,  // IR: b32 = r64 * scale, restart inner PCG at x32 = 0
+  kPhIrResid = 2,   // IR: x64 += x32 / scale (after an inner solve), then
+                    // r64 = b64 - A x64 (FP64), decide, restart the inner PCG
+                    // on b32 = r64 * scale at x32 = 0
 };
 
 // SysDesc.flags
@@ -82,7 +83,8 @@ struct SysDesc {
   float* z;
   float* b;                  // right-hand side of the FP32 PCG
   float* part;               // 3 * n_tiles per-tile dot partials
-  double* part64;            // 3 * n_tiles FP64 partials (IR / FP64-accumulated dots)
+  double* part64;            // 5 * n_tiles FP64 partials (FP64-accumulated dots; the
+                             // FP64 residual's r.r and b.b at 3 and 4 * n_tiles)
   double* x64;               // IR solution
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual

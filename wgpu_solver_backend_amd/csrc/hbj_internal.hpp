@@ -20,14 +20,14 @@ struct LaunchSet {
   uint32_t n_sys;
   uint32_t spmv_variant;  // streaming variant of the SpMV kernel (bit-identical results)
   const SysDesc* one;     // host copy of the only system's descriptor, or null (table)
+  uint32_t dot64;         // the systems accumulate dots in FP64 (kFlagDot64, all alike)
 };
-constexpr uint32_t kSpmvVariantCount = 10;
-constexpr uint32_t kSpmvDefault = 8;  // tile-local columns, 512 entries per pass
+constexpr uint32_t kSpmvVariantCount = 4;
+constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pass
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
-void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
-void launch_pcg_reduce2(const LaunchSet& ls, hipStream_t s);
+void launch_pcg_reduce(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s);
 void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t n_tiles,
                        hipStream_t s);

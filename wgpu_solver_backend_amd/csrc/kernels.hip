@@ -29,6 +29,8 @@
 // so an FP32 solve here matches the CPU restatement bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hbj_device.hpp"
 #include "hbj_internal.hpp"
 
@@ -316,18 +318,37 @@ __device__ __forceinline__ double buf_gather<double>(__amdgpu_buffer_rsrc_t r, u
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i << 3), 0, 0));
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Element access through a raw buffer resource over a whole array: the base
+// stays in scalar registers and a lane supplies one 32-bit byte offset
+// instead of a 64-bit address (fewer vector registers; arrays hold fewer
+// than 2^30 4-byte or 2^29 8-byte elements, which the host checks).
+template <typename T>
+__device__ __forceinline__ T bld(const T* base, uint32_t i) {
+  const __amdgpu_buffer_rsrc_t r = buf_rsrc(base, 0xFFFFFFFFu);
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i * 8u), 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(T* base, uint32_t i, T v) {
+  const __amdgpu_buffer_rsrc_t r = buf_rsrc(base, 0xFFFFFFFFu);
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(i * 8u), 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)(i * 4u), 0, 0);
+}
+
 // The wave's entry range [wb, we); false when it has no rows or no entries.
-// Read with readlane so the values (and the buffer resources built from
-// them) sit in scalar registers: a resource the compiler cannot prove
-// uniform turns every buffer load into a loop over lanes.  Called with the
-// whole wave active.
 __device__ __forceinline__ bool wave_range(uint32_t mb, uint32_t me, uint32_t nrows, uint32_t& wb,
                                            uint32_t& we) {
-  const uint32_t wr0 = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+  const uint32_t wr0 = (threadIdx.x >> 6) * 64u;
   if (wr0 >= nrows) return false;
   const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  wb = __builtin_amdgcn_readfirstlane(mb);  // lane 0: the wave's first row
-  we = __builtin_amdgcn_readlane(me, (int)wrows - 1);
+  wb = __shfl(mb, 0, 64);
+  we = __shfl(me, (int)wrows - 1, 64);
   return wb < we;
 }
 
@@ -343,12 +364,12 @@ __device__ __forceinline__ TA sum_slice(TA sum, const TA* s_prod_wave, uint32_t 
 }
 
 // dword loads, 64*PER entries per pass (lane l takes entries cs + 64k + l).
-// rv covers the gathered vector.
-template <typename TV, typename TA, int PER>
+// g(c) returns the multiplied vector's entry at column c (one or more buffer
+// gathers; every column of the matrix is a valid argument).
+template <typename TA, int PER, typename G>
 __device__ __forceinline__ TA spmv_wave(uint32_t mb, uint32_t me,
                                         const uint32_t* __restrict__ col,
-                                        const float* __restrict__ val,
-                                        __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
+                                        const float* __restrict__ val, G g, uint32_t nrows,
                                         TA* s_prod_wave) {
   constexpr uint32_t chunk = 64u * PER;
   const uint32_t lane = threadIdx.x & 63u;
@@ -372,7 +393,7 @@ __device__ __forceinline__ TA spmv_wave(uint32_t mb, uint32_t me,
     const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
     TA pr[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) pr[k] = (TA)a[k] * (TA)buf_gather<TV>(rv, c[k]);
+    for (int k = 0; k < PER; ++k) pr[k] = (TA)a[k] * (TA)g(c[k]);
     load(cs + chunk);  // prefetch the next pass
 #pragma unroll
     for (int k = 0; k < PER; ++k) s_prod_wave[k * 64 + lane] = pr[k];
@@ -426,11 +447,10 @@ __device__ __forceinline__ void park_groups(TA* s_prod_wave, const TA (&pr)[K][4
     for (int j = 0; j < 4; ++j) s_prod_wave[4u * ((uint32_t)k * 64u + lane) + (uint32_t)j] = pr[k][j];
 }
 
-template <typename TV, typename TA, int K>
+template <typename TA, int K, typename G>
 __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
                                             const uint32_t* __restrict__ col,
-                                            const float* __restrict__ val,
-                                            __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
+                                            const float* __restrict__ val, G g, uint32_t nrows,
                                             TA* s_prod_wave) {
   constexpr uint32_t chunk = 256u * K;
   uint32_t wb, we;
@@ -446,122 +466,25 @@ __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) pr[k][j] = (TA)a[k][j] * (TA)buf_gather<TV>(rv, c[k][j]);
+      for (int j = 0; j < 4; ++j) pr[k][j] = (TA)a[k][j] * (TA)g(c[k][j]);
     load_groups<K>(gs, cs + chunk, c, a);  // prefetch the next pass
     park_groups<TA, K>(s_prod_wave, pr);
     wave_sync();
     sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
     wave_sync();
-  }
-  return sum;
-}
-
-// Row-walk form: the wave stages (col, value) pairs of 256*K consecutive
-// entries in LDS with 16-byte loads, then every lane walks its own row's
-// entries, gathering v[col] itself.  Lanes of one gather instruction are
-// then neighbouring rows at the same position in their rows.  Sums are the
-// same sequential per-row sums as in spmv_wave.
-template <typename TV, typename TA, int K>
-__device__ __forceinline__ TA spmv_wave_walk(uint32_t mb, uint32_t me,
-                                             const uint32_t* __restrict__ col,
-                                             const float* __restrict__ val,
-                                             __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
-                                             uint2* s_pair) {
-  constexpr uint32_t chunk = 256u * K;
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t wb, we;
-  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
-  const GroupSrc gs = group_src(col, val, wb & ~3u, we);
-  u32x4 c[K];
-  f32x4v a[K];
-  load_groups<K>(gs, gs.cs0, c, a);
-  TA sum = TA(0);
-  for (uint32_t cs = gs.cs0; cs < we; cs += chunk) {
-    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      uint4* dst = reinterpret_cast<uint4*>(s_pair + 4u * ((uint32_t)k * 64u + lane));
-      dst[0] = uint4{c[k][0], __float_as_uint(a[k][0]), c[k][1], __float_as_uint(a[k][1])};
-      dst[1] = uint4{c[k][2], __float_as_uint(a[k][2]), c[k][3], __float_as_uint(a[k][3])};
-    }
-    load_groups<K>(gs, cs + chunk, c, a);  // prefetch the next pass
-    wave_sync();
-    const uint32_t lo = mb > cs ? mb : cs;
-    const uint32_t hi = me < ce ? me : ce;
-#pragma unroll 8
-    for (uint32_t e = lo; e < hi; ++e) {
-      const uint2 pr = s_pair[e - cs];
-      sum = sum + (TA)__uint_as_float(pr.y) * (TA)buf_gather<TV>(rv, pr.x);
-    }
-    wave_sync();
-  }
-  return sum;
-}
-
-// spmv_wave_vec with the gather software-pipelined: the gather of pass i+1
-// and the column/value loads of pass i+2 are in flight while pass i is
-// summed, so the dependent load chain (columns, then v[col]) is not waited
-// on once per pass.  The loop is unrolled twice over two register sets
-// (set 0 holds even passes, set 1 odd ones), so no loaded value is copied
-// across the back edge: such a copy would wait for the loads just issued.
-// Same products and sums as spmv_wave.
-template <typename TV, typename TA, int K>
-__device__ __forceinline__ TA spmv_wave_pipe(uint32_t mb, uint32_t me,
-                                             const uint32_t* __restrict__ col,
-                                             const float* __restrict__ val,
-                                             __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
-                                             TA* s_prod_wave) {
-  constexpr uint32_t chunk = 256u * K;
-  uint32_t wb, we;
-  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
-  const GroupSrc gs = group_src(col, val, wb & ~3u, we);
-  u32x4 c0[K], c1[K];
-  f32x4v a0[K], a1[K];
-  TV g[K][4];
-  TA sum = TA(0);
-  // one pass: products of the pass in (a, g), gather of the next pass from
-  // cn, loads of the pass after it into (c, a), sum of this pass
-  auto step = [&](uint32_t cs, f32x4v (&a)[K], u32x4 (&c)[K], const u32x4 (&cn)[K]) {
-    const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
-    TA pr[K][4];
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) pr[k][j] = (TA)a[k][j] * (TA)g[k][j];
-    park_groups<TA, K>(s_prod_wave, pr);
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) g[k][j] = buf_gather<TV>(rv, cn[k][j]);
-    load_groups<K>(gs, cs + 2u * chunk, c, a);
-    wave_sync();
-    sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
-    wave_sync();
-  };
-  load_groups<K>(gs, gs.cs0, c0, a0);
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) g[k][j] = buf_gather<TV>(rv, c0[k][j]);
-  load_groups<K>(gs, gs.cs0 + chunk, c1, a1);
-  for (uint32_t cs = gs.cs0;; cs += 2u * chunk) {
-    step(cs, a0, c0, c1);  // pass in set 0; next pass's columns in set 1
-    if (cs + chunk >= we) break;
-    step(cs + chunk, a1, c1, c0);
-    if (cs + 2u * chunk >= we) break;
   }
   return sum;
 }
 
 // Tile-local form (FP32): the tile's distinct columns are listed at setup
 // (d.tcols, sorted) and every entry carries its 16-bit index into that list
-// (d.lidx).  stage() copies x at the listed columns into LDS (s_v, coalesced:
-// the list is sorted); the entries then gather from LDS, and stream 6 bytes
-// each (value + local index) instead of 8.  The first pass's loads are
-// issued before stage() so they overlap it.  stage() holds workgroup
-// barriers and runs in every wave, including waves without rows.  Products
-// are a_k * x[c_k] with the same x values, summed as in spmv_wave.
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// (d.lidx).  stage() copies the multiplied vector at the listed columns into
+// LDS (s_v, coalesced: the list is sorted); the entries then gather from
+// LDS, and stream 6 bytes each (value + local index) instead of 8.  The
+// first pass's loads are issued before stage() so they overlap it.
+// stage() holds workgroup barriers and runs in every wave, including waves
+// without rows.  Products are a_k * v[c_k] with the same v values, summed
+// as in spmv_wave.
 
 template <int K, typename Stage>
 __device__ __forceinline__ float spmv_wave_local(uint32_t mb, uint32_t me,
@@ -577,7 +500,7 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t mb, uint32_t me,
   // ranges reach four entries past the wave's last one (zero padding)
   const __amdgpu_buffer_rsrc_t ri = buf_rsrc(lidx + cs0, (wend - cs0 + 4u) * 2u);
   const __amdgpu_buffer_rsrc_t ra = buf_rsrc(val + cs0, (wend - cs0 + 4u) * 4u);
-  u32x2 ci[K];   // four 16-bit local indices per group
+  u32x2 ci[K];  // four 16-bit local indices per group
   f32x4v a[K];
   auto load = [&](uint32_t cs) {
 #pragma unroll
@@ -610,11 +533,12 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t mb, uint32_t me,
   return sum;
 }
 
-// Copies x[tcols[off + i]] to s_v[i] for i < m (m <= kLocalMax), then a
-// barrier.  Run by the whole workgroup.  Loads are unconditional (positions
-// past m re-read the list's first entry).
+// s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
+// Run by the whole workgroup.  Loads are unconditional (positions past m
+// re-read the list's first entry).
+template <typename G>
 __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, uint32_t off,
-                                            uint32_t m, __amdgpu_buffer_rsrc_t rv, float* s_v) {
+                                            uint32_t m, G g, float* s_v) {
   constexpr int kSteps = (int)(kLocalMax / kTile);
   const uint32_t t = threadIdx.x;
   if (m > 0u) {
@@ -626,7 +550,7 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
     }
     float x[kSteps];
 #pragma unroll
-    for (int k = 0; k < kSteps; ++k) x[k] = buf_gather<float>(rv, c[k]);
+    for (int k = 0; k < kSteps; ++k) x[k] = g(c[k]);
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
       const uint32_t i = t + (uint32_t)k * kTile;
@@ -636,40 +560,16 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
   __syncthreads();
 }
 
-// Streaming variants of the FP32 SpMV wave loop (all bit-identical; the
-// autotuner picks one per system): 0 = dword loads, 512 entries per pass;
-// 1 = dword, 1024; 2 = 16-byte loads, 512; 3 = 16-byte loads, 1024;
-// 4 = row walk, 512; 5 = row walk, 1024; 6 = 16-byte loads with the gather
-// pipelined, 512; 7 = the same, 1024; 8 = tile-local columns, 512; 9 = the
-// same, 1024 (8 and 9 fall back to variant 0 on tiles without a local
+// Streaming variants of the FP32 SpMV (all bit-identical; the autotuner
+// picks one per system): 0 = dword loads, 512 entries per pass, gathers from
+// HBM; 1 = 16-byte loads, 1024, gathers from HBM; 2 = tile-local columns,
+// 512; 3 = tile-local, 1024 (2 and 3 use variant 0 on tiles without a local
 // column list).  kSlice = LDS floats per wave.
 template <int V> struct SpmvCfg;
 template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<2> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<3> { static constexpr int kSlice = 1024; };
-template <> struct SpmvCfg<4> { static constexpr int kSlice = 1024; };
-template <> struct SpmvCfg<5> { static constexpr int kSlice = 2048; };
-template <> struct SpmvCfg<6> { static constexpr int kSlice = 512; };
-template <> struct SpmvCfg<7> { static constexpr int kSlice = 1024; };
-template <> struct SpmvCfg<8> { static constexpr int kSlice = 512; };
-template <> struct SpmvCfg<9> { static constexpr int kSlice = 1024; };
-
-template <int V>
-__device__ __forceinline__ float spmv_rows(uint32_t mb, uint32_t me, const uint32_t* col,
-                                           const float* val, __amdgpu_buffer_rsrc_t rv, uint32_t nrows,
-                                           float* slice) {
-  if constexpr (V == 0) return spmv_wave<float, float, 8>(mb, me, col, val, rv, nrows, slice);
-  if constexpr (V == 1) return spmv_wave<float, float, 16>(mb, me, col, val, rv, nrows, slice);
-  if constexpr (V == 2) return spmv_wave_vec<float, float, 2>(mb, me, col, val, rv, nrows, slice);
-  if constexpr (V == 3) return spmv_wave_vec<float, float, 4>(mb, me, col, val, rv, nrows, slice);
-  if constexpr (V == 4)
-    return spmv_wave_walk<float, float, 2>(mb, me, col, val, rv, nrows, reinterpret_cast<uint2*>(slice));
-  if constexpr (V == 5)
-    return spmv_wave_walk<float, float, 4>(mb, me, col, val, rv, nrows, reinterpret_cast<uint2*>(slice));
-  if constexpr (V == 6) return spmv_wave_pipe<float, float, 2>(mb, me, col, val, rv, nrows, slice);
-  if constexpr (V == 7) return spmv_wave_pipe<float, float, 4>(mb, me, col, val, rv, nrows, slice);
-}
 
 // --------------------------------------------------- block-Jacobi tile body
 
@@ -708,7 +608,7 @@ __device__ __forceinline__ void bj_stage(const float* __restrict__ lu, uint32_t 
   const float* src = lu + (size_t)(bfirst + g) * kLuStride;
   for (uint32_t i = threadIdx.x; i < cnt * kLuStride; i += kTile) {
     const uint32_t jb = i / kLuStride, k = i - jb * kLuStride;
-    s_lu[jb * 37u + k] = src[i];
+    s_lu[jb * 37u + k] = bld(src, i);
   }
 }
 
@@ -797,12 +697,21 @@ __device__ __forceinline__ bool halo_row(uint32_t row0, uint32_t nrows, uint32_t
 
 // ------------------------------------------------------------ PCG kernels
 //
-// Per iteration: k_pcg_spmv (tiles) -> k_pcg_reduce1 (one workgroup per
-// system) -> k_pcg_update (tiles) -> k_pcg_reduce2 (systems) ->
-// k_pcg_pupdate (tiles).  Tiles write plain per-tile partials; the reduce
-// kernels read them after the kernel boundary, so no workgroup ever waits on
-// or synchronises with another one.  SysState is written only by the reduce
-// kernels and read by the kernels after them.
+// Per iteration four launches:
+//   k_pcg_spmv     (tiles)   q = A p and the tile's p.Ap partial [init:
+//                            q = A x0 and b.b; refinement: the FP64 residual]
+//   k_pcg_update   (tiles)   every workgroup reduces the p.Ap partials itself
+//                            (the same tree, so alpha is bit-identical in all
+//                            of them), then x += alpha p, r -= alpha q,
+//                            z = M^-1 r and the tile's r.r and r.z partials
+//   k_pcg_reduce   (systems) beta, breakdown checks, stop test and the state
+//                            transitions, in the reference's order
+//   k_pcg_pupdate  (tiles)   p = z + beta p [refinement: the fold]
+// Tiles write plain per-tile partials that later kernels read after the
+// kernel boundary; no workgroup waits on or synchronises with another.
+// Within a kernel no workgroup writes state another workgroup of it reads:
+// k_pcg_update's workgroup 0 records alpha, p.Ap and ||b||^2 for
+// k_pcg_reduce, which alone moves the state machine.
 //
 // Every kernel issues the loads that do not depend on the system's state
 // (row bounds, own-row vector entries, factors, partials) before it reads
@@ -813,9 +722,10 @@ template <int V, bool kOne>
 __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   // per-wave product slices: SpmvCfg<V>::kSlice floats, or 64*kPer/2 doubles
   constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
+  constexpr bool kLocal = V == 2 || V == 3;
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
   __shared__ __align__(16) double s_prod[kWaves * kSliceF / 2];
-  __shared__ float s_v[(V == 8 || V == 9) ? kLocalMax : 1];  // tile-local copy of x
+  __shared__ float s_v[kLocal ? kLocalMax : 1];  // tile-local copy of the multiplied vector
   double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
   double* s_rb = s_prod + kTile;
   TileRef tr;
@@ -831,32 +741,35 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
 
   if (phase == kPhInit || phase == kPhIter) {
     const bool init = phase == kPhInit;
-    const float* v = init ? d.x : d.p;
+    // the multiplied vector: x0 at init (compute.rs:294), p per iteration
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.p, d.n * 4u);
+    auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
     float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
-    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(v, d.n * 4u);
     float q;
-    if constexpr (V == 8 || V == 9) {
+    if constexpr (kLocal) {
       // workgroup-uniform: keep it in scalar registers (uniform branch)
       const uint2 tl0 = d.tloc[tr.tile];
       const uint2 tl = make_uint2(__builtin_amdgcn_readfirstlane(tl0.x),
                                   __builtin_amdgcn_readfirstlane(tl0.y));
       if (tl.y != kLocalOff) {
-        q = spmv_wave_local<V == 8 ? 2 : 4>(mb, me, d.lidx, d.val, tr.nrows, s_v, slice, [&]() {
-          stage_local(d.tcols, tl.x, tl.y, rv, s_v);
+        q = spmv_wave_local<V == 2 ? 2 : 4>(mb, me, d.lidx, d.val, tr.nrows, s_v, slice, [&]() {
+          stage_local(d.tcols, tl.x, tl.y, vec_at, s_v);
         });
       } else {
-        q = spmv_rows<0>(mb, me, d.col, d.val, rv, tr.nrows, slice);
+        q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
       }
+    } else if constexpr (V == 1) {
+      q = spmv_wave_vec<float, 4>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
     } else {
-      q = spmv_rows<V>(mb, me, d.col, d.val, rv, tr.nrows, slice);
+      q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
     }
     __syncthreads();  // all waves are done with their slices
     // dot(b,b) at init (compute.rs:235-242), dot(p,Ap) per iteration (:353-360)
     float w = 0.0f;
     double w64 = 0.0;
     if (own) {
-      d.q[row] = q;
-      const float a = init ? d.b[row] : v[row];
+      bst(d.q, row, q);
+      const float a = init ? bld(d.b, row) : vec_at(row);
       const float c = init ? a : q;
       w = a * c;
       w64 = (double)a * (double)c;
@@ -870,9 +783,11 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
     }
   } else if (phase == kPhIrResid) {
     // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
+    const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
     double* slice = s_prod + wave * (kSliceF / 2);
-    const double ax = spmv_wave<double, double, kPer / 2>(mb, me, d.col, d.val,
-                                                          buf_rsrc(d.x64, d.n * 8u), tr.nrows, slice);
+    const double ax = spmv_wave<double, kPer / 2>(
+        mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows,
+        slice);
     __syncthreads();
     double rr = 0.0, bb = 0.0;
     if (own) {
@@ -884,86 +799,34 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
     }
     tree256x2<double>(rr, bb, s_ra, s_rb);
     if (t == 0) {
-      d.part64[tr.tile] = rr;
-      d.part64[d.n_tiles + tr.tile] = bb;
+      d.part64[3u * d.n_tiles + tr.tile] = rr;
+      d.part64[4u * d.n_tiles + tr.tile] = bb;
     }
   }
 }
 
-// After k_pcg_spmv: ||b||^2 (init), p.Ap and alpha (iteration), or the FP64
-// residual decision (refinement).  One workgroup per system.  The p.Ap
-// partials of an iteration are fetched together with the state.
-template <bool kOne>
-__global__ __launch_bounds__(kTile) void k_pcg_reduce1(const SysArgs sa) {
-  __shared__ double s_a[256];
-  __shared__ double s_b[256];
-  if (blockIdx.x >= sa.n_sys) return;
-  const SysDesc d = kOne ? sa.one : sa.descs[blockIdx.x];
-  SysState* st = d.st;
-  if (!st) return;  // empty serving slot
-  const uint32_t t = threadIdx.x, nt = d.n_tiles;
-  const bool dot64 = (d.flags & kFlagDot64) != 0;
-  const bool pre = nt > 1u && nt <= kFetchMax;
-  float vf[4][4];
-  double vd[4][4];
-  if (pre) {
-    if (dot64) level_fetch<double>(d.part64, nt, vd);
-    else level_fetch<float>(d.part, nt, vf);
-  }
-  const int32_t status = st->status, phase = st->phase;
-  const float rz_old = st->rz_old;
-  const double rz64_old = st->rz64_old;
-  if (status != kRunning) return;  // finished system
-  if (phase == kPhInit) {
-    const float tot = reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_a));
-    if (t == 0) {
-      st->b_norm2 = tot;
-      if (tot == 0.0f) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
-    }
-  } else if (phase == kPhIter) {
-    if (dot64) {
-      const double tot = pre ? reduce_fetched<double>(vd, nt, s_a)
-                             : reduce_partials<double>(d.part64, nt, s_a);
-      if (t == 0) {
-        const float alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
-        st->p_ap = (float)tot;
-        st->p_ap64 = tot;
-        st->alpha = alpha;
-        st->neg_alpha = -alpha;
-      }
-    } else {
-      const float tot = pre ? reduce_fetched<float>(vf, nt, reinterpret_cast<float*>(s_a))
-                            : reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_a));
-      if (t == 0) {
-        // pcg_update_scalars.wgsl:59-62, 70-71
-        const float alpha = tot != 0.0f ? rz_old / tot : 0.0f;
-        st->p_ap = tot;
-        st->alpha = alpha;
-        st->neg_alpha = -alpha;
-      }
-    }
-  } else if (phase == kPhIrResid) {
-    const double rr = reduce_partials<double>(d.part64, nt, s_a);
-    const double bb = reduce_partials<double>(d.part64 + nt, nt, s_b);
-    if (t == 0) {
-      st->rr64 = rr;
-      st->bb64 = bb;
-      if (bb == 0.0 || rr <= st->target2 * bb) {
-        st->status = 0;
-      } else if (st->outer >= st->max_outer) {
-        st->status = 5;  // HBJ_E_NOT_CONVERGED
-      } else {
-        const double nr = sqrt(rr);
-        st->scale = 1.0 / nr;
-        st->inv_scale = nr;
-        st->outer += 1u;
-        st->phase = kPhIrReinit;
-      }
-    }
-  }
+// Refinement decision from the FP64 residual partials (k_pcg_spmv, phase
+// kPhIrResid): 0 = converged, 5 = out of outer steps, kRunning = restart the
+// inner solve on r64 * (1/||r64||).  Computed identically by k_pcg_update
+// (every workgroup) and k_pcg_reduce.
+__device__ __forceinline__ int32_t ir_decide(const SysDesc& d, double target2, uint32_t outer,
+                                             uint32_t max_outer, double* lds_a, double* lds_b,
+                                             double& rr, double& bb) {
+  rr = reduce_partials<double>(d.part64 + 3u * d.n_tiles, d.n_tiles, lds_a);
+  bb = reduce_partials<double>(d.part64 + 4u * d.n_tiles, d.n_tiles, lds_b);
+  if (bb == 0.0 || rr <= target2 * bb) return 0;
+  if (outer >= max_outer) return 5;  // HBJ_E_NOT_CONVERGED
+  return kRunning;
 }
 
-template <bool kOne>
+// kDot64: the table's systems accumulate their dots in FP64 (refinement);
+// a table never mixes the two.
+//
+// Loads come in two rounds: the state with the row data that does not depend
+// on it (x, q, z, tile blocks), then r (the state points to it) with p, the
+// p.Ap partials, block starts and factors.  alpha is reduced after the
+// second round, so the partials wait no longer than r does.
+template <bool kOne, bool kDot64>
 __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
@@ -971,97 +834,141 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
   __shared__ uint32_t s_blk[2];
+  using TP = typename std::conditional<kDot64, double, float>::type;
   TileRef tr;
   if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
-  const SysState* st = d.st;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t;
+  SysState* st = d.st;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t, nt = d.n_tiles;
   const bool own = t < tr.nrows;
-  // state-independent loads first: own-row x, p, q, z, the halo row's q and
-  // the tile's blocks
-  float xi = 0.0f, pi = 0.0f, qi = 0.0f, zi_old = 0.0f, hq = 0.0f;
+  float xi = 0.0f, qi = 0.0f, zi_old = 0.0f, hq = 0.0f;
   if (own) {
-    xi = d.x[row];
-    pi = d.p[row];
-    qi = d.q[row];
-    if (d.flags & kFlagOversize) zi_old = d.z[row];  // rows no block writes keep z
+    xi = bld(d.x, row);
+    qi = bld(d.q, row);
+    if (d.flags & kFlagOversize) zi_old = bld(d.z, row);  // rows no block writes keep z
   }
   uint32_t hrow = 0, hslot = 0;
   const bool halo = halo_row(tr.row0, tr.nrows, d.n, hrow, hslot);
-  if (halo) hq = d.q[hrow];
+  if (halo) hq = bld(d.q, hrow);
   const uint2 blk = d.tile_blk ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
   const int32_t status = st->status, phase = st->phase;
   const float* __restrict__ r_in = st->r_in;
   float* __restrict__ r_out = st->r_out;
-  const float na = st->neg_alpha;
-  const float alpha = st->alpha;
-  const double scale = st->scale;
+  const float rz_old = st->rz_old;
+  const double rz64_old = st->rz64_old, target2 = st->target2;
+  const uint32_t outer = st->outer, max_outer = st->max_outer;
   if (status != kRunning) return;
-  if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
+  if (phase != kPhInit && phase != kPhIter && phase != kPhIrResid) return;
 
+  // the rare phases decide first: ||b||^2 at init, the refinement step
+  double scale = 0.0;
+  if (phase == kPhInit) {
+    const float bn = reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_ra));
+    if (tr.tile == 0 && t == 0) st->b_norm2 = bn;
+    if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
+  } else if (phase == kPhIrResid) {
+    double rr, bb;
+    if (ir_decide(d, target2, outer, max_outer, s_ra, s_rb, rr, bb) != kRunning) return;
+    scale = 1.0 / sqrt(rr);
+  }
+  // second round: p.Ap partials, r sources, p, block starts, factors
+  const TP* part_pap = kDot64 ? reinterpret_cast<const TP*>(d.part64)
+                              : reinterpret_cast<const TP*>(d.part);
+  const bool iter = phase == kPhIter;
+  const bool pre = iter && nt > 1u && nt <= kFetchMax;
+  TP vp[4][4];
+  if (pre) level_fetch<TP>(part_pap, nt, vp);
+  // r source: r_in (iteration), b (init), r64 * scale (refinement restart)
+  auto rsrc = [&](uint32_t i) -> float {
+    if (iter) return bld(r_in, i);
+    if (phase == kPhInit) return bld(d.b, i);
+    return (float)(bld(d.r64, i) * scale);  // b32 = r64*scale
+  };
+  const float rs = own ? rsrc(row) : 0.0f;
+  const float rhs = halo ? rsrc(hrow) : 0.0f;
+  const float pi = (own && iter) ? bld(d.p, row) : 0.0f;
   const uint2 tb = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
                                                   tr.nrows, s_blk);
-  // r of this iteration: compute.rs:395-403 (iteration), :294,301-308 (init),
-  // b32 = r64*scale (refinement restart)
-  auto rnew = [&](uint32_t i, float q) -> float {
-    if (phase == kPhIter) return r_in[i] + na * q;
-    if (phase == kPhInit) return d.b[i] + (-1.0f) * q;
-    return (float)(d.r64[i] * scale);
-  };
-  const float rn = own ? rnew(row, qi) : 0.0f;
-  const float rh = halo ? rnew(hrow, hq) : 0.0f;
   uint32_t bs_s = 0, bs_e = 0;
   if (t < tb.y) {
-    bs_s = d.bstart[tb.x + t];
-    bs_e = d.bstart[tb.x + t + 1u];
+    bs_s = bld(d.bstart, tb.x + t);
+    bs_e = bld(d.bstart, tb.x + t + 1u);
   }
-  // the first factor group goes to LDS while r arrives
   bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
-  if (own) {
-    if (phase == kPhIter) {
-      d.x[row] = xi + alpha * pi;  // compute.rs:386-394
-    } else if (phase == kPhIrReinit) {
-      d.b[row] = rn;
-      d.x[row] = 0.0f;
+
+  float alpha = 0.0f;
+  if (iter) {
+    TP* lds = reinterpret_cast<TP*>(s_ra);
+    const TP tot = pre ? reduce_fetched<TP>(vp, nt, lds) : reduce_partials<TP>(part_pap, nt, lds);
+    if constexpr (kDot64) {
+      alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
+      if (tr.tile == 0 && t == 0) {
+        st->p_ap = (float)tot;
+        st->p_ap64 = tot;
+        st->alpha = alpha;
+      }
+    } else {
+      alpha = tot != 0.0f ? rz_old / tot : 0.0f;  // pcg_update_scalars.wgsl:59-62, 70-71
+      if (tr.tile == 0 && t == 0) {
+        st->p_ap = tot;
+        st->alpha = alpha;
+      }
     }
-    r_out[row] = rn;
+    __syncthreads();  // s_ra is reused below
+  }
+  const float na = -alpha;
+  // r of this iteration: compute.rs:395-403 (iteration), :294,301-308 (init)
+  auto rnew = [&](float r, float q) -> float {
+    if (iter) return r + na * q;
+    if (phase == kPhInit) return r + (-1.0f) * q;
+    return r;
+  };
+  const float rn = own ? rnew(rs, qi) : 0.0f;
+  if (own) {
+    if (iter) {
+      bst(d.x, row, xi + alpha * pi);  // compute.rs:386-394
+    } else if (phase == kPhIrResid) {
+      bst(d.b, row, rn);
+      bst(d.x, row, 0.0f);
+    }
+    bst(r_out, row, rn);
     s_rh[t + kHalo] = rn;
     s_z[t] = zi_old;
   }
-  if (halo) s_rh[hslot] = rh;
+  if (halo) s_rh[hslot] = rnew(rhs, hq);
   __syncthreads();
   bj_tile(d.lu, tb, d.n, tr.row0, tr.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
   float zi = 0.0f;
   if (own) {
     zi = s_z[t];
-    d.z[row] = zi;
+    bst(d.z, row, zi);
   }
   // r.r (compute.rs:406-413) and r.z (:419-426)
-  if (d.flags & kFlagDot64) {
+  if constexpr (kDot64) {
     double a = own ? (double)rn * (double)rn : 0.0;
     double b = own ? (double)rn * (double)zi : 0.0;
     tree256x2<double>(a, b, s_ra, s_rb);
     if (t == 0) {
-      d.part64[d.n_tiles + tr.tile] = a;
-      d.part64[2u * d.n_tiles + tr.tile] = b;
+      d.part64[nt + tr.tile] = a;
+      d.part64[2u * nt + tr.tile] = b;
     }
   } else {
     float a = own ? rn * rn : 0.0f;
     float b = own ? rn * zi : 0.0f;
     tree256x2<float>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
     if (t == 0) {
-      d.part[d.n_tiles + tr.tile] = a;
-      d.part[2u * d.n_tiles + tr.tile] = b;
+      d.part[nt + tr.tile] = a;
+      d.part[2u * nt + tr.tile] = b;
     }
   }
 }
 
 // After k_pcg_update: r.z (and r.r) -> beta, the breakdown checks and the
-// stop test in the reference's order, the restart bookkeeping of iterative
-// refinement, and the swap of the double-buffered r.  The r.r and r.z
+// stop test in the reference's order, the refinement bookkeeping, and the
+// swap of the double-buffered r.  One workgroup per system.  The r.r and r.z
 // partials are fetched together with the state.
 template <bool kOne>
-__global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysArgs sa) {
+__global__ __launch_bounds__(kTile) void k_pcg_reduce(const SysArgs sa) {
   __shared__ double s_a[256];
   __shared__ double s_b[256];
   if (blockIdx.x >= sa.n_sys) return;
@@ -1084,7 +991,29 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysArgs sa) {
   }
   const int32_t status = st->status, phase = st->phase;
   if (status != kRunning) return;  // finished system
-  if (phase != kPhInit && phase != kPhIter && phase != kPhIrReinit) return;
+  if (phase == kPhInit && st->b_norm2 == 0.0f) {
+    if (threadIdx.x == 0) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
+    return;
+  }
+  if (phase == kPhIrResid) {
+    double rr64, bb64;
+    const int32_t dec = ir_decide(d, st->target2, st->outer, st->max_outer, s_a, s_b, rr64, bb64);
+    __syncthreads();  // s_a / s_b are reused below
+    if (threadIdx.x == 0) {
+      st->rr64 = rr64;
+      st->bb64 = bb64;
+    }
+    if (dec != kRunning) {
+      if (threadIdx.x == 0) st->status = dec;
+      return;
+    }
+    if (threadIdx.x == 0) {
+      const double nr = sqrt(rr64);
+      st->scale = 1.0 / nr;
+      st->inv_scale = nr;
+      st->outer += 1u;
+    }
+  }
   float rr, rz;
   double rz64 = 0.0;
   if (dot64) {
@@ -1110,7 +1039,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysArgs sa) {
     if (st->max_iter == 0) st->status = 5;  // loop body never runs (compute.rs:336,491)
     return;
   }
-  if (phase == kPhIrReinit) {
+  if (phase == kPhIrResid) {
+    // restart of the inner solve on b32 = r64 * scale (written by k_pcg_update)
     st->rz_old = rz;
     st->rz64_old = rz64;
     st->b_norm2 = rr;  // ||b32||^2 of the restarted inner solve
@@ -1137,12 +1067,16 @@ __global__ __launch_bounds__(kTile) void k_pcg_reduce2(const SysArgs sa) {
     st->rz_old = rz;
     st->rz64_old = rz64;
   } else if (d.flags & kFlagIr) {
-    st->phase = kPhIrResid;  // fold the correction, then a new FP64 residual
+    st->phase = kPhIrResid;  // k_pcg_pupdate folds the correction; then a new FP64 residual
   } else {
     st->status = fin;
   }
 }
 
+// After k_pcg_reduce: p = (p * beta) + z, two roundings
+// (scale_from_scalar_results.wgsl:37, axpy.wgsl:41 with alpha = 1), or p = z
+// on the first iteration (compute.rs:313-314); after an inner solve of the
+// refinement (phase just set to kPhIrResid) the fold x64 += x32 / scale.
 template <bool kOne>
 __global__ __launch_bounds__(kTile) void k_pcg_pupdate(const TileArgs ta) {
   TileRef tr;
@@ -1151,20 +1085,20 @@ __global__ __launch_bounds__(kTile) void k_pcg_pupdate(const TileArgs ta) {
   const SysState* st = d.st;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   if (t >= tr.nrows) return;
-  const float zi = d.z[row], pi = d.p[row];
+  const float zi = bld(d.z, row), pi = bld(d.p, row);
   const int32_t status = st->status, phase = st->phase;
   const uint32_t iter = st->iter;
   const float beta = st->beta;
   if (status != kRunning) return;
   if (phase == kPhIter) {
     if (iter == 0u) {
-      d.p[row] = zi;  // p0 = z0 (compute.rs:313-314)
+      bst(d.p, row, zi);
     } else {
-      const float pb = pi * beta;  // scale_from_scalar_results.wgsl:37
-      d.p[row] = pb + zi;          // axpy.wgsl:41 with alpha = 1
+      const float pb = pi * beta;
+      bst(d.p, row, pb + zi);
     }
   } else if (phase == kPhIrResid) {
-    d.x64[row] = d.x64[row] + (double)d.x[row] * st->inv_scale;
+    bst(d.x64, row, bld(d.x64, row) + (double)bld(d.x, row) * st->inv_scale);
   }
 }
 
@@ -1183,7 +1117,9 @@ __global__ __launch_bounds__(kTile) void k_true_resid(const SysDesc* __restrict_
   const uint32_t nrows = (d.n - row0) < (uint32_t)kTile ? d.n - row0 : (uint32_t)kTile;
   uint32_t mb, me;
   lane_bounds(d.row_ptr, row0, nrows, mb, me);
-  const double ax = spmv_wave<float, double, kPer / 2>(mb, me, d.col, d.val, buf_rsrc(x, d.n * 4u), nrows,
+  const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x, d.n * 4u);
+  const double ax = spmv_wave<double, kPer / 2>(mb, me, d.col, d.val,
+                                                [&](uint32_t c) { return buf_gather<float>(rx, c); }, nrows,
                                                        s_prod + (threadIdx.x >> 6) * 64 * (kPer / 2));
   double rr = 0.0, bb = 0.0;
   if (threadIdx.x < nrows) {
@@ -1258,7 +1194,9 @@ __global__ __launch_bounds__(kTile) void k_op_spmv(uint32_t n, const uint32_t* _
   const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
   uint32_t mb, me;
   lane_bounds(rp, row0, nrows, mb, me);
-  const float q = spmv_wave<float, float, kPer>(mb, me, col, val, buf_rsrc(x, n * 4u), nrows,
+  const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x, n * 4u);
+  const float q = spmv_wave<float, kPer>(mb, me, col, val,
+                                         [&](uint32_t c) { return buf_gather<float>(rx, c); }, nrows,
                                                 s_prod + (threadIdx.x >> 6) * 64 * kPer);
   if (threadIdx.x < nrows) y[row0 + threadIdx.x] = q;
 }
@@ -1338,34 +1276,29 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 1: launch_spmv_v<1>(ls, s); break;
     case 2: launch_spmv_v<2>(ls, s); break;
     case 3: launch_spmv_v<3>(ls, s); break;
-    case 4: launch_spmv_v<4>(ls, s); break;
-    case 5: launch_spmv_v<5>(ls, s); break;
-    case 6: launch_spmv_v<6>(ls, s); break;
-    case 7: launch_spmv_v<7>(ls, s); break;
-    case 8: launch_spmv_v<8>(ls, s); break;
-    case 9: launch_spmv_v<9>(ls, s); break;
     default: launch_spmv_v<0>(ls, s); break;
   }
 }
-void launch_pcg_reduce1(const LaunchSet& ls, hipStream_t s) {
-  const SysArgs a = sys_args(ls);
-  if (ls.one) hipLaunchKernelGGL(k_pcg_reduce1<true>, dim3(1), dim3(kTile), 0, s, a);
-  else hipLaunchKernelGGL(k_pcg_reduce1<false>, dim3(ls.n_sys), dim3(kTile), 0, s, a);
-}
-void launch_pcg_reduce2(const LaunchSet& ls, hipStream_t s) {
-  const SysArgs a = sys_args(ls);
-  if (ls.one) hipLaunchKernelGGL(k_pcg_reduce2<true>, dim3(1), dim3(kTile), 0, s, a);
-  else hipLaunchKernelGGL(k_pcg_reduce2<false>, dim3(ls.n_sys), dim3(kTile), 0, s, a);
-}
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
-  if (ls.one) hipLaunchKernelGGL(k_pcg_update<true>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
-  else hipLaunchKernelGGL(k_pcg_update<false>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+  const dim3 g(ls.n_tiles), b(kTile);
+  if (ls.dot64) {
+    if (ls.one) hipLaunchKernelGGL((k_pcg_update<true, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_pcg_update<false, true>), g, b, 0, s, a);
+  } else {
+    if (ls.one) hipLaunchKernelGGL((k_pcg_update<true, false>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_pcg_update<false, false>), g, b, 0, s, a);
+  }
 }
 void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
   if (ls.one) hipLaunchKernelGGL(k_pcg_pupdate<true>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
   else hipLaunchKernelGGL(k_pcg_pupdate<false>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+}
+void launch_pcg_reduce(const LaunchSet& ls, hipStream_t s) {
+  const SysArgs a = sys_args(ls);
+  if (ls.one) hipLaunchKernelGGL(k_pcg_reduce<true>, dim3(1), dim3(kTile), 0, s, a);
+  else hipLaunchKernelGGL(k_pcg_reduce<false>, dim3(ls.n_sys), dim3(kTile), 0, s, a);
 }
 
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {

@@ -387,7 +387,7 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   ALLOC(ra, n);
   ALLOC(rb, n);
   ALLOC(part, 3ull * nt);
-  ALLOC(part64, 3ull * nt);
+  ALLOC(part64, 5ull * nt);
   ALLOC(resid2, 2ull * nt);
   ALLOC(d_desc1, 1);
   ALLOC(tile_blk, nt);

@@ -258,7 +258,7 @@ uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
   // min(nnz, kLocalMax) columns per tile, 8 bytes per tile
   const uint64_t tcols = std::min<uint64_t>(c(nnz), (uint64_t)kLocalMax * nt);
   return 4 * c(n + 1) + 8 * (uint64_t)(nnz + 4) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
-         4 * 3 * nt + 8 * 3 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n) +
+         4 * 3 * nt + 8 * 5 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n) +
          2 * ((uint64_t)nnz + 8) + 8 * nt + 4 * tcols;
 }
 

@@ -31,10 +31,13 @@ def log(msg):
 
 
 def kernel_bytes(n, nnz, nb):
-    """Algorithmic HBM bytes per launch of the three PCG kernels."""
-    spmv = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * n          # CSR + row_ptr + p (once) + q
-    update = 4 * n * 7 + 144 * nb + 4 * (nb + 1)         # x rw, p, r_in, q, r_out, z + LU + starts
-    pupdate = 12 * n                                     # p rw, z
+    """Algorithmic HBM bytes per launch of the streaming PCG kernels: the SpMV
+    in CSR accounting (8 bytes per entry, row_ptr, p once, q), the update
+    (x rw, p, r_in, q, r_out, z, the 6x6 factors and block starts) and the
+    p-update (p rw, z)."""
+    spmv = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * n
+    update = 4 * n * 7 + 144 * nb + 4 * (nb + 1)
+    pupdate = 12 * n
     return spmv, update, pupdate
 
 
@@ -80,6 +83,10 @@ def bench_plain(args):
     system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
     system.set_rhs(L["rhs"], np.zeros(n, np.float32))
     max_iter, tol = 5000, 1e-6
+    best, tune_us = system.autotune(40)
+    info = system.info()
+    log(f"autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 2).tolist()}; "
+        f"{info['local_tiles']}/{info['n_tiles']} tiles gather from LDS")
     copy_bytes = 256 << 20
     copy_us = ctx.stream_copy_us(copy_bytes, 10)
     copy_gbps = 2 * copy_bytes / (copy_us * 1e-6) / 1e9
@@ -109,6 +116,7 @@ def bench_plain(args):
         f"kernel avg us {avg_us.round(2).tolist()} ({cnt.tolist()} samples)")
     b1, b2, b3 = kernel_bytes(n, nnz, nb)
     spmv_gbps = b1 / (avg_us[0] * 1e-6) / 1e9
+    layout_gbps = info["spmv_layout_bytes"] / (avg_us[0] * 1e-6) / 1e9
     iter_us = float(avg_us.sum())
     iter_bytes = b1 + b2 + b3
     key = f"plain_N{N}"
@@ -134,17 +142,23 @@ def bench_plain(args):
         "true_rel_residual_fp64": res.rel_res_true_fp64,
         "spmv_gbps": spmv_gbps,
         "stream_copy_gbps": copy_gbps,
-        "iteration": {"us": iter_us, "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
-                      "kernel_us": {"spmv": avg_us[0], "reduce_alpha": avg_us[1],
-                                    "update_precond": avg_us[2], "reduce_beta": avg_us[3],
-                                    "p_update": avg_us[4]},
+        "iteration": {"us": iter_us, "wall_us": elapsed / args.steps / int(res.iterations + 1) * 1e6,
+                      "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
+                      "kernel_us": {"spmv": avg_us[0], "update": avg_us[1],
+                                    "reduce": avg_us[2], "p_update": avg_us[3]},
                       "kernel_gbps": {"spmv": spmv_gbps,
-                                      "update_precond": b2 / (avg_us[2] * 1e-6) / 1e9,
-                                      "p_update": b3 / (avg_us[4] * 1e-6) / 1e9}},
+                                      "update": b2 / (avg_us[1] * 1e-6) / 1e9,
+                                      "p_update": b3 / (avg_us[3] * 1e-6) / 1e9}},
+        "spmv": {"variant": best, "autotune_us_per_iter": [float(x) for x in tune_us],
+                 "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"],
+                 "layout_bytes": info["spmv_layout_bytes"], "layout_gbps": layout_gbps},
         "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": spmv_gbps,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": spmv_gbps / HBM_PEAK_GBPS,
                      "traffic": (traffic or {}).get("k_pcg_spmv_bytes"),
-                     "algorithmic_bytes": b1},
+                     "algorithmic_bytes": b1,
+                     "note": "achieved = CSR algorithmic bytes / kernel time; the tile-local "
+                             "layout streams layout_bytes (6 bytes per entry), so the HBM "
+                             "bandwidth it draws is spmv.layout_gbps"},
     }
     if not args.no_cpu:
         log("cpu baseline (oracle on host cores)")

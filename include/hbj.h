@@ -141,6 +141,37 @@ hbj_status hbj_system_get_x(hbj_system* sys, float* x);
 
 /* Launches per PCG iteration. */
 #define HBJ_ITER_KERNELS 4
+/* Bit-identical SpMV streaming variants a system can use: 0 = dword loads,
+ * gathers from HBM; 1 = 16-byte loads, gathers from HBM; 2 = tile-local
+ * column lists in LDS, 512 entries per wave pass (default); 3 = tile-local,
+ * 1024 entries per pass. */
+#define HBJ_SPMV_VARIANTS 4
+
+typedef struct {
+  uint32_t n, nnz, n_blocks, n_tiles;
+  uint32_t local_tiles;    /* tiles whose SpMV gathers from an LDS copy of x */
+  uint32_t spmv_variant;   /* SpMV variant of the next solve */
+  uint64_t local_columns;  /* total length of the tiles' column lists */
+  uint64_t device_bytes;   /* device memory held by the system */
+  uint64_t spmv_csr_bytes; /* one SpMV in CSR accounting: 8 nnz + 12 n + 4 */
+  uint64_t spmv_layout_bytes; /* one SpMV in the tile-local layout: 6 bytes per
+                                 entry of a local tile, 8 per other entry, the
+                                 column lists, row_ptr, x once and q */
+} hbj_system_info;
+
+/* No reference counterpart (introspection for measurement). */
+hbj_status hbj_system_get_info(const hbj_system* sys, hbj_system_info* out);
+/* Selects the SpMV variant (< HBJ_SPMV_VARIANTS) of later solves; results do
+ * not depend on it.  No reference counterpart. */
+hbj_status hbj_system_set_spmv_variant(hbj_system* sys, uint32_t variant);
+/* Autotuner: runs `iters` PCG iterations from the current right-hand side
+ * with each SpMV variant (after one untimed run each), keeps the fastest for
+ * later solves and returns it in *best; us_per_iter[HBJ_SPMV_VARIANTS] (may
+ * be null) receives each variant's mean time per iteration.  Overwrites the
+ * system's solution vector, not its right-hand side or x0.  No reference
+ * counterpart. */
+hbj_status hbj_system_autotune(hbj_system* sys, uint32_t iters, uint32_t* best,
+                               double* us_per_iter);
 
 /* Sample HIP-event timings of the per-iteration kernels during subsequent
  * solves of this system: every `every`-th iteration (0 = off). */

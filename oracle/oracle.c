@@ -329,10 +329,23 @@ void or_spmv64(uint32_t n_rows, const uint32_t* row_ptr, const uint32_t* col_idx
   }
 }
 
+/* FP64 dot with a fixed summation order whatever the thread count: blocks
+ * of 4096 summed sequentially, then the block sums in order. */
 static double dot64(uint32_t n, const double* a, const double* b) {
+  enum { kBlk = 4096 };
+  const int64_t nblk = ((int64_t)n + kBlk - 1) / kBlk;
+  double part_stack[64];
+  double* part = nblk <= 64 ? part_stack : (double*)malloc(sizeof(double) * (size_t)nblk);
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1 && nblk > 1)
+  for (int64_t k = 0; k < nblk; ++k) {
+    const int64_t i1 = (k + 1) * kBlk < (int64_t)n ? (k + 1) * kBlk : (int64_t)n;
+    double s = 0.0;
+    for (int64_t i = k * kBlk; i < i1; ++i) s += a[i] * b[i];
+    part[k] = s;
+  }
   double s = 0.0;
-#pragma omp parallel for num_threads(g_threads) reduction(+ : s) schedule(static) if (g_threads > 1)
-  for (int64_t i = 0; i < (int64_t)n; ++i) s += a[i] * b[i];
+  for (int64_t k = 0; k < nblk; ++k) s += part[k];
+  if (part != part_stack) free(part);
   return s;
 }
 
@@ -415,7 +428,8 @@ int or_pcg64(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const
   }
   size_t sz = sizeof(double) * (n ? n : 1);
   double *r = (double*)malloc(sz), *z = (double*)malloc(sz), *p = (double*)malloc(sz),
-         *q = (double*)malloc(sz);
+         *q = (double*)malloc(sz), *xbest = (double*)malloc(sz);
+  double best = INFINITY; /* lowest true relative residual seen (at replacements) */
   double bb = dot64(n, b, b);
   int status = OR_E_NOT_CONVERGED;
   if (bb == 0.0) {
@@ -440,6 +454,12 @@ int or_pcg64(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const
     if ((k + 1) % 50 == 0) { /* residual replacement keeps r honest */
       or_spmv64(n, row_ptr, col_idx, values, x, q);
       for (uint32_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
+      const double t = sqrt(dot64(n, r, r) / bb);
+      if (t < best) { /* past the attainable accuracy the iteration only drifts */
+        best = t;
+        memcpy(xbest, x, sz);
+      }
+      if (!isfinite(t)) break;
     }
     double rr = dot64(n, r, r);
     *iterations = k + 1;
@@ -459,11 +479,19 @@ int or_pcg64(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const
     rz = rz_new;
     for (uint32_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
   }
+  if (status != OR_OK && best < INFINITY) {
+    /* not converged: return the best iterate if the last one is worse */
+    or_spmv64(n, row_ptr, col_idx, values, x, q);
+    double tr = 0.0;
+    for (uint32_t i = 0; i < n; ++i) tr += (b[i] - q[i]) * (b[i] - q[i]);
+    if (!(sqrt(tr / bb) <= best)) memcpy(x, xbest, sz);
+  }
 done:
   free(lu);
   free(r);
   free(z);
   free(p);
   free(q);
+  free(xbest);
   return status;
 }

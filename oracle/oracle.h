@@ -95,7 +95,10 @@ double or_true_rel_residual64(uint32_t n, const uint32_t* row_ptr, const uint32_
 
 /* Block-Jacobi PCG entirely in double (LU of the same blocks in double),
  * run until ||r||/||b|| <= rel_tol (recomputing r = b - A x every 50
- * iterations).  Returns OR_OK or OR_E_NOT_CONVERGED; x in/out. */
+ * iterations).  Returns OR_OK or OR_E_NOT_CONVERGED; x in/out (when not
+ * converged: the iterate with the lowest true residual among those checked).
+ * FP64 dots use a fixed summation order, so results do not depend on the
+ * thread count. */
 int or_pcg64(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const float* values,
              const uint32_t* block_starts, uint32_t n_starts, const double* b, double* x,
              uint32_t max_iter, double rel_tol, uint32_t* iterations);

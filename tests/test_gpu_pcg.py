@@ -172,3 +172,32 @@ def test_full_size_lattice_50_bitwise(ctx, hbj, oracle):
     assert res.status == st == 0 and res.iterations == it
     assert np.array_equal(bits(x), bits(xo))
     sys_.close()
+
+
+@pytest.mark.gpu
+def test_spmv_variants_bitwise_and_autotune(ctx, hbj, oracle):
+    """Every SpMV streaming variant gives the same x bits and iteration count
+    (they differ only in how entries are staged); the autotuner picks one of
+    them and the solve after it is unchanged."""
+    L = hbj.lattice(10, seed=4)
+    n = L["n"]
+    st, it, xo, _ = oracle_solve(oracle, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                 L["rhs"], np.zeros(n, F), 2000, 1e-6, 0.0)
+    sys_ = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    sys_.set_rhs(L["rhs"])
+    info = sys_.info()
+    assert info["n"] == n and info["nnz"] == L["nnz"] and info["local_tiles"] == info["n_tiles"]
+    assert info["spmv_csr_bytes"] == 8 * L["nnz"] + 12 * n + 4
+    assert 0 < info["spmv_layout_bytes"] < info["spmv_csr_bytes"]
+    for v in range(hbj.SPMV_VARIANTS):
+        sys_.set_spmv_variant(v)
+        assert sys_.info()["spmv_variant"] == v
+        res = sys_.solve_resident(2000, 1e-6, 0.0)
+        assert res.iterations == it and np.array_equal(bits(sys_.get_x()), bits(xo)), v
+    best, us = sys_.autotune(20)
+    assert 0 <= best < hbj.SPMV_VARIANTS and np.all(us > 0) and us[best] == us.min()
+    res = sys_.solve_resident(2000, 1e-6, 0.0)
+    assert res.iterations == it and np.array_equal(bits(sys_.get_x()), bits(xo))
+    with pytest.raises(hbj.SolverError):
+        sys_.set_spmv_variant(hbj.SPMV_VARIANTS)
+    sys_.close()

@@ -29,7 +29,8 @@ __all__ = [
     "SpmvExecutor", "DotScalarExecutor", "VecOpsExecutor", "BlockJacobiExecutor",
     "PcgUpdateScalarsExecutor", "pcg_block_jacobi_csr_wgpu", "System", "Batch", "Server",
     "IrParams", "lattice", "lattice_size", "beam_element_stiffness", "text_to_bin",
-    "load_case_dir", "write_x_bin", "read_x_bin", "compare_x", "CLI_PATH",
+    "load_case_dir", "write_x_bin", "read_x_bin", "compare_x", "CLI_PATH", "ITER_KERNELS",
+    "ITER_KERNEL_NAMES", "SPMV_VARIANTS",
 ]
 
 import os as _os
@@ -38,6 +39,8 @@ CLI_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "hbj_cli"
 # launches per PCG iteration (HBJ_ITER_KERNELS in include/hbj.h)
 ITER_KERNELS = 4
 ITER_KERNEL_NAMES = ("spmv", "update", "reduce", "p_update")
+# bit-identical SpMV variants (HBJ_SPMV_VARIANTS)
+SPMV_VARIANTS = 4
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "ZERO_PIVOT", 3: "BREAKDOWN_PAP", 4: "BREAKDOWN_RZ",
                 5: "NOT_CONVERGED", 6: "DEVICE", 7: "OOM", 8: "NO_DEVICE", 9: "REJECTED"}
@@ -273,6 +276,26 @@ class System:
     @property
     def device_bytes(self):
         return int(_lib.lib().hbj_system_device_bytes(self._h))
+
+    def info(self):
+        """Sizes, tile-local SpMV coverage, device bytes, the SpMV variant of
+        the next solve and the bytes one SpMV streams (CSR accounting and the
+        tile-local layout), as a dict."""
+        inf = _lib.hbj_system_info()
+        _check(_lib.lib().hbj_system_get_info(self._h, ctypes.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in inf._fields_}
+
+    def set_spmv_variant(self, variant):
+        """Select one of the bit-identical SpMV variants (0..SPMV_VARIANTS-1)."""
+        _check(_lib.lib().hbj_system_set_spmv_variant(self._h, int(variant)))
+
+    def autotune(self, iters=50):
+        """Time `iters` PCG iterations with every SpMV variant, keep the fastest.
+        Returns (best_variant, microseconds per iteration of each variant)."""
+        best = ctypes.c_uint32(0)
+        us = np.zeros(SPMV_VARIANTS, np.float64)
+        _check(_lib.lib().hbj_system_autotune(self._h, int(iters), ctypes.byref(best), us))
+        return int(best.value), us
 
     def solve(self, b, x0, max_iter, rel_tol, abs_tol, raise_on_error=True):
         """hbj_solve: returns (x, result).  x is x0 unless status is OK."""

@@ -44,6 +44,12 @@ class hbj_server_stats(ctypes.Structure):
                 ("peak_active", c_u64)]
 
 
+class hbj_system_info(ctypes.Structure):
+    _fields_ = [("n", c_u32), ("nnz", c_u32), ("n_blocks", c_u32), ("n_tiles", c_u32),
+                ("local_tiles", c_u32), ("spmv_variant", c_u32), ("local_columns", c_u64),
+                ("device_bytes", c_u64), ("spmv_csr_bytes", c_u64), ("spmv_layout_bytes", c_u64)]
+
+
 class hbj_compare_stats(ctypes.Structure):
     _fields_ = [("n", c_u32), ("n_worst", c_u32), ("l2_ref", c_f64), ("l2_err", c_f64),
                 ("rel_l2", c_f64), ("rmse", c_f64), ("max_abs_err", c_f32), ("reserved0", c_f32)]
@@ -67,6 +73,9 @@ _SIGS = {
     "hbj_system_create": (c_i32, [c_void_p, c_u32, c_u32, u32p, u32p, f32p, u32p, c_u32, P(c_void_p)]),
     "hbj_system_destroy": (None, [c_void_p]),
     "hbj_system_device_bytes": (c_u64, [c_void_p]),
+    "hbj_system_get_info": (c_i32, [c_void_p, P(hbj_system_info)]),
+    "hbj_system_set_spmv_variant": (c_i32, [c_void_p, c_u32]),
+    "hbj_system_autotune": (c_i32, [c_void_p, c_u32, P(c_u32), f64p]),
     "hbj_solve": (c_i32, [c_void_p, f32p, f32p, c_u32, c_f32, c_f32, P(hbj_result)]),
     "hbj_system_set_rhs": (c_i32, [c_void_p, f32p, c_void_p]),
     "hbj_system_solve_resident": (c_i32, [c_void_p, c_u32, c_f32, c_f32, P(hbj_result)]),

@@ -218,6 +218,12 @@ struct Engine {
   }
 
   uint32_t spmv_variant = kSpmvDefault;
+  void set_spmv_variant(uint32_t v) {
+    if (v != spmv_variant) {
+      spmv_variant = v;
+      drop_graph();
+    }
+  }
   // A table of one system launches the single-system kernels, which take the
   // descriptor as a kernel argument instead of loading it.
   bool one_mode = false;
@@ -480,6 +486,9 @@ struct hbj_system {
   uint16_t* lidx = nullptr;   // tile-local SpMV tables (build_tile_local)
   uint2* tloc = nullptr;
   uint32_t* tcols = nullptr;
+  uint32_t local_tiles = 0;
+  uint64_t local_columns = 0, spmv_layout_bytes = 0;
+  int spmv_variant = -1;      // chosen variant (-1: the engine's default)
   uint64_t bytes = 0;
   Engine eng;
   bool eng_ready = false;
@@ -551,6 +560,7 @@ struct hbj_system {
       CHK(eng.init(ctx->device, 1, n_tiles));
       eng_ready = true;
     }
+    if (spmv_variant >= 0) eng.set_spmv_variant((uint32_t)spmv_variant);
     if (eng_mode != mode) {
       std::vector<uint2> tiles(n_tiles);
       for (uint32_t t = 0; t < n_tiles; ++t) tiles[t] = make_uint2(0u, t);

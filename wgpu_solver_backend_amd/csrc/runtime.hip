@@ -354,6 +354,21 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   auto* s = new hbj_system();
   s->ctx = ctx;
   s->n = n;
+  {
+    uint64_t csr_entries = 0;
+    for (uint32_t t = 0; t < (uint32_t)tloc_h.size(); ++t) {
+      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
+      const uint64_t e = row_ptr[r1] - row_ptr[r0];
+      if (tloc_h[t].y == kLocalOff) {
+        csr_entries += e;
+      } else {
+        s->local_tiles += 1;
+        s->local_columns += tloc_h[t].y;
+        s->spmv_layout_bytes += 6 * e + 4ull * tloc_h[t].y + 8;
+      }
+    }
+    s->spmv_layout_bytes += 8 * csr_entries + 12ull * n + 4;
+  }
   s->nnz = nnz;
   s->nb = nb;
   s->n_tiles = (n + kTile - 1) / kTile;
@@ -447,6 +462,79 @@ void hbj_system_destroy(hbj_system* s) {
 }
 
 uint64_t hbj_system_device_bytes(const hbj_system* s) { return s ? s->bytes : 0; }
+
+hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
+  clear_error();
+  if (!s || !out) { set_error("hbj_system_get_info: bad argument"); return HBJ_E_INVALID; }
+  std::memset(out, 0, sizeof *out);
+  out->n = s->n;
+  out->nnz = s->nnz;
+  out->n_blocks = s->nb;
+  out->n_tiles = s->n_tiles;
+  out->local_tiles = s->local_tiles;
+  out->spmv_variant = s->spmv_variant >= 0 ? (uint32_t)s->spmv_variant
+                                           : (s->eng_ready ? s->eng.spmv_variant : kSpmvDefault);
+  out->local_columns = s->local_columns;
+  out->device_bytes = s->bytes;
+  out->spmv_csr_bytes = 8ull * s->nnz + 12ull * s->n + 4;
+  out->spmv_layout_bytes = s->spmv_layout_bytes;
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_set_spmv_variant(hbj_system* s, uint32_t v) {
+  clear_error();
+  if (!s || v >= kSpmvVariantCount) {
+    set_error("hbj_system_set_spmv_variant: need a system and a variant < %u", kSpmvVariantCount);
+    return HBJ_E_INVALID;
+  }
+  s->spmv_variant = (int)v;
+  if (s->eng_ready) s->eng.set_spmv_variant(v);
+  return HBJ_OK;
+}
+
+hbj_status hbj_system_autotune(hbj_system* s, uint32_t iters, uint32_t* best, double* us) {
+  clear_error();
+  if (!s || iters == 0) { set_error("hbj_system_autotune: need a system and iters > 0"); return HBJ_E_INVALID; }
+  if (s->n == 0) {
+    if (best) *best = kSpmvDefault;
+    return HBJ_OK;
+  }
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int keep = s->spmv_variant;
+  CHK(s->ensure_engine(0));
+  const uint32_t timing = s->eng.timing_every;
+  s->eng.timing_every = 0;
+  double best_us = 0;
+  uint32_t bv = kSpmvDefault;
+  for (uint32_t v = 0; v < kSpmvVariantCount; ++v) {
+    s->eng.set_spmv_variant(v);
+    double t = 0;
+    for (int rep = 0; rep < 2; ++rep) {  // the first run captures the graph
+      HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, s->eng.stream));
+      if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, s->eng.stream));
+      // tolerances 0: runs `iters` iterations unless the residual is exactly 0
+      SysState init = s->init_state(false, iters, 0.0f, 0.0f, nullptr), fin;
+      double ms = 0;
+      const hbj_status st = s->eng.run(&init, &fin, &ms);
+      if (st != HBJ_OK) {
+        s->eng.timing_every = timing;
+        return st;
+      }
+      t = ms * 1e3 / (double)(fin.iter ? fin.iter : 1);
+    }
+    if (us) us[v] = t;
+    if (v == 0 || t < best_us) {
+      best_us = t;
+      bv = v;
+    }
+  }
+  s->eng.timing_every = timing;
+  s->spmv_variant = (int)bv;
+  s->eng.set_spmv_variant(bv);
+  (void)keep;
+  if (best) *best = bv;
+  return HBJ_OK;
+}
 
 hbj_status hbj_system_set_rhs(hbj_system* s, const float* b, const float* x0) {
   clear_error();

@@ -154,7 +154,7 @@ def bench_plain(args):
                  "layout_bytes": info["spmv_layout_bytes"], "layout_gbps": layout_gbps},
         "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": spmv_gbps,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": spmv_gbps / HBM_PEAK_GBPS,
-                     "traffic": (traffic or {}).get("k_pcg_spmv_bytes"),
+                     "traffic": ((traffic or {}).get("k_pcg_spmv_bytes_by_variant") or {}).get(str(best)),
                      "algorithmic_bytes": b1,
                      "note": "achieved = CSR algorithmic bytes / kernel time; the tile-local "
                              "layout streams layout_bytes (6 bytes per entry), so the HBM "

@@ -232,7 +232,7 @@ struct Engine {
   uint32_t dot64 = 0;
   LaunchSet ls() const {
     return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys, spmv_variant,
-                     one_mode ? &h_one : nullptr, dot64};
+                     one_mode ? &h_one : nullptr, dot64, nullptr, nullptr};
   }
 
   // one PCG iteration of every system in the table: SpMV, update, the
@@ -263,7 +263,7 @@ struct Engine {
   }
 
   hbj_status ensure_timing_events(uint32_t T) {
-    const size_t need = (kKernels + 1u) * ((T + timing_every - 1) / timing_every + 1);
+    const size_t need = 2u * kKernels * ((T + timing_every - 1) / timing_every + 1);
     for (int b = 0; b < 2; ++b)
       while (tev[b].size() < need) {
         hipEvent_t e;
@@ -280,16 +280,18 @@ struct Engine {
       const LaunchSet l = ls();
       for (uint32_t i = 0; i < T; ++i, ++trip) {
         if (trip % timing_every == 0) {
-          hipEvent_t* e = &tev[pool][(kKernels + 1) * used];
-          HIPCHK(hipEventRecord(e[0], stream));
-          launch_pcg_spmv(l, stream);
-          HIPCHK(hipEventRecord(e[1], stream));
-          launch_pcg_update(l, stream);
-          HIPCHK(hipEventRecord(e[2], stream));
-          launch_pcg_reduce(l, stream);
-          HIPCHK(hipEventRecord(e[3], stream));
-          launch_pcg_pupdate(l, stream);
-          HIPCHK(hipEventRecord(e[4], stream));
+          // a start/stop pair per kernel, set by the kernel's own dispatch
+          hipEvent_t* e = &tev[pool][2 * kKernels * used];
+          LaunchSet lt = l;
+          auto timed = [&](int k, void (*fn)(const LaunchSet&, hipStream_t)) {
+            lt.ev_start = e[2 * k];
+            lt.ev_stop = e[2 * k + 1];
+            fn(lt, stream);
+          };
+          timed(0, launch_pcg_spmv);
+          timed(1, launch_pcg_update);
+          timed(2, launch_pcg_reduce);
+          timed(3, launch_pcg_pupdate);
           ++used;
         } else {
           launch_iter();
@@ -311,10 +313,10 @@ struct Engine {
 
   hbj_status harvest_timing(int pool) {
     for (int u = 0; u < tev_used[pool]; ++u) {
-      hipEvent_t* e = &tev[pool][(kKernels + 1) * u];
+      hipEvent_t* e = &tev[pool][2 * kKernels * u];
       for (int k = 0; k < kKernels; ++k) {
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, e[2 * k], e[2 * k + 1]));
         t_sum[k] += ms * 1000.0;
         t_cnt[k] += 1;
       }

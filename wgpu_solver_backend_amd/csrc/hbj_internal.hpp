@@ -21,6 +21,7 @@ struct LaunchSet {
   uint32_t spmv_variant;  // streaming variant of the SpMV kernel (bit-identical results)
   const SysDesc* one;     // host copy of the only system's descriptor, or null (table)
   uint32_t dot64;         // the systems accumulate dots in FP64 (kFlagDot64, all alike)
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // kernel timing (optional)
 };
 constexpr uint32_t kSpmvVariantCount = 4;
 constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pass

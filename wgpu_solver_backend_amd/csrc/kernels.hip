@@ -27,6 +27,7 @@
 //   block_jacobi.wgsl:93-122 forward then backward substitution
 //   pcg_update_scalars.wgsl:56-67  alpha, beta with zero guards
 // so an FP32 solve here matches the CPU restatement bit for bit.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -1245,6 +1246,17 @@ __global__ void k_op_update_scalars(float* s, uint32_t i_pap, uint32_t i_rz_new,
 
 // ------------------------------------------------------- launch wrappers
 
+// Launches a PCG kernel; when the launch set carries timing events they take
+// the kernel's own start and end timestamps (hipExtLaunchKernelGGL), so the
+// measured time excludes the dispatch gap before the kernel.
+template <typename... A, typename K>
+static void launch(const LaunchSet& ls, K k, dim3 g, dim3 b, hipStream_t s, A... args) {
+  if (ls.ev_start)
+    hipExtLaunchKernelGGL(k, g, b, 0, s, ls.ev_start, ls.ev_stop, 0, args...);
+  else
+    hipLaunchKernelGGL(k, g, b, 0, s, args...);
+}
+
 static TileArgs tile_args(const LaunchSet& ls) {
   TileArgs a{};
   if (ls.one) a.one = *ls.one;
@@ -1266,9 +1278,9 @@ template <int V>
 static void launch_spmv_v(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
   if (ls.one)
-    hipLaunchKernelGGL((k_pcg_spmv<V, true>), dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+    launch(ls, k_pcg_spmv<V, true>, dim3(ls.n_tiles), dim3(kTile), s, a);
   else
-    hipLaunchKernelGGL((k_pcg_spmv<V, false>), dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+    launch(ls, k_pcg_spmv<V, false>, dim3(ls.n_tiles), dim3(kTile), s, a);
 }
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
@@ -1283,22 +1295,22 @@ void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
   const dim3 g(ls.n_tiles), b(kTile);
   if (ls.dot64) {
-    if (ls.one) hipLaunchKernelGGL((k_pcg_update<true, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_pcg_update<false, true>), g, b, 0, s, a);
+    if (ls.one) launch(ls, k_pcg_update<true, true>, g, b, s, a);
+    else launch(ls, k_pcg_update<false, true>, g, b, s, a);
   } else {
-    if (ls.one) hipLaunchKernelGGL((k_pcg_update<true, false>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_pcg_update<false, false>), g, b, 0, s, a);
+    if (ls.one) launch(ls, k_pcg_update<true, false>, g, b, s, a);
+    else launch(ls, k_pcg_update<false, false>, g, b, s, a);
   }
 }
 void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
-  if (ls.one) hipLaunchKernelGGL(k_pcg_pupdate<true>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
-  else hipLaunchKernelGGL(k_pcg_pupdate<false>, dim3(ls.n_tiles), dim3(kTile), 0, s, a);
+  if (ls.one) launch(ls, k_pcg_pupdate<true>, dim3(ls.n_tiles), dim3(kTile), s, a);
+  else launch(ls, k_pcg_pupdate<false>, dim3(ls.n_tiles), dim3(kTile), s, a);
 }
 void launch_pcg_reduce(const LaunchSet& ls, hipStream_t s) {
   const SysArgs a = sys_args(ls);
-  if (ls.one) hipLaunchKernelGGL(k_pcg_reduce<true>, dim3(1), dim3(kTile), 0, s, a);
-  else hipLaunchKernelGGL(k_pcg_reduce<false>, dim3(ls.n_sys), dim3(kTile), 0, s, a);
+  if (ls.one) launch(ls, k_pcg_reduce<true>, dim3(1), dim3(kTile), s, a);
+  else launch(ls, k_pcg_reduce<false>, dim3(ls.n_sys), dim3(kTile), s, a);
 }
 
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {

@@ -119,9 +119,18 @@ hbj_status worker_loop(hbj_server* S) {
   HIPCHK(hipSetDevice(S->ctx->device));
   Engine& E = S->eng;
   const uint32_t cap = S->cfg.max_active;
-  std::vector<Request*> slots(cap, nullptr);   // in the table (running or retiring)
-  std::vector<Request*> retiring;              // final, still referenced by an in-flight chunk
-  int inflight = -1;                           // pool of the chunk whose poll is pending
+  // A request belongs to the client once finish() marks it done (the client
+  // may then free it), so the worker keeps what it still needs -- the slot
+  // and the system, which an in-flight chunk may still reference -- apart.
+  struct Retire {
+    uint32_t slot;
+    hbj_system* sys;
+    uint64_t bytes;
+  };
+  std::vector<Request*> slots(cap, nullptr);        // running requests
+  std::vector<hbj_system*> slot_sys(cap, nullptr);  // the slot's system (running or retiring)
+  std::vector<Retire> retiring;                     // final, still referenced by an in-flight chunk
+  int inflight = -1;                                // pool of the chunk whose poll is pending
   int cur = 0;
   uint32_t active = 0;
   const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 16;
@@ -137,7 +146,7 @@ hbj_status worker_loop(hbj_server* S) {
   auto harvest = [&](const SysState* polled) -> hbj_status {
     for (uint32_t i = 0; i < cap; ++i) {
       Request* r = slots[i];
-      if (!r || r->done || polled[i].status == kRunning) continue;
+      if (!r || polled[i].status == kRunning) continue;
       hbj_result res{};
       const SysState& s = polled[i];
       res.status = s.status;
@@ -160,8 +169,10 @@ hbj_status worker_loop(hbj_server* S) {
         err = "PCG(BlockJacobiGpu): dot(p,Ap) is zero (breakdown)";
       else if (s.status == HBJ_E_BREAKDOWN_RZ)
         err = "PCG(BlockJacobiGpu): rz_old is zero (breakdown)";
-      finish(S, r, s.status, err);
-      retiring.push_back(r);
+      retiring.push_back(Retire{i, r->sys, r->est_bytes});
+      r->sys = nullptr;
+      slots[i] = nullptr;
+      finish(S, r, s.status, err);  // r is the client's from here on
       --active;
     }
     return HBJ_OK;
@@ -187,24 +198,23 @@ hbj_status worker_loop(hbj_server* S) {
         CHK(drain());
         CHK(harvest(E.h_poll[p]));
       }
-      for (Request* r : retiring) {
-        slots[r->slot] = nullptr;
+      for (const Retire& rt : retiring) {
+        slot_sys[rt.slot] = nullptr;
         SysState idle;
         std::memset(&idle, 0, sizeof idle);  // status 0: never iterated again
-        CHK(E.write_slot((uint32_t)r->slot, SysDesc{}, idle));
-        const uint64_t b = r->est_bytes;
-        hbj_system_destroy(r->sys);
-        r->sys = nullptr;
+        CHK(E.write_slot(rt.slot, SysDesc{}, idle));
+        hbj_system_destroy(rt.sys);
         std::lock_guard<std::mutex> g(S->mu);
-        S->bytes_in_use -= b;
+        S->bytes_in_use -= rt.bytes;
         S->resident -= 1;
         S->cv_admit.notify_all();
       }
       retiring.clear();
       for (Request* r : incoming) {
         uint32_t slot = 0;
-        while (slots[slot]) ++slot;  // resident <= cap guarantees a free slot
+        while (slot_sys[slot]) ++slot;  // resident <= cap guarantees a free slot
         slots[slot] = r;
+        slot_sys[slot] = r->sys;
         r->slot = (int)slot;
         CHK(r->sys->ensure_ir_buffers());
         if (r->sys->flags & kFlagOversize) HIPCHK(hipMemset(r->sys->z, 0, 4ull * r->n));

@@ -602,14 +602,29 @@ __device__ __forceinline__ uint2 tile_blocks(const uint32_t* __restrict__ bs,
   return make_uint2(s_blk[0], s_blk[1]);
 }
 
-// Copies the factors of blocks [bfirst+g, bfirst+g+cnt) into s_lu (stride 37
-// words per block to spread LDS banks).  No barrier.
+// Copies the factors of blocks [bfirst+g, bfirst+g+cnt) (cnt <= kLuStage)
+// into s_lu (stride 37 words per block to spread LDS banks).  No barrier.
+// All of a thread's loads are issued before any is used: range-checked
+// buffer loads, so positions past the group read 0 and need no branch (a
+// load per loop trip would cost a memory round trip per trip).
 __device__ __forceinline__ void bj_stage(const float* __restrict__ lu, uint32_t bfirst,
                                          uint32_t g, uint32_t cnt, float* s_lu) {
-  const float* src = lu + (size_t)(bfirst + g) * kLuStride;
-  for (uint32_t i = threadIdx.x; i < cnt * kLuStride; i += kTile) {
-    const uint32_t jb = i / kLuStride, k = i - jb * kLuStride;
-    s_lu[jb * 37u + k] = bld(src, i);
+  constexpr int kPerThread = (kLuStage * kLuStride + kTile - 1) / kTile;
+  const uint32_t len = cnt * kLuStride;
+  const __amdgpu_buffer_rsrc_t r = buf_rsrc(lu + (size_t)(bfirst + g) * kLuStride, len * 4u);
+  float v[kPerThread];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const uint32_t i = threadIdx.x + (uint32_t)k * kTile;
+    v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
+  }
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const uint32_t i = threadIdx.x + (uint32_t)k * kTile;
+    if (i < len) {
+      const uint32_t jb = i / kLuStride, kk = i - jb * kLuStride;
+      s_lu[jb * 37u + kk] = v[k];
+    }
   }
 }
 
@@ -823,10 +838,11 @@ __device__ __forceinline__ int32_t ir_decide(const SysDesc& d, double target2, u
 // kDot64: the table's systems accumulate their dots in FP64 (refinement);
 // a table never mixes the two.
 //
-// Loads come in two rounds: the state with the row data that does not depend
-// on it (x, q, z, tile blocks), then r (the state points to it) with p, the
-// p.Ap partials, block starts and factors.  alpha is reduced after the
-// second round, so the partials wait no longer than r does.
+// Loads come in two rounds, each issued in full before any of it is used
+// (every load is unconditional: threads without a row read row0): first the
+// state with what does not depend on it (x, q, z, tile blocks, and the p.Ap
+// partials of an iteration, reduced to alpha right after), then what the
+// state points to (r) with p, the block starts and the factors.
 template <bool kOne, bool kDot64>
 __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __shared__ float s_rh[kTile + 2 * kHalo];
@@ -842,15 +858,19 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   SysState* st = d.st;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, nt = d.n_tiles;
   const bool own = t < tr.nrows;
-  float xi = 0.0f, qi = 0.0f, zi_old = 0.0f, hq = 0.0f;
-  if (own) {
-    xi = bld(d.x, row);
-    qi = bld(d.q, row);
-    if (d.flags & kFlagOversize) zi_old = bld(d.z, row);  // rows no block writes keep z
-  }
+  const uint32_t rowc = own ? row : tr.row0;
   uint32_t hrow = 0, hslot = 0;
   const bool halo = halo_row(tr.row0, tr.nrows, d.n, hrow, hslot);
-  if (halo) hq = bld(d.q, hrow);
+  const uint32_t hrowc = halo ? hrow : tr.row0;
+  // round 1
+  const TP* part_pap = kDot64 ? reinterpret_cast<const TP*>(d.part64)
+                              : reinterpret_cast<const TP*>(d.part);
+  const bool pre = nt > 1u && nt <= kFetchMax;
+  TP vp[4][4];
+  if (pre) level_fetch<TP>(part_pap, nt, vp);
+  const float xi = bld(d.x, rowc), qi = bld(d.q, rowc), hq = bld(d.q, hrowc);
+  // rows no block writes keep z
+  const float zi_old = (d.flags & kFlagOversize) ? bld(d.z, rowc) : 0.0f;
   const uint2 blk = d.tile_blk ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
   const int32_t status = st->status, phase = st->phase;
   const float* __restrict__ r_in = st->r_in;
@@ -860,44 +880,12 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   const uint32_t outer = st->outer, max_outer = st->max_outer;
   if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrResid) return;
+  const bool iter = phase == kPhIter, restart = phase == kPhIrResid;
 
-  // the rare phases decide first: ||b||^2 at init, the refinement step
-  double scale = 0.0;
-  if (phase == kPhInit) {
-    const float bn = reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_ra));
-    if (tr.tile == 0 && t == 0) st->b_norm2 = bn;
-    if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
-  } else if (phase == kPhIrResid) {
-    double rr, bb;
-    if (ir_decide(d, target2, outer, max_outer, s_ra, s_rb, rr, bb) != kRunning) return;
-    scale = 1.0 / sqrt(rr);
-  }
-  // second round: p.Ap partials, r sources, p, block starts, factors
-  const TP* part_pap = kDot64 ? reinterpret_cast<const TP*>(d.part64)
-                              : reinterpret_cast<const TP*>(d.part);
-  const bool iter = phase == kPhIter;
-  const bool pre = iter && nt > 1u && nt <= kFetchMax;
-  TP vp[4][4];
-  if (pre) level_fetch<TP>(part_pap, nt, vp);
-  // r source: r_in (iteration), b (init), r64 * scale (refinement restart)
-  auto rsrc = [&](uint32_t i) -> float {
-    if (iter) return bld(r_in, i);
-    if (phase == kPhInit) return bld(d.b, i);
-    return (float)(bld(d.r64, i) * scale);  // b32 = r64*scale
-  };
-  const float rs = own ? rsrc(row) : 0.0f;
-  const float rhs = halo ? rsrc(hrow) : 0.0f;
-  const float pi = (own && iter) ? bld(d.p, row) : 0.0f;
-  const uint2 tb = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
-                                                  tr.nrows, s_blk);
-  uint32_t bs_s = 0, bs_e = 0;
-  if (t < tb.y) {
-    bs_s = bld(d.bstart, tb.x + t);
-    bs_e = bld(d.bstart, tb.x + t + 1u);
-  }
-  bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
-
+  // alpha (pcg_update_scalars.wgsl:59-62, 70-71), ||b||^2 at init, or the
+  // refinement restart's scale
   float alpha = 0.0f;
+  double scale = 0.0;
   if (iter) {
     TP* lds = reinterpret_cast<TP*>(s_ra);
     const TP tot = pre ? reduce_fetched<TP>(vp, nt, lds) : reduce_partials<TP>(part_pap, nt, lds);
@@ -909,26 +897,50 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
         st->alpha = alpha;
       }
     } else {
-      alpha = tot != 0.0f ? rz_old / tot : 0.0f;  // pcg_update_scalars.wgsl:59-62, 70-71
+      alpha = tot != 0.0f ? rz_old / tot : 0.0f;
       if (tr.tile == 0 && t == 0) {
         st->p_ap = tot;
         st->alpha = alpha;
       }
     }
-    __syncthreads();  // s_ra is reused below
+  } else if (!restart) {
+    const float bn = reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_ra));
+    if (tr.tile == 0 && t == 0) st->b_norm2 = bn;
+    if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
+  } else {
+    double rr, bb;
+    if (ir_decide(d, target2, outer, max_outer, s_ra, s_rb, rr, bb) != kRunning) return;
+    scale = 1.0 / sqrt(rr);
   }
+  __syncthreads();  // s_ra / s_rb are reused below
   const float na = -alpha;
+
+  // round 2: r source (r_in per iteration, b at init; r64 * scale = b32 at
+  // a refinement restart), p, block starts, the first factor group
+  const float* rsrc_arr = iter ? r_in : d.b;
+  float rs = bld(rsrc_arr, rowc), rhs = bld(rsrc_arr, hrowc);
+  if (restart) {
+    rs = (float)(bld(d.r64, rowc) * scale);
+    rhs = (float)(bld(d.r64, hrowc) * scale);
+  }
+  const float pi = bld(d.p, rowc);
+  const uint2 tb = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
+                                                  tr.nrows, s_blk);
+  const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+  const uint32_t bs_s = bld(d.bstart, bi), bs_e = bld(d.bstart, bi + 1u);
+  bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+
   // r of this iteration: compute.rs:395-403 (iteration), :294,301-308 (init)
   auto rnew = [&](float r, float q) -> float {
     if (iter) return r + na * q;
-    if (phase == kPhInit) return r + (-1.0f) * q;
+    if (!restart) return r + (-1.0f) * q;
     return r;
   };
   const float rn = own ? rnew(rs, qi) : 0.0f;
   if (own) {
     if (iter) {
       bst(d.x, row, xi + alpha * pi);  // compute.rs:386-394
-    } else if (phase == kPhIrResid) {
+    } else if (restart) {
       bst(d.b, row, rn);
       bst(d.x, row, 0.0f);
     }

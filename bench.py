@@ -31,14 +31,14 @@ def log(msg):
 
 
 def kernel_bytes(n, nnz, nb):
-    """Algorithmic HBM bytes per launch of the streaming PCG kernels: the SpMV
-    in CSR accounting (8 bytes per entry, row_ptr, p once, q), the update
-    (x rw, p, r_in, q, r_out, z, the 6x6 factors and block starts) and the
-    p-update (p rw, z)."""
+    """Algorithmic HBM bytes per launch of the PCG kernels: the SpMV in CSR
+    accounting (8 bytes per entry, row_ptr, p once, q), the update (x rw, p,
+    r_in, q, r_out, z, the 6x6 factors and block starts) and the finish
+    (p rw, z; its partials and state are a few KB)."""
     spmv = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * n
     update = 4 * n * 7 + 144 * nb + 4 * (nb + 1)
-    pupdate = 12 * n
-    return spmv, update, pupdate
+    finish = 12 * n
+    return spmv, update, finish
 
 
 def traffic_from_profiles(workload_key):
@@ -145,10 +145,10 @@ def bench_plain(args):
         "iteration": {"us": iter_us, "wall_us": elapsed / args.steps / int(res.iterations + 1) * 1e6,
                       "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
                       "kernel_us": {"spmv": avg_us[0], "update": avg_us[1],
-                                    "reduce": avg_us[2], "p_update": avg_us[3]},
+                                    "finish": avg_us[2]},
                       "kernel_gbps": {"spmv": spmv_gbps,
                                       "update": b2 / (avg_us[1] * 1e-6) / 1e9,
-                                      "p_update": b3 / (avg_us[3] * 1e-6) / 1e9}},
+                                      "finish": b3 / (avg_us[2] * 1e-6) / 1e9}},
         "spmv": {"variant": best, "autotune_us_per_iter": [float(x) for x in tune_us],
                  "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"],
                  "layout_bytes": info["spmv_layout_bytes"], "layout_gbps": layout_gbps},

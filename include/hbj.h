@@ -140,7 +140,7 @@ hbj_status hbj_system_solve_resident(hbj_system* sys, uint32_t max_iter, float r
 hbj_status hbj_system_get_x(hbj_system* sys, float* x);
 
 /* Launches per PCG iteration. */
-#define HBJ_ITER_KERNELS 4
+#define HBJ_ITER_KERNELS 3
 /* Bit-identical SpMV streaming variants a system can use: 0 = dword loads,
  * gathers from HBM; 1 = 16-byte loads, gathers from HBM; 2 = tile-local
  * column lists in LDS, 512 entries per wave pass (default); 3 = tile-local,
@@ -178,8 +178,8 @@ hbj_status hbj_system_autotune(hbj_system* sys, uint32_t iters, uint32_t* best,
 hbj_status hbj_system_set_kernel_timing(hbj_system* sys, uint32_t every);
 /* avg_us[HBJ_ITER_KERNELS], counts[HBJ_ITER_KERNELS]: mean duration and
  * samples of the launches of an iteration -- SpMV + p.Ap partials; alpha +
- * x/r update + block solve + r.r/r.z partials; their reduction (beta, stop
- * test); p-update.  Clears the accumulators. */
+ * x/r update + block solve + r.r/r.z partials; beta, stop test and state
+ * transition + p-update.  Clears the accumulators. */
 hbj_status hbj_system_kernel_timing(hbj_system* sys, double* avg_us, uint64_t* counts);
 
 /* ------------------------------------------------ mixed precision (IR) */

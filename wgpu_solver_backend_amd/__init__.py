@@ -37,8 +37,8 @@ import os as _os
 
 CLI_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "hbj_cli")
 # launches per PCG iteration (HBJ_ITER_KERNELS in include/hbj.h)
-ITER_KERNELS = 4
-ITER_KERNEL_NAMES = ("spmv", "update", "reduce", "p_update")
+ITER_KERNELS = 3
+ITER_KERNEL_NAMES = ("spmv", "update", "finish")
 # bit-identical SpMV variants (HBJ_SPMV_VARIANTS)
 SPMV_VARIANTS = 4
 
@@ -350,8 +350,8 @@ class System:
 
     def kernel_timing(self):
         """Mean HIP-event duration (us) and sample count of each launch of an
-        iteration: (spmv, update with alpha and the block solve, reduce,
-        p-update)."""
+        iteration: (spmv, update with alpha and the block solve, finish with
+        beta, the stop test and the p-update)."""
         avg = np.zeros(ITER_KERNELS, np.float64)
         cnt = np.zeros(ITER_KERNELS, np.uint64)
         _check(_lib.lib().hbj_system_kernel_timing(self._h, avg, cnt))

@@ -121,16 +121,20 @@ struct Engine {
   uint64_t trip = 0;  // iterations enqueued so far (for sampling)
   std::vector<hipEvent_t> tev[2];
   std::vector<int> tev_used;  // per pool: number of 4-event groups recorded
-  static constexpr int kKernels = HBJ_ITER_KERNELS;  // spmv, update, reduce, p-update
-  double t_sum[kKernels] = {0, 0, 0, 0};
-  uint64_t t_cnt[kKernels] = {0, 0, 0, 0};
+  static constexpr int kKernels = HBJ_ITER_KERNELS;  // spmv, update, finish
+  double t_sum[kKernels] = {0, 0, 0};
+  uint64_t t_cnt[kKernels] = {0, 0, 0};
+  // SysState of system i: d_states[i] (parity 0) and d_states[cap + i]
+  // (parity 1); chunks hold an even number of iterations, so copy 0 is
+  // current whenever the host looks.
+  uint32_t cap = 0;
 
   hbj_status init(int dev, uint32_t nsys, uint32_t ntiles) {
     device = dev;
     n_sys = nsys;
     n_tiles = ntiles;
     chunk = env_u32("HBJ_CHUNK", 16);
-    if (chunk < 1) chunk = 1;
+    chunk = chunk < 2 ? 2u : (chunk & ~1u);  // even: the state parity returns to 0
     use_graph = env_u32("HBJ_GRAPH", 1) != 0;
     remap = env_u32("HBJ_XCD_REMAP", 1);
     spmv_variant = env_u32("HBJ_SPMV_VARIANT", kSpmvDefault) % kSpmvVariantCount;
@@ -140,7 +144,8 @@ struct Engine {
     CHK(dalloc(&d_descs, nsys));
     CHK(dalloc(&d_tiles, ntiles));
     tiles_cap = ntiles ? ntiles : 1;
-    CHK(dalloc(&d_states, nsys));
+    cap = nsys ? nsys : 1;
+    CHK(dalloc(&d_states, 2ull * cap));
     CHK(halloc(&h_init, nsys));
     CHK(halloc(&h_poll[0], nsys));
     CHK(halloc(&h_poll[1], nsys));
@@ -232,24 +237,24 @@ struct Engine {
   uint32_t dot64 = 0;
   LaunchSet ls() const {
     return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys, spmv_variant,
-                     one_mode ? &h_one : nullptr, dot64, nullptr, nullptr};
+                     one_mode ? &h_one : nullptr, dot64, 0u, cap, nullptr, nullptr};
   }
 
-  // one PCG iteration of every system in the table: SpMV, update, the
-  // per-system reduction and the p-update
-  void launch_iter() {
-    const LaunchSet l = ls();
+  // one PCG iteration (parity par) of every system in the table: SpMV,
+  // update, finish
+  void launch_iter(uint32_t par) {
+    LaunchSet l = ls();
+    l.par = par;
     launch_pcg_spmv(l, stream);
     launch_pcg_update(l, stream);
-    launch_pcg_reduce(l, stream);
-    launch_pcg_pupdate(l, stream);
+    launch_pcg_finish(l, stream);
   }
 
   hbj_status capture(uint32_t T) {
     drop_graph();
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t i = 0; i < T; ++i) launch_iter();
+    for (uint32_t i = 0; i < T; ++i) launch_iter(i & 1u);
     HIPCHK(hipStreamEndCapture(stream, &g));
     hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -283,6 +288,7 @@ struct Engine {
           // a start/stop pair per kernel, set by the kernel's own dispatch
           hipEvent_t* e = &tev[pool][2 * kKernels * used];
           LaunchSet lt = l;
+          lt.par = i & 1u;
           auto timed = [&](int k, void (*fn)(const LaunchSet&, hipStream_t)) {
             lt.ev_start = e[2 * k];
             lt.ev_stop = e[2 * k + 1];
@@ -290,11 +296,10 @@ struct Engine {
           };
           timed(0, launch_pcg_spmv);
           timed(1, launch_pcg_update);
-          timed(2, launch_pcg_reduce);
-          timed(3, launch_pcg_pupdate);
+          timed(2, launch_pcg_finish);
           ++used;
         } else {
-          launch_iter();
+          launch_iter(i & 1u);
         }
       }
       tev_used[pool] = used;
@@ -306,7 +311,7 @@ struct Engine {
       HIPCHK(hipGraphLaunch(gexec, stream));
       return HBJ_OK;
     }
-    for (uint32_t i = 0; i < T; ++i) launch_iter();
+    for (uint32_t i = 0; i < T; ++i) launch_iter(i & 1u);
     HIPCHK(hipGetLastError());
     return HBJ_OK;
   }
@@ -358,6 +363,8 @@ struct Engine {
     }
     std::memcpy(h_init, init, sizeof(SysState) * n_sys);
     HIPCHK(hipMemcpyAsync(d_states, h_init, sizeof(SysState) * n_sys, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_states + cap, h_init, sizeof(SysState) * n_sys, hipMemcpyHostToDevice,
+                          stream));
     const double t0 = now_ms();
     uint32_t c = 0;
     int final_pool = -1;
@@ -421,6 +428,7 @@ struct Engine {
     d.st = d_states + slot;
     HIPCHK(hipMemcpy(d_descs + slot, &d, sizeof d, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_states + slot, &s, sizeof s, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_states + cap + slot, &s, sizeof s, hipMemcpyHostToDevice));
     return HBJ_OK;
   }
 

@@ -21,6 +21,8 @@ struct LaunchSet {
   uint32_t spmv_variant;  // streaming variant of the SpMV kernel (bit-identical results)
   const SysDesc* one;     // host copy of the only system's descriptor, or null (table)
   uint32_t dot64;         // the systems accumulate dots in FP64 (kFlagDot64, all alike)
+  uint32_t par;           // iteration parity: which of a system's two SysState copies is read
+  uint32_t st_stride;     // distance between a system's two SysState copies
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // kernel timing (optional)
 };
 constexpr uint32_t kSpmvVariantCount = 4;
@@ -28,8 +30,7 @@ constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pas
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
-void launch_pcg_reduce(const LaunchSet& ls, hipStream_t s);
-void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s);
+void launch_pcg_finish(const LaunchSet& ls, hipStream_t s);
 void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t n_tiles,
                        hipStream_t s);
 

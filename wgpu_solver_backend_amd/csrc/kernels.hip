@@ -9,9 +9,10 @@
 //                 system's last tile reduces the partials and sets
 //                 alpha = rz_old / p.q on the device.
 //   k_pcg_update  x += alpha p, r -= alpha q, z = M^-1 r (6x6 LU solves,
-//                 factors staged in LDS), r.r and r.z partials; the last
-//                 tile computes beta and the stop test and writes status.
-//   k_pcg_pupdate p = z + beta p.
+//                 factors staged in LDS), r.r and r.z partials.
+//   k_pcg_finish  every workgroup reduces r.r and r.z, computes beta, the
+//                 stop test and the state transition (one workgroup writes
+//                 it), then p = z + beta p over its rows.
 //
 // The same three kernels run the init step (r0 = b - A x0 ...) and, for
 // iterative refinement, the FP64 residual / restart / fold steps, chosen
@@ -237,18 +238,18 @@ struct TileRef {
 // (kOne kernels: tile = workgroup, no table or descriptor load before the
 // first data load); otherwise `tiles` maps workgroups to (system, tile) and
 // `descs` holds the systems.
+//
+// Each system has two SysState copies, st and st + st_stride (st_stride =
+// the engine's system capacity): the kernels of an iteration with parity
+// `par` read copy par, and only k_pcg_finish's state writer writes the next
+// iteration's state into copy par ^ 1, so no kernel writes state that
+// another workgroup of it reads.
 struct TileArgs {
   SysDesc one;
   const SysDesc* descs;
   const uint2* tiles;
   uint32_t n_tiles, remap;
-};
-
-// Arguments of the per-system reduce kernels (one workgroup per system).
-struct SysArgs {
-  SysDesc one;
-  const SysDesc* descs;
-  uint32_t n_sys;
+  uint32_t par, st_stride;
 };
 
 template <bool kOne>
@@ -713,21 +714,24 @@ __device__ __forceinline__ bool halo_row(uint32_t row0, uint32_t nrows, uint32_t
 
 // ------------------------------------------------------------ PCG kernels
 //
-// Per iteration four launches:
+// Per iteration three launches:
 //   k_pcg_spmv     (tiles)   q = A p and the tile's p.Ap partial [init:
 //                            q = A x0 and b.b; refinement: the FP64 residual]
 //   k_pcg_update   (tiles)   every workgroup reduces the p.Ap partials itself
 //                            (the same tree, so alpha is bit-identical in all
 //                            of them), then x += alpha p, r -= alpha q,
 //                            z = M^-1 r and the tile's r.r and r.z partials
-//   k_pcg_reduce   (systems) beta, breakdown checks, stop test and the state
-//                            transitions, in the reference's order
-//   k_pcg_pupdate  (tiles)   p = z + beta p [refinement: the fold]
+//   k_pcg_finish   (tiles)   every workgroup reduces r.r and r.z itself, then
+//                            beta, breakdown checks, stop test and the state
+//                            transition in the reference's order; then
+//                            p = z + beta p [refinement: the fold]
 // Tiles write plain per-tile partials that later kernels read after the
 // kernel boundary; no workgroup waits on or synchronises with another.
 // Within a kernel no workgroup writes state another workgroup of it reads:
-// k_pcg_update's workgroup 0 records alpha, p.Ap and ||b||^2 for
-// k_pcg_reduce, which alone moves the state machine.
+// the state is double-buffered by iteration parity, every kernel reads copy
+// `par`, k_pcg_update's workgroup 0 records alpha, p.Ap and ||b||^2 into it
+// (fields no workgroup of k_pcg_update reads), and k_pcg_finish's tile-0
+// workgroup writes the next state into copy par ^ 1.
 //
 // Every kernel issues the loads that do not depend on the system's state
 // (row bounds, own-row vector entries, factors, partials) before it reads
@@ -747,7 +751,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   TileRef tr;
   if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
-  const SysState* st = d.st;
+  const SysState* st = d.st + ta.par * ta.st_stride;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
   uint32_t mb, me;
@@ -823,8 +827,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
 
 // Refinement decision from the FP64 residual partials (k_pcg_spmv, phase
 // kPhIrResid): 0 = converged, 5 = out of outer steps, kRunning = restart the
-// inner solve on r64 * (1/||r64||).  Computed identically by k_pcg_update
-// (every workgroup) and k_pcg_reduce.
+// inner solve on r64 * (1/||r64||).  Computed identically by every workgroup
+// of k_pcg_update and of k_pcg_finish.
 __device__ __forceinline__ int32_t ir_decide(const SysDesc& d, double target2, uint32_t outer,
                                              uint32_t max_outer, double* lds_a, double* lds_b,
                                              double& rr, double& bb) {
@@ -855,7 +859,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   TileRef tr;
   if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
-  SysState* st = d.st;
+  SysState* st = d.st + ta.par * ta.st_stride;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, nt = d.n_tiles;
   const bool own = t < tr.nrows;
   const uint32_t rowc = own ? row : tr.row0;
@@ -976,142 +980,193 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   }
 }
 
-// After k_pcg_update: r.z (and r.r) -> beta, the breakdown checks and the
-// stop test in the reference's order, the refinement bookkeeping, and the
-// swap of the double-buffered r.  One workgroup per system.  The r.r and r.z
-// partials are fetched together with the state.
-template <bool kOne>
-__global__ __launch_bounds__(kTile) void k_pcg_reduce(const SysArgs sa) {
-  __shared__ double s_a[256];
-  __shared__ double s_b[256];
-  if (blockIdx.x >= sa.n_sys) return;
-  const SysDesc d = kOne ? sa.one : sa.descs[blockIdx.x];
-  SysState* st = d.st;
-  if (!st) return;  // empty serving slot
-  const uint32_t nt = d.n_tiles;
-  const bool dot64 = (d.flags & kFlagDot64) != 0;
-  const bool pre = nt > 1u && nt <= kFetchMax;
-  float af[4][4], bf[4][4];
-  double ad[4][4], bd[4][4];
-  if (pre) {
-    if (dot64) {
-      level_fetch<double>(d.part64 + nt, nt, ad);
-      level_fetch<double>(d.part64 + 2u * nt, nt, bd);
-    } else {
-      level_fetch<float>(d.part + nt, nt, af);
-      level_fetch<float>(d.part + 2u * nt, nt, bf);
-    }
-  }
-  const int32_t status = st->status, phase = st->phase;
-  if (status != kRunning) return;  // finished system
-  if (phase == kPhInit && st->b_norm2 == 0.0f) {
-    if (threadIdx.x == 0) st->status = 0;  // Ok(0), x untouched (compute.rs:252-254)
+// The state transition at the end of an iteration, in the reference's
+// order: the late scalar update (pcg_update_scalars.wgsl:64-67), then the
+// breakdown checks and the stop test (compute.rs:468-488); at init
+// compute.rs:316-330; the refinement bookkeeping.  `ns` holds the
+// iteration's state on entry (with alpha, p.Ap and ||b||^2 recorded by
+// k_pcg_update) and the next iteration's on return.  rr, rz (rz64): this
+// iteration's r.r and r.z; dec, rr64, bb64: the refinement decision (phase
+// kPhIrResid; kRunning = restart) and the residual's norms.
+__device__ __forceinline__ void pcg_transition(SysState& ns, uint32_t flags, float rr, float rz,
+                                               double rz64, int32_t dec, double rr64, double bb64) {
+  const int32_t phase = ns.phase;
+  if (phase == kPhInit && ns.b_norm2 == 0.0f) {
+    ns.status = 0;  // Ok(0), x untouched (compute.rs:252-254)
     return;
   }
   if (phase == kPhIrResid) {
-    double rr64, bb64;
-    const int32_t dec = ir_decide(d, st->target2, st->outer, st->max_outer, s_a, s_b, rr64, bb64);
-    __syncthreads();  // s_a / s_b are reused below
-    if (threadIdx.x == 0) {
-      st->rr64 = rr64;
-      st->bb64 = bb64;
-    }
+    ns.rr64 = rr64;
+    ns.bb64 = bb64;
     if (dec != kRunning) {
-      if (threadIdx.x == 0) st->status = dec;
+      ns.status = dec;
       return;
     }
-    if (threadIdx.x == 0) {
-      const double nr = sqrt(rr64);
-      st->scale = 1.0 / nr;
-      st->inv_scale = nr;
-      st->outer += 1u;
-    }
+    const double nr = sqrt(rr64);
+    ns.scale = 1.0 / nr;
+    ns.inv_scale = nr;
+    ns.outer += 1u;
   }
-  float rr, rz;
-  double rz64 = 0.0;
-  if (dot64) {
-    rr = (float)(pre ? reduce_fetched<double>(ad, nt, s_a)
-                     : reduce_partials<double>(d.part64 + nt, nt, s_a));
-    rz64 = pre ? reduce_fetched<double>(bd, nt, s_b) : reduce_partials<double>(d.part64 + 2u * nt, nt, s_b);
-    rz = (float)rz64;
-  } else {
-    float* fa = reinterpret_cast<float*>(s_a);
-    float* fb = reinterpret_cast<float*>(s_b);
-    rr = pre ? reduce_fetched<float>(af, nt, fa) : reduce_partials<float>(d.part + nt, nt, fa);
-    rz = pre ? reduce_fetched<float>(bf, nt, fb) : reduce_partials<float>(d.part + 2u * nt, nt, fb);
-  }
-  if (threadIdx.x != 0) return;
-  float* r_in = st->r_in;
-  st->r_in = st->r_out;  // r of this iteration becomes the input of the next
-  st->r_out = r_in;
+  float* r_in = ns.r_in;
+  ns.r_in = ns.r_out;  // r of this iteration becomes the input of the next
+  ns.r_out = r_in;
   if (phase == kPhInit) {
-    st->rz_old = rz;  // compute.rs:316-330
-    st->rz64_old = rz64;
-    st->phase = kPhIter;
-    st->iter = 0;
-    if (st->max_iter == 0) st->status = 5;  // loop body never runs (compute.rs:336,491)
+    ns.rz_old = rz;  // compute.rs:316-330
+    ns.rz64_old = rz64;
+    ns.phase = kPhIter;
+    ns.iter = 0;
+    if (ns.max_iter == 0) ns.status = 5;  // loop body never runs (compute.rs:336,491)
     return;
   }
   if (phase == kPhIrResid) {
     // restart of the inner solve on b32 = r64 * scale (written by k_pcg_update)
-    st->rz_old = rz;
-    st->rz64_old = rz64;
-    st->b_norm2 = rr;  // ||b32||^2 of the restarted inner solve
-    st->phase = kPhIter;
-    st->iter = 0;
+    ns.rz_old = rz;
+    ns.rz64_old = rz64;
+    ns.b_norm2 = rr;  // ||b32||^2 of the restarted inner solve
+    ns.phase = kPhIter;
+    ns.iter = 0;
     return;
   }
-  // pcg_update_scalars.wgsl:64-67 (late update), then compute.rs:468-488
-  const float rz_old = st->rz_old, p_ap = st->p_ap;
+  const float rz_old = ns.rz_old, p_ap = ns.p_ap;
   float beta = rz_old != 0.0f ? rz / rz_old : 0.0f;
-  if (dot64) beta = st->rz64_old != 0.0 ? (float)(rz64 / st->rz64_old) : 0.0f;
-  st->beta = beta;
-  st->rz_new = rz;
-  st->rr = rr;
-  const uint32_t it = st->iter + 1u;
-  st->iter = it;
-  st->total_iter += 1u;
+  if (flags & kFlagDot64) beta = ns.rz64_old != 0.0 ? (float)(rz64 / ns.rz64_old) : 0.0f;
+  ns.beta = beta;
+  ns.rz_new = rz;
+  ns.rr = rr;
+  const uint32_t it = ns.iter + 1u;
+  ns.iter = it;
+  ns.total_iter += 1u;
   int32_t fin = kRunning;
   if (p_ap == 0.0f) fin = 3;            // HBJ_E_BREAKDOWN_PAP
   else if (rz_old == 0.0f) fin = 4;     // HBJ_E_BREAKDOWN_RZ
-  else if (rr <= st->abs_tol2 || rr <= st->rel_tol2 * st->b_norm2) fin = 0;
-  else if (it >= st->max_iter) fin = 5;  // HBJ_E_NOT_CONVERGED
+  else if (rr <= ns.abs_tol2 || rr <= ns.rel_tol2 * ns.b_norm2) fin = 0;
+  else if (it >= ns.max_iter) fin = 5;  // HBJ_E_NOT_CONVERGED
   if (fin == kRunning) {
-    st->rz_old = rz;
-    st->rz64_old = rz64;
-  } else if (d.flags & kFlagIr) {
-    st->phase = kPhIrResid;  // k_pcg_pupdate folds the correction; then a new FP64 residual
+    ns.rz_old = rz;
+    ns.rz64_old = rz64;
+  } else if (flags & kFlagIr) {
+    ns.phase = kPhIrResid;  // fold the correction (below), then a new FP64 residual
   } else {
-    st->status = fin;
+    ns.status = fin;
   }
 }
 
-// After k_pcg_reduce: p = (p * beta) + z, two roundings
+// Tiles per workgroup of k_pcg_finish for a single system.
+constexpr uint32_t kFinishTiles = 4;
+
+// After k_pcg_update.  Every workgroup reduces r.r and r.z (and, at a
+// refinement decision, the residual's norms) from the partials -- the same
+// trees, so the same bits -- and computes the state transition; the
+// workgroup holding the system's tile 0 writes the next iteration's state
+// (a final state into both copies: a finished system may get no more
+// workgroups once its tiles are dropped from the table).  Then, over the
+// workgroup's rows: p = (p * beta) + z, two roundings
 // (scale_from_scalar_results.wgsl:37, axpy.wgsl:41 with alpha = 1), or p = z
 // on the first iteration (compute.rs:313-314); after an inner solve of the
 // refinement (phase just set to kPhIrResid) the fold x64 += x32 / scale.
-template <bool kOne>
-__global__ __launch_bounds__(kTile) void k_pcg_pupdate(const TileArgs ta) {
+// kOne: workgroup b covers tiles b + k * gridDim.x, k < kFinishTiles.
+template <bool kOne, bool kDot64>
+__global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
+  __shared__ double s_a[256];
+  __shared__ double s_b[256];
+  __shared__ SysState s_ns;
+  constexpr uint32_t kWords = sizeof(SysState) / 4;
+  static_assert(kWords <= (uint32_t)kTile && sizeof(SysState) % 4 == 0, "state copy by dwords");
   TileRef tr;
-  if (!get_tile<kOne>(ta, tr)) return;
+  uint32_t step = 0;
+  if constexpr (kOne) {
+    if (blockIdx.x >= ta.n_tiles) return;
+    tr.d = ta.one;
+    tr.tile = blockIdx.x;
+    step = gridDim.x;
+  } else {
+    if (!get_tile<false>(ta, tr)) return;
+  }
   const SysDesc& d = tr.d;
-  const SysState* st = d.st;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t;
-  if (t >= tr.nrows) return;
-  const float zi = bld(d.z, row), pi = bld(d.p, row);
-  const int32_t status = st->status, phase = st->phase;
-  const uint32_t iter = st->iter;
-  const float beta = st->beta;
-  if (status != kRunning) return;
-  if (phase == kPhIter) {
-    if (iter == 0u) {
-      bst(d.p, row, zi);
-    } else {
-      const float pb = pi * beta;
-      bst(d.p, row, pb + zi);
+  SysState* in = d.st + ta.par * ta.st_stride;
+  SysState* out = d.st + (ta.par ^ 1u) * ta.st_stride;
+  const uint32_t t = threadIdx.x, nt = d.n_tiles;
+  const bool writer = tr.tile == 0;
+  // r.r and r.z partials fetched together with the state
+  using TP = typename std::conditional<kDot64, double, float>::type;
+  const TP* part = kDot64 ? reinterpret_cast<const TP*>(d.part64) : reinterpret_cast<const TP*>(d.part);
+  const bool pre = nt > 1u && nt <= kFetchMax;
+  TP va[4][4], vb[4][4];
+  if (pre) {
+    level_fetch<TP>(part + nt, nt, va);
+    level_fetch<TP>(part + 2u * nt, nt, vb);
+  }
+  if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(in)[t];
+  __syncthreads();
+  const int32_t status = s_ns.status, phase = s_ns.phase;
+  if (status != kRunning) {  // finished: carry the state to the other copy
+    if (writer && t < kWords)
+      reinterpret_cast<uint32_t*>(out)[t] = reinterpret_cast<const uint32_t*>(&s_ns)[t];
+    return;
+  }
+  int32_t dec = kRunning;
+  double rr64 = 0.0, bb64 = 0.0;
+  if (phase == kPhIrResid) {
+    dec = ir_decide(d, s_ns.target2, s_ns.outer, s_ns.max_outer, s_a, s_b, rr64, bb64);
+    __syncthreads();  // s_a / s_b are reused below
+  }
+  float rr = 0.0f, rz = 0.0f;
+  double rz64 = 0.0;
+  const bool reduce = !(phase == kPhInit && s_ns.b_norm2 == 0.0f) &&
+                      !(phase == kPhIrResid && dec != kRunning);
+  if (reduce) {
+    TP* la = reinterpret_cast<TP*>(s_a);
+    TP* lb = reinterpret_cast<TP*>(s_b);
+    const TP a = pre ? reduce_fetched<TP>(va, nt, la) : reduce_partials<TP>(part + nt, nt, la);
+    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
+    rr = (float)a;
+    rz = (float)b;
+    if constexpr (kDot64) rz64 = b;
+  }
+  if (t == 0) pcg_transition(s_ns, d.flags, rr, rz, rz64, dec, rr64, bb64);
+  __syncthreads();
+  const int32_t nstatus = s_ns.status, nphase = s_ns.phase;
+  const uint32_t niter = s_ns.iter;
+  const float beta = s_ns.beta;
+  const double inv_scale = s_ns.inv_scale;
+  if (writer && t < kWords) {
+    const uint32_t w = reinterpret_cast<const uint32_t*>(&s_ns)[t];
+    reinterpret_cast<uint32_t*>(out)[t] = w;
+    if (nstatus != kRunning) reinterpret_cast<uint32_t*>(in)[t] = w;
+  }
+  if (nstatus != kRunning) return;
+  if (nphase == kPhIter) {
+    // all of the workgroup's loads first (rows past the system read row 0)
+    constexpr uint32_t kT = kOne ? kFinishTiles : 1u;
+    float zv[kT], pv[kT];
+    uint32_t rw[kT];
+#pragma unroll
+    for (uint32_t k = 0; k < kT; ++k) {
+      const uint32_t tile = tr.tile + k * step;
+      const uint32_t row = tile * (uint32_t)kTile + t;
+      rw[k] = (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
+      const uint32_t rc = rw[k] != 0xFFFFFFFFu ? row : 0u;
+      zv[k] = bld(d.z, rc);
+      pv[k] = bld(d.p, rc);
     }
-  } else if (phase == kPhIrResid) {
-    bst(d.x64, row, bld(d.x64, row) + (double)bld(d.x, row) * st->inv_scale);
+#pragma unroll
+    for (uint32_t k = 0; k < kT; ++k) {
+      if (rw[k] == 0xFFFFFFFFu) continue;
+      if (niter == 0u) {
+        bst(d.p, rw[k], zv[k]);
+      } else {
+        const float pb = pv[k] * beta;
+        bst(d.p, rw[k], pb + zv[k]);
+      }
+    }
+  } else if (nphase == kPhIrResid) {
+    const uint32_t kT = kOne ? kFinishTiles : 1u;
+    for (uint32_t k = 0; k < kT; ++k) {
+      const uint32_t tile = tr.tile + k * step;
+      const uint32_t row = tile * (uint32_t)kTile + t;
+      if (tile < nt && row < d.n)
+        bst(d.x64, row, bld(d.x64, row) + (double)bld(d.x, row) * inv_scale);
+    }
   }
 }
 
@@ -1276,13 +1331,8 @@ static TileArgs tile_args(const LaunchSet& ls) {
   a.tiles = ls.tiles;
   a.n_tiles = ls.n_tiles;
   a.remap = ls.remap;
-  return a;
-}
-static SysArgs sys_args(const LaunchSet& ls) {
-  SysArgs a{};
-  if (ls.one) a.one = *ls.one;
-  a.descs = ls.descs;
-  a.n_sys = ls.n_sys;
+  a.par = ls.par;
+  a.st_stride = ls.st_stride;
   return a;
 }
 
@@ -1314,15 +1364,19 @@ void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
     else launch(ls, k_pcg_update<false, false>, g, b, s, a);
   }
 }
-void launch_pcg_pupdate(const LaunchSet& ls, hipStream_t s) {
-  const TileArgs a = tile_args(ls);
-  if (ls.one) launch(ls, k_pcg_pupdate<true>, dim3(ls.n_tiles), dim3(kTile), s, a);
-  else launch(ls, k_pcg_pupdate<false>, dim3(ls.n_tiles), dim3(kTile), s, a);
+template <bool kDot64>
+static void launch_finish_t(const LaunchSet& ls, const TileArgs& a, hipStream_t s) {
+  if (ls.one) {
+    const uint32_t g = (ls.n_tiles + kFinishTiles - 1) / kFinishTiles;
+    launch(ls, k_pcg_finish<true, kDot64>, dim3(g ? g : 1), dim3(kTile), s, a);
+  } else {
+    launch(ls, k_pcg_finish<false, kDot64>, dim3(ls.n_tiles), dim3(kTile), s, a);
+  }
 }
-void launch_pcg_reduce(const LaunchSet& ls, hipStream_t s) {
-  const SysArgs a = sys_args(ls);
-  if (ls.one) launch(ls, k_pcg_reduce<true>, dim3(1), dim3(kTile), s, a);
-  else launch(ls, k_pcg_reduce<false>, dim3(ls.n_sys), dim3(kTile), s, a);
+void launch_pcg_finish(const LaunchSet& ls, hipStream_t s) {
+  const TileArgs a = tile_args(ls);
+  if (ls.dot64) launch_finish_t<true>(ls, a, s);
+  else launch_finish_t<false>(ls, a, s);
 }
 
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {

@@ -286,6 +286,8 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   S->ctx = ctx;
   S->cfg = *cfg;
   if (S->cfg.chunk_iters == 0) S->cfg.chunk_iters = 16;
+  // even: a system's state copy 0 is current at every chunk boundary
+  S->cfg.chunk_iters = S->cfg.chunk_iters < 2 ? 2u : (S->cfg.chunk_iters & ~1u);
   hipError_t e = hipSetDevice(ctx->device);
   hbj_status s = e == hipSuccess ? S->eng.init(ctx->device, cfg->max_active, 1) : HBJ_E_DEVICE;
   if (s == HBJ_OK) {

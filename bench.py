@@ -326,8 +326,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=50)
-    ap.add_argument("--sample-every", type=int, default=8,
-                    help="HIP-event timing of every k-th PCG iteration's kernels")
+    ap.add_argument("--sample-every", type=int, default=32,
+                    help="HIP-event timing of every k-th PCG iteration's kernels (inside the "
+                         "timed solves; each sample costs a few microseconds)")
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])

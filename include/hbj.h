@@ -165,9 +165,10 @@ hbj_status hbj_system_get_info(const hbj_system* sys, hbj_system_info* out);
  * not depend on it.  No reference counterpart. */
 hbj_status hbj_system_set_spmv_variant(hbj_system* sys, uint32_t variant);
 /* Autotuner: runs `iters` PCG iterations from the current right-hand side
- * with each SpMV variant (after one untimed run each), keeps the fastest for
- * later solves and returns it in *best; us_per_iter[HBJ_SPMV_VARIANTS] (may
- * be null) receives each variant's mean time per iteration.  Overwrites the
+ * with each SpMV variant (one untimed run, then the best of three), keeps the
+ * fastest for later solves and returns it in *best; us_per_iter
+ * [HBJ_SPMV_VARIANTS] (may be null) receives each variant's time per
+ * iteration.  Overwrites the
  * system's solution vector, not its right-hand side or x0.  No reference
  * counterpart. */
 hbj_status hbj_system_autotune(hbj_system* sys, uint32_t iters, uint32_t* best,

@@ -101,6 +101,9 @@ def test_resident_api_and_warm_start(ctx, hbj, oracle):
     avg, cnt = sys_.kernel_timing()
     assert res_t.iterations == it and np.array_equal(bits(sys_.get_x()), bits(xo))
     assert np.all(cnt > 0) and np.all(avg > 0)
+    # samples of launch sets 0, 4, 8, ... that found the system running: the
+    # init set and one per iteration (the chunk's trailing launches drop out)
+    assert np.all(cnt == -(-(it + 1) // 4)), (cnt, it)
     sys_.set_kernel_timing(0)
 
 

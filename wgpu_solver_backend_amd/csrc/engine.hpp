@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -120,7 +121,11 @@ struct Engine {
   uint32_t timing_every = 0;
   uint64_t trip = 0;  // iterations enqueued so far (for sampling)
   std::vector<hipEvent_t> tev[2];
-  std::vector<int> tev_used;  // per pool: number of 4-event groups recorded
+  std::vector<int> tev_used;  // per pool: number of sampled launch sets recorded
+  std::vector<uint64_t> tev_trip[2];  // per pool: each sample's launch set within the run
+  // harvested samples of the current run: launch set, then one time per kernel
+  std::vector<std::array<double, HBJ_ITER_KERNELS + 1>> t_pending;
+  uint64_t run_trip0 = 0;     // `trip` when the current run started
   static constexpr int kKernels = HBJ_ITER_KERNELS;  // spmv, update, finish
   double t_sum[kKernels] = {0, 0, 0};
   uint64_t t_cnt[kKernels] = {0, 0, 0};
@@ -282,9 +287,10 @@ struct Engine {
     if (timing_every) {
       CHK(ensure_timing_events(T));
       int used = 0;
+      tev_trip[pool].clear();
       const LaunchSet l = ls();
       for (uint32_t i = 0; i < T; ++i, ++trip) {
-        if (trip % timing_every == 0) {
+        if ((trip - run_trip0) % timing_every == 0) {
           // a start/stop pair per kernel, set by the kernel's own dispatch
           hipEvent_t* e = &tev[pool][2 * kKernels * used];
           LaunchSet lt = l;
@@ -297,6 +303,7 @@ struct Engine {
           timed(0, launch_pcg_spmv);
           timed(1, launch_pcg_update);
           timed(2, launch_pcg_finish);
+          tev_trip[pool].push_back(trip - run_trip0);
           ++used;
         } else {
           launch_iter(i & 1u);
@@ -319,15 +326,38 @@ struct Engine {
   hbj_status harvest_timing(int pool) {
     for (int u = 0; u < tev_used[pool]; ++u) {
       hipEvent_t* e = &tev[pool][2 * kKernels * u];
+      std::array<double, kKernels + 1> smp;
+      smp[0] = (double)tev_trip[pool][u];
       for (int k = 0; k < kKernels; ++k) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e[2 * k], e[2 * k + 1]));
-        t_sum[k] += ms * 1000.0;
-        t_cnt[k] += 1;
+        smp[k + 1] = ms * 1000.0;
       }
+      t_pending.push_back(smp);
     }
     tev_used[pool] = 0;
     return HBJ_OK;
+  }
+
+  // Keeps the samples of launch sets in which some system still ran: a
+  // system's `sets` counts the launch sets that found it running, so the
+  // run's live sets are [0, max over systems of the sets it added).  Later
+  // sampled launches (the rest of the chunk that finished the last system,
+  // and the chunk queued behind it) exit at once and would bias the means.
+  void commit_timing(const SysState* init, const SysState* out, uint32_t n) {
+    uint64_t live = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint64_t s = (uint64_t)(out[i].sets - init[i].sets);
+      live = s > live ? s : live;
+    }
+    for (const auto& smp : t_pending) {
+      if ((uint64_t)smp[0] >= live) continue;
+      for (int k = 0; k < kKernels; ++k) {
+        t_sum[k] += smp[k + 1];
+        t_cnt[k] += 1;
+      }
+    }
+    t_pending.clear();
   }
 
   static bool all_done(const SysState* s, uint32_t n) {
@@ -368,6 +398,8 @@ struct Engine {
     const double t0 = now_ms();
     uint32_t c = 0;
     int final_pool = -1;
+    run_trip0 = trip;
+    t_pending.clear();
     for (;;) {
       if (c >= max_chunks) {
         HIPCHK(hipStreamSynchronize(stream));
@@ -401,6 +433,7 @@ struct Engine {
     if (timing_every) CHK(harvest_timing(final_pool));
     if (ms) *ms = now_ms() - t0;
     std::memcpy(out, h_poll[final_pool], sizeof(SysState) * n_sys);
+    if (timing_every) commit_timing(init, out, n_sys);
     return HBJ_OK;
   }
 

@@ -63,7 +63,8 @@ struct alignas(128) SysState {
   float* r_out;        // r written this iteration
   // iterative refinement
   uint32_t outer, max_outer;
-  uint32_t total_iter, pad0;
+  uint32_t total_iter;
+  uint32_t sets;       // launch sets (spmv, update, finish) that found the system running
   double target2;      // (target relative residual)^2
   double rr64, bb64;   // ||r64||^2, ||b64||^2 of the last residual
   double scale;        // b32 = r64 * scale

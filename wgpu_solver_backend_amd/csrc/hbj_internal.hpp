@@ -25,6 +25,8 @@ struct LaunchSet {
   uint32_t st_stride;     // distance between a system's two SysState copies
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // kernel timing (optional)
 };
+// SpMV variants: 0 dword loads with HBM gathers, 1 16-byte loads with HBM
+// gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass.
 constexpr uint32_t kSpmvVariantCount = 4;
 constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pass
 

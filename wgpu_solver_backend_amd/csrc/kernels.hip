@@ -229,6 +229,38 @@ __device__ T reduce_partials(const T* __restrict__ part, uint32_t len, T* lds,
   return lds[0];
 }
 
+// Values the code knows to be equal in every lane (a system's state, a
+// tile's block range), moved to scalar registers.  The compiler cannot always
+// prove it (a load from memory the kernel also writes, an LDS read), and a
+// buffer descriptor or branch built from a value it takes as divergent turns
+// into a per-lane loop.
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ int32_t uni(int32_t v) {
+  return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+// (the builtin returns int: each half goes back through uint32_t so that
+// widening to 64 bits cannot sign-extend it)
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint64_t lo = uni((uint32_t)a);
+  const uint64_t hi = uni((uint32_t)(a >> 32));
+  return reinterpret_cast<T*>((hi << 32) | lo);
+}
+// A struct read from memory, every dword moved to scalar registers.
+template <typename T>
+__device__ __forceinline__ T uni_copy(const T& src) {
+  static_assert(sizeof(T) % 4 == 0, "dword copy");
+  T out;
+  const uint32_t* in = reinterpret_cast<const uint32_t*>(&src);
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = uni(in[i]);
+  return out;
+}
+
 struct TileRef {
   SysDesc d;
   uint32_t tile, row0, nrows;
@@ -252,6 +284,8 @@ struct TileArgs {
   uint32_t par, st_stride;
 };
 
+// This workgroup's table entry (blockIdx.x, or with `remap` its
+// XCD-contiguous renumbering); false past the table.
 template <bool kOne>
 __device__ __forceinline__ bool get_tile(const TileArgs& a, TileRef& tr) {
   const uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -261,8 +295,8 @@ __device__ __forceinline__ bool get_tile(const TileArgs& a, TileRef& tr) {
     tr.tile = b;
   } else {
     const uint2 td = a.tiles[b];
-    tr.d = a.descs[td.x];
-    tr.tile = td.y;
+    tr.d = uni_copy(a.descs[uni(td.x)]);
+    tr.tile = uni(td.y);
   }
   tr.row0 = tr.tile * (uint32_t)kTile;
   const uint32_t left = tr.d.n - tr.row0;
@@ -322,6 +356,7 @@ __device__ __forceinline__ double buf_gather<double>(__amdgpu_buffer_rsrc_t r, u
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
+
 // Element access through a raw buffer resource over a whole array: the base
 // stays in scalar registers and a lane supplies one 32-bit byte offset
 // instead of a 64-bit address (fewer vector registers; arrays hold fewer
@@ -346,11 +381,13 @@ __device__ __forceinline__ void bst(T* base, uint32_t i, T v) {
 // The wave's entry range [wb, we); false when it has no rows or no entries.
 __device__ __forceinline__ bool wave_range(uint32_t mb, uint32_t me, uint32_t nrows, uint32_t& wb,
                                            uint32_t& we) {
-  const uint32_t wr0 = (threadIdx.x >> 6) * 64u;
+  const uint32_t wr0 = uni((uint32_t)(threadIdx.x >> 6)) * 64u;
   if (wr0 >= nrows) return false;
   const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
-  wb = __shfl(mb, 0, 64);
-  we = __shfl(me, (int)wrows - 1, 64);
+  // lane reads into scalar registers: the range is wave-uniform by
+  // construction, and buffer descriptors built from it must be too
+  wb = uni(mb);
+  we = (uint32_t)__builtin_amdgcn_readlane((int)me, (int)wrows - 1);
   return wb < we;
 }
 
@@ -738,25 +775,29 @@ __device__ __forceinline__ bool halo_row(uint32_t row0, uint32_t nrows, uint32_t
 // SysState, so a workgroup waits for one memory round trip where the state
 // and its data would otherwise be two or three dependent ones.
 
-template <int V, bool kOne>
-__global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
-  // per-wave product slices: SpmvCfg<V>::kSlice floats, or 64*kPer/2 doubles
-  constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
-  constexpr bool kLocal = V == 2 || V == 3;
+// per-wave product slices: SpmvCfg<V>::kSlice floats, or 64*kPer/2 doubles
+template <int V>
+struct SpmvLds {
+  static constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
+  static constexpr bool kLocal = V == 2 || V == 3;
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
-  __shared__ __align__(16) double s_prod[kWaves * kSliceF / 2];
-  __shared__ float s_v[kLocal ? kLocalMax : 1];  // tile-local copy of the multiplied vector
+};
+
+// One tile of k_pcg_spmv.
+template <int V>
+__device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr, double* s_prod,
+                                          float* s_v) {
+  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+  constexpr bool kLocal = SpmvLds<V>::kLocal;
   double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
   double* s_rb = s_prod + kTile;
-  TileRef tr;
-  if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
   const SysState* st = d.st + ta.par * ta.st_stride;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
   uint32_t mb, me;
   lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
-  const int32_t status = st->status, phase = st->phase;
+  const int32_t status = uni(st->status), phase = uni(st->phase);
   if (status != kRunning) return;
 
   if (phase == kPhInit || phase == kPhIter) {
@@ -769,8 +810,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
     if constexpr (kLocal) {
       // workgroup-uniform: keep it in scalar registers (uniform branch)
       const uint2 tl0 = d.tloc[tr.tile];
-      const uint2 tl = make_uint2(__builtin_amdgcn_readfirstlane(tl0.x),
-                                  __builtin_amdgcn_readfirstlane(tl0.y));
+      const uint2 tl = make_uint2(uni(tl0.x), uni(tl0.y));
       if (tl.y != kLocalOff) {
         q = spmv_wave_local<V == 2 ? 2 : 4>(mb, me, d.lidx, d.val, tr.nrows, s_v, slice, [&]() {
           stage_local(d.tcols, tl.x, tl.y, vec_at, s_v);
@@ -825,6 +865,21 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   }
 }
 
+// One workgroup per table entry.  (A persistent grid in which every
+// workgroup loops over the same number of tiles removes the partly occupied
+// last round, but the loop keeps lane-index arithmetic live across tiles --
+// 48 -> 80 VGPRs for variant 2 -- and each workgroup waits for its tiles'
+// staging one after the other; measured slower, 38.3 vs 36.8 us per launch
+// on the 50^3 lattice.)
+template <int V, bool kOne>
+__global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
+  __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  TileRef tr;
+  if (!get_tile<kOne>(ta, tr)) return;
+  spmv_tile<V>(ta, tr, s_prod, s_v);
+}
+
 // Refinement decision from the FP64 residual partials (k_pcg_spmv, phase
 // kPhIrResid): 0 = converged, 5 = out of outer steps, kRunning = restart the
 // inner solve on r64 * (1/||r64||).  Computed identically by every workgroup
@@ -876,12 +931,12 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   // rows no block writes keep z
   const float zi_old = (d.flags & kFlagOversize) ? bld(d.z, rowc) : 0.0f;
   const uint2 blk = d.tile_blk ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
-  const int32_t status = st->status, phase = st->phase;
-  const float* __restrict__ r_in = st->r_in;
-  float* __restrict__ r_out = st->r_out;
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  const float* __restrict__ r_in = uni(st->r_in);
+  float* __restrict__ r_out = uni(st->r_out);
   const float rz_old = st->rz_old;
   const double rz64_old = st->rz64_old, target2 = st->target2;
-  const uint32_t outer = st->outer, max_outer = st->max_outer;
+  const uint32_t outer = uni(st->outer), max_outer = uni(st->max_outer);
   if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrResid) return;
   const bool iter = phase == kPhIter, restart = phase == kPhIrResid;
@@ -921,15 +976,16 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
 
   // round 2: r source (r_in per iteration, b at init; r64 * scale = b32 at
   // a refinement restart), p, block starts, the first factor group
-  const float* rsrc_arr = iter ? r_in : d.b;
+  const float* rsrc_arr = uni(iter ? r_in : d.b);
   float rs = bld(rsrc_arr, rowc), rhs = bld(rsrc_arr, hrowc);
   if (restart) {
     rs = (float)(bld(d.r64, rowc) * scale);
     rhs = (float)(bld(d.r64, hrowc) * scale);
   }
   const float pi = bld(d.p, rowc);
-  const uint2 tb = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
-                                                  tr.nrows, s_blk);
+  const uint2 tb0 = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
+                                                   tr.nrows, s_blk);
+  const uint2 tb = make_uint2(uni(tb0.x), uni(tb0.y));
   const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
   const uint32_t bs_s = bld(d.bstart, bi), bs_e = bld(d.bstart, bi + 1u);
   bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
@@ -991,6 +1047,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
 __device__ __forceinline__ void pcg_transition(SysState& ns, uint32_t flags, float rr, float rz,
                                                double rz64, int32_t dec, double rr64, double bb64) {
   const int32_t phase = ns.phase;
+  ns.sets += 1u;
   if (phase == kPhInit && ns.b_norm2 == 0.0f) {
     ns.status = 0;  // Ok(0), x untouched (compute.rs:252-254)
     return;
@@ -1098,7 +1155,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   }
   if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(in)[t];
   __syncthreads();
-  const int32_t status = s_ns.status, phase = s_ns.phase;
+  const int32_t status = uni(s_ns.status), phase = uni(s_ns.phase);
   if (status != kRunning) {  // finished: carry the state to the other copy
     if (writer && t < kWords)
       reinterpret_cast<uint32_t*>(out)[t] = reinterpret_cast<const uint32_t*>(&s_ns)[t];
@@ -1125,8 +1182,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   }
   if (t == 0) pcg_transition(s_ns, d.flags, rr, rz, rz64, dec, rr64, bb64);
   __syncthreads();
-  const int32_t nstatus = s_ns.status, nphase = s_ns.phase;
-  const uint32_t niter = s_ns.iter;
+  const int32_t nstatus = uni(s_ns.status), nphase = uni(s_ns.phase);
+  const uint32_t niter = uni(s_ns.iter);
   const float beta = s_ns.beta;
   const double inv_scale = s_ns.inv_scale;
   if (writer && t < kWords) {
@@ -1339,10 +1396,9 @@ static TileArgs tile_args(const LaunchSet& ls) {
 template <int V>
 static void launch_spmv_v(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
-  if (ls.one)
-    launch(ls, k_pcg_spmv<V, true>, dim3(ls.n_tiles), dim3(kTile), s, a);
-  else
-    launch(ls, k_pcg_spmv<V, false>, dim3(ls.n_tiles), dim3(kTile), s, a);
+  const dim3 g(ls.n_tiles), b(kTile);
+  if (ls.one) launch(ls, k_pcg_spmv<V, true>, g, b, s, a);
+  else launch(ls, k_pcg_spmv<V, false>, g, b, s, a);
 }
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {

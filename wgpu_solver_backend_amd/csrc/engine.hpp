@@ -555,6 +555,8 @@ struct hbj_system {
     d.x64 = x64;
     d.b64 = b64;
     d.r64 = r64;
+    d.r[0] = ra;
+    d.r[1] = rb;
     d.tile_blk = tile_blk;
     d.lidx = lidx;
     d.tloc = tloc;
@@ -576,8 +578,6 @@ struct hbj_system {
     s.max_iter = ir ? ip->max_inner : max_iter;
     s.rel_tol2 = ir ? ip->inner_rel_tol * ip->inner_rel_tol : rel_tol * rel_tol;  // compute.rs:256
     s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257
-    s.r_in = ra;
-    s.r_out = rb;
     if (ir) {
       s.max_outer = ip->max_outer;
       s.target2 = ip->target_rel_res * ip->target_rel_res;

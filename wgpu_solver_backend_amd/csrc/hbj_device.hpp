@@ -49,6 +49,8 @@ enum Phase : int32_t {
 constexpr uint32_t kFlagOversize = 1u;  // some block has more than 6 rows
 constexpr uint32_t kFlagIr = 2u;        // iterative refinement enabled
 constexpr uint32_t kFlagDot64 = 4u;     // inner dots accumulated in FP64
+constexpr uint32_t kFlagBlocks6 = 8u;   // block b is rows [6b, min(6b + 6, n)): block
+                                        // ranges and starts are arithmetic
 
 struct alignas(128) SysState {
   // FP32 PCG scalars, the reference's mailbox slots (compute.rs:271-277)
@@ -59,8 +61,6 @@ struct alignas(128) SysState {
   int32_t phase;       // Phase
   uint32_t iter;       // iterations of the current (inner) PCG
   uint32_t max_iter;   // per inner PCG
-  float* r_in;         // r of the previous iteration (double-buffered)
-  float* r_out;        // r written this iteration
   // iterative refinement
   uint32_t outer, max_outer;
   uint32_t total_iter;
@@ -89,6 +89,8 @@ struct SysDesc {
   double* x64;               // IR solution
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual
+  float* r[2];               // r, double-buffered: a launch set of parity par reads
+                             // r[par] and writes r[par ^ 1]
   const uint2* tile_blk;     // per tile: first block touching it, number of such blocks
   const uint16_t* lidx;      // nnz (+8 zero padding): entry's index in its tile's column list
   const uint2* tloc;         // per tile: offset of its column list in tcols, length (or kLocalOff)

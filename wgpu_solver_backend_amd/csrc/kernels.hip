@@ -641,30 +641,28 @@ __device__ __forceinline__ uint2 tile_blocks(const uint32_t* __restrict__ bs,
 }
 
 // Copies the factors of blocks [bfirst+g, bfirst+g+cnt) (cnt <= kLuStage)
-// into s_lu (stride 37 words per block to spread LDS banks).  No barrier.
-// All of a thread's loads are issued before any is used: range-checked
-// buffer loads, so positions past the group read 0 and need no branch (a
-// load per loop trip would cost a memory round trip per trip).
+// into s_lu, kLuStride words per block in order, with buffer loads that
+// write LDS directly (gfx950 "buffer_load_dword ... lds": lane l of a wave
+// writes word M0 + 4 l), so the copy holds no registers and needs no store
+// instructions.  Range-checked: positions past the group write 0.  The
+// loads complete asynchronously; a reader first calls lds_dma_wait() and
+// then the barrier that publishes s_lu.  s_lu holds kLuPer * kTile words.
+constexpr int kLuPer = (kLuStage * kLuStride + kTile - 1) / kTile;
 __device__ __forceinline__ void bj_stage(const float* __restrict__ lu, uint32_t bfirst,
                                          uint32_t g, uint32_t cnt, float* s_lu) {
-  constexpr int kPerThread = (kLuStage * kLuStride + kTile - 1) / kTile;
   const uint32_t len = cnt * kLuStride;
   const __amdgpu_buffer_rsrc_t r = buf_rsrc(lu + (size_t)(bfirst + g) * kLuStride, len * 4u);
-  float v[kPerThread];
+  const uint32_t wave = uni((uint32_t)(threadIdx.x >> 6));
 #pragma unroll
-  for (int k = 0; k < kPerThread; ++k) {
+  for (int k = 0; k < kLuPer; ++k) {
     const uint32_t i = threadIdx.x + (uint32_t)k * kTile;
-    v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
-  }
-#pragma unroll
-  for (int k = 0; k < kPerThread; ++k) {
-    const uint32_t i = threadIdx.x + (uint32_t)k * kTile;
-    if (i < len) {
-      const uint32_t jb = i / kLuStride, kk = i - jb * kLuStride;
-      s_lu[jb * 37u + kk] = v[k];
-    }
+    auto* dst = (__attribute__((address_space(3))) void*)(s_lu + (uint32_t)k * kTile + wave * 64u);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, dst, 4, i * 4u, 0, 0, 0);
   }
 }
+// Waits for this thread's outstanding vector-memory loads, the LDS-writing
+// ones included (s_waitcnt vmcnt(0)).
+__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Rows on each side of a tile whose r a straddling block's solve reads: the
 // leading (at most 6) rows of a block that meets the tile lie within 5 rows
@@ -675,7 +673,8 @@ constexpr uint32_t kHalo = 8;
 // Thread t solves block blk.x+t (blk.y blocks meet the tile; [bs_s, bs_e)
 // are that block's start and end, loaded by the caller).  Factors are
 // staged through LDS in groups of kLuStage blocks, the first group by the
-// caller (bj_stage(..., 0, ...) before the barrier that publishes s_rh).
+// caller (bj_stage(..., 0, ...) and lds_dma_wait() before the barrier that
+// publishes s_rh).
 // s_rh[row + kHalo - row0] holds r for rows [row0-kHalo, row0+nrows+kHalo):
 // the tile's own rows and the halo rows of straddling blocks (recomputed by
 // the owner's formula, bit-identical to the owner's value).
@@ -688,6 +687,7 @@ __device__ __forceinline__ void bj_tile(const float* __restrict__ lu, uint2 blk,
     const uint32_t cnt = (nblk - g) < (uint32_t)kLuStage ? nblk - g : (uint32_t)kLuStage;
     if (g > 0) {
       bj_stage(lu, bfirst, g, cnt, s_lu);
+      lds_dma_wait();
       __syncthreads();
     }
     const uint32_t j = t - g;
@@ -696,27 +696,34 @@ __device__ __forceinline__ void bj_tile(const float* __restrict__ lu, uint2 blk,
       if (s < n && e > s) {  // block_jacobi.wgsl:74-76
         const uint32_t m = (e - s) < (uint32_t)kBlock ? e - s : (uint32_t)kBlock;
         if (s + m > row0) {
-          const float* L = s_lu + j * 37u;
+          // All 36 factors and 6 right-hand sides are read unconditionally
+          // (the LDS slots exist for any m; kHalo >= kBlock - 1 keeps R in
+          // s_rh), so the reads issue together instead of one guarded read
+          // at a time.  Rows i >= m compute values no row < m uses: the
+          // forward pass only reads y[jj < i], and the backward pass keeps
+          // the running sum for jj >= m.  Every row < m performs the same
+          // operations in the same order as the plain guarded loops.
+          const float* Ls = s_lu + j * (uint32_t)kLuStride;
           const float* R = s_rh + (s + kHalo - row0);
-          float y[kBlock], xx[kBlock];
+          float L[kBlock * kBlock], y[kBlock], xx[kBlock];
+#pragma unroll
+          for (int k = 0; k < kBlock * kBlock; ++k) L[k] = Ls[k];
 #pragma unroll
           for (int i = 0; i < kBlock; ++i) {
-            if ((uint32_t)i < m) {
-              float sum = R[i];
+            float sum = R[i];
 #pragma unroll
-              for (int jj = 0; jj < i; ++jj) sum = sum - L[i * 6 + jj] * y[jj];
-              y[i] = sum;
-            }
+            for (int jj = 0; jj < i; ++jj) sum = sum - L[i * 6 + jj] * y[jj];
+            y[i] = sum;
           }
 #pragma unroll
           for (int i = kBlock - 1; i >= 0; --i) {
-            if ((uint32_t)i < m) {
-              float sum = y[i];
+            float sum = y[i];
 #pragma unroll
-              for (int jj = i + 1; jj < kBlock; ++jj)
-                if ((uint32_t)jj < m) sum = sum - L[i * 6 + jj] * xx[jj];
-              xx[i] = sum / L[i * 6 + i];
+            for (int jj = i + 1; jj < kBlock; ++jj) {
+              const float u = sum - L[i * 6 + jj] * xx[jj];
+              sum = (uint32_t)jj < m ? u : sum;
             }
+            xx[i] = sum / L[i * 6 + i];
           }
 #pragma unroll
           for (int i = 0; i < kBlock; ++i) {
@@ -906,7 +913,7 @@ template <bool kOne, bool kDot64>
 __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
-  __shared__ float s_lu[kLuStage * 37];
+  __shared__ float s_lu[kLuPer * kTile];
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
   __shared__ uint32_t s_blk[2];
@@ -930,10 +937,29 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   const float xi = bld(d.x, rowc), qi = bld(d.q, rowc), hq = bld(d.q, hrowc);
   // rows no block writes keep z
   const float zi_old = (d.flags & kFlagOversize) ? bld(d.z, rowc) : 0.0f;
-  const uint2 blk = d.tile_blk ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
+  // r of the previous iteration (the source of an iteration's update), p
+  // (a select, not d.r[par]: a runtime index into the by-value descriptor
+  // would copy it to scratch memory)
+  const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
+  float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
+  const float ri_prev = bld(r_in, rowc), rh_prev = bld(r_in, hrowc);
+  const float pi = bld(d.p, rowc);
+  // blocks touching the tile, their row ranges and factors: arithmetic with
+  // 6-row blocks, so the factor loads go out now; otherwise after the table
+  // lookup below
+  const bool b6 = (d.flags & kFlagBlocks6) != 0u;
+  uint2 tb = make_uint2(0u, 0u);
+  uint32_t bs_s = 0, bs_e = 0;
+  if (b6) {
+    const uint32_t first = tr.row0 / 6u, last = (tr.row0 + tr.nrows - 1u) / 6u;
+    tb = make_uint2(first, last - first + 1u);
+    const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+    bs_s = 6u * bi;
+    bs_e = 6u * bi + 6u < d.n ? 6u * bi + 6u : d.n;
+    bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+  }
+  const uint2 blk = (!b6 && d.tile_blk) ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
   const int32_t status = uni(st->status), phase = uni(st->phase);
-  const float* __restrict__ r_in = uni(st->r_in);
-  float* __restrict__ r_out = uni(st->r_out);
   const float rz_old = st->rz_old;
   const double rz64_old = st->rz64_old, target2 = st->target2;
   const uint32_t outer = uni(st->outer), max_outer = uni(st->max_outer);
@@ -974,21 +1000,25 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __syncthreads();  // s_ra / s_rb are reused below
   const float na = -alpha;
 
-  // round 2: r source (r_in per iteration, b at init; r64 * scale = b32 at
-  // a refinement restart), p, block starts, the first factor group
-  const float* rsrc_arr = uni(iter ? r_in : d.b);
-  float rs = bld(rsrc_arr, rowc), rhs = bld(rsrc_arr, hrowc);
+  // round 2, only where the state chooses another r source (b at init;
+  // r64 * scale = b32 at a refinement restart) or the blocks are general
+  float rs = ri_prev, rhs = rh_prev;
   if (restart) {
     rs = (float)(bld(d.r64, rowc) * scale);
     rhs = (float)(bld(d.r64, hrowc) * scale);
+  } else if (!iter) {
+    rs = bld(d.b, rowc);
+    rhs = bld(d.b, hrowc);
   }
-  const float pi = bld(d.p, rowc);
-  const uint2 tb0 = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
-                                                   tr.nrows, s_blk);
-  const uint2 tb = make_uint2(uni(tb0.x), uni(tb0.y));
-  const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
-  const uint32_t bs_s = bld(d.bstart, bi), bs_e = bld(d.bstart, bi + 1u);
-  bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+  if (!b6) {
+    const uint2 tb0 = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
+                                                     tr.nrows, s_blk);
+    tb = make_uint2(uni(tb0.x), uni(tb0.y));
+    const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+    bs_s = bld(d.bstart, bi);
+    bs_e = bld(d.bstart, bi + 1u);
+    bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+  }
 
   // r of this iteration: compute.rs:395-403 (iteration), :294,301-308 (init)
   auto rnew = [&](float r, float q) -> float {
@@ -1009,6 +1039,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
     s_z[t] = zi_old;
   }
   if (halo) s_rh[hslot] = rnew(rhs, hq);
+  lds_dma_wait();  // the factor copy into s_lu
   __syncthreads();
   bj_tile(d.lu, tb, d.n, tr.row0, tr.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
   float zi = 0.0f;
@@ -1064,9 +1095,6 @@ __device__ __forceinline__ void pcg_transition(SysState& ns, uint32_t flags, flo
     ns.inv_scale = nr;
     ns.outer += 1u;
   }
-  float* r_in = ns.r_in;
-  ns.r_in = ns.r_out;  // r of this iteration becomes the input of the next
-  ns.r_out = r_in;
   if (phase == kPhInit) {
     ns.rz_old = rz;  // compute.rs:316-330
     ns.rz64_old = rz64;
@@ -1333,7 +1361,7 @@ __global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t 
                                                            float* __restrict__ z) {
   __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
-  __shared__ float s_lu[kLuStage * 37];
+  __shared__ float s_lu[kLuPer * kTile];
   __shared__ uint32_t s_blk[2];
   const uint32_t row0 = blockIdx.x * kTile;
   const uint32_t nrows = (n - row0) < (uint32_t)kTile ? n - row0 : (uint32_t)kTile;
@@ -1351,6 +1379,7 @@ __global__ __launch_bounds__(kTile) void k_op_block_jacobi(uint32_t n, uint32_t 
     bs_e = bs[blk.x + t + 1u];
   }
   bj_stage(lu, blk.x, 0, blk.y < (uint32_t)kLuStage ? blk.y : (uint32_t)kLuStage, s_lu);
+  lds_dma_wait();
   __syncthreads();
   bj_tile(lu, blk, n, row0, nrows, bs_s, bs_e, s_rh, s_z, s_lu);
   if (t < nrows) z[row0 + t] = s_z[t];

@@ -375,8 +375,12 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   s->nnz = nnz;
   s->nb = nb;
   s->n_tiles = (n + kTile - 1) / kTile;
-  for (uint32_t b = 0; b < nb; ++b)
+  bool blocks6 = nb > 0 && block_starts[nb] == n && n <= 6ull * nb && n > 6ull * (nb - 1);
+  for (uint32_t b = 0; b < nb; ++b) {
     if (block_starts[b + 1] - block_starts[b] > (uint32_t)kBlock) s->flags |= kFlagOversize;
+    if (block_starts[b] != 6ull * b) blocks6 = false;
+  }
+  if (blocks6) s->flags |= kFlagBlocks6;
   auto fail = [&](hbj_status st) {
     s->release();
     delete s;

@@ -56,16 +56,54 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane l receives lane l + OFF's 32-bit value, for the lanes a tree step
+// keeps (l < OFF): offsets 32 and 16 cross rows of 16 lanes with gfx950's
+// v_permlane32_swap / v_permlane16_swap (swapping a value with itself
+// leaves lane l + OFF's value in lane l of the second result), offsets 8..1
+// shift inside a row with DPP row_shl.  All VALU: no LDS round trip, unlike
+// a ds_bpermute shuffle.  Other lanes receive values no tree step uses.
+// kLean keeps ds_bpermute for offsets 32 and 16: a swap overwrites both of
+// its registers, so the compiler copies the value first, and a kernel whose
+// occupancy is set by its register count can lose a wave per SIMD to that.
+template <int OFF, bool kLean>
+__device__ __forceinline__ uint32_t lane_down_u32(uint32_t v) {
+  static_assert(OFF == 32 || OFF == 16 || OFF == 8 || OFF == 4 || OFF == 2 || OFF == 1, "offset");
+  if constexpr (kLean && (OFF == 32 || OFF == 16)) return (uint32_t)__shfl_down((int)v, OFF, 64);
+  else if constexpr (OFF == 32) return __builtin_amdgcn_permlane32_swap(v, v, false, false)[1];
+  else if constexpr (OFF == 16) return __builtin_amdgcn_permlane16_swap(v, v, false, false)[1];
+  else return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + OFF, 0xF, 0xF, false);
+}
+template <int OFF, bool kLean, typename T>
+__device__ __forceinline__ T lane_down(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, lane_down_u32<OFF, kLean>(__builtin_bit_cast(uint32_t, v)));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = lane_down_u32<OFF, kLean>((uint32_t)u);
+    const uint64_t hi = lane_down_u32<OFF, kLean>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(T, (hi << 32) | lo);
+  }
+}
+// Offsets 32..1 of the reference's stride-halving tree inside one wave:
+// w[l] = w[l] + w[l + off] for l < off.  Result in lane 0.
+template <typename T, bool kLean = false>
+__device__ __forceinline__ T wave_tail(T w) {
+  w = w + lane_down<32, kLean>(w);
+  w = w + lane_down<16, kLean>(w);
+  w = w + lane_down<8, kLean>(w);
+  w = w + lane_down<4, kLean>(w);
+  w = w + lane_down<2, kLean>(w);
+  w = w + lane_down<1, kLean>(w);
+  return w;
+}
+
 // The reference's 256-slot tree (dot_partials.wgsl:57-69) computed by one
 // wave: lane l holds slots l, l+64, l+128, l+192 as v0..v3.  Offset 128 pairs
 // (l, l+128) and (l+64, l+192); offset 64 pairs the two results; offsets
-// 32..1 are lane shuffles.  Same pairs, same order.  Result in lane 0.
-template <typename T>
+// 32..1 are lane exchanges.  Same pairs, same order.  Result in lane 0.
+template <typename T, bool kLean = false>
 __device__ __forceinline__ T wave_tree256(T v0, T v1, T v2, T v3) {
-  T w = (v0 + v2) + (v1 + v3);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) w = w + __shfl_down(w, off, 64);
-  return w;
+  return wave_tail<T, kLean>((v0 + v2) + (v1 + v3));
 }
 
 // The same tree over a workgroup of 256 threads (slot t = thread t): offsets
@@ -78,17 +116,13 @@ __device__ __forceinline__ T tree256(T v, T* sh) {
   if (t < 128) sh[t] = sh[t] + sh[t + 128];
   __syncthreads();
   T w = T(0);
-  if (t < 64) {
-    w = sh[t] + sh[t + 64];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) w = w + __shfl_down(w, off, 64);
-  }
+  if (t < 64) w = wave_tail(sh[t] + sh[t + 64]);
   __syncthreads();
   return w;
 }
 
 // Two independent trees in one pass (same pairing for each component).
-template <typename T>
+template <typename T, bool kLean = false>
 __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
   const int t = threadIdx.x;
   sha[t] = a;
@@ -101,13 +135,8 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
   __syncthreads();
   T wa = T(0), wb = T(0);
   if (t < 64) {
-    wa = sha[t] + sha[t + 64];
-    wb = shb[t] + shb[t + 64];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      wa = wa + __shfl_down(wa, off, 64);
-      wb = wb + __shfl_down(wb, off, 64);
-    }
+    wa = wave_tail<T, kLean>(sha[t] + sha[t + 64]);
+    wb = wave_tail<T, kLean>(shb[t] + shb[t + 64]);
   }
   __syncthreads();
   a = wa;
@@ -118,7 +147,7 @@ __device__ __forceinline__ void tree256x2(T& a, T& b, T* sha, T* shb) {
 // values, zero padded) for g < ceil(len/256).  Chunk g is reduced by wave
 // g % 4; each wave issues the loads of four of its chunks before reducing
 // them.
-template <typename T>
+template <typename T, bool kLean = false>
 __device__ __forceinline__ void level_pass(const T* __restrict__ src, uint32_t len, T* dst) {
   constexpr int B = 4;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -138,7 +167,7 @@ __device__ __forceinline__ void level_pass(const T* __restrict__ src, uint32_t l
     for (int m = 0; m < B; ++m) {
       const uint32_t g = g0 + 4u * m;
       if (g >= out_len) continue;
-      const T r = wave_tree256(v[m][0], v[m][1], v[m][2], v[m][3]);
+      const T r = wave_tree256<T, kLean>(v[m][0], v[m][1], v[m][2], v[m][3]);
       if (lane == 0) dst[g] = r;
     }
   }
@@ -161,7 +190,7 @@ __device__ __forceinline__ void level_fetch(const T* __restrict__ src, uint32_t 
     }
 }
 
-template <typename T>
+template <typename T, bool kLean = false>
 __device__ __forceinline__ T reduce_fetched(const T (&v)[4][4], uint32_t len, T* lds) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t out_len = (len + 255u) / 256u;
@@ -169,7 +198,7 @@ __device__ __forceinline__ T reduce_fetched(const T (&v)[4][4], uint32_t len, T*
   for (int m = 0; m < 4; ++m) {
     const uint32_t g = wave + 4u * m;
     if (g < out_len) {
-      const T r = wave_tree256(v[m][0], v[m][1], v[m][2], v[m][3]);
+      const T r = wave_tree256<T, kLean>(v[m][0], v[m][1], v[m][2], v[m][3]);
       if (lane == 0) lds[g] = r;
     }
   }
@@ -182,7 +211,7 @@ __device__ __forceinline__ T reduce_fetched(const T (&v)[4][4], uint32_t len, T*
         const uint32_t i = lane + 64u * j;
         w[j] = i < out_len ? lds[i] : T(0);
       }
-      const T r = wave_tree256(w[0], w[1], w[2], w[3]);
+      const T r = wave_tree256<T, kLean>(w[0], w[1], w[2], w[3]);
       if (lane == 0) lds[0] = r;
     }
     __syncthreads();
@@ -196,20 +225,20 @@ __device__ __forceinline__ T reduce_fetched(const T (&v)[4][4], uint32_t len, T*
 // go to lds when they fit (at most 256, i.e. up to 65,536 partials -- every
 // PCG system) and otherwise to `scratch` (global, ceil(len/256) values),
 // which a second level reduces into lds.  Returns the value in every thread.
-template <typename T>
+template <typename T, bool kLean = false>
 __device__ T reduce_partials(const T* __restrict__ part, uint32_t len, T* lds,
                              T* scratch = nullptr) {
   if (len <= 1u) return part[0];
   uint32_t out_len = (len + 255u) / 256u;
   if (out_len > 256u) {
-    level_pass<T>(part, len, scratch);
+    level_pass<T, kLean>(part, len, scratch);
     __threadfence_block();
     __syncthreads();
     len = out_len;
     out_len = (len + 255u) / 256u;
-    level_pass<T>(scratch, len, lds);
+    level_pass<T, kLean>(scratch, len, lds);
   } else {
-    level_pass<T>(part, len, lds);
+    level_pass<T, kLean>(part, len, lds);
   }
   __syncthreads();
   if (out_len > 1u) {  // out_len <= 256: one more level, one wave
@@ -221,7 +250,7 @@ __device__ T reduce_partials(const T* __restrict__ part, uint32_t len, T* lds,
         const uint32_t i = lane + 64u * j;
         v[j] = i < out_len ? lds[i] : T(0);
       }
-      const T r = wave_tree256(v[0], v[1], v[2], v[3]);
+      const T r = wave_tree256<T, kLean>(v[0], v[1], v[2], v[3]);
       if (lane == 0) lds[0] = r;
     }
     __syncthreads();
@@ -891,11 +920,12 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
 // kPhIrResid): 0 = converged, 5 = out of outer steps, kRunning = restart the
 // inner solve on r64 * (1/||r64||).  Computed identically by every workgroup
 // of k_pcg_update and of k_pcg_finish.
+template <bool kLean = false>
 __device__ __forceinline__ int32_t ir_decide(const SysDesc& d, double target2, uint32_t outer,
                                              uint32_t max_outer, double* lds_a, double* lds_b,
                                              double& rr, double& bb) {
-  rr = reduce_partials<double>(d.part64 + 3u * d.n_tiles, d.n_tiles, lds_a);
-  bb = reduce_partials<double>(d.part64 + 4u * d.n_tiles, d.n_tiles, lds_b);
+  rr = reduce_partials<double, kLean>(d.part64 + 3u * d.n_tiles, d.n_tiles, lds_a);
+  bb = reduce_partials<double, kLean>(d.part64 + 4u * d.n_tiles, d.n_tiles, lds_b);
   if (bb == 0.0 || rr <= target2 * bb) return 0;
   if (outer >= max_outer) return 5;  // HBJ_E_NOT_CONVERGED
   return kRunning;
@@ -973,7 +1003,9 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   double scale = 0.0;
   if (iter) {
     TP* lds = reinterpret_cast<TP*>(s_ra);
-    const TP tot = pre ? reduce_fetched<TP>(vp, nt, lds) : reduce_partials<TP>(part_pap, nt, lds);
+    // lean trees: this kernel's occupancy is set by its register count
+    const TP tot =
+        pre ? reduce_fetched<TP, true>(vp, nt, lds) : reduce_partials<TP, true>(part_pap, nt, lds);
     if constexpr (kDot64) {
       alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
       if (tr.tile == 0 && t == 0) {
@@ -989,12 +1021,12 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
       }
     }
   } else if (!restart) {
-    const float bn = reduce_partials<float>(d.part, nt, reinterpret_cast<float*>(s_ra));
+    const float bn = reduce_partials<float, true>(d.part, nt, reinterpret_cast<float*>(s_ra));
     if (tr.tile == 0 && t == 0) st->b_norm2 = bn;
     if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
   } else {
     double rr, bb;
-    if (ir_decide(d, target2, outer, max_outer, s_ra, s_rb, rr, bb) != kRunning) return;
+    if (ir_decide<true>(d, target2, outer, max_outer, s_ra, s_rb, rr, bb) != kRunning) return;
     scale = 1.0 / sqrt(rr);
   }
   __syncthreads();  // s_ra / s_rb are reused below
@@ -1051,7 +1083,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   if constexpr (kDot64) {
     double a = own ? (double)rn * (double)rn : 0.0;
     double b = own ? (double)rn * (double)zi : 0.0;
-    tree256x2<double>(a, b, s_ra, s_rb);
+    tree256x2<double, true>(a, b, s_ra, s_rb);
     if (t == 0) {
       d.part64[nt + tr.tile] = a;
       d.part64[2u * nt + tr.tile] = b;
@@ -1059,7 +1091,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   } else {
     float a = own ? rn * rn : 0.0f;
     float b = own ? rn * zi : 0.0f;
-    tree256x2<float>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
+    tree256x2<float, true>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
     if (t == 0) {
       d.part[nt + tr.tile] = a;
       d.part[2u * nt + tr.tile] = b;

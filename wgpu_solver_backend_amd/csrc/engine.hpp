@@ -527,7 +527,7 @@ struct hbj_system {
   double* h_part2 = nullptr;  // pinned, 2 * n_tiles
   uint2* tile_blk = nullptr;  // per tile: first block touching it, count
   uint16_t* lidx = nullptr;   // tile-local SpMV tables (build_tile_local)
-  uint2* tloc = nullptr;
+  TileHdr* thdr = nullptr;
   uint32_t* tcols = nullptr;
   uint32_t local_tiles = 0;
   uint64_t local_columns = 0, spmv_layout_bytes = 0;
@@ -559,7 +559,7 @@ struct hbj_system {
     d.r[1] = rb;
     d.tile_blk = tile_blk;
     d.lidx = lidx;
-    d.tloc = tloc;
+    d.thdr = thdr;
     d.tcols = tcols;
     d.st = nullptr;
     d.n = n;
@@ -637,7 +637,7 @@ struct hbj_system {
     dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
     dfree(d_desc1);
     dfree(tile_blk);
-    dfree(lidx); dfree(tloc); dfree(tcols);
+    dfree(lidx); dfree(thdr); dfree(tcols);
     hfree(h_part2);
   }
 };

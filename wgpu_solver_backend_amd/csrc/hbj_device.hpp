@@ -33,9 +33,20 @@ constexpr int32_t kRunning = -1;
 constexpr uint32_t kMaxTiles = 65536;
 // Tile-local SpMV tables: a tile whose entries reference at most kLocalMax
 // distinct columns gathers from an LDS copy of those x entries; others
-// (tloc.y == kLocalOff) gather from HBM.
+// (TileHdr.cnt == kLocalOff) gather from HBM.
 constexpr uint32_t kLocalMax = 2048;
 constexpr uint32_t kLocalOff = 0xFFFFFFFFu;
+
+// Per 256-row tile, read by one scalar load at the start of the SpMV: where
+// its column list is, and the entry ranges of its four 64-row waves, so the
+// entry stream and the list's loads go out together one round trip later
+// (no row_ptr round trip in between).
+struct alignas(32) TileHdr {
+  uint32_t off;    // offset of the tile's column list in tcols
+  uint32_t cnt;    // its length, or kLocalOff
+  uint32_t eb[5];  // row_ptr at the tile's rows 0, 64, 128, 192 and its end (clamped)
+  uint32_t pad;
+};
 
 enum Phase : int32_t {
   kPhInit = 0,      // r0 = b - A x0, z0 = M^-1 r0, p0 = z0, rz_old = r0.z0
@@ -93,7 +104,7 @@ struct SysDesc {
                              // r[par] and writes r[par ^ 1]
   const uint2* tile_blk;     // per tile: first block touching it, number of such blocks
   const uint16_t* lidx;      // nnz (+8 zero padding): entry's index in its tile's column list
-  const uint2* tloc;         // per tile: offset of its column list in tcols, length (or kLocalOff)
+  const TileHdr* thdr;       // per tile: column list and wave entry ranges
   const uint32_t* tcols;     // concatenated sorted distinct column lists of the tiles
   SysState* st;
   uint32_t n, nb, n_tiles, flags;

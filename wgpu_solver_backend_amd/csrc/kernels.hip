@@ -408,9 +408,15 @@ __device__ __forceinline__ void bst(T* base, uint32_t i, T v) {
 }
 
 // The wave's entry range [wb, we); false when it has no rows or no entries.
-__device__ __forceinline__ bool wave_range(uint32_t mb, uint32_t me, uint32_t nrows, uint32_t& wb,
-                                           uint32_t& we) {
-  const uint32_t wr0 = uni((uint32_t)(threadIdx.x >> 6)) * 64u;
+// First tile row of this wave's 64 rows in a 256-thread tile (thread t owns
+// row t); a two-wave tile passes its two row ranges explicitly.
+__device__ __forceinline__ uint32_t own_wave_row0() { return uni((uint32_t)(threadIdx.x >> 6)) * 64u; }
+
+// Entry range [wb, we) of the 64 rows [wr0, wr0 + 64) of a tile with nrows
+// rows; mb/me are lane l's bounds of row wr0 + l.  False if the wave has no
+// rows or no entries.
+__device__ __forceinline__ bool wave_range(uint32_t mb, uint32_t me, uint32_t nrows, uint32_t wr0,
+                                           uint32_t& wb, uint32_t& we) {
   if (wr0 >= nrows) return false;
   const uint32_t wrows = (nrows - wr0) < 64u ? nrows - wr0 : 64u;
   // lane reads into scalar registers: the range is wave-uniform by
@@ -438,11 +444,11 @@ template <typename TA, int PER, typename G>
 __device__ __forceinline__ TA spmv_wave(uint32_t mb, uint32_t me,
                                         const uint32_t* __restrict__ col,
                                         const float* __restrict__ val, G g, uint32_t nrows,
-                                        TA* s_prod_wave) {
+                                        TA* s_prod_wave, uint32_t wr0 = own_wave_row0()) {
   constexpr uint32_t chunk = 64u * PER;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t wb, we;
-  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
+  if (!wave_range(mb, me, nrows, wr0, wb, we)) return TA(0);
   const __amdgpu_buffer_rsrc_t rc = buf_rsrc(col + wb, (we - wb) * 4u);
   const __amdgpu_buffer_rsrc_t ra = buf_rsrc(val + wb, (we - wb) * 4u);
   uint32_t c[PER];
@@ -522,7 +528,7 @@ __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
                                             TA* s_prod_wave) {
   constexpr uint32_t chunk = 256u * K;
   uint32_t wb, we;
-  if (!wave_range(mb, me, nrows, wb, we)) return TA(0);
+  if (!wave_range(mb, me, nrows, own_wave_row0(), wb, we)) return TA(0);
   const GroupSrc gs = group_src(col, val, wb & ~3u, we);
   u32x4 c[K];
   f32x4v a[K];
@@ -554,66 +560,88 @@ __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
 // without rows.  Products are a_k * v[c_k] with the same v values, summed
 // as in spmv_wave.
 
-template <int K, typename Stage>
-__device__ __forceinline__ float spmv_wave_local(uint32_t mb, uint32_t me,
+// kDepth passes in flight: 1 = the next pass is loaded while this one is
+// summed; 2 = the two next passes (two register buffers, the loop unrolled
+// by two so that neither is copied).
+// [wb, we): the wave's entries (wave-uniform); mb/me: the lane's row.
+template <int K, int kDepth = 1, typename Stage>
+__device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint32_t mb, uint32_t me,
                                                  const uint16_t* __restrict__ lidx,
-                                                 const float* __restrict__ val, uint32_t nrows,
-                                                 const float* s_v, float* s_prod_wave,
-                                                 Stage stage) {
+                                                 const float* __restrict__ val, const float* s_v,
+                                                 float* s_prod_wave, Stage stage) {
+  static_assert(kDepth == 1 || kDepth == 2, "prefetch depth");
   constexpr uint32_t chunk = 256u * K;
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t wb = 0, we = 0;
-  const bool has = wave_range(mb, me, nrows, wb, we);
+  const bool has = wb < we;
   const uint32_t cs0 = has ? (wb & ~3u) : 0u, wend = has ? we : 0u;
   // ranges reach four entries past the wave's last one (zero padding)
   const __amdgpu_buffer_rsrc_t ri = buf_rsrc(lidx + cs0, (wend - cs0 + 4u) * 2u);
   const __amdgpu_buffer_rsrc_t ra = buf_rsrc(val + cs0, (wend - cs0 + 4u) * 4u);
-  u32x2 ci[K];  // four 16-bit local indices per group
-  f32x4v a[K];
-  auto load = [&](uint32_t cs) {
+  struct Pass {
+    u32x2 ci[K];  // four 16-bit local indices per group
+    f32x4v a[K];
+  };
+  auto load = [&](Pass& b, uint32_t cs) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint32_t e = cs - cs0 + 4u * ((uint32_t)k * 64u + lane);
-      ci[k] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(ri, (int)(e * 2u), 0, kNt));
-      a[k] = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(e * 4u), 0, kNt));
+      b.ci[k] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(ri, (int)(e * 2u), 0, kNt));
+      b.a[k] = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(e * 4u), 0, kNt));
     }
   };
-  load(cs0);
-  stage();
-  if (!has) return 0.0f;
   float sum = 0.0f;
-  for (uint32_t cs = cs0; cs < we; cs += chunk) {
+  // products of pass b (entries [cs, cs + chunk)), parked and summed by the
+  // owner lanes; `next` is loaded in between
+  auto run = [&](const Pass& b, uint32_t cs, Pass& next, uint32_t next_cs) {
     const uint32_t ce = (we - cs) < chunk ? we : cs + chunk;
     float pr[K][4];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      pr[k][0] = a[k][0] * s_v[ci[k][0] & 0xFFFFu];
-      pr[k][1] = a[k][1] * s_v[ci[k][0] >> 16];
-      pr[k][2] = a[k][2] * s_v[ci[k][1] & 0xFFFFu];
-      pr[k][3] = a[k][3] * s_v[ci[k][1] >> 16];
+      pr[k][0] = b.a[k][0] * s_v[b.ci[k][0] & 0xFFFFu];
+      pr[k][1] = b.a[k][1] * s_v[b.ci[k][0] >> 16];
+      pr[k][2] = b.a[k][2] * s_v[b.ci[k][1] & 0xFFFFu];
+      pr[k][3] = b.a[k][3] * s_v[b.ci[k][1] >> 16];
     }
-    load(cs + chunk);  // prefetch the next pass
+    load(next, next_cs);
     park_groups<float, K>(s_prod_wave, pr);
     wave_sync();
     sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
     wave_sync();
+  };
+  if constexpr (kDepth == 1) {
+    Pass p0;
+    load(p0, cs0);
+    stage();
+    if (!has) return 0.0f;
+    for (uint32_t cs = cs0; cs < we; cs += chunk) run(p0, cs, p0, cs + chunk);
+  } else {
+    Pass p0, p1;
+    load(p0, cs0);
+    load(p1, cs0 + chunk);
+    stage();
+    if (!has) return 0.0f;
+    for (uint32_t cs = cs0; cs < we; cs += 2u * chunk) {
+      run(p0, cs, p0, cs + 2u * chunk);
+      if (cs + chunk >= we) break;
+      run(p1, cs + chunk, p1, cs + 3u * chunk);
+    }
   }
   return sum;
 }
 
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
-// Run by the whole workgroup.  Loads are unconditional (positions past m
-// re-read the list's first entry).
-template <typename G>
+// Run by the whole workgroup of NT threads.  Loads are unconditional
+// (positions past m re-read the list's first entry).
+template <int NT = kTile, typename G>
 __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, uint32_t off,
                                             uint32_t m, G g, float* s_v) {
-  constexpr int kSteps = (int)(kLocalMax / kTile);
+  constexpr int kSteps = (int)(kLocalMax / NT);
   const uint32_t t = threadIdx.x;
   if (m > 0u) {
     uint32_t c[kSteps];
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
-      const uint32_t i = t + (uint32_t)k * kTile;
+      const uint32_t i = t + (uint32_t)k * NT;
       c[k] = tcols[off + (i < m ? i : 0u)];
     }
     float x[kSteps];
@@ -621,7 +649,7 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
     for (int k = 0; k < kSteps; ++k) x[k] = g(c[k]);
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
-      const uint32_t i = t + (uint32_t)k * kTile;
+      const uint32_t i = t + (uint32_t)k * NT;
       if (i < m) s_v[i] = x[k];
     }
   }
@@ -631,13 +659,16 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
 // Streaming variants of the FP32 SpMV (all bit-identical; the autotuner
 // picks one per system): 0 = dword loads, 512 entries per pass, gathers from
 // HBM; 1 = 16-byte loads, 1024, gathers from HBM; 2 = tile-local columns,
-// 512; 3 = tile-local, 1024 (2 and 3 use variant 0 on tiles without a local
-// column list).  kSlice = LDS floats per wave.
+// 512; 3 = tile-local, 1024; 4 / 5 = 2 / 3 with two passes in flight (2..5
+// use variant 0 on tiles without a local column list).  kSlice = LDS floats
+// per wave.
 template <int V> struct SpmvCfg;
 template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<2> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<3> { static constexpr int kSlice = 1024; };
+template <> struct SpmvCfg<4> { static constexpr int kSlice = 512; };
+template <> struct SpmvCfg<5> { static constexpr int kSlice = 1024; };
 
 // --------------------------------------------------- block-Jacobi tile body
 
@@ -815,7 +846,9 @@ __device__ __forceinline__ bool halo_row(uint32_t row0, uint32_t nrows, uint32_t
 template <int V>
 struct SpmvLds {
   static constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
-  static constexpr bool kLocal = V == 2 || V == 3;
+  static constexpr bool kLocal = V >= 2;
+  static constexpr int K = (V == 2 || V == 4) ? 2 : 4;  // groups per lane and pass (local)
+  static constexpr int kDepth = V >= 4 ? 2 : 1;          // passes in flight (local)
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
 };
 
@@ -831,12 +864,42 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
   const SysState* st = d.st + ta.par * ta.st_stride;
   const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
   const bool own = t < tr.nrows;
+  // the tile header (tile-local variants) goes out with the state's loads
+  TileHdr hdr{};
+  if constexpr (kLocal) hdr = uni_copy(d.thdr[tr.tile]);
   uint32_t mb, me;
   lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   if (status != kRunning) return;
 
-  if (phase == kPhInit || phase == kPhIter) {
+  if (phase == kPhIrResid) {
+    // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
+    const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
+    double* slice = s_prod + wave * (kSliceF / 2);
+    const double ax = spmv_wave<double, kPer / 2>(
+        mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows,
+        slice);
+    __syncthreads();
+    double rr = 0.0, bb = 0.0;
+    if (own) {
+      const double bi = d.b64[row];
+      const double ri = bi - ax;
+      d.r64[row] = ri;
+      rr = ri * ri;
+      bb = bi * bi;
+    }
+    tree256x2<double>(rr, bb, s_ra, s_rb);
+    if (t == 0) {
+      d.part64[3u * d.n_tiles + tr.tile] = rr;
+      d.part64[4u * d.n_tiles + tr.tile] = bb;
+    }
+    return;
+  }
+  if (phase != kPhInit && phase != kPhIter) return;
+  // (the refinement residual above returns on its own: joining it with this
+  // path before the entry stream starts made the compiler wait for every
+  // outstanding load there, the row bounds included)
+  {
     const bool init = phase == kPhInit;
     // the multiplied vector: x0 at init (compute.rs:294), p per iteration
     const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.p, d.n * 4u);
@@ -844,12 +907,15 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
     float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
     float q;
     if constexpr (kLocal) {
-      // workgroup-uniform: keep it in scalar registers (uniform branch)
-      const uint2 tl0 = d.tloc[tr.tile];
-      const uint2 tl = make_uint2(uni(tl0.x), uni(tl0.y));
-      if (tl.y != kLocalOff) {
-        q = spmv_wave_local<V == 2 ? 2 : 4>(mb, me, d.lidx, d.val, tr.nrows, s_v, slice, [&]() {
-          stage_local(d.tcols, tl.x, tl.y, vec_at, s_v);
+      if (hdr.cnt != kLocalOff) {
+        // the wave's entries from the header (selects, not an index: a
+        // runtime index into the struct would place it in scratch memory)
+        const uint32_t w = uni(wave);
+        const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
+        const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
+        q = spmv_wave_local<SpmvLds<V>::K, SpmvLds<V>::kDepth>(wb, we, mb, me, d.lidx, d.val, s_v,
+                                                               slice, [&]() {
+          stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
         });
       } else {
         q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
@@ -876,27 +942,6 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
     } else {
       const float pd = tree256<float>(w, reinterpret_cast<float*>(s_ra));
       if (t == 0) d.part[tr.tile] = pd;
-    }
-  } else if (phase == kPhIrResid) {
-    // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
-    const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
-    double* slice = s_prod + wave * (kSliceF / 2);
-    const double ax = spmv_wave<double, kPer / 2>(
-        mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows,
-        slice);
-    __syncthreads();
-    double rr = 0.0, bb = 0.0;
-    if (own) {
-      const double bi = d.b64[row];
-      const double ri = bi - ax;
-      d.r64[row] = ri;
-      rr = ri * ri;
-      bb = bi * bi;
-    }
-    tree256x2<double>(rr, bb, s_ra, s_rb);
-    if (t == 0) {
-      d.part64[3u * d.n_tiles + tr.tile] = rr;
-      d.part64[4u * d.n_tiles + tr.tile] = bb;
     }
   }
 }
@@ -1467,6 +1512,8 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 1: launch_spmv_v<1>(ls, s); break;
     case 2: launch_spmv_v<2>(ls, s); break;
     case 3: launch_spmv_v<3>(ls, s); break;
+    case 4: launch_spmv_v<4>(ls, s); break;
+    case 5: launch_spmv_v<5>(ls, s); break;
     default: launch_spmv_v<0>(ls, s); break;
   }
 }

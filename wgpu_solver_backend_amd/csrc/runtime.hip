@@ -367,7 +367,7 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
       } else {
         s->local_tiles += 1;
         s->local_columns += tloc_h[t].y;
-        s->spmv_layout_bytes += 6 * e + 4ull * tloc_h[t].y + 8;
+        s->spmv_layout_bytes += 6 * e + 4ull * tloc_h[t].y + sizeof(TileHdr);
       }
     }
     s->spmv_layout_bytes += 8 * csr_entries + 12ull * n + 4;
@@ -414,7 +414,7 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   ALLOC(d_desc1, 1);
   ALLOC(tile_blk, nt);
   ALLOC(lidx, lidx_h.size());
-  ALLOC(tloc, nt);
+  ALLOC(thdr, nt);
   ALLOC(tcols, tcols_h.size());
 #undef ALLOC
   if (st != HBJ_OK) return fail(st);
@@ -430,7 +430,19 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   up(s->bs, block_starts, 4ull * n_starts);
   up(s->lu, lu_h.data(), 144ull * nb);
   up(s->lidx, lidx_h.data(), sizeof(uint16_t) * lidx_h.size());
-  up(s->tloc, tloc_h.data(), sizeof(uint2) * tloc_h.size());
+  {
+    std::vector<TileHdr> th(tloc_h.size());
+    for (uint32_t t = 0; t < (uint32_t)tloc_h.size(); ++t) {
+      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
+      TileHdr& h = th[t];
+      h.off = tloc_h[t].x;
+      h.cnt = tloc_h[t].y;
+      for (uint32_t k = 0; k < 4; ++k) h.eb[k] = row_ptr[std::min<uint32_t>(r0 + 64u * k, r1)];
+      h.eb[4] = row_ptr[r1];
+      h.pad = 0;
+    }
+    up(s->thdr, th.data(), sizeof(TileHdr) * th.size());
+  }
   up(s->tcols, tcols_h.data(), sizeof(uint32_t) * tcols_h.size());
   {
     // per tile: the first block touching rows [256t, 256t+256) and how many do

@@ -265,11 +265,11 @@ uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
   auto c = [](uint64_t k) { return k ? k : 1; };
   const uint64_t nt = c((n + kTile - 1) / kTile), nb = n_starts ? n_starts - 1 : 0;
   // tile-local SpMV tables: 2 bytes per entry, a list of at most
-  // min(nnz, kLocalMax) columns per tile, 8 bytes per tile
+  // min(nnz, kLocalMax) columns per tile, a 32-byte header per tile
   const uint64_t tcols = std::min<uint64_t>(c(nnz), (uint64_t)kLocalMax * nt);
   return 4 * c(n + 1) + 8 * (uint64_t)(nnz + 4) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
          4 * 3 * nt + 8 * 5 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n) +
-         2 * ((uint64_t)nnz + 8) + 8 * nt + 4 * tcols;
+         2 * ((uint64_t)nnz + 8) + sizeof(TileHdr) * nt + 4 * tcols;
 }
 
 }  // namespace

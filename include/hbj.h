@@ -161,6 +161,12 @@ typedef struct {
 
 /* No reference counterpart (introspection for measurement). */
 hbj_status hbj_system_get_info(const hbj_system* sys, hbj_system_info* out);
+/* Copies the system's LU blocks (36 * n_blocks floats, the layout of
+ * hbj_build_lu_blocks_6; built on the device from the CSR when the system
+ * was created) to lu_out.  No reference counterpart (introspection; the
+ * reference's builder is build_lu_blocks_from_csr_block_starts_6,
+ * compute.rs:84-181, which hbj_build_lu_blocks_6 restates on the host). */
+hbj_status hbj_system_get_lu(const hbj_system* sys, float* lu_out);
 /* Selects the SpMV variant (< HBJ_SPMV_VARIANTS) of later solves; results do
  * not depend on it.  No reference counterpart. */
 hbj_status hbj_system_set_spmv_variant(hbj_system* sys, uint32_t variant);

@@ -204,3 +204,59 @@ def test_spmv_variants_bitwise_and_autotune(ctx, hbj, oracle):
     with pytest.raises(hbj.SolverError):
         sys_.set_spmv_variant(hbj.SPMV_VARIANTS)
     sys_.close()
+
+
+def _csr_from_dense(A):
+    """CSR of A's nonzeros plus its whole diagonal (explicit zeros kept)."""
+    rp, ci, va = [0], [], []
+    for i in range(A.shape[0]):
+        keep = A[i] != 0
+        keep[i] = True
+        nz = np.nonzero(keep)[0]
+        ci.extend(nz.tolist())
+        va.extend(A[i, nz].tolist())
+        rp.append(len(ci))
+    return np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, F)
+
+
+def test_device_lu_blocks_bitwise_vs_host_builder(ctx, hbj, oracle):
+    """The LU blocks a system builds on the device from its CSR are the host
+    builder's (build_lu_blocks_from_csr_block_starts_6 restated) bit for bit:
+    the reference's datasets (including 3-row blocks), lattices, and a matrix
+    with an oversize block and duplicate CSR entries (last write wins)."""
+    cases = [oracle.load_dataset(k) for k in (1, 2, 3)]
+    for N in (10, 50):
+        L = hbj.lattice(N, seed=3)
+        cases.append({"n": L["n"], "row_ptr": L["row_ptr"], "col_idx": L["col_idx"],
+                      "values": L["values"], "block_starts": L["block_starts"]})
+    rng = np.random.default_rng(5)
+    A = (rng.standard_normal((14, 14)) * 0.1).astype(F) + 4 * np.eye(14, dtype=F)
+    rp, ci, va = _csr_from_dense(A)
+    # duplicate (row 2, col 3) entry after the original: the later value wins
+    k = int(rp[2]) + int(np.searchsorted(ci[rp[2]:rp[3]], 3)) + 1
+    ci = np.insert(ci, k, 3).astype(np.uint32)
+    va = np.insert(va, k, np.float32(7.25)).astype(F)
+    rp = rp.copy()
+    rp[3:] += 1
+    cases.append({"n": 14, "row_ptr": rp, "col_idx": ci, "values": va,
+                  "block_starts": np.array([0, 8, 11, 14], np.uint32)})
+    for d in cases:
+        n = int(d["n"])
+        st, ref = oracle.build_lu_blocks_6(n, d["row_ptr"], d["col_idx"], d["values"], d["block_starts"])
+        assert st == 0
+        s = hbj.System(ctx, d["row_ptr"], d["col_idx"], d["values"], d["block_starts"])
+        got = s.lu()
+        assert got.shape[0] == len(d["block_starts"]) - 1
+        assert np.array_equal(got.reshape(-1).view(np.uint32), np.asarray(ref, F).reshape(-1).view(np.uint32))
+
+
+def test_device_lu_reports_first_failing_block(ctx, hbj):
+    """Two blocks with zero pivots (block 0 at k = 3, block 1 at k = 0): the
+    error names the first block's pivot, as the host loop does."""
+    A = np.eye(12, dtype=F)
+    A[3, 3] = 0.0
+    A[6, 6] = 0.0
+    rp, ci, va = _csr_from_dense(A)
+    with pytest.raises(hbj.SolverError, match="LU zero pivot at k = 3") as e:
+        hbj.System(ctx, rp, ci, va, np.array([0, 6, 12], np.uint32))
+    assert e.value.status_name == "ZERO_PIVOT"

@@ -285,6 +285,14 @@ class System:
         _check(_lib.lib().hbj_system_get_info(self._h, ctypes.byref(inf)))
         return {f: getattr(inf, f) for f, _ in inf._fields_}
 
+    def lu(self):
+        """The system's LU blocks (float32 [n_blocks, 6, 6]), as built on the
+        device from its CSR when the system was created."""
+        nb = self.info()["n_blocks"]
+        out = np.zeros((nb, 6, 6), np.float32)
+        _check(_lib.lib().hbj_system_get_lu(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
     def set_spmv_variant(self, variant):
         """Select one of the bit-identical SpMV variants (0..SPMV_VARIANTS-1)."""
         _check(_lib.lib().hbj_system_set_spmv_variant(self._h, int(variant)))

@@ -38,6 +38,11 @@ void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t
                        hipStream_t s);
 
 void launch_stream_copy(const void* src, void* dst, uint64_t n4, hipStream_t s);
+// LU blocks of a CSR matrix on the device (block_starts already validated);
+// *fail (set to ~0 first) receives (first failing block << 32 | pivot index)
+void launch_build_lu6(uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const float* val,
+                      const uint32_t* bs, uint32_t nb, float* lu, unsigned long long* fail,
+                      hipStream_t s);
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s);
 void launch_op_scale(uint32_t n, float beta, float* x, hipStream_t s);
 void launch_op_dot(uint32_t n, const float* a, const float* b, float* part, float* scratch,

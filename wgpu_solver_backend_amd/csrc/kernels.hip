@@ -1387,6 +1387,67 @@ void launch_stream_copy(const void* src, void* dst, uint64_t n4, hipStream_t s) 
   hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const f32x4*)src, (f32x4*)dst, n4);
 }
 
+// ------------------------------------------------------ LU block build
+
+// build_lu_blocks_from_csr_block_starts_6 (compute.rs:84-181) with
+// lu_factor_inplace_6 (:37-71), one thread per block: extract the block's
+// leading m x m (m = min(6, size)) from its rows' CSR entries in CSR order
+// (a duplicate's last write wins) into an identity-filled 6x6, then the
+// no-pivot LU in place, operation for operation as on the host.  Blocks
+// with end > n or start >= end stay zero (:141-143).  The 6x6 lives in LDS
+// (stride 37 words per thread) because the extraction indexes it at run
+// time.  A zero pivot records (block << 32 | k) with an atomic minimum, so
+// the host reports the first failing block as the host loop does; the
+// other blocks are still written.  block_starts is validated by the caller.
+__global__ __launch_bounds__(kTile) void k_build_lu6(uint32_t n, const uint32_t* __restrict__ row_ptr,
+                                                     const uint32_t* __restrict__ col,
+                                                     const float* __restrict__ val,
+                                                     const uint32_t* __restrict__ bs, uint32_t nb,
+                                                     float* __restrict__ lu,
+                                                     unsigned long long* __restrict__ fail) {
+  __shared__ float s_m[kTile * 37];
+  const uint32_t blk = blockIdx.x * kTile + threadIdx.x;
+  if (blk >= nb) return;
+  float* mat = s_m + threadIdx.x * 37u;
+  for (int i = 0; i < 36; ++i) mat[i] = 0.0f;
+  const uint32_t off = bs[blk], end = bs[blk + 1];
+  if (!(end > n || off >= end)) {
+    const uint32_t m = end - off < 6u ? end - off : 6u;
+    for (int i = 0; i < 6; ++i) mat[i * 6 + i] = 1.0f;
+    for (uint32_t il = 0; il < m; ++il) {
+      const uint32_t i = off + il;
+      const uint32_t k1 = row_ptr[i + 1];
+      for (uint32_t k = row_ptr[i]; k < k1; ++k) {
+        const uint32_t j = col[k];
+        if (j >= off && j < off + m) mat[il * 6u + (j - off)] = val[k];
+      }
+    }
+    for (uint32_t k = 0; k < m; ++k) {
+      const float pivot = mat[k * 6u + k];
+      if (pivot == 0.0f) {
+        atomicMin(fail, ((unsigned long long)blk << 32) | k);
+        break;
+      }
+      for (uint32_t i = k + 1; i < m; ++i) mat[i * 6u + k] /= pivot;
+      for (uint32_t i = k + 1; i < m; ++i) {
+        const float l = mat[i * 6u + k];
+        if (l == 0.0f) continue;  // compute.rs:62
+        for (uint32_t j = k + 1; j < m; ++j) mat[i * 6u + j] -= l * mat[k * 6u + j];
+      }
+    }
+  }
+  float* dst = lu + 36ull * blk;
+  for (int i = 0; i < 36; ++i) dst[i] = mat[i];
+}
+
+void launch_build_lu6(uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const float* val,
+                      const uint32_t* bs, uint32_t nb, float* lu, unsigned long long* fail,
+                      hipStream_t s) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_build_lu6, dim3((nb + kTile - 1) / kTile), dim3(kTile), 0, s, n, row_ptr,
+                     col, val, bs, nb, lu, fail);
+}
+
 // ------------------------------------------------- single-op kernels (KATs)
 
 __global__ void k_op_axpy(uint32_t n, float alpha, const float* __restrict__ x, float* __restrict__ y) {

@@ -347,9 +347,6 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
     return HBJ_E_INVALID;
   }
   const uint32_t nb = n_starts - 1;
-  std::vector<float> lu_h(36ull * (nb ? nb : 1));
-  CHK(hbj_build_lu_blocks_6(n, row_ptr, n + 1, col_idx, nnz, values, nnz, block_starts, n_starts,
-                            lu_h.data()));
   std::vector<uint16_t> lidx_h;
   std::vector<uint2> tloc_h;
   std::vector<uint32_t> tcols_h;
@@ -428,7 +425,6 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   up(s->col, col_idx, 4ull * nnz);
   up(s->val, values, 4ull * nnz);
   up(s->bs, block_starts, 4ull * n_starts);
-  up(s->lu, lu_h.data(), 144ull * nb);
   up(s->lidx, lidx_h.data(), sizeof(uint16_t) * lidx_h.size());
   {
     std::vector<TileHdr> th(tloc_h.size());
@@ -454,6 +450,24 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
       tb[t] = make_uint2(bf, bl - bf + 1);
     }
     up(s->tile_blk, tb.data(), sizeof(uint2) * nt);
+  }
+  // LU blocks on the device from the uploaded CSR (compute.rs:84-181; the
+  // block starts were validated above, so the only error left is a zero
+  // pivot, reported for the first failing block as the host builder does)
+  if (e == hipSuccess && nb) {
+    // the residual check's partials (>= 16 bytes) are free during setup
+    unsigned long long* d_fail = reinterpret_cast<unsigned long long*>(s->resid2);
+    unsigned long long h_fail = 0;
+    e = hipMemset(d_fail, 0xFF, sizeof h_fail);
+    if (e == hipSuccess) {
+      launch_build_lu6(n, s->rp, s->col, s->val, s->bs, nb, s->lu, d_fail, nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h_fail, d_fail, sizeof h_fail, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h_fail != ~0ull) {
+      set_error("LU zero pivot at k = %u", (uint32_t)(h_fail & 0xFFFFFFFFu));
+      return fail(HBJ_E_ZERO_PIVOT);
+    }
   }
   // the padding past nnz is read by 16-byte loads: keep it zero (column 0)
   if (e == hipSuccess) e = hipMemset(s->col + nnz, 0, 16);
@@ -481,6 +495,15 @@ void hbj_system_destroy(hbj_system* s) {
 }
 
 uint64_t hbj_system_device_bytes(const hbj_system* s) { return s ? s->bytes : 0; }
+
+hbj_status hbj_system_get_lu(const hbj_system* s, float* lu_out) {
+  clear_error();
+  if (!s || (s->nb && !lu_out)) { set_error("hbj_system_get_lu: bad argument"); return HBJ_E_INVALID; }
+  if (s->nb == 0) return HBJ_OK;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipMemcpy(lu_out, s->lu, 144ull * s->nb, hipMemcpyDeviceToHost));
+  return HBJ_OK;
+}
 
 hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
   clear_error();

@@ -144,8 +144,8 @@ hbj_status hbj_system_get_x(hbj_system* sys, float* x);
 /* Bit-identical SpMV streaming variants a system can use: 0 = dword loads,
  * gathers from HBM; 1 = 16-byte loads, gathers from HBM; 2 = tile-local
  * column lists in LDS, 512 entries per wave pass (default); 3 = tile-local,
- * 1024 entries per pass; 4 / 5 = 2 / 3 with two passes in flight per wave. */
-#define HBJ_SPMV_VARIANTS 6
+ * 1024 entries per pass. */
+#define HBJ_SPMV_VARIANTS 4
 
 typedef struct {
   uint32_t n, nnz, n_blocks, n_tiles;

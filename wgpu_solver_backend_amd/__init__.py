@@ -40,7 +40,7 @@ CLI_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "hbj_cli"
 ITER_KERNELS = 3
 ITER_KERNEL_NAMES = ("spmv", "update", "finish")
 # bit-identical SpMV variants (HBJ_SPMV_VARIANTS)
-SPMV_VARIANTS = 6
+SPMV_VARIANTS = 4
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "ZERO_PIVOT", 3: "BREAKDOWN_PAP", 4: "BREAKDOWN_RZ",
                 5: "NOT_CONVERGED", 6: "DEVICE", 7: "OOM", 8: "NO_DEVICE", 9: "REJECTED"}

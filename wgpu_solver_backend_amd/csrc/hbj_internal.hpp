@@ -26,9 +26,8 @@ struct LaunchSet {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // kernel timing (optional)
 };
 // SpMV variants: 0 dword loads with HBM gathers, 1 16-byte loads with HBM
-// gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass,
-// 4 / 5 the same as 2 / 3 with two passes in flight per wave.
-constexpr uint32_t kSpmvVariantCount = 6;
+// gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass.
+constexpr uint32_t kSpmvVariantCount = 4;
 constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pass
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);

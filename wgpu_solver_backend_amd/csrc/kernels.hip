@@ -409,7 +409,7 @@ __device__ __forceinline__ void bst(T* base, uint32_t i, T v) {
 
 // The wave's entry range [wb, we); false when it has no rows or no entries.
 // First tile row of this wave's 64 rows in a 256-thread tile (thread t owns
-// row t); a two-wave tile passes its two row ranges explicitly.
+// row t).
 __device__ __forceinline__ uint32_t own_wave_row0() { return uni((uint32_t)(threadIdx.x >> 6)) * 64u; }
 
 // Entry range [wb, we) of the 64 rows [wr0, wr0 + 64) of a tile with nrows
@@ -444,11 +444,11 @@ template <typename TA, int PER, typename G>
 __device__ __forceinline__ TA spmv_wave(uint32_t mb, uint32_t me,
                                         const uint32_t* __restrict__ col,
                                         const float* __restrict__ val, G g, uint32_t nrows,
-                                        TA* s_prod_wave, uint32_t wr0 = own_wave_row0()) {
+                                        TA* s_prod_wave) {
   constexpr uint32_t chunk = 64u * PER;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t wb, we;
-  if (!wave_range(mb, me, nrows, wr0, wb, we)) return TA(0);
+  if (!wave_range(mb, me, nrows, own_wave_row0(), wb, we)) return TA(0);
   const __amdgpu_buffer_rsrc_t rc = buf_rsrc(col + wb, (we - wb) * 4u);
   const __amdgpu_buffer_rsrc_t ra = buf_rsrc(val + wb, (we - wb) * 4u);
   uint32_t c[PER];
@@ -560,16 +560,12 @@ __device__ __forceinline__ TA spmv_wave_vec(uint32_t mb, uint32_t me,
 // without rows.  Products are a_k * v[c_k] with the same v values, summed
 // as in spmv_wave.
 
-// kDepth passes in flight: 1 = the next pass is loaded while this one is
-// summed; 2 = the two next passes (two register buffers, the loop unrolled
-// by two so that neither is copied).
 // [wb, we): the wave's entries (wave-uniform); mb/me: the lane's row.
-template <int K, int kDepth = 1, typename Stage>
+template <int K, typename Stage>
 __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint32_t mb, uint32_t me,
                                                  const uint16_t* __restrict__ lidx,
                                                  const float* __restrict__ val, const float* s_v,
                                                  float* s_prod_wave, Stage stage) {
-  static_assert(kDepth == 1 || kDepth == 2, "prefetch depth");
   constexpr uint32_t chunk = 256u * K;
   const uint32_t lane = threadIdx.x & 63u;
   const bool has = wb < we;
@@ -608,24 +604,11 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
     sum = sum_slice(sum, s_prod_wave, mb, me, cs, ce);
     wave_sync();
   };
-  if constexpr (kDepth == 1) {
-    Pass p0;
-    load(p0, cs0);
-    stage();
-    if (!has) return 0.0f;
-    for (uint32_t cs = cs0; cs < we; cs += chunk) run(p0, cs, p0, cs + chunk);
-  } else {
-    Pass p0, p1;
-    load(p0, cs0);
-    load(p1, cs0 + chunk);
-    stage();
-    if (!has) return 0.0f;
-    for (uint32_t cs = cs0; cs < we; cs += 2u * chunk) {
-      run(p0, cs, p0, cs + 2u * chunk);
-      if (cs + chunk >= we) break;
-      run(p1, cs + chunk, p1, cs + 3u * chunk);
-    }
-  }
+  Pass p0;
+  load(p0, cs0);
+  stage();  // the tile's LDS copy of the vector, while the first pass loads
+  if (!has) return 0.0f;
+  for (uint32_t cs = cs0; cs < we; cs += chunk) run(p0, cs, p0, cs + chunk);  // prefetch next
   return sum;
 }
 
@@ -659,16 +642,13 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
 // Streaming variants of the FP32 SpMV (all bit-identical; the autotuner
 // picks one per system): 0 = dword loads, 512 entries per pass, gathers from
 // HBM; 1 = 16-byte loads, 1024, gathers from HBM; 2 = tile-local columns,
-// 512; 3 = tile-local, 1024; 4 / 5 = 2 / 3 with two passes in flight (2..5
-// use variant 0 on tiles without a local column list).  kSlice = LDS floats
-// per wave.
+// 512; 3 = tile-local, 1024 (2 and 3 use variant 0 on tiles without a local
+// column list).  kSlice = LDS floats per wave.
 template <int V> struct SpmvCfg;
 template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<2> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<3> { static constexpr int kSlice = 1024; };
-template <> struct SpmvCfg<4> { static constexpr int kSlice = 512; };
-template <> struct SpmvCfg<5> { static constexpr int kSlice = 1024; };
 
 // --------------------------------------------------- block-Jacobi tile body
 
@@ -847,8 +827,7 @@ template <int V>
 struct SpmvLds {
   static constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
   static constexpr bool kLocal = V >= 2;
-  static constexpr int K = (V == 2 || V == 4) ? 2 : 4;  // groups per lane and pass (local)
-  static constexpr int kDepth = V >= 4 ? 2 : 1;          // passes in flight (local)
+  static constexpr int K = V == 2 ? 2 : 4;  // groups per lane and pass (local)
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
 };
 
@@ -913,8 +892,7 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
         const uint32_t w = uni(wave);
         const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
         const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
-        q = spmv_wave_local<SpmvLds<V>::K, SpmvLds<V>::kDepth>(wb, we, mb, me, d.lidx, d.val, s_v,
-                                                               slice, [&]() {
+        q = spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [&]() {
           stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
         });
       } else {
@@ -1573,8 +1551,6 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 1: launch_spmv_v<1>(ls, s); break;
     case 2: launch_spmv_v<2>(ls, s); break;
     case 3: launch_spmv_v<3>(ls, s); break;
-    case 4: launch_spmv_v<4>(ls, s); break;
-    case 5: launch_spmv_v<5>(ls, s); break;
     default: launch_spmv_v<0>(ls, s); break;
   }
 }

@@ -49,27 +49,36 @@ def traffic_from_profiles(workload_key):
     return None
 
 
-def cpu_baseline(L, iters_gpu, sample_iters):
+def cpu_baseline(L, iters_gpu, sample_iters, sample_iters_1core):
     """The oracle (FP32 restatement of the reference loop) on the host cores,
-    on a bounded sample: `sample_iters` PCG iterations of the same system,
-    scaled to solves/s with the GPU's iteration count."""
+    on a bounded sample: `sample_iters` PCG iterations of the same system on
+    up to 16 threads (and `sample_iters_1core` on one), scaled to solves/s
+    with the GPU's iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_lib
     oracle_lib.build()
-    threads = min(16, os.cpu_count() or 1)
-    oracle_lib.set_threads(threads)
     st, lu = oracle_lib.build_lu_blocks_6(L["n"], L["row_ptr"], L["col_idx"], L["values"],
                                           L["block_starts"])
     x0 = np.zeros(L["n"], np.float32)
-    t0 = time.perf_counter()
-    st, it, _, _ = oracle_lib.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"],
-                                  L["rhs"], x0, sample_iters, 1e-6, 0.0)
-    dt = time.perf_counter() - t0
-    per_iter = dt / max(it, 1)
-    solve_s = per_iter * (iters_gpu + 1)
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{it} PCG iterations of the same 50^3 system in {dt:.1f} s "
-                      f"({per_iter * 1e3:.2f} ms/iteration), scaled to {iters_gpu} iterations"}
+
+    def sample(threads, iters):
+        oracle_lib.set_threads(threads)
+        t0 = time.perf_counter()
+        _, it, _, _ = oracle_lib.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"],
+                                     L["rhs"], x0, iters, 1e-6, 0.0)
+        dt = time.perf_counter() - t0
+        per_iter = dt / max(it, 1)
+        return 1.0 / (per_iter * (iters_gpu + 1)), it, dt, per_iter
+
+    side = round((L["n"] / 6) ** (1.0 / 3.0))
+    threads = min(16, os.cpu_count() or 1)
+    v, it, dt, per_iter = sample(threads, sample_iters)
+    v1, it1, dt1, per_iter1 = sample(1, sample_iters_1core)
+    return {"value": v, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{it} PCG iterations of the same {side}^3 system in {dt:.1f} s "
+                      f"({per_iter * 1e3:.2f} ms/iteration), scaled to {iters_gpu} iterations",
+            "single_core": {"value": v1, "unit": "solves/s", "cores": 1,
+                            "sample": f"{it1} iterations in {dt1:.1f} s ({per_iter1 * 1e3:.2f} ms/iteration)"}}
 
 
 def bench_plain(args):
@@ -162,7 +171,7 @@ def bench_plain(args):
     }
     if not args.no_cpu:
         log("cpu baseline (oracle on host cores)")
-        out["cpu_baseline"] = cpu_baseline(L, int(res.iterations), args.cpu_iters)
+        out["cpu_baseline"] = cpu_baseline(L, int(res.iterations), args.cpu_iters, args.cpu_iters_1core)
     else:
         out["cpu_baseline"] = None
     system.close()
@@ -329,7 +338,10 @@ def main():
     ap.add_argument("--sample-every", type=int, default=32,
                     help="HIP-event timing of every k-th PCG iteration's kernels (inside the "
                          "timed solves; each sample costs a few microseconds)")
-    ap.add_argument("--cpu-iters", type=int, default=300)
+    ap.add_argument("--cpu-iters", type=int, default=1000,
+                    help="oracle PCG iterations timed on up to 16 host threads")
+    ap.add_argument("--cpu-iters-1core", type=int, default=100,
+                    help="oracle PCG iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])
     ap.add_argument("--inner-tol", type=float, default=1e-4)

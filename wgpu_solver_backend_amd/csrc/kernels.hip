@@ -613,18 +613,18 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
 }
 
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
-// Run by the whole workgroup of NT threads.  Loads are unconditional
-// (positions past m re-read the list's first entry).
-template <int NT = kTile, typename G>
+// Run by the whole workgroup.  Loads are unconditional (positions past m
+// re-read the list's first entry).
+template <typename G>
 __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, uint32_t off,
                                             uint32_t m, G g, float* s_v) {
-  constexpr int kSteps = (int)(kLocalMax / NT);
+  constexpr int kSteps = (int)(kLocalMax / kTile);
   const uint32_t t = threadIdx.x;
   if (m > 0u) {
     uint32_t c[kSteps];
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
-      const uint32_t i = t + (uint32_t)k * NT;
+      const uint32_t i = t + (uint32_t)k * kTile;
       c[k] = tcols[off + (i < m ? i : 0u)];
     }
     float x[kSteps];
@@ -632,7 +632,7 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
     for (int k = 0; k < kSteps; ++k) x[k] = g(c[k]);
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
-      const uint32_t i = t + (uint32_t)k * NT;
+      const uint32_t i = t + (uint32_t)k * kTile;
       if (i < m) s_v[i] = x[k];
     }
   }

@@ -91,6 +91,19 @@ hbj_status hbj_build_lu_blocks_6(uint32_t n, const uint32_t* row_ptr, uint32_t r
                                  const float* values, uint32_t values_len,
                                  const uint32_t* block_starts, uint32_t n_starts, float* lu_out);
 
+/* Tile-local SpMV tables of a CSR matrix, as hbj_system_create builds them:
+ * per tile of 256 rows the sorted distinct columns its entries reference
+ * (tile_cnt[t] of them at tcols + tile_off[t]; tile_cnt[t] = 0xFFFFFFFF and no
+ * list when there are more than 2048) and per entry its index in its tile's
+ * list (lidx, nnz entries; 0 in tiles without a list).  tile_off / tile_cnt
+ * hold ceil(n / 256) entries; *n_tcols receives the lists' total length, and
+ * a tcols_cap below it is an error (query with tcols_cap = 0).  No reference
+ * counterpart: the setup of this SpMV's layout, exposed for tests. */
+hbj_status hbj_spmv_tile_tables(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                                const uint32_t* col_idx, uint32_t* tile_off, uint32_t* tile_cnt,
+                                uint16_t* lidx, uint32_t* tcols, uint64_t tcols_cap,
+                                uint64_t* n_tcols);
+
 /* ------------------------------------- single-op executors (device) */
 /* Each runs the same device code as the fused solver kernels on host buffers. */
 
@@ -217,7 +230,9 @@ hbj_status hbj_batch_solve_ir(hbj_batch* batch, const hbj_ir_params* params,
 /* ------------------------------------------------------------ serving */
 
 typedef struct hbj_server_config {
-  uint64_t hbm_budget_bytes;   /* admission limit on device bytes of resident requests */
+  uint64_t hbm_budget_bytes;   /* admission limit on device bytes of resident requests;
+                                  the server allocates it at creation and places
+                                  each resident system in a block of it */
   uint32_t max_active;         /* most systems advanced together */
   uint32_t chunk_iters;        /* iterations between host status polls */
 } hbj_server_config;
@@ -227,11 +242,19 @@ typedef struct hbj_server_stats {
   uint64_t device_bytes_in_use;
   uint64_t peak_device_bytes;
   uint64_t peak_active;
+  uint64_t chunks;         /* chunks of iterations enqueued */
+  uint64_t table_changes;  /* slot-table rebuilds (arrivals / retirements folded in) */
+  uint64_t arena_waits;    /* admissions that waited for a block to come back
+                              (the budget allowed them, no free range was long enough) */
+  double admit_ms;         /* host time spent making requests resident (summed) */
+  double table_ms;         /* worker time spent on table rebuilds (summed) */
 } hbj_server_stats;
 
 /* Continuous-batching request loop: a host thread admits queued requests as
  * HBM budget and slots free up, uploads them while others iterate, advances
- * all resident requests together, and retires each as it converges. */
+ * all resident requests together, and retires each as it converges; the
+ * slot table changes between chunks of iterations without waiting for the
+ * device. */
 hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_server** out);
 void hbj_server_destroy(hbj_server* srv);
 /* Copies the request; returns a ticket.  b64 in FP64; solved with IR (params). */
@@ -244,6 +267,10 @@ hbj_status hbj_server_submit(hbj_server* srv, uint32_t n, uint32_t nnz, const ui
 hbj_status hbj_server_wait(hbj_server* srv, uint64_t ticket, double* x, hbj_result* result,
                            double* latency_ms);
 hbj_status hbj_server_stats_get(hbj_server* srv, hbj_server_stats* stats);
+/* Device bytes a request of these sizes reserves from the budget (its block
+ * in the server's arena, tile-local column lists counted at their largest).
+ * No reference counterpart: sizing of budgets. */
+uint64_t hbj_server_request_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts);
 
 /* Calibration: `reps` launches of a plain streaming copy of `bytes` (read
  * bytes + write bytes per launch); *avg_us = mean launch duration (HIP events).

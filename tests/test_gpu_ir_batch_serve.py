@@ -187,3 +187,33 @@ def test_server_rejects_and_reports_failures(ctx, hbj):
     s = srv.stats()
     assert s["rejected"] == 1 and s["completed"] == 2
     srv.close()
+
+
+def test_server_waits_for_a_free_range(ctx, hbj):
+    """The arena holds exactly the budget.  A (short solve) takes the front,
+    B (long solve) the next block; C fits the budget once A is done but not
+    into either free range (A's old block, the tail), so it waits until B
+    gives its block back -- and every result is still the one solved alone."""
+    print("server arena: sides 10 / 12 / 11, E over 3 decades", flush=True)
+    p = hbj.IrParams(target_rel_res=1e-8)
+    Ls = [hbj.lattice(N, seed=200 + i, e_log10_half_range=1.5) for i, N in enumerate((10, 12, 11))]
+    alone = []
+    for L in Ls:
+        s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+        alone.append(s.solve_ir(L["rhs"].astype(np.float64), None, p))
+        s.close()
+    assert alone[0][1].iterations < alone[1][1].iterations  # A finishes first
+    a, b, c = (hbj.Server.request_bytes(L["n"], L["nnz"], L["block_starts"].size) for L in Ls)
+    assert a < c
+    budget = a + b + c - a // 2  # tail after A and B: c - a/2 < c
+    srv = hbj.Server(ctx, budget, max_active=4, chunk_iters=8)
+    tickets = [srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                          L["rhs"].astype(np.float64), p) for L in Ls]
+    for t, L, (xa, ra) in zip(tickets, Ls, alone):
+        st, x, res, ms = srv.wait(t, n=L["n"])
+        assert st == 0 and res.iterations == ra.iterations
+        assert np.array_equal(bits(x), bits(xa))
+    stats = srv.stats()
+    assert stats["arena_waits"] >= 1
+    assert stats["peak_device_bytes"] <= budget
+    srv.close()

@@ -138,6 +138,28 @@ def build_lu_blocks_from_csr_block_starts_6(n, row_ptr, col_idx, values, block_s
     return out[: 36 * nb]
 
 
+def spmv_tile_tables(row_ptr, col_idx):
+    """Tile-local SpMV tables of a CSR matrix (hbj_spmv_tile_tables): per 256-row
+    tile (offset, count) into the column lists (count 0xFFFFFFFF: no list), each
+    entry's index in its tile's list, and the concatenated sorted lists."""
+    rp, ci = _u32(row_ptr), _u32(col_idx)
+    n, nnz = rp.size - 1, ci.size
+    nt = (n + 255) // 256
+    off = np.zeros(max(nt, 1), np.uint32)
+    cnt = np.zeros(max(nt, 1), np.uint32)
+    lidx = np.zeros(max(nnz, 1), np.uint16)
+    total = ctypes.c_uint64()
+    L = _lib.lib()
+    args = (n, nnz, _nz(rp, np.uint32), _nz(ci, np.uint32), off.ctypes.data, cnt.ctypes.data,
+            lidx.ctypes.data)
+    st = L.hbj_spmv_tile_tables(*args, None, 0, ctypes.byref(total))  # sizes the lists
+    tcols = np.zeros(max(total.value, 1), np.uint32)
+    if st != 0 and total.value > 0:
+        st = L.hbj_spmv_tile_tables(*args, tcols.ctypes.data, total.value, ctypes.byref(total))
+    _check(st)
+    return off[:nt], cnt[:nt], lidx[:nnz], tcols[: total.value]
+
+
 # ------------------------------------------------------------- executors
 
 class VecOpsExecutor:
@@ -439,6 +461,11 @@ class Server:
         if raise_on_error:
             _check(st, res)
         return st, (x[:n] if x is not None else None), res, lat.value
+
+    @staticmethod
+    def request_bytes(n, nnz, n_starts):
+        """Device bytes a request of these sizes reserves from the budget."""
+        return int(_lib.lib().hbj_server_request_bytes(int(n), int(nnz), int(n_starts)))
 
     def stats(self):
         s = hbj_server_stats()

@@ -41,7 +41,8 @@ class hbj_server_config(ctypes.Structure):
 class hbj_server_stats(ctypes.Structure):
     _fields_ = [("submitted", c_u64), ("completed", c_u64), ("rejected", c_u64), ("active", c_u64),
                 ("queued", c_u64), ("device_bytes_in_use", c_u64), ("peak_device_bytes", c_u64),
-                ("peak_active", c_u64)]
+                ("peak_active", c_u64), ("chunks", c_u64), ("table_changes", c_u64),
+                ("arena_waits", c_u64), ("admit_ms", c_f64), ("table_ms", c_f64)]
 
 
 class hbj_system_info(ctypes.Structure):
@@ -64,6 +65,8 @@ _SIGS = {
     "hbj_ctx_describe": (c_i32, [c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
     "hbj_lu_factor_inplace_6": (c_i32, [f32p, c_u32]),
     "hbj_build_lu_blocks_6": (c_i32, [c_u32, u32p, c_u32, u32p, c_u32, f32p, c_u32, u32p, c_u32, f32p]),
+    "hbj_spmv_tile_tables": (c_i32, [c_u32, c_u32, u32p, u32p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_u64, P(c_u64)]),
     "hbj_op_axpy": (c_i32, [c_void_p, c_u32, c_f32, f32p, f32p]),
     "hbj_op_scale": (c_i32, [c_void_p, c_u32, c_f32, f32p]),
     "hbj_op_dot": (c_i32, [c_void_p, c_u32, f32p, f32p, P(c_f32)]),
@@ -96,6 +99,7 @@ _SIGS = {
                                   P(hbj_ir_params), P(c_u64)]),
     "hbj_server_wait": (c_i32, [c_void_p, c_u64, c_void_p, P(hbj_result), P(c_f64)]),
     "hbj_server_stats_get": (c_i32, [c_void_p, P(hbj_server_stats)]),
+    "hbj_server_request_bytes": (c_u64, [c_u32, c_u32, c_u32]),
     "hbj_case_load": (c_i32, [c_char_p, P(c_void_p)]),
     "hbj_case_free": (None, [c_void_p]),
     "hbj_case_view": (c_i32, [c_void_p, P(c_u32), P(c_u32), P(c_u32)] + [P(c_void_p)] * 6),

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "hbj_internal.hpp"
+#include "ranges.hpp"
 
 using namespace hbj;
 
@@ -437,20 +438,41 @@ struct Engine {
     return HBJ_OK;
   }
 
-  // ---- step-wise use (serving): the caller owns the loop.  Slot tables are
-  // only written while the stream is idle (after wait_poll of every chunk).
+  // ---- step-wise use (serving): the caller owns the loop and changes the
+  // table between chunks without waiting for the device.  Slot and tile
+  // writes are copies on the engine stream from pinned staging, so they land
+  // after the chunk in flight and before the next one.  The writes made
+  // before the chunk of pool p is enqueued use staging half p
+  // (stage_begin(p)); the caller starts a half again only after the chunk
+  // that followed its previous writes has been polled, or after
+  // sync_stream().
   uint32_t tiles_cap = 0;
+  struct Staging {
+    SysDesc* desc = nullptr;   // cap records
+    SysState* st = nullptr;    // cap records
+    uint2* tiles = nullptr;    // tiles_cap entries
+    uint32_t used = 0;         // slot records written since stage_begin
+  } stg[2];
 
-  hbj_status reserve_tiles(uint32_t need) {
-    if (need <= tiles_cap) return HBJ_OK;
-    HIPCHK(hipStreamSynchronize(stream));
-    dfree(d_tiles);
-    CHK(dalloc(&d_tiles, need));
-    tiles_cap = need;
+  hbj_status stage_begin(int pool) {
+    Staging& g = stg[pool];
+    if (!g.desc) {
+      CHK(halloc(&g.desc, cap));
+      CHK(halloc(&g.st, cap));
+      CHK(halloc(&g.tiles, tiles_cap));
+    }
+    g.used = 0;
     return HBJ_OK;
   }
 
-  hbj_status write_slot(uint32_t slot, SysDesc d, const SysState& s) {
+  // slot <- (d, s); both of the slot's state copies get s.  One write per
+  // slot between stage_begin calls.
+  hbj_status stage_slot(int pool, uint32_t slot, SysDesc d, const SysState& s) {
+    Staging& g = stg[pool];
+    if (slot >= cap || g.used >= cap) {
+      set_error("engine: slot %u / staging record %u beyond %u slots", slot, g.used, cap);
+      return HBJ_E_INVALID;
+    }
     if (d.n) {  // a system (not an idle slot): the table's dot accumulation follows it
       const uint32_t want = (d.flags & kFlagDot64) ? 1u : 0u;
       if (want != dot64) {
@@ -459,18 +481,43 @@ struct Engine {
       }
     }
     d.st = d_states + slot;
-    HIPCHK(hipMemcpy(d_descs + slot, &d, sizeof d, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_states + slot, &s, sizeof s, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_states + cap + slot, &s, sizeof s, hipMemcpyHostToDevice));
+    const uint32_t k = g.used++;
+    g.desc[k] = d;
+    g.st[k] = s;
+    HIPCHK(hipMemcpyAsync(d_descs + slot, &g.desc[k], sizeof d, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_states + slot, &g.st[k], sizeof s, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_states + cap + slot, &g.st[k], sizeof s, hipMemcpyHostToDevice, stream));
     return HBJ_OK;
   }
 
-  hbj_status write_tiles(const std::vector<uint2>& tiles) {
-    CHK(reserve_tiles((uint32_t)tiles.size()));
-    if (!tiles.empty())
-      HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
-    n_tiles = (uint32_t)tiles.size();
+  // The tile table of the next chunks.  Growing it waits for the stream (the
+  // chunk in flight reads the table) and at least doubles it.
+  hbj_status stage_tiles(int pool, const std::vector<uint2>& tiles) {
+    const uint32_t need = (uint32_t)tiles.size();
+    if (need > tiles_cap) {
+      HIPCHK(hipStreamSynchronize(stream));
+      const uint32_t nc = need > 2 * tiles_cap ? need : 2 * tiles_cap;
+      dfree(d_tiles);
+      CHK(dalloc(&d_tiles, nc));
+      for (Staging& g : stg)
+        if (g.tiles) {
+          hfree(g.tiles);
+          CHK(halloc(&g.tiles, nc));
+        }
+      tiles_cap = nc;
+    }
+    if (need) {
+      std::memcpy(stg[pool].tiles, tiles.data(), sizeof(uint2) * need);
+      HIPCHK(hipMemcpyAsync(d_tiles, stg[pool].tiles, sizeof(uint2) * need, hipMemcpyHostToDevice,
+                            stream));
+    }
+    n_tiles = need;
     drop_graph();
+    return HBJ_OK;
+  }
+
+  hbj_status sync_stream() {
+    HIPCHK(hipStreamSynchronize(stream));
     return HBJ_OK;
   }
 
@@ -497,6 +544,9 @@ struct Engine {
       if (ev[b]) (void)hipEventDestroy(ev[b]);
       ev[b] = nullptr;
       hfree(h_poll[b]);
+      hfree(stg[b].desc);
+      hfree(stg[b].st);
+      hfree(stg[b].tiles);
     }
     hfree(h_init);
     dfree(d_descs);
@@ -504,6 +554,58 @@ struct Engine {
     dfree(d_states);
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
+  }
+};
+
+// A server's device memory: one allocation of its HBM budget, handed out in
+// blocks (FreeRanges).  Taking and returning blocks touch no HIP API, so
+// retiring a system never waits for the device, as hipFree does (it returns
+// only once all work on the device has finished, other streams' included).
+struct DeviceArena {
+  char* base = nullptr;
+  uint64_t size = 0;
+  std::mutex mu;
+  FreeRanges ranges;
+
+  hbj_status init(uint64_t bytes) {
+    size = FreeRanges::round(bytes);
+    CHK(dalloc(&base, size));
+    ranges.reset(size);
+    return HBJ_OK;
+  }
+  bool take(uint64_t bytes, uint64_t* off) {
+    std::lock_guard<std::mutex> g(mu);
+    return ranges.take(bytes, off);
+  }
+  void give(uint64_t off, uint64_t bytes) {
+    std::lock_guard<std::mutex> g(mu);
+    ranges.give(off, bytes);
+  }
+  void destroy() {
+    dfree(base);
+    ranges.reset(0);
+    size = 0;
+  }
+};
+
+// Offsets of a block's pieces (256-byte aligned, in the order added); assign()
+// points each piece's pointer into the block.
+struct BlockLayout {
+  struct Piece {
+    void** ptr;
+    uint64_t off;
+  };
+  std::vector<Piece> pieces;
+  uint64_t size = 0;
+  template <typename T>
+  void add(T** p, uint64_t count) {
+    if (count == 0) count = 1;
+    size = (size + 255) & ~255ull;
+    pieces.push_back(Piece{reinterpret_cast<void**>(p), size});
+    size += count * sizeof(T);
+  }
+  void assign(char* base) const {
+    for (const Piece& pc : pieces) *pc.ptr = base + pc.off;
   }
 };
 
@@ -524,7 +626,7 @@ struct hbj_system {
   double* resid2 = nullptr;  // 2 * n_tiles FP64 partials of the residual check
   double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
   float* b32 = nullptr;
-  double* h_part2 = nullptr;  // pinned, 2 * n_tiles
+  double* h_part2 = nullptr;  // pinned, 2 * n_tiles (allocated by the first residual check)
   uint2* tile_blk = nullptr;  // per tile: first block touching it, count
   uint16_t* lidx = nullptr;   // tile-local SpMV tables (build_tile_local)
   TileHdr* thdr = nullptr;
@@ -537,6 +639,42 @@ struct hbj_system {
   bool eng_ready = false;
   int eng_mode = -1;  // 0 plain FP32, 1 IR
   SysDesc* d_desc1 = nullptr;  // plain descriptor for the FP64 check kernel
+  // Every device buffer above is a piece of one block: from hipMalloc, or
+  // from a server's arena.  The FP64 refinement buffers are in it when the
+  // system was created for refinement, else in a block of their own
+  // allocated by the first refinement call.
+  char* block = nullptr;
+  uint64_t block_bytes = 0;
+  DeviceArena* arena = nullptr;
+  uint64_t arena_off = 0;
+  char* ir_block = nullptr;
+
+  // The pieces of the main block for a system of n rows, nnz entries,
+  // n_starts block starts and n_tcols tile-local columns.
+  void layout(BlockLayout& L, uint32_t n_, uint32_t nnz_, uint32_t n_starts, uint64_t n_tcols,
+              bool with_ir) {
+    const uint32_t nt = n_ ? (n_ + kTile - 1) / kTile : 1u;
+    const uint64_t nb_ = n_starts ? n_starts - 1 : 0;
+    L.add(&rp, n_ + 1ull);
+    L.add(&col, nnz_ + 4ull);  // 4 entries of padding for 16-byte loads
+    L.add(&val, nnz_ + 4ull);
+    L.add(&bs, (uint64_t)n_starts);
+    L.add(&lu, 36 * (nb_ ? nb_ : 1));
+    for (float** v : {&x, &x0, &b, &p, &q, &z, &ra, &rb}) L.add(v, n_);
+    L.add(&part, 3ull * nt);
+    L.add(&part64, 5ull * nt);
+    L.add(&resid2, 2ull * nt);
+    L.add(&d_desc1, 1);
+    L.add(&tile_blk, nt);
+    L.add(&lidx, nnz_ + 8ull);
+    L.add(&thdr, nt);
+    L.add(&tcols, n_tcols);
+    if (with_ir) layout_ir(L, n_);
+  }
+  void layout_ir(BlockLayout& L, uint32_t n_) {
+    for (double** v : {&x64, &x0_64, &b64, &r64}) L.add(v, n_);
+    L.add(&b32, n_);
+  }
 
   SysDesc desc(bool ir) const {
     SysDesc d{};
@@ -588,11 +726,11 @@ struct hbj_system {
   hbj_status ensure_ir_buffers() {
     if (x64) return HBJ_OK;
     HIPCHK(hipSetDevice(ctx->device));
-    CHK(dalloc(&x64, n, &bytes));
-    CHK(dalloc(&x0_64, n, &bytes));
-    CHK(dalloc(&b64, n, &bytes));
-    CHK(dalloc(&r64, n, &bytes));
-    CHK(dalloc(&b32, n, &bytes));
+    BlockLayout L;
+    layout_ir(L, n);
+    CHK(dalloc(&ir_block, L.size));
+    bytes += L.size;
+    L.assign(ir_block);
     HIPCHK(hipMemset(x0_64, 0, sizeof(double) * n));
     HIPCHK(hipMemset(b64, 0, sizeof(double) * n));
     return HBJ_OK;
@@ -615,6 +753,7 @@ struct hbj_system {
 
   // ||b - A x|| / ||b|| in FP64 for the FP32 solution in `x` (plain mode).
   hbj_status true_residual(hipStream_t s, double* out) {
+    if (!h_part2) CHK(halloc(&h_part2, 2ull * n_tiles));
     launch_true_resid(d_desc1, x, resid2, n_tiles, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h_part2, resid2, sizeof(double) * 2 * n_tiles, hipMemcpyDeviceToHost, s));
@@ -628,17 +767,28 @@ struct hbj_system {
     return HBJ_OK;
   }
 
+  // A system whose block came from an arena and that was never solved on
+  // its own (no engine, no residual check, no separate refinement block)
+  // returns its block without any HIP call.
   void release() {
     if (eng_ready) eng.destroy();
     eng_ready = false;
-    dfree(rp); dfree(col); dfree(bs); dfree(val); dfree(lu);
-    dfree(x); dfree(x0); dfree(b); dfree(p); dfree(q); dfree(z);
-    dfree(ra); dfree(rb); dfree(part); dfree(part64); dfree(resid2);
-    dfree(x64); dfree(x0_64); dfree(b64); dfree(r64); dfree(b32);
-    dfree(d_desc1);
-    dfree(tile_blk);
-    dfree(lidx); dfree(thdr); dfree(tcols);
+    if (arena) {
+      arena->give(arena_off, block_bytes);
+      arena = nullptr;
+      block = nullptr;
+    }
+    dfree(block);
+    dfree(ir_block);
     hfree(h_part2);
   }
 };
 
+namespace hbj {
+// hbj_system_create with its options: with_ir puts the FP64 refinement
+// buffers into the system's block (zeroed); arena (optional) supplies the
+// block instead of hipMalloc.
+hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                         const uint32_t* col_idx, const float* values, const uint32_t* block_starts,
+                         uint32_t n_starts, bool with_ir, DeviceArena* arena, hbj_system** out);
+}  // namespace hbj

@@ -334,6 +334,17 @@ hbj_status hbj_op_update_scalars(hbj_ctx* ctx, float* slots, uint32_t n_slots, u
 hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
                              const uint32_t* col_idx, const float* values,
                              const uint32_t* block_starts, uint32_t n_starts, hbj_system** out) {
+  return hbj::system_create(ctx, n, nnz, row_ptr, col_idx, values, block_starts, n_starts, false,
+                            nullptr, out);
+}
+
+}  // extern "C"
+
+namespace hbj {
+
+hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                         const uint32_t* col_idx, const float* values, const uint32_t* block_starts,
+                         uint32_t n_starts, bool with_ir, DeviceArena* arena, hbj_system** out) {
   clear_error();
   if (!ctx || !out || (nnz && !values)) {
     set_error("hbj_system_create: bad argument");
@@ -388,35 +399,23 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
     set_error("hipSetDevice failed: %s", hipGetErrorString(e));
     return fail(HBJ_E_DEVICE);
   }
-  hbj_status st = HBJ_OK;
   const uint32_t nt = s->n_tiles ? s->n_tiles : 1;
-#define ALLOC(ptr, cnt) \
-  if (st == HBJ_OK) st = dalloc(&s->ptr, (cnt), &s->bytes)
-  ALLOC(rp, n + 1);
-  ALLOC(col, nnz + 4ull);  // 4 entries of padding for 16-byte loads
-  ALLOC(val, nnz + 4ull);
-  ALLOC(bs, n_starts);
-  ALLOC(lu, 36ull * (nb ? nb : 1));
-  ALLOC(x, n);
-  ALLOC(x0, n);
-  ALLOC(b, n);
-  ALLOC(p, n);
-  ALLOC(q, n);
-  ALLOC(z, n);
-  ALLOC(ra, n);
-  ALLOC(rb, n);
-  ALLOC(part, 3ull * nt);
-  ALLOC(part64, 5ull * nt);
-  ALLOC(resid2, 2ull * nt);
-  ALLOC(d_desc1, 1);
-  ALLOC(tile_blk, nt);
-  ALLOC(lidx, lidx_h.size());
-  ALLOC(thdr, nt);
-  ALLOC(tcols, tcols_h.size());
-#undef ALLOC
-  if (st != HBJ_OK) return fail(st);
-  st = halloc(&s->h_part2, 2ull * nt);
-  if (st != HBJ_OK) return fail(st);
+  BlockLayout L;
+  s->layout(L, n, nnz, n_starts, tcols_h.size(), with_ir);
+  if (arena) {
+    if (!arena->take(L.size, &s->arena_off)) {
+      set_error("device arena: no free range of %llu bytes", (unsigned long long)L.size);
+      return fail(HBJ_E_OOM);
+    }
+    s->arena = arena;
+    s->block = arena->base + s->arena_off;
+  } else {
+    hbj_status st = dalloc(&s->block, L.size);
+    if (st != HBJ_OK) return fail(st);
+  }
+  s->block_bytes = L.size;
+  s->bytes = L.size;
+  L.assign(s->block);
   auto up = [&](void* dst, const void* src, size_t bytes) {
     if (bytes == 0 || e != hipSuccess) return;
     e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -475,6 +474,9 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   if (e == hipSuccess && n) e = hipMemset(s->z, 0, 4ull * n);  // wgpu zero-initialises buffers
   if (e == hipSuccess && n) e = hipMemset(s->x0, 0, 4ull * n);
   if (e == hipSuccess && n) e = hipMemset(s->b, 0, 4ull * n);
+  if (with_ir && n)  // x64 starts at x0_64 = 0 and b64 = 0 until a right-hand side is set
+    for (double* v : {s->x64, s->x0_64, s->b64})
+      if (e == hipSuccess) e = hipMemset(v, 0, 8ull * n);
   if (e == hipSuccess) {
     SysDesc d1 = s->desc(false);
     e = hipMemcpy(s->d_desc1, &d1, sizeof d1, hipMemcpyHostToDevice);
@@ -486,6 +488,10 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
   *out = s;
   return HBJ_OK;
 }
+
+}  // namespace hbj
+
+extern "C" {
 
 void hbj_system_destroy(hbj_system* s) {
   if (!s) return;

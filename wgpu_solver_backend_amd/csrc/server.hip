@@ -3,14 +3,20 @@
 // Two host threads share one device:
 //   admitter  takes queued requests in arrival order while the HBM budget
 //             and the slot count allow, and turns each into a resident
-//             system (validation, LU blocks, uploads, FP64 b) -- this host
-//             work overlaps the iterations already running on the GPU;
+//             system (validation, tile tables, uploads, LU blocks, FP64 b)
+//             in a block of the server's device arena -- this host work
+//             overlaps the iterations already running on the GPU;
 //   worker    owns the slot table.  It enqueues chunks of PCG iterations
-//             (one launch of each kernel covers every resident request),
-//             polls the per-slot status words of the previous chunk while the
-//             next one runs, retires requests whose status is final (x copied
-//             out, latency stamped) and folds newly prepared requests into
-//             the table between chunks.
+//             (one launch of each kernel covers every resident request) and
+//             always has the next chunk queued before it waits for the
+//             previous one's status poll.  Arrivals and retirements change
+//             the table between two enqueues, without waiting for the
+//             device: the slot and tile writes are copies on the engine's
+//             stream, so they land between the chunk in flight and the next.
+//             A finished request's x is copied out at once; its system's
+//             block goes back to the arena once the last chunk that may
+//             still reference the system has been polled (no hipFree, which
+//             would wait for the device).
 // Each request's arithmetic only depends on its own system (tile partials and
 // last-tile reductions are per system), so its result is the same alone or
 // in any mix.
@@ -54,8 +60,13 @@ struct hbj_server {
   uint64_t next_ticket = 1;
   uint64_t bytes_in_use = 0;      // reserved by admitted requests
   uint32_t resident = 0;          // admitted and not yet released
+  DeviceArena arena;              // device memory of the resident systems
+  uint64_t arena_gives = 0;       // systems whose blocks went back to the arena
+  bool arena_blocked = false;     // the queue head found no free range ...
+  uint64_t blocked_at = 0;        // ... when arena_gives had this value
   hbj_server_stats st{};
   bool stop = false;
+  bool failed = false;            // the worker stopped on a device failure
   std::thread admitter, worker;
   Engine eng;
 };
@@ -76,27 +87,41 @@ void admitter_main(hbj_server* S) {
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
     S->cv_admit.wait(lk, [&] {
-      if (S->stop && S->queue.empty()) return true;
+      if (S->failed || (S->stop && S->queue.empty())) return true;
       if (S->queue.empty()) return false;
+      if (S->arena_blocked && S->arena_gives == S->blocked_at) return false;
       const Request* h = S->queue.front();
       return S->resident < S->cfg.max_active && S->bytes_in_use + h->est_bytes <= S->cfg.hbm_budget_bytes;
     });
-    if (S->queue.empty()) break;  // stop requested and nothing left
+    if (S->failed || S->queue.empty()) break;  // device failure, or stop requested and nothing left
     Request* r = S->queue.front();
     S->queue.pop_front();
     S->bytes_in_use += r->est_bytes;
     S->resident += 1;
+    S->arena_blocked = false;
     lk.unlock();
+    const double t0 = now_ms();
     hbj_system* sys = nullptr;
-    hbj_status s = hbj_system_create(S->ctx, r->n, r->nnz, r->rp.data(), r->col.data(), r->val.data(),
-                                     r->bs.data(), r->n_starts, &sys);
+    hbj_status s = system_create(S->ctx, r->n, r->nnz, r->rp.data(), r->col.data(), r->val.data(),
+                                 r->bs.data(), r->n_starts, true, &S->arena, &sys);
     if (s == HBJ_OK) s = hbj_system_set_rhs64(sys, r->b64.data(), nullptr);
     std::string err = s == HBJ_OK ? "" : hbj_last_error();
+    const double t1 = now_ms();
     lk.lock();
+    S->st.admit_ms += t1 - t0;
     if (s != HBJ_OK) {
       if (sys) hbj_system_destroy(sys);
       S->bytes_in_use -= r->est_bytes;
       S->resident -= 1;
+      if (s == HBJ_E_OOM && !sys && S->resident > 0) {
+        // the arena has no free range this large right now (its free space
+        // is split): retry once a resident system has given its block back
+        S->queue.push_front(r);
+        S->arena_blocked = true;
+        S->blocked_at = S->arena_gives;
+        S->st.arena_waits += 1;
+        continue;
+      }
       finish(S, r, s, err);
       S->cv_admit.notify_all();
       continue;
@@ -119,34 +144,32 @@ hbj_status worker_loop(hbj_server* S) {
   HIPCHK(hipSetDevice(S->ctx->device));
   Engine& E = S->eng;
   const uint32_t cap = S->cfg.max_active;
-  // A request belongs to the client once finish() marks it done (the client
-  // may then free it), so the worker keeps what it still needs -- the slot
-  // and the system, which an in-flight chunk may still reference -- apart.
-  struct Retire {
-    uint32_t slot;
+  const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 16;
+  // Chunks are numbered in enqueue order.  A request placed in a slot is
+  // advanced from chunk since[slot] on, so polls of earlier chunks (which
+  // show the slot's previous request) are not read for it.
+  std::vector<Request*> slots(cap, nullptr);
+  std::vector<uint64_t> since(cap, 0);
+  // Retired systems: a chunk enqueued before the retirement may still
+  // reference them; their blocks go back once chunk `after` has been polled.
+  struct Grave {
     hbj_system* sys;
     uint64_t bytes;
+    uint64_t after;
   };
-  std::vector<Request*> slots(cap, nullptr);        // running requests
-  std::vector<hbj_system*> slot_sys(cap, nullptr);  // the slot's system (running or retiring)
-  std::vector<Retire> retiring;                     // final, still referenced by an in-flight chunk
-  int inflight = -1;                                // pool of the chunk whose poll is pending
+  std::vector<Grave> graves;
+  uint64_t next_chunk = 0;
+  int inflight = -1;  // pool of the chunk whose poll is pending
+  uint64_t inflight_id = 0;
   int cur = 0;
   uint32_t active = 0;
-  const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 16;
+  bool table_dirty = false;
 
-  auto drain = [&]() -> hbj_status {
-    if (inflight >= 0) {
-      CHK(E.wait_poll(inflight));
-      inflight = -1;
-    }
-    return HBJ_OK;
-  };
   // retire every slot whose polled status is final
-  auto harvest = [&](const SysState* polled) -> hbj_status {
+  auto harvest = [&](const SysState* polled, uint64_t chunk_id) -> hbj_status {
     for (uint32_t i = 0; i < cap; ++i) {
       Request* r = slots[i];
-      if (!r || polled[i].status == kRunning) continue;
+      if (!r || since[i] > chunk_id || polled[i].status == kRunning) continue;
       hbj_result res{};
       const SysState& s = polled[i];
       res.status = s.status;
@@ -157,8 +180,9 @@ hbj_status worker_loop(hbj_server* S) {
       res.rel_res_recurrence = s.b_norm2 > 0 ? std::sqrt((double)s.rr / (double)s.b_norm2) : 0.0;
       res.rel_res_true_fp64 = s.bb64 > 0 ? std::sqrt(s.rr64 / s.bb64) : 0.0;
       r->x.assign(r->n, 0.0);
-      // the in-flight chunk no longer writes this system (its status is final)
+      // the chunks still queued no longer write this system (its status is final)
       CHK(hbj_system_get_x64(r->sys, r->x.data()));
+      graves.push_back(Grave{r->sys, r->est_bytes, next_chunk - 1});
       std::lock_guard<std::mutex> g(S->mu);
       r->res = res;
       r->res.solve_ms = now_ms() - r->t_submit;
@@ -169,20 +193,41 @@ hbj_status worker_loop(hbj_server* S) {
         err = "PCG(BlockJacobiGpu): dot(p,Ap) is zero (breakdown)";
       else if (s.status == HBJ_E_BREAKDOWN_RZ)
         err = "PCG(BlockJacobiGpu): rz_old is zero (breakdown)";
-      retiring.push_back(Retire{i, r->sys, r->est_bytes});
       r->sys = nullptr;
       slots[i] = nullptr;
       finish(S, r, s.status, err);  // r is the client's from here on
       --active;
+      table_dirty = true;
     }
     return HBJ_OK;
+  };
+  // return the blocks of systems no chunk after `polled_id` references
+  auto bury = [&](uint64_t polled_id) {
+    size_t keep = 0;
+    uint64_t freed = 0, count = 0;
+    for (const Grave& g : graves) {
+      if (g.after <= polled_id) {
+        hbj_system_destroy(g.sys);  // an arena system: no HIP call
+        freed += g.bytes;
+        ++count;
+      } else {
+        graves[keep++] = g;
+      }
+    }
+    graves.resize(keep);
+    if (!count) return;
+    std::lock_guard<std::mutex> g(S->mu);
+    S->bytes_in_use -= freed;
+    S->resident -= (uint32_t)count;
+    S->arena_gives += count;
+    S->cv_admit.notify_all();
   };
 
   for (;;) {
     std::vector<Request*> incoming;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (active == 0 && retiring.empty() && inflight < 0) {
+      if (active == 0 && inflight < 0) {
         S->cv_work.wait(lk, [&] { return !S->ready.empty() || (S->stop && S->queue.empty() && S->resident == 0); });
         if (S->ready.empty() && S->stop && S->queue.empty() && S->resident == 0) break;
       }
@@ -191,59 +236,55 @@ hbj_status worker_loop(hbj_server* S) {
         S->ready.pop_front();
       }
     }
-    if (!incoming.empty() || !retiring.empty()) {
-      // table change: let the in-flight chunk finish first
-      if (inflight >= 0) {
-        const int p = inflight;
-        CHK(drain());
-        CHK(harvest(E.h_poll[p]));
-      }
-      for (const Retire& rt : retiring) {
-        slot_sys[rt.slot] = nullptr;
-        SysState idle;
-        std::memset(&idle, 0, sizeof idle);  // status 0: never iterated again
-        CHK(E.write_slot(rt.slot, SysDesc{}, idle));
-        hbj_system_destroy(rt.sys);
-        std::lock_guard<std::mutex> g(S->mu);
-        S->bytes_in_use -= rt.bytes;
-        S->resident -= 1;
-        S->cv_admit.notify_all();
-      }
-      retiring.clear();
+    bool staged = false;
+    if (!incoming.empty() || table_dirty) {
+      const double t0 = now_ms();
+      CHK(E.stage_begin(cur));
       for (Request* r : incoming) {
         uint32_t slot = 0;
-        while (slot_sys[slot]) ++slot;  // resident <= cap guarantees a free slot
+        while (slots[slot]) ++slot;  // resident <= cap guarantees a free slot
         slots[slot] = r;
-        slot_sys[slot] = r->sys;
+        since[slot] = next_chunk;
         r->slot = (int)slot;
-        CHK(r->sys->ensure_ir_buffers());
-        if (r->sys->flags & kFlagOversize) HIPCHK(hipMemset(r->sys->z, 0, 4ull * r->n));
-        HIPCHK(hipMemset(r->sys->x64, 0, 8ull * r->n));
-        CHK(E.write_slot(slot, r->sys->desc(true), r->sys->init_state(true, 0, 0, 0, &r->params)));
+        CHK(E.stage_slot(cur, slot, r->sys->desc(true), r->sys->init_state(true, 0, 0, 0, &r->params)));
         ++active;
       }
       std::vector<uint2> tiles;
       for (uint32_t i = 0; i < cap; ++i)
         if (slots[i])
           for (uint32_t t = 0; t < slots[i]->sys->n_tiles; ++t) tiles.push_back(make_uint2(i, t));
-      CHK(E.write_tiles(tiles));
-      {
-        std::lock_guard<std::mutex> g(S->mu);
-        S->st.active = active;
-        if (active > S->st.peak_active) S->st.peak_active = active;
-      }
+      CHK(E.stage_tiles(cur, tiles));
+      table_dirty = false;
+      staged = true;
+      std::lock_guard<std::mutex> g(S->mu);
+      S->st.active = active;
+      if (active > S->st.peak_active) S->st.peak_active = active;
+      S->st.table_changes += 1;
+      S->st.table_ms += now_ms() - t0;
     }
-    if (active == 0) continue;
-    CHK(E.enqueue_poll(T, cur));
+    int enq = -1;
+    uint64_t enq_id = 0;
+    if (active > 0) {
+      CHK(E.enqueue_poll(T, cur));
+      enq = cur;
+      enq_id = next_chunk++;
+    } else if (staged) {
+      CHK(E.sync_stream());  // nothing follows the staged copies: let them land
+    }
     if (inflight >= 0) {
-      const int p = inflight;
-      CHK(E.wait_poll(p));
-      CHK(harvest(E.h_poll[p]));
+      CHK(E.wait_poll(inflight));
+      CHK(harvest(E.h_poll[inflight], inflight_id));
+      bury(inflight_id);
+      inflight = -1;
     }
-    inflight = cur;
-    cur ^= 1;
+    if (enq >= 0) {
+      inflight = enq;
+      inflight_id = enq_id;
+      cur ^= 1;
+    }
     std::lock_guard<std::mutex> g(S->mu);
     S->st.active = active;
+    if (enq >= 0) S->st.chunks += 1;
   }
   return HBJ_OK;
 }
@@ -255,21 +296,23 @@ void worker_main(hbj_server* S) {
     const std::string err = hbj_last_error();
     std::lock_guard<std::mutex> g(S->mu);
     S->stop = true;
+    S->failed = true;
     for (auto& kv : S->reqs)
       if (!kv.second->done) finish(S, kv.second, s, err);
     S->cv_admit.notify_all();
   }
 }
 
+// The block a request's system takes in the arena, with its tile-local
+// column lists at their largest (min(nnz, kLocalMax) per tile): an upper
+// bound of the block system_create carves once the lists are built.
 uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
-  auto c = [](uint64_t k) { return k ? k : 1; };
-  const uint64_t nt = c((n + kTile - 1) / kTile), nb = n_starts ? n_starts - 1 : 0;
-  // tile-local SpMV tables: 2 bytes per entry, a list of at most
-  // min(nnz, kLocalMax) columns per tile, a 32-byte header per tile
-  const uint64_t tcols = std::min<uint64_t>(c(nnz), (uint64_t)kLocalMax * nt);
-  return 4 * c(n + 1) + 8 * (uint64_t)(nnz + 4) + 4 * c(n_starts) + 4 * c(36 * (nb ? nb : 1)) + 8 * 4 * c(n) +
-         4 * 3 * nt + 8 * 5 * nt + 8 * 2 * nt + sizeof(SysDesc) + 8 * nt + 4 * 8 * c(n) + 4 * c(n) +
-         2 * ((uint64_t)nnz + 8) + sizeof(TileHdr) * nt + 4 * tcols;
+  const uint64_t nt = n ? (n + kTile - 1) / kTile : 1u;
+  const uint64_t tcols = std::min<uint64_t>(nnz ? nnz : 1u, (uint64_t)kLocalMax * nt);
+  hbj_system probe;
+  BlockLayout L;
+  probe.layout(L, n, nnz, n_starts, tcols, true);
+  return (L.size + 255) & ~255ull;
 }
 
 }  // namespace
@@ -289,19 +332,27 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   // even: a system's state copy 0 is current at every chunk boundary
   S->cfg.chunk_iters = S->cfg.chunk_iters < 2 ? 2u : (S->cfg.chunk_iters & ~1u);
   hipError_t e = hipSetDevice(ctx->device);
-  hbj_status s = e == hipSuccess ? S->eng.init(ctx->device, cfg->max_active, 1) : HBJ_E_DEVICE;
+  // tile table sized for the budget (a 256-row tile holds at least 72 bytes
+  // per row of vectors), at most 2^18 tiles to start; it grows if needed
+  const uint64_t tiles0 = std::min<uint64_t>(cfg->hbm_budget_bytes / (72ull * kTile) + 1, 1ull << 18);
+  hbj_status s = e == hipSuccess ? S->eng.init(ctx->device, cfg->max_active, (uint32_t)tiles0) : HBJ_E_DEVICE;
   if (s == HBJ_OK) {
     S->eng.use_graph = false;  // the table changes between chunks
     S->eng.n_tiles = 0;
-    // every slot starts idle: a descriptor whose state says "finished", so
-    // the per-system kernels (one workgroup per slot) skip it
-    SysState idle;
-    std::memset(&idle, 0, sizeof idle);
-    for (uint32_t i = 0; i < cfg->max_active && s == HBJ_OK; ++i)
-      s = S->eng.write_slot(i, SysDesc{}, idle);
+    // slots start zeroed (status 0: final); no tile references a free slot
+    const uint32_t cap = S->eng.cap;
+    e = hipMemset(S->eng.d_descs, 0, sizeof(SysDesc) * cap);
+    if (e == hipSuccess) e = hipMemset(S->eng.d_states, 0, sizeof(SysState) * 2 * cap);
+    if (e != hipSuccess) {
+      set_error("hbj_server_create: %s", hipGetErrorString(e));
+      s = HBJ_E_DEVICE;
+    }
   }
+  // the resident systems' device memory: one allocation of the budget
+  if (s == HBJ_OK) s = S->arena.init(cfg->hbm_budget_bytes);
   if (s != HBJ_OK) {
     S->eng.destroy();
+    S->arena.destroy();
     delete S;
     return s;
   }
@@ -331,6 +382,7 @@ void hbj_server_destroy(hbj_server* S) {
     delete kv.second;
   }
   S->eng.destroy();
+  S->arena.destroy();
   delete S;
 }
 
@@ -407,6 +459,10 @@ hbj_status hbj_server_wait(hbj_server* S, uint64_t ticket, double* x, hbj_result
   S->reqs.erase(it);
   delete r;
   return (hbj_status)st;
+}
+
+uint64_t hbj_server_request_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
+  return estimate_bytes(n, nnz, n_starts);
 }
 
 hbj_status hbj_server_stats_get(hbj_server* S, hbj_server_stats* out) {

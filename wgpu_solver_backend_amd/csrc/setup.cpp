@@ -93,6 +93,58 @@ static void parallel_ranges(uint32_t count, uint32_t serial, Fn fn) {
   for (auto& x : th) x.join();
 }
 
+// One tile's table: its sorted distinct columns into u (false and u cleared
+// when there are more than kLocalMax), each entry's index in u into lidx.
+// Distinct columns are bits of a bitmap over [cmin, cmax], so the list comes
+// out sorted and an entry's index is the number of set bits below its column
+// (a per-word prefix count plus a popcount): linear in the entries and the
+// span / 64.  A tile whose columns spread over more than 64 times twice its
+// entry count (plus 4096 words) sorts its columns instead.
+static bool tile_table(const uint32_t* col, uint32_t e0, uint32_t e1, std::vector<uint64_t>& bits,
+                       std::vector<uint32_t>& rank, std::vector<uint32_t>& u, uint16_t* lidx) {
+  u.clear();
+  if (e0 == e1) return true;
+  uint32_t cmin = col[e0], cmax = col[e0];
+  for (uint32_t e = e0 + 1; e < e1; ++e) {
+    cmin = col[e] < cmin ? col[e] : cmin;
+    cmax = col[e] > cmax ? col[e] : cmax;
+  }
+  const uint32_t words = ((cmax - cmin) >> 6) + 1;
+  if ((uint64_t)words > 2ull * (e1 - e0) + 4096ull) {
+    u.assign(col + e0, col + e1);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    if (u.size() > kLocalMax) {
+      u.clear();
+      return false;
+    }
+    for (uint32_t e = e0; e < e1; ++e)
+      lidx[e] = (uint16_t)(std::lower_bound(u.begin(), u.end(), col[e]) - u.begin());
+    return true;
+  }
+  bits.assign(words, 0ull);
+  for (uint32_t e = e0; e < e1; ++e) {
+    const uint32_t c = col[e] - cmin;
+    bits[c >> 6] |= 1ull << (c & 63u);
+  }
+  rank.resize(words);
+  uint32_t total = 0;
+  for (uint32_t w = 0; w < words; ++w) {
+    rank[w] = total;
+    total += (uint32_t)__builtin_popcountll(bits[w]);
+  }
+  if (total > kLocalMax) return false;
+  u.reserve(total);
+  for (uint32_t w = 0; w < words; ++w)
+    for (uint64_t m = bits[w]; m; m &= m - 1)
+      u.push_back(cmin + (w << 6) + (uint32_t)__builtin_ctzll(m));
+  for (uint32_t e = e0; e < e1; ++e) {
+    const uint32_t c = col[e] - cmin;
+    lidx[e] = (uint16_t)(rank[c >> 6] + (uint32_t)__builtin_popcountll(bits[c >> 6] & ((1ull << (c & 63u)) - 1ull)));
+  }
+  return true;
+}
+
 void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
                       std::vector<uint16_t>& lidx, std::vector<uint2>& tloc,
                       std::vector<uint32_t>& tcols) {
@@ -101,18 +153,14 @@ void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
   std::vector<std::vector<uint32_t>> lists(nt);
   tloc.assign(nt, make_uint2(0u, 0u));
   lidx.assign((size_t)nnz + 8u, 0u);
-  // sorted distinct columns of each tile
   parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
+    std::vector<uint64_t> bits;
+    std::vector<uint32_t> rank;
     for (uint32_t t = t0; t < t1; ++t) {
       const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
-      std::vector<uint32_t>& u = lists[t];
-      u.assign(col + row_ptr[r0], col + row_ptr[r1]);
-      std::sort(u.begin(), u.end());
-      u.erase(std::unique(u.begin(), u.end()), u.end());
-      if (u.size() > kLocalMax) {
-        u.clear();
+      // entries of a tile without a list keep index 0
+      if (!tile_table(col, row_ptr[r0], row_ptr[r1], bits, rank, lists[t], lidx.data()))
         tloc[t].y = kLocalOff;
-      }
     }
   });
   size_t total = 0;
@@ -122,16 +170,9 @@ void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
     total += lists[t].size();
   }
   tcols.resize(total ? total : 1u, 0u);
-  // each entry's position in its tile's list
   parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
-    for (uint32_t t = t0; t < t1; ++t) {
-      const std::vector<uint32_t>& u = lists[t];
-      if (u.empty()) continue;
-      std::copy(u.begin(), u.end(), tcols.begin() + tloc[t].x);
-      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
-      for (uint32_t e = row_ptr[r0]; e < row_ptr[r1]; ++e)
-        lidx[e] = (uint16_t)(std::lower_bound(u.begin(), u.end(), col[e]) - u.begin());
-    }
+    for (uint32_t t = t0; t < t1; ++t)
+      std::copy(lists[t].begin(), lists[t].end(), tcols.begin() + tloc[t].x);
   });
 }
 
@@ -261,6 +302,40 @@ hbj_status hbj_build_lu_blocks_6(uint32_t n, const uint32_t* row_ptr, uint32_t r
       return HBJ_E_ZERO_PIVOT;
     }
   }
+  return HBJ_OK;
+}
+
+hbj_status hbj_spmv_tile_tables(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                                const uint32_t* col_idx, uint32_t* tile_off, uint32_t* tile_cnt,
+                                uint16_t* lidx, uint32_t* tcols, uint64_t tcols_cap,
+                                uint64_t* n_tcols) {
+  clear_error();
+  const uint32_t bs[2] = {0u, n};
+  const hbj_status v = validate_case(n, nnz, row_ptr, col_idx, bs, n ? 2u : 1u);
+  if (v != HBJ_OK) return v;
+  const uint32_t nt = (n + kTile - 1) / kTile;
+  if ((nt && (!tile_off || !tile_cnt)) || (nnz && !lidx) || !n_tcols) {
+    set_error("hbj_spmv_tile_tables: null output");
+    return HBJ_E_INVALID;
+  }
+  std::vector<uint16_t> lh;
+  std::vector<uint2> tl;
+  std::vector<uint32_t> tc;
+  build_tile_local(n, row_ptr, col_idx, lh, tl, tc);
+  uint64_t total = 0;
+  for (uint32_t t = 0; t < nt; ++t) total += tl[t].y == kLocalOff ? 0u : tl[t].y;
+  *n_tcols = total;
+  if (total > tcols_cap || (total && !tcols)) {
+    set_error("hbj_spmv_tile_tables: tcols holds %llu columns, %llu needed",
+              (unsigned long long)tcols_cap, (unsigned long long)total);
+    return HBJ_E_INVALID;
+  }
+  for (uint32_t t = 0; t < nt; ++t) {
+    tile_off[t] = tl[t].x;
+    tile_cnt[t] = tl[t].y;
+  }
+  if (nnz) std::memcpy(lidx, lh.data(), 2ull * nnz);
+  if (total) std::memcpy(tcols, tc.data(), 4ull * total);
   return HBJ_OK;
 }
 

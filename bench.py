@@ -268,8 +268,9 @@ def bench_batch(args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
             "data": "synthetic",
             "config": {"workload": f"batched: {len(systems)} beam lattices, sides log-uniform [4,24] "
-                                   f"({min(ns):,}..{max(ns):,} unknowns, sum {sum(ns):,}), FP32 inner + "
-                                   f"FP64 refinement to 1e-8, one tile table, graph-replayed chunks",
+                                   f"({min(ns):,}..{max(ns):,} unknowns, sum {sum(ns):,}), FP32 inner "
+                                   f"(rel tol {args.inner_tol:g}) + FP64 refinement to 1e-8, one tile "
+                                   f"table, graph-replayed chunks",
                        "systems": len(systems)},
             "per_system_path_solves_per_s": alone_sps, "speedup_vs_per_system": batched_sps / alone_sps,
             "worst_true_rel_residual_fp64": worst, "iterations_identical_alone_vs_batched": same,
@@ -293,6 +294,7 @@ def bench_serve(args):
     tickets = [None] * count
     lat = [None] * count
     status = [None] * count
+    iters = [(0, 0)] * count
 
     def waiter():
         for i in range(count):
@@ -300,6 +302,7 @@ def bench_serve(args):
                 time.sleep(0.0005)
             st, x, res, ms = srv.wait(tickets[i])
             lat[i], status[i] = ms, st
+            iters[i] = (res.iterations, res.outer_iterations)
 
     th = threading.Thread(target=waiter)
     th.start()
@@ -317,16 +320,24 @@ def bench_serve(args):
     stats = srv.stats()
     srv.close()
     ok = sum(1 for s in status if s == 0)
+    # bytes the requests' iterations stream in CSR accounting: per inner PCG
+    # iteration 8 nnz + 76.7 n (SpMV, dots, updates, block solves fused), per
+    # FP64 residual step 8 nnz + 28 n + 4
+    moved = sum(it * (8 * L["nnz"] + 76.7 * L["n"]) + (outer + 1) * (8 * L["nnz"] + 28 * L["n"] + 4)
+                for (it, outer), L in zip(iters, reqs))
     return {"metric": METRIC, "value": ok / elapsed, "unit": "solves/s", "n_gpus": 1,
             "steps": count, "warmup": 0, "ms_per_step": elapsed / count * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
             "data": "synthetic",
             "config": {"workload": f"serving: Poisson arrivals at {args.rate:g} req/s (seed 0), {count} "
-                                   f"requests, lattice sides log-uniform [6,40], FP64-verified 1e-8, "
+                                   f"requests, lattice sides log-uniform [6,40], FP64-verified 1e-8 (FP32 inner "
+                                   f"rel tol {args.inner_tol:g}), "
                                    f"HBM budget {args.budget_gb:g} GiB, <= {args.max_active} resident"},
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "max": float(max(lat))},
-            "offered_rate": count / (t_next - t0), "completed_ok": ok, "server": stats}
+            "offered_rate": count / (t_next - t0), "completed_ok": ok,
+            "inner_iterations_total": int(sum(i for i, _ in iters)),
+            "streamed_gbps": moved / elapsed / 1e9, "server": stats}
 
 
 def main():
@@ -344,7 +355,10 @@ def main():
                     help="oracle PCG iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="inner FP32 PCG relative tolerance of the refinement loop (default: 1e-4 "
+                         "for --workload ir, 1e-3 for batch and serve; the fastest in a sweep over "
+                         "1e-2..1e-5 on one MI355X)")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="counter runs: solves capped at this many iterations, no headline line")
     ap.add_argument("--batch", type=int, default=256)
@@ -359,6 +373,8 @@ def main():
                                    "system across GPUs with no exchange, so N>1 is not run here",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.inner_tol is None:
+        args.inner_tol = 1e-4 if args.workload == "ir" else 1e-3
     out = {"plain": bench_plain, "ir": bench_ir, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
     if out is not None:
         print(json.dumps(out), flush=True)

@@ -41,6 +41,15 @@ def kernel_bytes(n, nnz, nb):
     return spmv, update, finish
 
 
+def kernel_bytes_one_reduction(n, nnz, nb):
+    """The same for the one-reduction PCG's two kernels: the SpMV w = A u with
+    r read for its dots, and the update (x, p, s_in, s_out, r_in, r_out, u
+    rw, w, the factors and block starts)."""
+    spmv = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * n + 4 * n
+    update = 4 * n * 10 + 144 * nb + 4 * (nb + 1)
+    return spmv, update, 0
+
+
 def traffic_from_profiles(workload_key):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}.json")
     if os.path.exists(path):
@@ -90,6 +99,9 @@ def bench_plain(args):
     ctx = H.GpuContext.create(0)
     log(ctx.describe())
     system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    one_red = args.alg == "one-reduction"
+    if one_red:
+        system.set_algorithm(H.ALG_PCG_ONE_REDUCTION)
     system.set_rhs(L["rhs"], np.zeros(n, np.float32))
     max_iter, tol = 5000, 1e-6
     best, tune_us = system.autotune(40)
@@ -123,7 +135,8 @@ def bench_plain(args):
     system.set_kernel_timing(0)
     log(f"timed: {args.steps} solves in {elapsed:.3f} s; {res.iterations} its; "
         f"kernel avg us {avg_us.round(2).tolist()} ({cnt.tolist()} samples)")
-    b1, b2, b3 = kernel_bytes(n, nnz, nb)
+    b1, b2, b3 = (kernel_bytes_one_reduction if one_red else kernel_bytes)(n, nnz, nb)
+    gbps = lambda b, us: b / (us * 1e-6) / 1e9 if us > 0 else None
     spmv_gbps = b1 / (avg_us[0] * 1e-6) / 1e9
     layout_gbps = info["spmv_layout_bytes"] / (avg_us[0] * 1e-6) / 1e9
     iter_us = float(avg_us.sum())
@@ -143,9 +156,12 @@ def bench_plain(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain FP32 block-Jacobi PCG, {N}^3 beam lattice ({n:,} unknowns, "
-                               f"{nnz:,} nnz, 6x6 blocks), tol {tol:g}, max {max_iter}, x0 = 0",
-                   "n": n, "nnz": nnz, "iterations": int(res.iterations), "seed": 0},
+        "config": {"workload": (f"plain FP32 block-Jacobi PCG, {N}^3 beam lattice ({n:,} unknowns, "
+                                f"{nnz:,} nnz, 6x6 blocks), tol {tol:g}, max {max_iter}, x0 = 0"
+                                + (", one-reduction (Chronopoulos-Gear) PCG: two launches per "
+                                   "iteration, not bit-identical to the reference" if one_red else "")),
+                   "n": n, "nnz": nnz, "iterations": int(res.iterations), "seed": 0,
+                   "algorithm": "one-reduction" if one_red else "pcg"},
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
                        "max": max(lat)},
         "true_rel_residual_fp64": res.rel_res_true_fp64,
@@ -155,15 +171,16 @@ def bench_plain(args):
                       "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
                       "kernel_us": {"spmv": avg_us[0], "update": avg_us[1],
                                     "finish": avg_us[2]},
-                      "kernel_gbps": {"spmv": spmv_gbps,
-                                      "update": b2 / (avg_us[1] * 1e-6) / 1e9,
-                                      "finish": b3 / (avg_us[2] * 1e-6) / 1e9}},
+                      "kernel_gbps": {"spmv": spmv_gbps, "update": gbps(b2, avg_us[1]),
+                                      "finish": gbps(b3, avg_us[2])}},
         "spmv": {"variant": best, "autotune_us_per_iter": [float(x) for x in tune_us],
                  "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"],
                  "layout_bytes": info["spmv_layout_bytes"], "layout_gbps": layout_gbps},
-        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": spmv_gbps,
+        "roofline": {"bound": "hbm", "kernel": "k_ccg_spmv" if one_red else "k_pcg_spmv",
+                     "achieved": spmv_gbps,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": spmv_gbps / HBM_PEAK_GBPS,
-                     "traffic": ((traffic or {}).get("k_pcg_spmv_bytes_by_variant") or {}).get(str(best)),
+                     "traffic": None if one_red else
+                     ((traffic or {}).get("k_pcg_spmv_bytes_by_variant") or {}).get(str(best)),
                      "algorithmic_bytes": b1,
                      "note": "achieved = CSR algorithmic bytes / kernel time; the tile-local "
                              "layout streams layout_bytes (6 bytes per entry), so the HBM "
@@ -355,6 +372,9 @@ def main():
                     help="oracle PCG iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])
+    ap.add_argument("--alg", default="pcg", choices=["pcg", "one-reduction"],
+                    help="plain workload: the reference's PCG (default, bit-identical to the "
+                         "reference) or the one-reduction PCG (two launches per iteration)")
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="inner FP32 PCG relative tolerance of the refinement loop (default: 1e-4 "
                          "for --workload ir, 1e-3 for batch and serve; the fastest in a sweep over "

@@ -172,6 +172,20 @@ typedef struct {
                                  column lists, row_ptr, x once and q */
 } hbj_system_info;
 
+/* Algorithm of a system's plain solves (hbj_solve, hbj_system_solve_resident,
+ * hbj_system_autotune).  HBJ_ALG_PCG is pcg_block_jacobi_csr_wgpu
+ * (compute.rs:189-495) and the default.  HBJ_ALG_PCG_ONE_REDUCTION is the
+ * Chronopoulos-Gear form of the same preconditioned CG: the iteration's
+ * dots (r,u), (w,u) with w = A u, and (r,r) are reduced at one point, so an
+ * iteration is two launches instead of three (and one all-reduce instead of
+ * two when a system spans GPUs).  Same iterates in exact arithmetic, same
+ * stop test, iteration count and errors; its rounding differs from the
+ * reference's, so its x is not bit-identical to HBJ_ALG_PCG's.  Refinement,
+ * batches and the server always use HBJ_ALG_PCG.  No reference counterpart. */
+#define HBJ_ALG_PCG 0
+#define HBJ_ALG_PCG_ONE_REDUCTION 1
+hbj_status hbj_system_set_algorithm(hbj_system* sys, uint32_t alg);
+
 /* No reference counterpart (introspection for measurement). */
 hbj_status hbj_system_get_info(const hbj_system* sys, hbj_system_info* out);
 /* Copies the system's LU blocks (36 * n_blocks floats, the layout of

@@ -35,6 +35,12 @@
  *                            counts of an independent numpy FP64 PCG on a
  *                            well-conditioned system; breakdown / b=0 /
  *                            max_iter error paths.
+ *   or_ccg                   not a reference function (the library's optional
+ *                            one-reduction PCG): an independent numpy FP64
+ *                            restatement of the same recurrences takes the
+ *                            FP64 PCG's iteration count; FP32 runs converge to
+ *                            x_ref / the scipy solution like or_pcg; error
+ *                            paths.
  *   or_pcg64, or_spmv64,
  *   or_true_rel_residual64   scipy spsolve / numpy residuals.
  */
@@ -312,6 +318,83 @@ int or_pcg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const f
   free(p);
   free(z);
   free(y);
+  return status;
+}
+
+/* One-reduction PCG (Chronopoulos-Gear), the HIP library's optional
+ * algorithm (hbj_system_set_algorithm), not a reference function: the same
+ * FP32 building blocks as or_pcg (or_spmv, or_dot, or_block_jacobi), in the
+ * library kernels' order.  u = M^-1 r, w = A u, gamma = (r,u), delta =
+ * (w,u); beta = gamma / gamma_prev, alpha = gamma / (delta - beta * (gamma /
+ * alpha_prev)) (first iteration: beta = 0, alpha = gamma / delta); p = u +
+ * beta p, s = w + beta s, x = x + alpha p, r = r + (-alpha) s.  Stop test on
+ * (r,r) after each update (never on r0) with or_pcg's thresholds; the same
+ * error statuses and texts. */
+int or_ccg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const float* values,
+           const float* lu, const uint32_t* bs, uint32_t n_starts, const float* b, float* x_io,
+           uint32_t max_iter, float rel_tol, float abs_tol, uint32_t* iterations) {
+  *iterations = 0;
+  float b_norm2 = or_dot(n, b, b);
+  if (b_norm2 == 0.0f) return OR_OK;
+  float rel_tol2 = rel_tol * rel_tol, abs_tol2 = abs_tol * abs_tol;
+  size_t nn = n ? n : 1, nb = sizeof(float) * nn;
+  float* x = (float*)malloc(nb);
+  float* r = (float*)calloc(nn, sizeof(float));
+  float* u = (float*)calloc(nn, sizeof(float));
+  float* w = (float*)calloc(nn, sizeof(float));
+  float* p = (float*)calloc(nn, sizeof(float));
+  float* s = (float*)calloc(nn, sizeof(float));
+  memcpy(x, x_io, nb);
+  or_spmv(n, row_ptr, col_idx, values, x, w);
+  for (uint32_t i = 0; i < n; ++i) r[i] = b[i] + (-1.0f) * w[i];
+  or_block_jacobi(n, lu, bs, n_starts, r, u);
+  float g_prev = 0.0f, a_prev = 0.0f;
+  int status = OR_E_NOT_CONVERGED;
+  for (uint32_t it = 0;; ++it) {
+    or_spmv(n, row_ptr, col_idx, values, u, w);
+    float g = or_dot(n, r, u), dl = or_dot(n, w, u), rr = or_dot(n, r, r);
+    *iterations = it;
+    if (it > 0 && (rr <= abs_tol2 || rr <= rel_tol2 * b_norm2)) {
+      memcpy(x_io, x, nb);
+      status = OR_OK;
+      break;
+    }
+    if (it >= max_iter) break;
+    float beta = 0.0f, den = dl;
+    if (it > 0) {
+      if (g_prev == 0.0f) {
+        set_err("PCG(BlockJacobiGpu): rz_old is zero (breakdown)");
+        status = OR_E_BREAKDOWN_RZ;
+        break;
+      }
+      beta = g / g_prev;
+      float ga = g / a_prev;
+      den = dl - beta * ga;
+    }
+    if (den == 0.0f) {
+      set_err("PCG(BlockJacobiGpu): dot(p,Ap) is zero (breakdown)");
+      status = OR_E_BREAKDOWN_PAP;
+      break;
+    }
+    float alpha = g / den, na = -alpha;
+    for (uint32_t i = 0; i < n; ++i) {
+      p[i] = u[i] + beta * p[i];
+      s[i] = w[i] + beta * s[i];
+      x[i] = x[i] + alpha * p[i];
+      r[i] = r[i] + na * s[i];
+    }
+    or_block_jacobi(n, lu, bs, n_starts, r, u);
+    g_prev = g;
+    a_prev = alpha;
+  }
+  if (status == OR_E_NOT_CONVERGED)
+    set_err("PCG(BlockJacobiGpu): did not converge in %u iterations", max_iter);
+  free(x);
+  free(r);
+  free(u);
+  free(w);
+  free(p);
+  free(s);
   return status;
 }
 

@@ -83,6 +83,13 @@ int or_pcg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const f
            const float* b, float* x, uint32_t max_iter, float rel_tol, float abs_tol,
            uint32_t* iterations, float* rr_hist);
 
+/* One-reduction (Chronopoulos-Gear) PCG of the HIP library's optional
+ * algorithm; x is written only on OR_OK. */
+int or_ccg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const float* values,
+           const float* lu, const uint32_t* block_starts, uint32_t n_starts,
+           const float* b, float* x, uint32_t max_iter, float rel_tol, float abs_tol,
+           uint32_t* iterations);
+
 /* ---- FP64 oracle (error reporting; not a restatement) ---- */
 
 /* y = A x in double, row order. */

@@ -48,6 +48,8 @@ def lib():
     L.or_update_scalars.argtypes = [_f32p, u32, u32, u32, u32, u32, u32]
     L.or_pcg.argtypes = [u32, _u32p, _u32p, _f32p, _f32p, _u32p, u32, _f32p, _f32p, u32, f32, f32,
                          ctypes.POINTER(u32), ctypes.c_void_p]
+    L.or_ccg.argtypes = [u32, _u32p, _u32p, _f32p, _f32p, _u32p, u32, _f32p, _f32p, u32, f32, f32,
+                         ctypes.POINTER(u32)]
     L.or_spmv64.argtypes = [u32, _u32p, _u32p, _f32p, _f64p, _f64p]
     L.or_true_rel_residual64.argtypes = [u32, _u32p, _u32p, _f32p, _f64p, _f64p]
     L.or_true_rel_residual64.restype = f64
@@ -145,6 +147,20 @@ def pcg(row_ptr, col_idx, values, lu, block_starts, b, x0, max_iter, rel_tol, ab
                       max_iter, rel_tol, abs_tol, ctypes.byref(it),
                       hist.ctypes.data if history else None)
     return st, it.value, x, (hist[: it.value] if history else None)
+
+
+def ccg(row_ptr, col_idx, values, lu, block_starts, b, x0, max_iter, rel_tol, abs_tol):
+    """One-reduction PCG: (status, iterations, x); x is x0 unless status == 0."""
+    n = len(row_ptr) - 1
+    x = _c(x0, np.float32).copy()
+    it = ctypes.c_uint32(0)
+    lu = _c(lu, np.float32)
+    if lu.size == 0:
+        lu = np.zeros(1, np.float32)
+    st = lib().or_ccg(n, _c(row_ptr, np.uint32), _c(col_idx, np.uint32), _c(values, np.float32),
+                      lu, _c(block_starts, np.uint32), len(block_starts), _c(b, np.float32), x,
+                      max_iter, rel_tol, abs_tol, ctypes.byref(it))
+    return st, it.value, x
 
 
 def true_rel_residual64(row_ptr, col_idx, values, b, x):

@@ -30,7 +30,7 @@ __all__ = [
     "PcgUpdateScalarsExecutor", "pcg_block_jacobi_csr_wgpu", "System", "Batch", "Server",
     "IrParams", "lattice", "lattice_size", "beam_element_stiffness", "text_to_bin",
     "load_case_dir", "write_x_bin", "read_x_bin", "compare_x", "CLI_PATH", "ITER_KERNELS",
-    "ITER_KERNEL_NAMES", "SPMV_VARIANTS",
+    "ITER_KERNEL_NAMES", "SPMV_VARIANTS", "ALG_PCG", "ALG_PCG_ONE_REDUCTION",
 ]
 
 import os as _os
@@ -41,6 +41,9 @@ ITER_KERNELS = 3
 ITER_KERNEL_NAMES = ("spmv", "update", "finish")
 # bit-identical SpMV variants (HBJ_SPMV_VARIANTS)
 SPMV_VARIANTS = 4
+# hbj_system_set_algorithm
+ALG_PCG = 0
+ALG_PCG_ONE_REDUCTION = 1
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "ZERO_PIVOT", 3: "BREAKDOWN_PAP", 4: "BREAKDOWN_RZ",
                 5: "NOT_CONVERGED", 6: "DEVICE", 7: "OOM", 8: "NO_DEVICE", 9: "REJECTED"}
@@ -318,6 +321,12 @@ class System:
     def set_spmv_variant(self, variant):
         """Select one of the bit-identical SpMV variants (0..SPMV_VARIANTS-1)."""
         _check(_lib.lib().hbj_system_set_spmv_variant(self._h, int(variant)))
+
+    def set_algorithm(self, alg):
+        """ALG_PCG (the reference's PCG, default) or ALG_PCG_ONE_REDUCTION
+        (Chronopoulos-Gear: one reduction point, two launches per iteration;
+        rounding differs from the reference's) for later plain solves."""
+        _check(_lib.lib().hbj_system_set_algorithm(self._h, int(alg)))
 
     def autotune(self, iters=50):
         """Time `iters` PCG iterations with every SpMV variant, keep the fastest.

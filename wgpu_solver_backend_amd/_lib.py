@@ -79,6 +79,7 @@ _SIGS = {
     "hbj_system_get_info": (c_i32, [c_void_p, P(hbj_system_info)]),
     "hbj_system_get_lu": (c_i32, [c_void_p, c_void_p]),
     "hbj_system_set_spmv_variant": (c_i32, [c_void_p, c_u32]),
+    "hbj_system_set_algorithm": (c_i32, [c_void_p, c_u32]),
     "hbj_system_autotune": (c_i32, [c_void_p, c_u32, P(c_u32), f64p]),
     "hbj_solve": (c_i32, [c_void_p, f32p, f32p, c_u32, c_f32, c_f32, P(hbj_result)]),
     "hbj_system_set_rhs": (c_i32, [c_void_p, f32p, c_void_p]),

@@ -246,11 +246,26 @@ struct Engine {
                      one_mode ? &h_one : nullptr, dot64, 0u, cap, nullptr, nullptr};
   }
 
+  // Algorithm of the table's systems: 0 the reference's PCG (three launches
+  // per iteration), 1 the one-reduction PCG (two).
+  uint32_t alg = 0;
+  void set_alg(uint32_t a) {
+    if (a != alg) {
+      alg = a;
+      drop_graph();
+    }
+  }
+
   // one PCG iteration (parity par) of every system in the table: SpMV,
-  // update, finish
+  // update, finish (one-reduction PCG: SpMV, update)
   void launch_iter(uint32_t par) {
     LaunchSet l = ls();
     l.par = par;
+    if (alg == 1) {
+      launch_ccg_spmv(l, stream);
+      launch_ccg_update(l, stream);
+      return;
+    }
     launch_pcg_spmv(l, stream);
     launch_pcg_update(l, stream);
     launch_pcg_finish(l, stream);
@@ -301,9 +316,14 @@ struct Engine {
             lt.ev_stop = e[2 * k + 1];
             fn(lt, stream);
           };
-          timed(0, launch_pcg_spmv);
-          timed(1, launch_pcg_update);
-          timed(2, launch_pcg_finish);
+          if (alg == 1) {  // two kernels: the third sample stays empty
+            timed(0, launch_ccg_spmv);
+            timed(1, launch_ccg_update);
+          } else {
+            timed(0, launch_pcg_spmv);
+            timed(1, launch_pcg_update);
+            timed(2, launch_pcg_finish);
+          }
           tev_trip[pool].push_back(trip - run_trip0);
           ++used;
         } else {
@@ -330,6 +350,8 @@ struct Engine {
       std::array<double, kKernels + 1> smp;
       smp[0] = (double)tev_trip[pool][u];
       for (int k = 0; k < kKernels; ++k) {
+        smp[k + 1] = -1.0;  // no launch of this kernel (one-reduction PCG: no third)
+        if (alg == 1 && k == 2) continue;
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e[2 * k], e[2 * k + 1]));
         smp[k + 1] = ms * 1000.0;
@@ -354,6 +376,7 @@ struct Engine {
     for (const auto& smp : t_pending) {
       if ((uint64_t)smp[0] >= live) continue;
       for (int k = 0; k < kKernels; ++k) {
+        if (smp[k + 1] < 0.0) continue;
         t_sum[k] += smp[k + 1];
         t_cnt[k] += 1;
       }
@@ -648,6 +671,11 @@ struct hbj_system {
   DeviceArena* arena = nullptr;
   uint64_t arena_off = 0;
   char* ir_block = nullptr;
+  // one-reduction PCG (hbj_system_set_algorithm): its s vectors, a block of
+  // their own allocated when the algorithm is chosen
+  uint32_t alg = 0;
+  float *sa = nullptr, *sb = nullptr;
+  char* ccg_block = nullptr;
 
   // The pieces of the main block for a system of n rows, nnz entries,
   // n_starts block starts and n_tcols tile-local columns.
@@ -695,6 +723,8 @@ struct hbj_system {
     d.r64 = r64;
     d.r[0] = ra;
     d.r[1] = rb;
+    d.s[0] = sa;
+    d.s[1] = sb;
     d.tile_blk = tile_blk;
     d.lidx = lidx;
     d.thdr = thdr;
@@ -707,12 +737,13 @@ struct hbj_system {
     return d;
   }
 
+  // ccg: start the one-reduction PCG (plain solves only)
   SysState init_state(bool ir, uint32_t max_iter, float rel_tol, float abs_tol,
-                      const hbj_ir_params* ip) const {
+                      const hbj_ir_params* ip, bool ccg = false) const {
     SysState s;
     std::memset(&s, 0, sizeof s);
     s.status = kRunning;
-    s.phase = ir ? kPhIrResid : kPhInit;
+    s.phase = ir ? kPhIrResid : (ccg ? kPhCcgInit : kPhInit);
     s.max_iter = ir ? ip->max_inner : max_iter;
     s.rel_tol2 = ir ? ip->inner_rel_tol * ip->inner_rel_tol : rel_tol * rel_tol;  // compute.rs:256
     s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257
@@ -736,11 +767,26 @@ struct hbj_system {
     return HBJ_OK;
   }
 
+  hbj_status ensure_ccg_buffers() {
+    if (sa) return HBJ_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    BlockLayout L;
+    L.add(&sa, n);
+    L.add(&sb, n);
+    CHK(dalloc(&ccg_block, L.size));
+    bytes += L.size;
+    L.assign(ccg_block);
+    return HBJ_OK;
+  }
+
+  // mode 0: plain solves (the system's algorithm), 1: refinement (always
+  // the reference's PCG inside)
   hbj_status ensure_engine(int mode) {
     if (!eng_ready) {
       CHK(eng.init(ctx->device, 1, n_tiles));
       eng_ready = true;
     }
+    eng.set_alg(mode == 0 ? alg : 0u);
     if (spmv_variant >= 0) eng.set_spmv_variant((uint32_t)spmv_variant);
     if (eng_mode != mode) {
       std::vector<uint2> tiles(n_tiles);
@@ -780,6 +826,7 @@ struct hbj_system {
     }
     dfree(block);
     dfree(ir_block);
+    dfree(ccg_block);
     hfree(h_part2);
   }
 };

@@ -54,6 +54,11 @@ enum Phase : int32_t {
   kPhIrResid = 2,   // IR: x64 += x32 / scale (after an inner solve), then
                     // r64 = b64 - A x64 (FP64), decide, restart the inner PCG
                     // on b32 = r64 * scale at x32 = 0
+  // One-reduction PCG (Chronopoulos-Gear), an alternative to the reference's
+  // PCG: two launches per iteration, one reduction point
+  kPhCcgInit = 3,   // q = A x0, ||b||^2; then r0 = b - q, u0 = M^-1 r0, p = s = 0
+  kPhCcg = 4,       // w = A u with (r,u), (w,u), (r,r); then alpha, beta, the
+                    // stop test, p, s, x, r and u = M^-1 r
 };
 
 // SysDesc.flags
@@ -102,6 +107,7 @@ struct SysDesc {
   double* r64;               // IR residual
   float* r[2];               // r, double-buffered: a launch set of parity par reads
                              // r[par] and writes r[par ^ 1]
+  float* s[2];               // one-reduction PCG: s = A p, double-buffered like r
   const uint2* tile_blk;     // per tile: first block touching it, number of such blocks
   const uint16_t* lidx;      // nnz (+8 zero padding): entry's index in its tile's column list
   const TileHdr* thdr;       // per tile: column list and wave entry ranges

@@ -33,6 +33,9 @@ constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pas
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_finish(const LaunchSet& ls, hipStream_t s);
+// one-reduction PCG (Chronopoulos-Gear): SpMV with its three partials, update
+void launch_ccg_spmv(const LaunchSet& ls, hipStream_t s);
+void launch_ccg_update(const LaunchSet& ls, hipStream_t s);
 void launch_true_resid(const SysDesc* d, const float* x, double* part2, uint32_t n_tiles,
                        hipStream_t s);
 

@@ -5,11 +5,11 @@
 //
 //   k_pcg_spmv    q = A p (each wave streams its 64 rows' CSR entries
 //                 coalesced, products parked in the wave's LDS slice, each
-//                 row summed by its owner lane), the tile's p.q partial; the
-//                 system's last tile reduces the partials and sets
-//                 alpha = rz_old / p.q on the device.
-//   k_pcg_update  x += alpha p, r -= alpha q, z = M^-1 r (6x6 LU solves,
-//                 factors staged in LDS), r.r and r.z partials.
+//                 row summed by its owner lane), the tile's p.q partial.
+//   k_pcg_update  every workgroup reduces the p.q partials to alpha =
+//                 rz_old / p.q, then x += alpha p, r -= alpha q, z = M^-1 r
+//                 (6x6 LU solves, factors staged in LDS), r.r and r.z
+//                 partials.
 //   k_pcg_finish  every workgroup reduces r.r and r.z, computes beta, the
 //                 stop test and the state transition (one workgroup writes
 //                 it), then p = z + beta p over its rows.
@@ -17,6 +17,8 @@
 // The same three kernels run the init step (r0 = b - A x0 ...) and, for
 // iterative refinement, the FP64 residual / restart / fold steps, chosen
 // per system by SysState.phase.
+// An optional one-reduction PCG (k_ccg_spmv, k_ccg_update) replaces the three
+// with two; it is not the reference's operation order.
 //
 // Arithmetic follows the reference operation for operation (the file is
 // compiled with -ffp-contract=off, IEEE division):
@@ -217,6 +219,49 @@ __device__ __forceinline__ T reduce_fetched(const T (&v)[4][4], uint32_t len, T*
     __syncthreads();
   }
   return lds[0];
+}
+
+// Three reduce_fetched in one pass, with the same trees and so the same
+// results: every wave reduces its chunks of all three, one barrier, the
+// second levels in waves 0, 1 and 2 side by side, one barrier.
+template <typename T, bool kLean = false>
+__device__ __forceinline__ void reduce_fetched3(const T (&va)[4][4], const T (&vb)[4][4],
+                                                const T (&vc)[4][4], uint32_t len, T* la, T* lb,
+                                                T* lc, T& a, T& b, T& c) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t out_len = (len + 255u) / 256u;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const uint32_t g = wave + 4u * m;
+    if (g < out_len) {
+      const T ra = wave_tree256<T, kLean>(va[m][0], va[m][1], va[m][2], va[m][3]);
+      const T rb = wave_tree256<T, kLean>(vb[m][0], vb[m][1], vb[m][2], vb[m][3]);
+      const T rc = wave_tree256<T, kLean>(vc[m][0], vc[m][1], vc[m][2], vc[m][3]);
+      if (lane == 0) {
+        la[g] = ra;
+        lb[g] = rb;
+        lc[g] = rc;
+      }
+    }
+  }
+  __syncthreads();
+  if (out_len > 1u) {
+    if (wave < 3u) {
+      T* l = wave == 0u ? la : wave == 1u ? lb : lc;
+      T w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t i = lane + 64u * j;
+        w[j] = i < out_len ? l[i] : T(0);
+      }
+      const T r = wave_tree256<T, kLean>(w[0], w[1], w[2], w[3]);
+      if (lane == 0) l[0] = r;
+    }
+    __syncthreads();
+  }
+  a = la[0];
+  b = lb[0];
+  c = lc[0];
 }
 
 // dot_scalar_exec.rs:204-237: while more than one partial is left, reduce
@@ -1310,6 +1355,269 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   }
 }
 
+// ------------------------------------------ one-reduction PCG (optional)
+//
+// Chronopoulos-Gear PCG: the three dots of an iteration -- (r,u), (w,u) with
+// w = A u, and (r,r) -- are reduced at one point, so an iteration is two
+// launches where the reference's PCG needs three (and, across GPUs, one
+// all-reduce where it needs two).  In exact arithmetic its iterates are the
+// reference's; its operations are not, so its results differ in rounding
+// and it is never the drop-in path (hbj_system_set_algorithm opts in).
+//   k_ccg_spmv    w = A u and the tile's (r,u), (w,u), (r,r) partials
+//                 [init: q = A x0 and b.b]
+//   k_ccg_update  every workgroup reduces the three partials (same trees,
+//                 same bits) and takes the same decision: stop test on
+//                 (r,r) with the reference's thresholds, or
+//                   beta = gamma / gamma_prev,
+//                   alpha = gamma / (delta - beta * (gamma / alpha_prev))
+//                   (first iteration: beta = 0, alpha = gamma / delta)
+//                 then p = u + beta p, s = w + beta s, x = x + alpha p,
+//                 r = r + (-alpha) s and u = M^-1 r; its tile-0 workgroup
+//                 writes the next state  [init: r = b + (-1) q, u = M^-1 r,
+//                 p = s = 0]
+// Vectors: u in SysDesc.z, w in SysDesc.q; r and s double-buffered by
+// parity (a tile reads its neighbours' rows for straddling blocks).
+
+template <int V, bool kOne>
+__global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
+  __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+  constexpr bool kLocal = SpmvLds<V>::kLocal;
+  TileRef tr;
+  if (!get_tile<kOne>(ta, tr)) return;
+  float* s_ga = reinterpret_cast<float*>(s_prod);  // tile reductions reuse the slices
+  float* s_gb = s_ga + kTile;
+  float* s_gc = s_gb + kTile;
+  const SysDesc& d = tr.d;
+  const SysState* st = d.st + ta.par * ta.st_stride;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
+  const bool own = t < tr.nrows;
+  const uint32_t rowc = own ? row : tr.row0;
+  TileHdr hdr{};
+  if constexpr (kLocal) hdr = uni_copy(d.thdr[tr.tile]);
+  uint32_t mb, me;
+  lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
+  const float* __restrict__ r_cur = ta.par ? d.r[1] : d.r[0];
+  const float ri = bld(r_cur, rowc), ui = bld(d.z, rowc);
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  if (status != kRunning) return;
+  if (phase != kPhCcgInit && phase != kPhCcg) return;
+  const bool init = phase == kPhCcgInit;
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.z, d.n * 4u);
+  auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
+  float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
+  float q;
+  if constexpr (kLocal) {
+    if (hdr.cnt != kLocalOff) {
+      const uint32_t w = uni(wave);
+      const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
+      const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
+      q = spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [&]() {
+        stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
+      });
+    } else {
+      q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+    }
+  } else if constexpr (V == 1) {
+    q = spmv_wave_vec<float, 4>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+  } else {
+    q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+  }
+  __syncthreads();  // all waves are done with their slices
+  if (own) bst(d.q, row, q);
+  if (init) {
+    float bb = 0.0f;
+    if (own) {
+      const float bi = bld(d.b, row);
+      bb = bi * bi;
+    }
+    const float pd = tree256<float>(bb, s_ga);
+    if (t == 0) d.part[tr.tile] = pd;
+    return;
+  }
+  float g = own ? ri * ui : 0.0f, dl = own ? q * ui : 0.0f;
+  const float rr = tree256<float>(own ? ri * ri : 0.0f, s_gc);
+  tree256x2<float>(g, dl, s_ga, s_gb);
+  if (t == 0) {
+    const uint32_t nt = d.n_tiles;
+    d.part[tr.tile] = g;
+    d.part[nt + tr.tile] = dl;
+    d.part[2u * nt + tr.tile] = rr;
+  }
+}
+
+template <bool kOne>
+__global__ __launch_bounds__(kTile) void k_ccg_update(const TileArgs ta) {
+  __shared__ float s_rh[kTile + 2 * kHalo];
+  __shared__ float s_z[kTile];
+  __shared__ float s_lu[kLuPer * kTile];
+  __shared__ float s_g[3][kTile];
+  __shared__ uint32_t s_blk[2];
+  __shared__ SysState s_ns;
+  TileRef tr;
+  if (!get_tile<kOne>(ta, tr)) return;
+  const SysDesc& d = tr.d;
+  SysState* st = d.st + ta.par * ta.st_stride;
+  SysState* st_next = d.st + (ta.par ^ 1u) * ta.st_stride;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t, nt = d.n_tiles;
+  const bool own = t < tr.nrows;
+  const uint32_t rowc = own ? row : tr.row0;
+  uint32_t hrow = 0, hslot = 0;
+  const bool halo = halo_row(tr.row0, tr.nrows, d.n, hrow, hslot);
+  const uint32_t hrowc = halo ? hrow : tr.row0;
+  // round 1: the partials, own and halo rows, the factors (6-row blocks)
+  const bool pre = nt > 1u && nt <= kFetchMax;
+  float vg[4][4], vd[4][4], vr[4][4];
+  if (pre) {
+    level_fetch<float>(d.part, nt, vg);
+    level_fetch<float>(d.part + nt, nt, vd);
+    level_fetch<float>(d.part + 2u * nt, nt, vr);
+  }
+  const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
+  float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
+  const float* __restrict__ s_in = ta.par ? d.s[1] : d.s[0];
+  float* __restrict__ s_out = ta.par ? d.s[0] : d.s[1];
+  const float xi = bld(d.x, rowc), pi = bld(d.p, rowc), ui = bld(d.z, rowc), wi = bld(d.q, rowc);
+  const float ri = bld(r_in, rowc), si = bld(s_in, rowc);
+  const float hr = bld(r_in, hrowc), hs = bld(s_in, hrowc), hw = bld(d.q, hrowc);
+  const bool b6 = (d.flags & kFlagBlocks6) != 0u;
+  uint2 tb = make_uint2(0u, 0u);
+  uint32_t bs_s = 0, bs_e = 0;
+  if (b6) {
+    const uint32_t first = tr.row0 / 6u, last = (tr.row0 + tr.nrows - 1u) / 6u;
+    tb = make_uint2(first, last - first + 1u);
+    const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+    bs_s = 6u * bi;
+    bs_e = 6u * bi + 6u < d.n ? 6u * bi + 6u : d.n;
+    bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+  }
+  const uint2 blk = (!b6 && d.tile_blk) ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  const uint32_t it = uni(st->iter), max_iter = uni(st->max_iter);
+  const float g_prev = st->rz_old, a_prev = st->alpha;
+  const float rel2 = st->rel_tol2, abs2 = st->abs_tol2, bn_sys = st->b_norm2;
+  if (status != kRunning) return;
+  if (phase != kPhCcgInit && phase != kPhCcg) return;
+  const bool init = phase == kPhCcgInit;
+
+  int32_t fin = kRunning;
+  float g = 0.0f, dl = 0.0f, rr = 0.0f, bn = 0.0f, alpha = 0.0f, beta = 0.0f;
+  if (init) {
+    bn = pre ? reduce_fetched<float, true>(vg, nt, s_g[0]) : reduce_partials<float, true>(d.part, nt, s_g[0]);
+    if (bn == 0.0f) fin = 0;  // Ok(0), x untouched (compute.rs:252-254)
+  } else {
+    if (pre) {
+      reduce_fetched3<float, true>(vg, vd, vr, nt, s_g[0], s_g[1], s_g[2], g, dl, rr);
+    } else {
+      g = reduce_partials<float, true>(d.part, nt, s_g[0]);
+      dl = reduce_partials<float, true>(d.part + nt, nt, s_g[1]);
+      rr = reduce_partials<float, true>(d.part + 2u * nt, nt, s_g[2]);
+    }
+    // r is r after `it` updates: the reference's stop test after an
+    // iteration (compute.rs:481), never on r0
+    if (it > 0u && (rr <= abs2 || rr <= rel2 * bn_sys)) {
+      fin = 0;
+    } else if (it >= max_iter) {
+      fin = 5;  // HBJ_E_NOT_CONVERGED
+    } else {
+      float den = dl;
+      if (it > 0u) {
+        if (g_prev == 0.0f) {
+          fin = 4;  // HBJ_E_BREAKDOWN_RZ
+        } else {
+          beta = g / g_prev;
+          const float ga = g / a_prev;
+          den = dl - beta * ga;
+        }
+      }
+      if (fin == kRunning) {
+        if (den == 0.0f) fin = 3;  // HBJ_E_BREAKDOWN_PAP: (p, A p) = 0
+        else alpha = g / den;
+      }
+    }
+  }
+  if (tr.tile == 0) {  // the next state, staged in LDS and copied out by dwords
+    constexpr uint32_t kWords = sizeof(SysState) / 4;
+    static_assert(kWords <= (uint32_t)kTile && sizeof(SysState) % 4 == 0, "state copy by dwords");
+    if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(st)[t];
+    __syncthreads();
+    if (t == 0) {
+      SysState& ns = s_ns;
+      ns.sets += 1u;
+      if (init) {
+        ns.b_norm2 = bn;
+        if (fin != kRunning) {
+          ns.status = fin;
+        } else {
+          ns.phase = kPhCcg;
+          ns.iter = 0;
+        }
+      } else {
+        ns.rr = rr;
+        ns.rz_new = g;
+        if (fin != kRunning) {
+          ns.status = fin;
+        } else {
+          ns.rz_old = g;
+          ns.alpha = alpha;
+          ns.neg_alpha = -alpha;
+          ns.beta = beta;
+          ns.iter = it + 1u;
+          ns.total_iter = it + 1u;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < kWords) {
+      const uint32_t w = reinterpret_cast<const uint32_t*>(&s_ns)[t];
+      reinterpret_cast<uint32_t*>(st_next)[t] = w;
+      if (fin != kRunning) reinterpret_cast<uint32_t*>(st)[t] = w;  // a final state into both copies
+    }
+  }
+  if (fin != kRunning) return;
+
+  // round 2: b at init; the block table when blocks are not 6 rows
+  float rs = ri, rhs_ = hr;
+  if (init) {
+    rs = bld(d.b, rowc);
+    rhs_ = bld(d.b, hrowc);
+  }
+  if (!b6) {
+    const uint2 tb0 = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
+                                                     tr.nrows, s_blk);
+    tb = make_uint2(uni(tb0.x), uni(tb0.y));
+    const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+    bs_s = bld(d.bstart, bi);
+    bs_e = bld(d.bstart, bi + 1u);
+    bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+  }
+  const float na = -alpha;
+  float rn = 0.0f;
+  if (own) {
+    if (init) {
+      rn = rs + (-1.0f) * wi;  // r0 = b - A x0 (q holds A x0)
+      bst(d.p, row, 0.0f);
+      bst(s_out, row, 0.0f);
+    } else {
+      const float pn = ui + beta * pi;
+      const float sn = wi + beta * si;
+      rn = rs + na * sn;
+      bst(d.p, row, pn);
+      bst(s_out, row, sn);
+      bst(d.x, row, xi + alpha * pn);
+    }
+    bst(r_out, row, rn);
+    s_rh[t + kHalo] = rn;
+    s_z[t] = ui;  // rows no block writes keep u
+  }
+  if (halo) s_rh[hslot] = init ? rhs_ + (-1.0f) * hw : rhs_ + na * (hw + beta * hs);
+  lds_dma_wait();  // the factor copy into s_lu
+  __syncthreads();
+  bj_tile(d.lu, tb, d.n, tr.row0, tr.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
+  if (own) bst(d.z, row, s_z[t]);
+}
+
 // FP64 check of an FP32 solution: per tile ||b - A x||^2 and ||b||^2 with
 // FP32 inputs promoted exactly to FP64 (host sums the n_tiles partials).
 __global__ __launch_bounds__(kTile) void k_true_resid(const SysDesc* __restrict__ dp,
@@ -1578,6 +1886,28 @@ void launch_pcg_finish(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
   if (ls.dot64) launch_finish_t<true>(ls, a, s);
   else launch_finish_t<false>(ls, a, s);
+}
+
+template <int V>
+static void launch_ccg_spmv_v(const LaunchSet& ls, hipStream_t s) {
+  const TileArgs a = tile_args(ls);
+  const dim3 g(ls.n_tiles), b(kTile);
+  if (ls.one) launch(ls, k_ccg_spmv<V, true>, g, b, s, a);
+  else launch(ls, k_ccg_spmv<V, false>, g, b, s, a);
+}
+void launch_ccg_spmv(const LaunchSet& ls, hipStream_t s) {
+  switch (ls.spmv_variant) {
+    case 1: launch_ccg_spmv_v<1>(ls, s); break;
+    case 2: launch_ccg_spmv_v<2>(ls, s); break;
+    case 3: launch_ccg_spmv_v<3>(ls, s); break;
+    default: launch_ccg_spmv_v<0>(ls, s); break;
+  }
+}
+void launch_ccg_update(const LaunchSet& ls, hipStream_t s) {
+  const TileArgs a = tile_args(ls);
+  const dim3 g(ls.n_tiles), b(kTile);
+  if (ls.one) launch(ls, k_ccg_update<true>, g, b, s, a);
+  else launch(ls, k_ccg_update<false>, g, b, s, a);
 }
 
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {

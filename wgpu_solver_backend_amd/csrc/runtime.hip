@@ -540,6 +540,21 @@ hbj_status hbj_system_set_spmv_variant(hbj_system* s, uint32_t v) {
   return HBJ_OK;
 }
 
+hbj_status hbj_system_set_algorithm(hbj_system* s, uint32_t alg) {
+  clear_error();
+  if (!s || alg > HBJ_ALG_PCG_ONE_REDUCTION) {
+    set_error("hbj_system_set_algorithm: need a system and an algorithm <= %d",
+              HBJ_ALG_PCG_ONE_REDUCTION);
+    return HBJ_E_INVALID;
+  }
+  if (alg == HBJ_ALG_PCG_ONE_REDUCTION) CHK(s->ensure_ccg_buffers());
+  if (alg != s->alg) {
+    s->alg = alg;
+    s->eng_mode = -1;  // the device descriptor is written again (s vectors)
+  }
+  return HBJ_OK;
+}
+
 hbj_status hbj_system_autotune(hbj_system* s, uint32_t iters, uint32_t* best, double* us) {
   clear_error();
   if (!s || iters == 0) { set_error("hbj_system_autotune: need a system and iters > 0"); return HBJ_E_INVALID; }
@@ -560,7 +575,7 @@ hbj_status hbj_system_autotune(hbj_system* s, uint32_t iters, uint32_t* best, do
       HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, s->eng.stream));
       if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, s->eng.stream));
       // tolerances 0: runs `iters` iterations unless the residual is exactly 0
-      SysState init = s->init_state(false, iters, 0.0f, 0.0f, nullptr), fin;
+      SysState init = s->init_state(false, iters, 0.0f, 0.0f, nullptr, s->alg == 1), fin;
       double ms = 0;
       const hbj_status st = s->eng.run(&init, &fin, &ms);
       if (st != HBJ_OK) {
@@ -639,7 +654,7 @@ hbj_status hbj_system_solve_resident(hbj_system* s, uint32_t max_iter, float rel
   hipStream_t st = s->eng.stream;
   HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, st));
   if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
-  SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr), fin;
+  SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr, s->alg == 1), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
   fill_result(fin, false, ms, res);

@@ -260,3 +260,65 @@ def test_device_lu_reports_first_failing_block(ctx, hbj):
     with pytest.raises(hbj.SolverError, match="LU zero pivot at k = 3") as e:
         hbj.System(ctx, rp, ci, va, np.array([0, 6, 12], np.uint32))
     assert e.value.status_name == "ZERO_PIVOT"
+
+
+def _arrow_spd(n=3000, dense=(5, 1700, 2999), seed=3):
+    """SPD CSR with a few dense rows/columns (rows of ~n entries: several SpMV
+    passes per row, and tiles referencing more than 2,048 columns, which
+    gather from HBM in every variant) around a random near-band part;
+    diagonally dominant.  Blocks of 1 to 6 rows."""
+    rng = np.random.default_rng(seed)
+    rows = [dict() for _ in range(n)]
+    for i in range(n):
+        for k in rng.integers(-30, 31, 6):
+            j = i + int(k)
+            if 0 <= j < n and j != i:
+                v = float(rng.uniform(-1.0, 1.0))
+                rows[i][j] = v
+                rows[j][i] = v
+    for dr in dense:
+        for j in range(n):
+            if j != dr:
+                v = float(rng.uniform(-0.01, 0.01))
+                rows[dr][j] = v
+                rows[j][dr] = v
+    for i in range(n):
+        rows[i][i] = sum(abs(v) for j, v in rows[i].items() if j != i) + 1.0
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        for j in sorted(rows[i]):
+            ci.append(j)
+            va.append(rows[i][j])
+        rp.append(len(ci))
+    sizes, starts, s = [6, 6, 3, 1, 4, 6, 2, 5, 6, 6], [0], 0
+    while s < n:
+        s = min(n, s + sizes[len(starts) % len(sizes)])
+        starts.append(s)
+    b = rng.uniform(-1.0, 1.0, n).astype(F)
+    return (np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, F),
+            np.array(starts, np.uint32), b)
+
+
+def test_long_rows_and_tiles_without_column_lists_bitwise(ctx, hbj, oracle):
+    print("arrow SPD n=3000: long rows, HBM-gathering tiles, irregular blocks", flush=True)
+    rp, ci, va, bs, b = _arrow_spd()
+    n = len(rp) - 1
+    assert np.diff(rp.astype(np.int64)).max() > 2048  # rows longer than one pass of any variant
+    off, cnt, _, _ = hbj.spmv_tile_tables(rp, ci)
+    assert 0 < int(np.sum(cnt == 0xFFFFFFFF)) < cnt.size  # both kinds of tiles
+    x0 = np.zeros(n, F)
+    st, it, xo, _ = oracle_solve(oracle, rp, ci, va, bs, b, x0, 2000, 1e-6, 0.0)
+    assert st == 0 and it > 5
+    st_lu, lu = oracle.build_lu_blocks_6(n, rp, ci, va, bs)
+    st_c, it_c, xc = oracle.ccg(rp, ci, va, lu, bs, b, x0, 2000, 1e-6, 0.0)
+    assert st_c == 0
+    sys_ = hbj.System(ctx, rp, ci, va, bs)
+    for alg, its, xref in ((hbj.ALG_PCG, it, xo), (hbj.ALG_PCG_ONE_REDUCTION, it_c, xc)):
+        sys_.set_algorithm(alg)
+        for v in range(hbj.SPMV_VARIANTS):
+            print(f"  algorithm {alg}, SpMV variant {v}", flush=True)
+            sys_.set_spmv_variant(v)
+            x, res = sys_.solve(b, x0, 2000, 1e-6, 0.0)
+            assert res.status == 0 and res.iterations == its, (alg, v)
+            assert np.array_equal(bits(x), bits(xref)), (alg, v)
+    sys_.close()

@@ -109,6 +109,11 @@ struct Engine {
   uint32_t n_sys = 0, n_tiles = 0;
   SysDesc* d_descs = nullptr;
   uint2* d_tiles = nullptr;
+  // k_pcg_finish's table: one entry per group of kFinishTiles consecutive
+  // tiles of a system (the entries of the tile table whose tile index is a
+  // multiple of kFinishTiles); same capacity as d_tiles
+  uint2* d_ftiles = nullptr;
+  uint32_t n_ftiles = 0;
   SysState* d_states = nullptr;
   SysState* h_init = nullptr;
   SysState* h_poll[2] = {nullptr, nullptr};
@@ -149,6 +154,7 @@ struct Engine {
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     CHK(dalloc(&d_descs, nsys));
     CHK(dalloc(&d_tiles, ntiles));
+    CHK(dalloc(&d_ftiles, ntiles));
     tiles_cap = ntiles ? ntiles : 1;
     cap = nsys ? nsys : 1;
     CHK(dalloc(&d_states, 2ull * cap));
@@ -164,6 +170,7 @@ struct Engine {
     for (uint32_t i = 0; i < descs.size(); ++i) descs[i].st = d_states + i;
     HIPCHK(hipMemcpy(d_descs, descs.data(), sizeof(SysDesc) * descs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
+    CHK(upload_groups(tiles));
     n_sys = (uint32_t)descs.size();
     n_tiles = (uint32_t)tiles.size();
     one_mode = n_sys == 1 && env_u32("HBJ_ONE", 1) != 0;
@@ -180,6 +187,21 @@ struct Engine {
     return HBJ_OK;
   }
 
+  static std::vector<uint2> groups_of(const std::vector<uint2>& tiles) {
+    std::vector<uint2> g;
+    g.reserve(tiles.size() / kFinishTiles + 1);
+    for (const uint2& t : tiles)
+      if (t.y % kFinishTiles == 0) g.push_back(t);
+    return g;
+  }
+  hbj_status upload_groups(const std::vector<uint2>& tiles) {
+    const std::vector<uint2> g = groups_of(tiles);
+    if (!g.empty())
+      HIPCHK(hipMemcpy(d_ftiles, g.data(), sizeof(uint2) * g.size(), hipMemcpyHostToDevice));
+    n_ftiles = (uint32_t)g.size();
+    return HBJ_OK;
+  }
+
   // Tile table compaction for many-system runs: once fewer than 3/4 of the
   // table's tiles belong to running systems, the table is rewritten with only
   // those (finished systems' tiles would otherwise still be dispatched and
@@ -193,6 +215,7 @@ struct Engine {
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipMemcpy(d_tiles, full_tiles.data(), sizeof(uint2) * full_tiles.size(),
                      hipMemcpyHostToDevice));
+    CHK(upload_groups(full_tiles));
     n_tiles = (uint32_t)full_tiles.size();
     table_full = true;
     drop_graph();
@@ -214,6 +237,7 @@ struct Engine {
       if (latest[t.x].status == kRunning) live.push_back(t);
     if (live.empty()) return HBJ_OK;  // everything finished in the last chunk
     HIPCHK(hipMemcpy(d_tiles, live.data(), sizeof(uint2) * live.size(), hipMemcpyHostToDevice));
+    CHK(upload_groups(live));
     n_tiles = (uint32_t)live.size();
     table_full = false;
     drop_graph();
@@ -242,7 +266,7 @@ struct Engine {
   // FP64-accumulated dots (kFlagDot64): one setting for the whole table
   uint32_t dot64 = 0;
   LaunchSet ls() const {
-    return LaunchSet{d_descs, d_tiles, n_tiles, remap, n_sys, spmv_variant,
+    return LaunchSet{d_descs, d_tiles, n_tiles, d_ftiles, n_ftiles, remap, n_sys, spmv_variant,
                      one_mode ? &h_one : nullptr, dot64, 0u, cap, nullptr, nullptr};
   }
 
@@ -474,6 +498,7 @@ struct Engine {
     SysDesc* desc = nullptr;   // cap records
     SysState* st = nullptr;    // cap records
     uint2* tiles = nullptr;    // tiles_cap entries
+    uint2* ftiles = nullptr;   // tiles_cap entries (finish groups)
     uint32_t used = 0;         // slot records written since stage_begin
   } stg[2];
 
@@ -483,6 +508,7 @@ struct Engine {
       CHK(halloc(&g.desc, cap));
       CHK(halloc(&g.st, cap));
       CHK(halloc(&g.tiles, tiles_cap));
+      CHK(halloc(&g.ftiles, tiles_cap));
     }
     g.used = 0;
     return HBJ_OK;
@@ -521,20 +547,29 @@ struct Engine {
       HIPCHK(hipStreamSynchronize(stream));
       const uint32_t nc = need > 2 * tiles_cap ? need : 2 * tiles_cap;
       dfree(d_tiles);
+      dfree(d_ftiles);
       CHK(dalloc(&d_tiles, nc));
+      CHK(dalloc(&d_ftiles, nc));
       for (Staging& g : stg)
         if (g.tiles) {
           hfree(g.tiles);
+          hfree(g.ftiles);
           CHK(halloc(&g.tiles, nc));
+          CHK(halloc(&g.ftiles, nc));
         }
       tiles_cap = nc;
     }
+    const std::vector<uint2> groups = groups_of(tiles);
     if (need) {
       std::memcpy(stg[pool].tiles, tiles.data(), sizeof(uint2) * need);
       HIPCHK(hipMemcpyAsync(d_tiles, stg[pool].tiles, sizeof(uint2) * need, hipMemcpyHostToDevice,
                             stream));
+      std::memcpy(stg[pool].ftiles, groups.data(), sizeof(uint2) * groups.size());
+      HIPCHK(hipMemcpyAsync(d_ftiles, stg[pool].ftiles, sizeof(uint2) * groups.size(),
+                            hipMemcpyHostToDevice, stream));
     }
     n_tiles = need;
+    n_ftiles = (uint32_t)groups.size();
     drop_graph();
     return HBJ_OK;
   }
@@ -570,10 +605,12 @@ struct Engine {
       hfree(stg[b].desc);
       hfree(stg[b].st);
       hfree(stg[b].tiles);
+      hfree(stg[b].ftiles);
     }
     hfree(h_init);
     dfree(d_descs);
     dfree(d_tiles);
+    dfree(d_ftiles);
     dfree(d_states);
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;

@@ -25,6 +25,9 @@ constexpr int kPer = 8;
 constexpr int kWaves = kTile / 64;
 // Blocks whose factors a tile stages in LDS (more than this: read from HBM).
 constexpr int kLuStage = 64;
+// Tiles of one system per workgroup of k_pcg_finish (its reductions are
+// redundant per workgroup, so fewer workgroups do less of them).
+constexpr uint32_t kFinishTiles = 4;
 
 // SysState.status while a system is still iterating; any other value is a
 // final hbj_status.

@@ -16,6 +16,8 @@ struct LaunchSet {
   const SysDesc* descs;
   const uint2* tiles;
   uint32_t n_tiles;
+  const uint2* ftiles;    // k_pcg_finish's table (many systems): the entries of `tiles`
+  uint32_t n_ftiles;      // whose tile is a multiple of kFinishTiles
   uint32_t remap;
   uint32_t n_sys;
   uint32_t spmv_variant;  // streaming variant of the SpMV kernel (bit-identical results)

@@ -1236,8 +1236,6 @@ __device__ __forceinline__ void pcg_transition(SysState& ns, uint32_t flags, flo
   }
 }
 
-// Tiles per workgroup of k_pcg_finish for a single system.
-constexpr uint32_t kFinishTiles = 4;
 
 // After k_pcg_update.  Every workgroup reduces r.r and r.z (and, at a
 // refinement decision, the residual's norms) from the partials -- the same
@@ -1249,7 +1247,10 @@ constexpr uint32_t kFinishTiles = 4;
 // (scale_from_scalar_results.wgsl:37, axpy.wgsl:41 with alpha = 1), or p = z
 // on the first iteration (compute.rs:313-314); after an inner solve of the
 // refinement (phase just set to kPhIrResid) the fold x64 += x32 / scale.
-// kOne: workgroup b covers tiles b + k * gridDim.x, k < kFinishTiles.
+// A workgroup covers kFinishTiles tiles of one system (so a system's
+// reductions run in a quarter as many workgroups): kOne, tiles b + k *
+// gridDim.x; a table (the engine's finish table, one entry per group),
+// tiles tile0 + k of the entry's system.
 template <bool kOne, bool kDot64>
 __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   __shared__ double s_a[256];
@@ -1258,14 +1259,14 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   constexpr uint32_t kWords = sizeof(SysState) / 4;
   static_assert(kWords <= (uint32_t)kTile && sizeof(SysState) % 4 == 0, "state copy by dwords");
   TileRef tr;
-  uint32_t step = 0;
+  uint32_t step = 1;
   if constexpr (kOne) {
     if (blockIdx.x >= ta.n_tiles) return;
     tr.d = ta.one;
     tr.tile = blockIdx.x;
     step = gridDim.x;
   } else {
-    if (!get_tile<false>(ta, tr)) return;
+    if (!get_tile<false>(ta, tr)) return;  // tr.tile: the group's first tile
   }
   const SysDesc& d = tr.d;
   SysState* in = d.st + ta.par * ta.st_stride;
@@ -1322,7 +1323,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   if (nstatus != kRunning) return;
   if (nphase == kPhIter) {
     // all of the workgroup's loads first (rows past the system read row 0)
-    constexpr uint32_t kT = kOne ? kFinishTiles : 1u;
+    constexpr uint32_t kT = kFinishTiles;
     float zv[kT], pv[kT];
     uint32_t rw[kT];
 #pragma unroll
@@ -1345,7 +1346,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
       }
     }
   } else if (nphase == kPhIrResid) {
-    const uint32_t kT = kOne ? kFinishTiles : 1u;
+    constexpr uint32_t kT = kFinishTiles;
     for (uint32_t k = 0; k < kT; ++k) {
       const uint32_t tile = tr.tile + k * step;
       const uint32_t row = tile * (uint32_t)kTile + t;
@@ -1879,7 +1880,10 @@ static void launch_finish_t(const LaunchSet& ls, const TileArgs& a, hipStream_t 
     const uint32_t g = (ls.n_tiles + kFinishTiles - 1) / kFinishTiles;
     launch(ls, k_pcg_finish<true, kDot64>, dim3(g ? g : 1), dim3(kTile), s, a);
   } else {
-    launch(ls, k_pcg_finish<false, kDot64>, dim3(ls.n_tiles), dim3(kTile), s, a);
+    TileArgs ag = a;
+    ag.tiles = ls.ftiles;
+    ag.n_tiles = ls.n_ftiles;
+    launch(ls, k_pcg_finish<false, kDot64>, dim3(ls.n_ftiles ? ls.n_ftiles : 1), dim3(kTile), s, ag);
   }
 }
 void launch_pcg_finish(const LaunchSet& ls, hipStream_t s) {

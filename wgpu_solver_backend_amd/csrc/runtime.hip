@@ -778,7 +778,7 @@ struct hbj_batch {
       }
       if (tiles.empty()) tiles.push_back(make_uint2(0, 0));
       CHK(eng.set_table(descs, tiles));
-      if (n_tiles == 0) eng.n_tiles = 0;
+      if (n_tiles == 0) eng.n_tiles = eng.n_ftiles = 0;
       mode = m;
     }
     return HBJ_OK;

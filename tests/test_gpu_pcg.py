@@ -322,3 +322,17 @@ def test_long_rows_and_tiles_without_column_lists_bitwise(ctx, hbj, oracle):
             assert res.status == 0 and res.iterations == its, (alg, v)
             assert np.array_equal(bits(x), bits(xref)), (alg, v)
     sys_.close()
+
+
+def test_one_tile_update_kernel_bitwise(ctx, hbj, oracle, monkeypatch):
+    """HBJ_UPDATE_GROUPS=0 selects the update kernel with one tile per
+    workgroup (the default covers two, or four in a table): same bits."""
+    monkeypatch.setenv("HBJ_UPDATE_GROUPS", "0")
+    L = hbj.lattice(9, seed=6)
+    x0 = np.zeros(L["n"], F)
+    st, it, xo, _ = oracle_solve(oracle, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                 L["rhs"], x0, 5000, 1e-6, 0.0)
+    sys_ = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    x, res = sys_.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert res.iterations == it and np.array_equal(bits(x), bits(xo))
+    sys_.close()

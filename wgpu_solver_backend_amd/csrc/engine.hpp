@@ -123,6 +123,7 @@ struct Engine {
   uint32_t chunk = 16;
   bool use_graph = true;
   uint32_t remap = 1;
+  uint32_t update_groups = 1;  // several tiles per k_pcg_update workgroup
   // kernel timing (HIP events around sampled iterations)
   uint32_t timing_every = 0;
   uint64_t trip = 0;  // iterations enqueued so far (for sampling)
@@ -148,6 +149,7 @@ struct Engine {
     chunk = chunk < 2 ? 2u : (chunk & ~1u);  // even: the state parity returns to 0
     use_graph = env_u32("HBJ_GRAPH", 1) != 0;
     remap = env_u32("HBJ_XCD_REMAP", 1);
+    update_groups = env_u32("HBJ_UPDATE_GROUPS", 1);
     spmv_variant = env_u32("HBJ_SPMV_VARIANT", kSpmvDefault) % kSpmvVariantCount;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -267,7 +269,7 @@ struct Engine {
   uint32_t dot64 = 0;
   LaunchSet ls() const {
     return LaunchSet{d_descs, d_tiles, n_tiles, d_ftiles, n_ftiles, remap, n_sys, spmv_variant,
-                     one_mode ? &h_one : nullptr, dot64, 0u, cap, nullptr, nullptr};
+                     one_mode ? &h_one : nullptr, dot64, 0u, update_groups, cap, nullptr, nullptr};
   }
 
   // Algorithm of the table's systems: 0 the reference's PCG (three launches

@@ -24,6 +24,8 @@ struct LaunchSet {
   const SysDesc* one;     // host copy of the only system's descriptor, or null (table)
   uint32_t dot64;         // the systems accumulate dots in FP64 (kFlagDot64, all alike)
   uint32_t par;           // iteration parity: which of a system's two SysState copies is read
+  uint32_t update_groups; // k_pcg_update_grp: two tiles per workgroup for a single
+                          // system, kFinishTiles for a table
   uint32_t st_stride;     // distance between a system's two SysState copies
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // kernel timing (optional)
 };

@@ -1167,6 +1167,189 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   }
 }
 
+// k_pcg_update with kUT consecutive tiles of one system per workgroup: the
+// p.Ap partials are reduced once per workgroup instead of once per tile,
+// and fewer workgroups are dispatched (a 50^3 system's 1,465 pairs fit in
+// one round of the GPU).  Each tile's operations are k_pcg_update's.  A
+// tile's loads are issued after the previous tile is done (holding two
+// tiles' would cost a wave per SIMD).  kOne: tiles kUT*b + u; a table:
+// entries of the engine's group table (tile0 a multiple of kUT).
+template <bool kOne, bool kDot64, uint32_t kUT>
+__global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
+  __shared__ float s_rh[kTile + 2 * kHalo];
+  __shared__ float s_z[kTile];
+  __shared__ float s_lu[kLuPer * kTile];
+  __shared__ double s_ra[kTile];
+  __shared__ double s_rb[kTile];
+  __shared__ uint32_t s_blk[2];
+  using TP = typename std::conditional<kDot64, double, float>::type;
+  TileRef tr;
+  if constexpr (kOne) {
+    tr.d = ta.one;
+    tr.tile = kUT * (ta.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
+    if (tr.tile >= tr.d.n_tiles) return;
+  } else {
+    if (!get_tile<false>(ta, tr)) return;  // tr.tile: the group's first tile
+  }
+  const SysDesc& d = tr.d;
+  const uint32_t nt = d.n_tiles, t = threadIdx.x, tile0 = tr.tile;
+  SysState* st = d.st + ta.par * ta.st_stride;
+  const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
+  float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
+  const bool b6 = (d.flags & kFlagBlocks6) != 0u;
+  struct Ld {
+    uint32_t tile, row0, nrows, row, rowc, hrow, hslot, hrowc, bs_s, bs_e;
+    bool own, halo;
+    float xi, qi, hq, zi_old, ri_prev, rh_prev, pi;
+    uint2 tb, blk;
+  };
+  // round 1 of a tile: everything that does not depend on the state
+  auto load = [&](uint32_t tile, Ld& L) {
+    L.tile = tile;
+    L.row0 = tile * (uint32_t)kTile;
+    const uint32_t left = d.n - L.row0;
+    L.nrows = left < (uint32_t)kTile ? left : (uint32_t)kTile;
+    L.own = t < L.nrows;
+    L.row = L.row0 + t;
+    L.rowc = L.own ? L.row : L.row0;
+    L.hrow = 0;
+    L.hslot = 0;
+    L.halo = halo_row(L.row0, L.nrows, d.n, L.hrow, L.hslot);
+    L.hrowc = L.halo ? L.hrow : L.row0;
+    L.xi = bld(d.x, L.rowc);
+    L.qi = bld(d.q, L.rowc);
+    L.hq = bld(d.q, L.hrowc);
+    L.zi_old = (d.flags & kFlagOversize) ? bld(d.z, L.rowc) : 0.0f;
+    L.ri_prev = bld(r_in, L.rowc);
+    L.rh_prev = bld(r_in, L.hrowc);
+    L.pi = bld(d.p, L.rowc);
+    L.tb = make_uint2(0u, 0u);
+    L.bs_s = L.bs_e = 0;
+    if (b6) {
+      const uint32_t first = L.row0 / 6u, last = (L.row0 + L.nrows - 1u) / 6u;
+      L.tb = make_uint2(first, last - first + 1u);
+      const uint32_t bi = t < L.tb.y ? L.tb.x + t : L.tb.x;
+      L.bs_s = 6u * bi;
+      L.bs_e = 6u * bi + 6u < d.n ? 6u * bi + 6u : d.n;
+      bj_stage(d.lu, L.tb.x, 0, L.tb.y < (uint32_t)kLuStage ? L.tb.y : (uint32_t)kLuStage, s_lu);
+    }
+    L.blk = (!b6 && d.tile_blk) ? d.tile_blk[tile] : make_uint2(0u, 0u);
+  };
+  Ld L;
+  const bool pre = nt > 1u && nt <= kFetchMax;
+  const TP* part_pap = kDot64 ? reinterpret_cast<const TP*>(d.part64)
+                              : reinterpret_cast<const TP*>(d.part);
+  TP vp[4][4];
+  if (pre) level_fetch<TP>(part_pap, nt, vp);
+  load(tile0, L);
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  const float rz_old = st->rz_old;
+  const double rz64_old = st->rz64_old, target2 = st->target2;
+  const uint32_t outer = uni(st->outer), max_outer = uni(st->max_outer);
+  if (status != kRunning) return;
+  if (phase != kPhInit && phase != kPhIter && phase != kPhIrResid) return;
+  const bool iter = phase == kPhIter, restart = phase == kPhIrResid;
+  float alpha = 0.0f;
+  double scale = 0.0;
+  if (iter) {
+    TP* lds = reinterpret_cast<TP*>(s_ra);
+    const TP tot =
+        pre ? reduce_fetched<TP, true>(vp, nt, lds) : reduce_partials<TP, true>(part_pap, nt, lds);
+    if constexpr (kDot64) {
+      alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
+      if (tile0 == 0 && t == 0) {
+        st->p_ap = (float)tot;
+        st->p_ap64 = tot;
+        st->alpha = alpha;
+      }
+    } else {
+      alpha = tot != 0.0f ? rz_old / tot : 0.0f;
+      if (tile0 == 0 && t == 0) {
+        st->p_ap = tot;
+        st->alpha = alpha;
+      }
+    }
+  } else if (!restart) {
+    const float bn = reduce_partials<float, true>(d.part, nt, reinterpret_cast<float*>(s_ra));
+    if (tile0 == 0 && t == 0) st->b_norm2 = bn;
+    if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
+  } else {
+    double rr, bb;
+    if (ir_decide<true>(d, target2, outer, max_outer, s_ra, s_rb, rr, bb) != kRunning) return;
+    scale = 1.0 / sqrt(rr);
+  }
+  __syncthreads();  // s_ra / s_rb are reused below
+  const float na = -alpha;
+  auto rnew = [&](float r, float q) -> float {
+    if (iter) return r + na * q;
+    if (!restart) return r + (-1.0f) * q;
+    return r;
+  };
+  for (uint32_t u = 0; u < kUT; ++u) {
+    if (u > 0u) {
+      if (tile0 + u >= nt) break;
+      load(tile0 + u, L);
+    }
+    float rs = L.ri_prev, rhs = L.rh_prev;
+    if (restart) {
+      rs = (float)(bld(d.r64, L.rowc) * scale);
+      rhs = (float)(bld(d.r64, L.hrowc) * scale);
+    } else if (!iter) {
+      rs = bld(d.b, L.rowc);
+      rhs = bld(d.b, L.hrowc);
+    }
+    uint2 tb = L.tb;
+    uint32_t bs_s = L.bs_s, bs_e = L.bs_e;
+    if (!b6) {
+      const uint2 tb0 = d.tile_blk ? L.blk : tile_blocks(d.bstart, nullptr, L.tile, d.nb, L.row0,
+                                                         L.nrows, s_blk);
+      tb = make_uint2(uni(tb0.x), uni(tb0.y));
+      const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+      bs_s = bld(d.bstart, bi);
+      bs_e = bld(d.bstart, bi + 1u);
+      bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+    }
+    const float rn = L.own ? rnew(rs, L.qi) : 0.0f;
+    if (L.own) {
+      if (iter) {
+        bst(d.x, L.row, L.xi + alpha * L.pi);  // compute.rs:386-394
+      } else if (restart) {
+        bst(d.b, L.row, rn);
+        bst(d.x, L.row, 0.0f);
+      }
+      bst(r_out, L.row, rn);
+      s_rh[t + kHalo] = rn;
+      s_z[t] = L.zi_old;
+    }
+    if (L.halo) s_rh[L.hslot] = rnew(rhs, L.hq);
+    lds_dma_wait();  // the factor copy into s_lu
+    __syncthreads();
+    bj_tile(d.lu, tb, d.n, L.row0, L.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
+    float zi = 0.0f;
+    if (L.own) {
+      zi = s_z[t];
+      bst(d.z, L.row, zi);
+    }
+    if constexpr (kDot64) {
+      double a = L.own ? (double)rn * (double)rn : 0.0;
+      double b = L.own ? (double)rn * (double)zi : 0.0;
+      tree256x2<double, true>(a, b, s_ra, s_rb);
+      if (t == 0) {
+        d.part64[nt + L.tile] = a;
+        d.part64[2u * nt + L.tile] = b;
+      }
+    } else {
+      float a = L.own ? rn * rn : 0.0f;
+      float b = L.own ? rn * zi : 0.0f;
+      tree256x2<float, true>(a, b, reinterpret_cast<float*>(s_ra), reinterpret_cast<float*>(s_rb));
+      if (t == 0) {
+        d.part[nt + L.tile] = a;
+        d.part[2u * nt + L.tile] = b;
+      }
+    }
+  }
+}
+
 // The state transition at the end of an iteration, in the reference's
 // order: the late scalar update (pcg_update_scalars.wgsl:64-67), then the
 // breakdown checks and the stop test (compute.rs:468-488); at init
@@ -1866,6 +2049,21 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
   const dim3 g(ls.n_tiles), b(kTile);
+  if (ls.update_groups && ls.one) {
+    const dim3 g2((ls.n_tiles + 1) / 2);
+    if (ls.dot64) launch(ls, k_pcg_update_grp<true, true, 2>, g2, b, s, a);
+    else launch(ls, k_pcg_update_grp<true, false, 2>, g2, b, s, a);
+    return;
+  }
+  if (ls.update_groups) {  // the finish table's groups of kFinishTiles tiles
+    TileArgs ag = a;
+    ag.tiles = ls.ftiles;
+    ag.n_tiles = ls.n_ftiles;
+    const dim3 g4(ls.n_ftiles ? ls.n_ftiles : 1);
+    if (ls.dot64) launch(ls, k_pcg_update_grp<false, true, kFinishTiles>, g4, b, s, ag);
+    else launch(ls, k_pcg_update_grp<false, false, kFinishTiles>, g4, b, s, ag);
+    return;
+  }
   if (ls.dot64) {
     if (ls.one) launch(ls, k_pcg_update<true, true>, g, b, s, a);
     else launch(ls, k_pcg_update<false, true>, g, b, s, a);

@@ -1236,7 +1236,9 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
     L.blk = (!b6 && d.tile_blk) ? d.tile_blk[tile] : make_uint2(0u, 0u);
   };
   Ld L;
-  const bool pre = nt > 1u && nt <= kFetchMax;
+  // FP64 partials are reduced after the state is read: prefetched they
+  // would hold 32 more VGPRs and cost the kernel its one-round fit
+  const bool pre = !kDot64 && nt > 1u && nt <= kFetchMax;
   const TP* part_pap = kDot64 ? reinterpret_cast<const TP*>(d.part64)
                               : reinterpret_cast<const TP*>(d.part);
   TP vp[4][4];
@@ -1459,7 +1461,10 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   // r.r and r.z partials fetched together with the state
   using TP = typename std::conditional<kDot64, double, float>::type;
   const TP* part = kDot64 ? reinterpret_cast<const TP*>(d.part64) : reinterpret_cast<const TP*>(d.part);
-  const bool pre = nt > 1u && nt <= kFetchMax;
+  // a table's FP64 partials are reduced after the state is read (prefetched
+  // they hold 64 VGPRs, which costs a table's many rounds of workgroups more
+  // than the extra load round; a single system's one round gains from them)
+  const bool pre = (kOne || !kDot64) && nt > 1u && nt <= kFetchMax;
   TP va[4][4], vb[4][4];
   if (pre) {
     level_fetch<TP>(part + nt, nt, va);

@@ -336,3 +336,26 @@ def test_one_tile_update_kernel_bitwise(ctx, hbj, oracle, monkeypatch):
     x, res = sys_.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
     assert res.iterations == it and np.array_equal(bits(x), bits(xo))
     sys_.close()
+
+
+def test_final_residual_matches_the_history_fixture(ctx, hbj, oracle):
+    """The GPU's iteration count and final r.r equal the committed oracle
+    histories (tests/golden/histories.npz) without running the oracle."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "histories.npz"))
+    cases = []
+    for k in (1, 2, 3):
+        d = oracle.load_dataset(k)
+        cases.append((f"ds{k}", d["row_ptr"], d["col_idx"], d["values"], d["block_starts"], d["b"],
+                      d["x0"], 2000, 1e-4, 1e-7))
+    L = hbj.lattice(10, seed=0)
+    cases.append(("lattice10", L["row_ptr"], L["col_idx"], L["values"], L["block_starts"], L["rhs"],
+                  np.zeros(L["n"], F), 5000, 1e-6, 0.0))
+    for name, rp, ci, va, bs, b, x0, mi, rel, ab in cases:
+        print(f"history fixture {name}", flush=True)
+        sys_ = hbj.System(ctx, rp, ci, va, bs)
+        x, res = sys_.solve(b, x0, mi, rel, ab)
+        st, it = fx[f"{name}_status_iterations"].tolist()
+        assert res.status == st and res.iterations == it, name
+        assert np.float32(res.rr).view(np.uint32) == fx[f"{name}_rr"][-1].view(np.uint32), name
+        sys_.close()

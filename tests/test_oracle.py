@@ -457,3 +457,28 @@ def test_fp64_oracle_vs_scipy(oracle, hbj):
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-12
     xd = spla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+
+
+def test_oracle_residual_histories_match_the_fixture(oracle, hbj):
+    """tests/golden/histories.npz (tests/golden/make_histories.py): the FP32
+    r.r of every iteration on the three datasets and the 10^3 lattice, bit for
+    bit, and the lattice generator's output by digest."""
+    import hashlib
+    fx = np.load(os.path.join(GOLDEN, "histories.npz"))
+    L = hbj.lattice(10, seed=0)
+    h = hashlib.sha256()
+    for k in ("row_ptr", "col_idx", "values", "rhs", "block_starts"):
+        h.update(np.ascontiguousarray(L[k]).tobytes())
+    assert h.hexdigest() == str(fx["lattice10_sha256"])
+    cases = []
+    for k in (1, 2, 3):
+        d = oracle.load_dataset(k)
+        cases.append((f"ds{k}", d["row_ptr"], d["col_idx"], d["values"], d["block_starts"], d["b"],
+                      d["x0"], 2000, 1e-4, 1e-7))
+    cases.append(("lattice10", L["row_ptr"], L["col_idx"], L["values"], L["block_starts"], L["rhs"],
+                  np.zeros(L["n"], F), 5000, 1e-6, 0.0))
+    for name, rp, ci, va, bs, b, x0, mi, rel, ab in cases:
+        lu = _lu(oracle, {"n": len(rp) - 1, "row_ptr": rp, "col_idx": ci, "values": va, "block_starts": bs})
+        st, it, x, hist = oracle.pcg(rp, ci, va, lu, bs, b, x0, mi, rel, ab, history=True)
+        assert [st, it] == fx[f"{name}_status_iterations"].tolist(), name
+        assert np.array_equal(hist.view(np.uint32), fx[f"{name}_rr"].view(np.uint32)), name

@@ -267,6 +267,8 @@ def bench_batch(args):
         s.set_rhs64(L["rhs"].astype(np.float64))
     params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
     batch = H.Batch(ctx, systems)
+    best, tune_us = batch.autotune(20, ir=True)
+    log(f"batch autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 1).tolist()}")
     for _ in range(args.warmup):
         batch.solve_ir(params)
     t0 = time.perf_counter()
@@ -294,6 +296,7 @@ def bench_batch(args):
                                    f"(rel tol {args.inner_tol:g}) + FP64 refinement to 1e-8, one tile "
                                    f"table, graph-replayed chunks",
                        "systems": len(systems)},
+            "spmv": {"variant": best, "autotune_us_per_iter": [float(u) for u in tune_us]},
             "per_system_path_solves_per_s": alone_sps, "speedup_vs_per_system": batched_sps / alone_sps,
             "worst_true_rel_residual_fp64": worst, "iterations_identical_alone_vs_batched": same,
             "status_all_ok": st == 0}

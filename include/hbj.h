@@ -240,6 +240,16 @@ hbj_status hbj_batch_solve(hbj_batch* batch, uint32_t max_iter, float rel_tol, f
                            hbj_result* results /* count */);
 hbj_status hbj_batch_solve_ir(hbj_batch* batch, const hbj_ir_params* params,
                               hbj_result* results /* count */);
+/* Times each of the HBJ_SPMV_VARIANTS bit-identical SpMV variants on the
+ * batch's own tile table (one untimed run, then the best of three runs of
+ * `iters` iterations of every system; ir != 0: inside the refinement loop on
+ * the systems' FP64 right-hand sides, else the plain PCG on their FP32 ones)
+ * and keeps the fastest for later solves; us (HBJ_SPMV_VARIANTS doubles, may
+ * be NULL) receives microseconds per iteration.  Overwrites the systems'
+ * solution vectors; results of later solves do not change.  No reference
+ * counterpart. */
+hbj_status hbj_batch_autotune(hbj_batch* batch, uint32_t iters, int ir, uint32_t* best,
+                              double* us);
 
 /* ------------------------------------------------------------ serving */
 

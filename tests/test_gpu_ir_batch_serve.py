@@ -83,6 +83,8 @@ def test_batch_fp32_identical_to_alone(ctx, hbj, oracle):
         alone.append((x, res.iterations))
         s.set_rhs(L["rhs"])
     batch = hbj.Batch(ctx, [s for _, s in sysl])
+    best, us = batch.autotune(10)  # any variant: the same bits
+    assert 0 <= best < hbj.SPMV_VARIANTS and np.all(us > 0) and us[best] == us.min()
     st, results = batch.solve(5000, 1e-6, 0.0)
     assert st == 0
     for (L, s), (xa, ita), r in zip(sysl, alone, results):
@@ -125,6 +127,8 @@ def test_batch_ir_identical_to_alone(ctx, hbj):
         alone.append((x, res))
         s.set_rhs64(L["rhs"].astype(np.float64))
     batch = hbj.Batch(ctx, [s for _, s in sysl])
+    best, us = batch.autotune(10, ir=True)
+    assert 0 <= best < hbj.SPMV_VARIANTS and np.all(us > 0)
     st, results = batch.solve_ir(p)
     assert st == 0
     for (L, s), (xa, ra), r in zip(sysl, alone, results):

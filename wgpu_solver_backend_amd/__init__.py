@@ -429,6 +429,13 @@ class Batch:
         st = _lib.lib().hbj_batch_solve_ir(self._h, ctypes.byref(p), res)
         return st, list(res)
 
+    def autotune(self, iters=20, ir=False):
+        """hbj_batch_autotune: (best SpMV variant, us per iteration of each)."""
+        best = ctypes.c_uint32()
+        us = np.zeros(SPMV_VARIANTS, np.float64)
+        _check(_lib.lib().hbj_batch_autotune(self._h, int(iters), int(bool(ir)), ctypes.byref(best), us))
+        return int(best.value), us
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().hbj_batch_destroy(self._h)

@@ -887,4 +887,68 @@ hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* 
   return worst;
 }
 
+hbj_status hbj_batch_autotune(hbj_batch* b, uint32_t iters, int ir, uint32_t* best, double* us) {
+  clear_error();
+  if (!b || iters == 0) { set_error("hbj_batch_autotune: need a batch and iters > 0"); return HBJ_E_INVALID; }
+  HIPCHK(hipSetDevice(b->ctx->device));
+  if (ir) {
+    for (auto* s : b->sys) CHK(s->ensure_ir_buffers());
+    if (b->mode != 1 && b->ready) b->mode = -1;
+  }
+  CHK(b->ensure(ir ? 1 : 0));
+  hipStream_t st = b->eng.stream;
+  const uint32_t S = (uint32_t)b->sys.size();
+  // every system runs exactly `iters` inner iterations: tolerances that
+  // square to 0 (refinement: one outer step, then out of steps)
+  hbj_ir_params p{};
+  p.max_outer = 1;
+  p.max_inner = iters;
+  p.inner_rel_tol = 1e-30f;
+  p.target_rel_res = 1e-300;
+  const uint32_t timing = b->eng.timing_every;
+  b->eng.timing_every = 0;
+  double best_us = 0;
+  uint32_t bv = kSpmvDefault;
+  std::vector<SysState> init(S), fin(S);
+  for (uint32_t v = 0; v < kSpmvVariantCount; ++v) {
+    b->eng.set_spmv_variant(v);
+    double tv = 0;
+    for (int rep = 0; rep < 4; ++rep) {  // the first run captures the graph; best of 3
+      for (uint32_t i = 0; i < S; ++i) {
+        hbj_system* s = b->sys[i];
+        init[i] = ir ? s->init_state(true, 0, 0, 0, &p) : s->init_state(false, iters, 0.0f, 0.0f, nullptr);
+        if (s->n == 0) {
+          init[i].status = HBJ_OK;
+          continue;
+        }
+        if (ir) HIPCHK(hipMemcpyAsync(s->x64, s->x0_64, 8ull * s->n, hipMemcpyDeviceToDevice, st));
+        else HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, st));
+        if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
+      }
+      double ms = 0;
+      const hbj_status rs = b->eng.run(init.data(), fin.data(), &ms);
+      if (rs != HBJ_OK) {
+        b->eng.timing_every = timing;
+        return rs;
+      }
+      uint32_t its = 1;
+      for (uint32_t i = 0; i < S; ++i) {
+        const uint32_t k = ir ? fin[i].total_iter : fin[i].iter;
+        its = k > its ? k : its;
+      }
+      const double u = ms * 1e3 / (double)its;
+      if (rep == 1 || (rep > 1 && u < tv)) tv = u;
+    }
+    if (us) us[v] = tv;
+    if (v == 0 || tv < best_us) {
+      best_us = tv;
+      bv = v;
+    }
+  }
+  b->eng.timing_every = timing;
+  b->eng.set_spmv_variant(bv);
+  if (best) *best = bv;
+  return HBJ_OK;
+}
+
 }  // extern "C"

@@ -1466,10 +1466,11 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   // than the extra load round; a single system's one round gains from them)
   const bool pre = (kOne || !kDot64) && nt > 1u && nt <= kFetchMax;
   TP va[4][4], vb[4][4];
-  if (pre) {
-    level_fetch<TP>(part + nt, nt, va);
-    level_fetch<TP>(part + 2u * nt, nt, vb);
-  }
+  // workgroups that will only need beta (below) do not fetch the r.r partials
+  const bool maybe_beta_only = !kDot64 && !writer && !(d.flags & kFlagIr);
+  const bool pre_a = pre && !maybe_beta_only;
+  if (pre_a) level_fetch<TP>(part + nt, nt, va);
+  if (pre) level_fetch<TP>(part + 2u * nt, nt, vb);
   if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(in)[t];
   __syncthreads();
   const int32_t status = uni(s_ns.status), phase = uni(s_ns.phase);
@@ -1488,10 +1489,39 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   double rz64 = 0.0;
   const bool reduce = !(phase == kPhInit && s_ns.b_norm2 == 0.0f) &&
                       !(phase == kPhIrResid && dec != kRunning);
+  // Without refinement only the state writer needs r.r (the stop test);
+  // the other workgroups need beta, from r.z alone, and update p even in
+  // the iteration that stops the system (harmless: nothing reads p then).
+  const bool beta_only = maybe_beta_only && phase == kPhIter;
+  if (reduce && beta_only) {
+    TP* lb = reinterpret_cast<TP*>(s_b);
+    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
+    const float rz_old = s_ns.rz_old;
+    const float beta = rz_old != 0.0f ? (float)b / rz_old : 0.0f;  // pcg_transition's beta
+    constexpr uint32_t kT = kFinishTiles;
+    float zv[kT], pv[kT];
+    uint32_t rw[kT];
+#pragma unroll
+    for (uint32_t k = 0; k < kT; ++k) {
+      const uint32_t tile = tr.tile + k * step;
+      const uint32_t row = tile * (uint32_t)kTile + t;
+      rw[k] = (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
+      const uint32_t rc = rw[k] != 0xFFFFFFFFu ? row : 0u;
+      zv[k] = bld(d.z, rc);
+      pv[k] = bld(d.p, rc);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kT; ++k) {
+      if (rw[k] == 0xFFFFFFFFu) continue;
+      const float pb = pv[k] * beta;
+      bst(d.p, rw[k], pb + zv[k]);
+    }
+    return;
+  }
   if (reduce) {
     TP* la = reinterpret_cast<TP*>(s_a);
     TP* lb = reinterpret_cast<TP*>(s_b);
-    const TP a = pre ? reduce_fetched<TP>(va, nt, la) : reduce_partials<TP>(part + nt, nt, la);
+    const TP a = pre_a ? reduce_fetched<TP>(va, nt, la) : reduce_partials<TP>(part + nt, nt, la);
     const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
     rr = (float)a;
     rz = (float)b;

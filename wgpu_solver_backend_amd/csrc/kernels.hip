@@ -1666,7 +1666,12 @@ __global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
   }
 }
 
-template <bool kOne>
+// kUT consecutive tiles of the system per workgroup (single system: tiles
+// kUT*b + u); the three reductions and the decision run once per workgroup.
+// kPre: the partials are fetched before the state (48 more VGPRs; the
+// two-tile form goes without: 53 VGPRs and one round of workgroups, 15.1 us
+// at 50^3 against 15.5 with the prefetch and 16.5 for one tile each).
+template <bool kOne, uint32_t kUT, bool kPre>
 __global__ __launch_bounds__(kTile) void k_ccg_update(const TileArgs ta) {
   __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
@@ -1675,43 +1680,72 @@ __global__ __launch_bounds__(kTile) void k_ccg_update(const TileArgs ta) {
   __shared__ uint32_t s_blk[2];
   __shared__ SysState s_ns;
   TileRef tr;
-  if (!get_tile<kOne>(ta, tr)) return;
+  if constexpr (kOne) {
+    tr.d = ta.one;
+    tr.tile = kUT * (ta.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
+    if (tr.tile >= tr.d.n_tiles) return;
+  } else {
+    static_assert(kUT == 1, "tables take one tile per workgroup");
+    if (!get_tile<false>(ta, tr)) return;
+  }
   const SysDesc& d = tr.d;
   SysState* st = d.st + ta.par * ta.st_stride;
   SysState* st_next = d.st + (ta.par ^ 1u) * ta.st_stride;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t, nt = d.n_tiles;
-  const bool own = t < tr.nrows;
-  const uint32_t rowc = own ? row : tr.row0;
-  uint32_t hrow = 0, hslot = 0;
-  const bool halo = halo_row(tr.row0, tr.nrows, d.n, hrow, hslot);
-  const uint32_t hrowc = halo ? hrow : tr.row0;
-  // round 1: the partials, own and halo rows, the factors (6-row blocks)
-  const bool pre = nt > 1u && nt <= kFetchMax;
+  const uint32_t t = threadIdx.x, nt = d.n_tiles, tile0 = tr.tile;
+  const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
+  float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
+  const float* __restrict__ s_in = ta.par ? d.s[1] : d.s[0];
+  float* __restrict__ s_out = ta.par ? d.s[0] : d.s[1];
+  const bool b6 = (d.flags & kFlagBlocks6) != 0u;
+  struct Ld {
+    uint32_t tile, row0, nrows, row, rowc, hrow, hslot, hrowc, bs_s, bs_e;
+    bool own, halo;
+    float xi, pi, ui, wi, ri, si, hr, hs, hw;
+    uint2 tb, blk;
+  };
+  // a tile's loads that do not depend on the state
+  auto load = [&](uint32_t tile, Ld& L) {
+    L.tile = tile;
+    L.row0 = tile * (uint32_t)kTile;
+    const uint32_t left = d.n - L.row0;
+    L.nrows = left < (uint32_t)kTile ? left : (uint32_t)kTile;
+    L.own = t < L.nrows;
+    L.row = L.row0 + t;
+    L.rowc = L.own ? L.row : L.row0;
+    L.hrow = 0;
+    L.hslot = 0;
+    L.halo = halo_row(L.row0, L.nrows, d.n, L.hrow, L.hslot);
+    L.hrowc = L.halo ? L.hrow : L.row0;
+    L.xi = bld(d.x, L.rowc);
+    L.pi = bld(d.p, L.rowc);
+    L.ui = bld(d.z, L.rowc);
+    L.wi = bld(d.q, L.rowc);
+    L.ri = bld(r_in, L.rowc);
+    L.si = bld(s_in, L.rowc);
+    L.hr = bld(r_in, L.hrowc);
+    L.hs = bld(s_in, L.hrowc);
+    L.hw = bld(d.q, L.hrowc);
+    L.tb = make_uint2(0u, 0u);
+    L.bs_s = L.bs_e = 0;
+    if (b6) {
+      const uint32_t first = L.row0 / 6u, last = (L.row0 + L.nrows - 1u) / 6u;
+      L.tb = make_uint2(first, last - first + 1u);
+      const uint32_t bi = t < L.tb.y ? L.tb.x + t : L.tb.x;
+      L.bs_s = 6u * bi;
+      L.bs_e = 6u * bi + 6u < d.n ? 6u * bi + 6u : d.n;
+      bj_stage(d.lu, L.tb.x, 0, L.tb.y < (uint32_t)kLuStage ? L.tb.y : (uint32_t)kLuStage, s_lu);
+    }
+    L.blk = (!b6 && d.tile_blk) ? d.tile_blk[tile] : make_uint2(0u, 0u);
+  };
+  const bool pre = kPre && nt > 1u && nt <= kFetchMax;
   float vg[4][4], vd[4][4], vr[4][4];
   if (pre) {
     level_fetch<float>(d.part, nt, vg);
     level_fetch<float>(d.part + nt, nt, vd);
     level_fetch<float>(d.part + 2u * nt, nt, vr);
   }
-  const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
-  float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
-  const float* __restrict__ s_in = ta.par ? d.s[1] : d.s[0];
-  float* __restrict__ s_out = ta.par ? d.s[0] : d.s[1];
-  const float xi = bld(d.x, rowc), pi = bld(d.p, rowc), ui = bld(d.z, rowc), wi = bld(d.q, rowc);
-  const float ri = bld(r_in, rowc), si = bld(s_in, rowc);
-  const float hr = bld(r_in, hrowc), hs = bld(s_in, hrowc), hw = bld(d.q, hrowc);
-  const bool b6 = (d.flags & kFlagBlocks6) != 0u;
-  uint2 tb = make_uint2(0u, 0u);
-  uint32_t bs_s = 0, bs_e = 0;
-  if (b6) {
-    const uint32_t first = tr.row0 / 6u, last = (tr.row0 + tr.nrows - 1u) / 6u;
-    tb = make_uint2(first, last - first + 1u);
-    const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
-    bs_s = 6u * bi;
-    bs_e = 6u * bi + 6u < d.n ? 6u * bi + 6u : d.n;
-    bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
-  }
-  const uint2 blk = (!b6 && d.tile_blk) ? d.tile_blk[tr.tile] : make_uint2(0u, 0u);
+  Ld L;
+  load(tile0, L);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   const uint32_t it = uni(st->iter), max_iter = uni(st->max_iter);
   const float g_prev = st->rz_old, a_prev = st->alpha;
@@ -1756,7 +1790,7 @@ __global__ __launch_bounds__(kTile) void k_ccg_update(const TileArgs ta) {
       }
     }
   }
-  if (tr.tile == 0) {  // the next state, staged in LDS and copied out by dwords
+  if (tile0 == 0) {  // the next state, staged in LDS and copied out by dwords
     constexpr uint32_t kWords = sizeof(SysState) / 4;
     static_assert(kWords <= (uint32_t)kTile && sizeof(SysState) % 4 == 0, "state copy by dwords");
     if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(st)[t];
@@ -1795,46 +1829,54 @@ __global__ __launch_bounds__(kTile) void k_ccg_update(const TileArgs ta) {
     }
   }
   if (fin != kRunning) return;
-
-  // round 2: b at init; the block table when blocks are not 6 rows
-  float rs = ri, rhs_ = hr;
-  if (init) {
-    rs = bld(d.b, rowc);
-    rhs_ = bld(d.b, hrowc);
-  }
-  if (!b6) {
-    const uint2 tb0 = d.tile_blk ? blk : tile_blocks(d.bstart, nullptr, tr.tile, d.nb, tr.row0,
-                                                     tr.nrows, s_blk);
-    tb = make_uint2(uni(tb0.x), uni(tb0.y));
-    const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
-    bs_s = bld(d.bstart, bi);
-    bs_e = bld(d.bstart, bi + 1u);
-    bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
-  }
   const float na = -alpha;
-  float rn = 0.0f;
-  if (own) {
-    if (init) {
-      rn = rs + (-1.0f) * wi;  // r0 = b - A x0 (q holds A x0)
-      bst(d.p, row, 0.0f);
-      bst(s_out, row, 0.0f);
-    } else {
-      const float pn = ui + beta * pi;
-      const float sn = wi + beta * si;
-      rn = rs + na * sn;
-      bst(d.p, row, pn);
-      bst(s_out, row, sn);
-      bst(d.x, row, xi + alpha * pn);
+  for (uint32_t u = 0; u < kUT; ++u) {
+    if (u > 0u) {
+      if (tile0 + u >= nt) break;
+      load(tile0 + u, L);
     }
-    bst(r_out, row, rn);
-    s_rh[t + kHalo] = rn;
-    s_z[t] = ui;  // rows no block writes keep u
+    // b at init; the block table when blocks are not 6 rows
+    float rs = L.ri, rhs_ = L.hr;
+    if (init) {
+      rs = bld(d.b, L.rowc);
+      rhs_ = bld(d.b, L.hrowc);
+    }
+    uint2 tb = L.tb;
+    uint32_t bs_s = L.bs_s, bs_e = L.bs_e;
+    if (!b6) {
+      const uint2 tb0 = d.tile_blk ? L.blk : tile_blocks(d.bstart, nullptr, L.tile, d.nb, L.row0,
+                                                         L.nrows, s_blk);
+      tb = make_uint2(uni(tb0.x), uni(tb0.y));
+      const uint32_t bi = t < tb.y ? tb.x + t : tb.x;
+      bs_s = bld(d.bstart, bi);
+      bs_e = bld(d.bstart, bi + 1u);
+      bj_stage(d.lu, tb.x, 0, tb.y < (uint32_t)kLuStage ? tb.y : (uint32_t)kLuStage, s_lu);
+    }
+    float rn = 0.0f;
+    if (L.own) {
+      if (init) {
+        rn = rs + (-1.0f) * L.wi;  // r0 = b - A x0 (q holds A x0)
+        bst(d.p, L.row, 0.0f);
+        bst(s_out, L.row, 0.0f);
+      } else {
+        const float pn = L.ui + beta * L.pi;
+        const float sn = L.wi + beta * L.si;
+        rn = rs + na * sn;
+        bst(d.p, L.row, pn);
+        bst(s_out, L.row, sn);
+        bst(d.x, L.row, L.xi + alpha * pn);
+      }
+      bst(r_out, L.row, rn);
+      s_rh[t + kHalo] = rn;
+      s_z[t] = L.ui;  // rows no block writes keep u
+    }
+    if (L.halo) s_rh[L.hslot] = init ? rhs_ + (-1.0f) * L.hw : rhs_ + na * (L.hw + beta * L.hs);
+    lds_dma_wait();  // the factor copy into s_lu
+    __syncthreads();
+    bj_tile(d.lu, tb, d.n, L.row0, L.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
+    if (L.own) bst(d.z, L.row, s_z[t]);
+    __syncthreads();  // s_z and s_rh are written again by the next tile
   }
-  if (halo) s_rh[hslot] = init ? rhs_ + (-1.0f) * hw : rhs_ + na * (hw + beta * hs);
-  lds_dma_wait();  // the factor copy into s_lu
-  __syncthreads();
-  bj_tile(d.lu, tb, d.n, tr.row0, tr.nrows, bs_s, bs_e, s_rh, s_z, s_lu);
-  if (own) bst(d.z, row, s_z[t]);
 }
 
 // FP64 check of an FP32 solution: per tile ||b - A x||^2 and ||b||^2 with
@@ -2142,9 +2184,14 @@ void launch_ccg_spmv(const LaunchSet& ls, hipStream_t s) {
 }
 void launch_ccg_update(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
-  const dim3 g(ls.n_tiles), b(kTile);
-  if (ls.one) launch(ls, k_ccg_update<true>, g, b, s, a);
-  else launch(ls, k_ccg_update<false>, g, b, s, a);
+  const dim3 b(kTile);
+  if (ls.one && ls.update_groups) {
+    launch(ls, k_ccg_update<true, 2, false>, dim3((ls.n_tiles + 1) / 2), b, s, a);
+  } else if (ls.one) {
+    launch(ls, k_ccg_update<true, 1, true>, dim3(ls.n_tiles), b, s, a);
+  } else {
+    launch(ls, k_ccg_update<false, 1, true>, dim3(ls.n_tiles), b, s, a);
+  }
 }
 
 void launch_op_axpy(uint32_t n, float alpha, const float* x, float* y, hipStream_t s) {

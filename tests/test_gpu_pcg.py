@@ -359,3 +359,49 @@ def test_final_residual_matches_the_history_fixture(ctx, hbj, oracle):
         assert res.status == st and res.iterations == it, name
         assert np.float32(res.rr).view(np.uint32) == fx[f"{name}_rr"][-1].view(np.uint32), name
         sys_.close()
+
+
+def _banded_spd(n, seed):
+    """Symmetric, diagonally dominant, entries within 40 of the diagonal."""
+    rng = np.random.default_rng(seed)
+    rows = [dict() for _ in range(n)]
+    for i in range(n):
+        for k in rng.integers(1, 41, 4):
+            j = i + int(k)
+            if j < n:
+                v = float(rng.uniform(-1.0, 1.0))
+                rows[i][j] = v
+                rows[j][i] = v
+    for i in range(n):
+        rows[i][i] = sum(abs(v) for v in rows[i].values()) + 1.0
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        for j in sorted(rows[i]):
+            ci.append(j)
+            va.append(rows[i][j])
+        rp.append(len(ci))
+    bs = np.array(list(range(0, n, 6)) + [n], np.uint32)  # 6-row blocks, the last one partial
+    return (np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, F), bs,
+            rng.uniform(-1.0, 1.0, n).astype(F))
+
+
+@pytest.mark.parametrize("n", [1, 5, 7, 255, 256, 257, 511, 513, 1000])
+def test_tile_and_block_edges_bitwise(ctx, hbj, oracle, n):
+    """Systems ending inside a tile and inside a 6-row block (the arithmetic
+    block path with a partial last block), both algorithms, every variant."""
+    print(f"edges n={n}", flush=True)
+    rp, ci, va, bs, b = _banded_spd(n, seed=n)
+    x0 = np.zeros(n, F)
+    st, it, xo, _ = oracle_solve(oracle, rp, ci, va, bs, b, x0, 500, 1e-6, 0.0)
+    st_lu, lu = oracle.build_lu_blocks_6(n, rp, ci, va, bs)
+    st_c, it_c, xc = oracle.ccg(rp, ci, va, lu, bs, b, x0, 500, 1e-6, 0.0)
+    sys_ = hbj.System(ctx, rp, ci, va, bs)
+    for alg, sts, its, xref in ((hbj.ALG_PCG, st, it, xo), (hbj.ALG_PCG_ONE_REDUCTION, st_c, it_c, xc)):
+        sys_.set_algorithm(alg)
+        for v in range(hbj.SPMV_VARIANTS):
+            sys_.set_spmv_variant(v)
+            x, res = sys_.solve(b, x0, 500, 1e-6, 0.0, raise_on_error=False)
+            assert res.status == sts and res.iterations == its, (alg, v)
+            if sts == 0:
+                assert np.array_equal(bits(x), bits(xref)), (alg, v)
+    sys_.close()

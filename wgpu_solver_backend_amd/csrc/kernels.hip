@@ -850,10 +850,13 @@ __device__ __forceinline__ bool halo_row(uint32_t row0, uint32_t nrows, uint32_t
 //                            (the same tree, so alpha is bit-identical in all
 //                            of them), then x += alpha p, r -= alpha q,
 //                            z = M^-1 r and the tile's r.r and r.z partials
-//   k_pcg_finish   (tiles)   every workgroup reduces r.r and r.z itself, then
-//                            beta, breakdown checks, stop test and the state
-//                            transition in the reference's order; then
-//                            p = z + beta p [refinement: the fold]
+//                            (k_pcg_update_grp: the same over two consecutive
+//                            tiles per workgroup, four in a table)
+//   k_pcg_finish   (tiles)   every workgroup reduces r.z (the state writer and
+//                            refinement also r.r), then beta, breakdown
+//                            checks, stop test and the state transition in
+//                            the reference's order; then p = z + beta p over
+//                            four tiles [refinement: the fold]
 // Tiles write plain per-tile partials that later kernels read after the
 // kernel boundary; no workgroup waits on or synchronises with another.
 // Within a kernel no workgroup writes state another workgroup of it reads:
@@ -1424,7 +1427,9 @@ __device__ __forceinline__ void pcg_transition(SysState& ns, uint32_t flags, flo
 
 // After k_pcg_update.  Every workgroup reduces r.r and r.z (and, at a
 // refinement decision, the residual's norms) from the partials -- the same
-// trees, so the same bits -- and computes the state transition; the
+// trees, so the same bits -- and computes the state transition (without
+// refinement the workgroups other than the state writer reduce r.z alone
+// and take beta from it; below); the
 // workgroup holding the system's tile 0 writes the next iteration's state
 // (a final state into both copies: a finished system may get no more
 // workgroups once its tiles are dropped from the table).  Then, over the

@@ -80,3 +80,32 @@ def test_run_pcg_case_failure_writes_metrics_only(hbj, tmp_path):
     m = json.load(open(tmp_path / "o" / "m.json"))
     assert m["converged"] is False and m["iterations"] is None
     assert not (tmp_path / "o" / "x.bin").exists()
+
+
+def test_run_pcg_case_one_reduction(hbj, oracle, tmp_path):
+    """--algorithm one-reduction: the optional PCG through the CLI; x is the
+    oracle's one-reduction x bit for bit and passes compare-x."""
+    d = oracle.load_dataset(2)
+    write_text_dataset(tmp_path / "ds2.txt", d, extra_section=False)
+    case, outd = tmp_path / "case", tmp_path / "out"
+    rc, out, err = run([hbj.CLI_PATH, "text-to-bin", "--input", str(tmp_path / "ds2.txt"),
+                        "--out-dir", str(case), "--force"])
+    assert rc == 0, err
+    rc, out, err = run([hbj.CLI_PATH, "run-pcg-case", "--case-dir", str(case), "--max-iters", "2000",
+                        "--rel-tol", "1e-4", "--abs-tol", "1e-7", "--algorithm", "one-reduction",
+                        "--out-x", str(outd / "x.bin"), "--out-metrics", str(outd / "metrics.json")])
+    assert rc == 0, err
+    m = json.load(open(outd / "metrics.json"))
+    st, lu = oracle.build_lu_blocks_6(int(d["n"]), d["row_ptr"], d["col_idx"], d["values"],
+                                      d["block_starts"])
+    sto, ito, xo = oracle.ccg(d["row_ptr"], d["col_idx"], d["values"], lu, d["block_starts"], d["b"],
+                              d["x0"], 2000, 1e-4, 1e-7)
+    assert m["converged"] is True and m["algorithm"] == "one-reduction" and m["iterations"] == ito
+    x = hbj.read_x_bin(str(outd / "x.bin"))
+    assert np.array_equal(x.view(np.uint32), xo.view(np.uint32))
+    rc, out, err = run([hbj.CLI_PATH, "compare-x", "--x-ref", str(case / "x_ref.bin"), "--x",
+                        str(outd / "x.bin"), "--rel-tol", "1e-3", "--abs-tol", "1e0"])
+    assert rc == 0 and "pass               : true" in out
+    rc, out, err = run([hbj.CLI_PATH, "run-pcg-case", "--case-dir", str(case), "--algorithm", "cg",
+                        "--out-x", str(outd / "y.bin"), "--out-metrics", str(outd / "m2.json")])
+    assert rc != 0 and "--algorithm must be pcg or one-reduction" in err

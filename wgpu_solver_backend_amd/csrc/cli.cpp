@@ -4,6 +4,8 @@
 //           block-jacobi-test | pcg-update-scalars-test
 //   hbj_cli run-pcg-case --case-dir D --out-x X --out-metrics M
 //                        [--max-iters 2000] [--rel-tol 1e-8] [--abs-tol 0]
+//                        [--algorithm pcg|one-reduction]   (no reference counterpart:
+//                        hbj_system_set_algorithm; pcg, the reference's, is the default)
 //   hbj_cli compare-x --x-ref R --x X [--rel-tol 1e-3] [--abs-tol 1e-4] [--top-k 10]
 //   hbj_cli text-to-bin --input T --out-dir D [--force]
 //
@@ -181,6 +183,9 @@ int run_pcg_case(const Args& a) {
   const float rel_tol = std::stof(a.get("rel-tol", "1e-8"));
   const float abs_tol = std::stof(a.get("abs-tol", "0"));
   const std::string out_x = a.get("out-x"), out_metrics = a.get("out-metrics");
+  const std::string algorithm = a.get("algorithm", "pcg");
+  if (algorithm != "pcg" && algorithm != "one-reduction")
+    die("--algorithm must be pcg or one-reduction, got " + algorithm);
   const auto t0 = Clock::now();
   auto tg = Clock::now();
   hbj_ctx* ctx = make_ctx();
@@ -204,8 +209,11 @@ int run_pcg_case(const Args& a) {
     hbj_case_view(cs, &n, &nnz, &ns, &rp, &ci, &va, &b, &x0, &bs);
     auto ts = Clock::now();
     hbj_system* sys = nullptr;
-    if (hbj_system_create(ctx, n, nnz, rp, ci, va, bs, ns, &sys) != HBJ_OK) {
+    if (hbj_system_create(ctx, n, nnz, rp, ci, va, bs, ns, &sys) != HBJ_OK ||
+        (algorithm == "one-reduction" &&
+         hbj_system_set_algorithm(sys, HBJ_ALG_PCG_ONE_REDUCTION) != HBJ_OK)) {
       err = hbj_last_error();
+      if (sys) hbj_system_destroy(sys);
     } else {
       t_setup = ms_since(ts);
       x.assign(x0, x0 + n);
@@ -238,6 +246,7 @@ int run_pcg_case(const Args& a) {
                   ",\n  \"converged\": " + (converged ? "true" : "false") +
                   ",\n  \"error\": " + (err.empty() ? "null" : json_str(err)) +
                   ",\n  \"true_rel_residual_fp64\": " + num(res.rel_res_true_fp64) +
+                  ",\n  \"algorithm\": " + json_str(algorithm) +
                   ",\n  \"timings_ms\": {\"total\": " + num(ms_since(t0)) + ", \"load_io\": " +
                   num(t_load) + ", \"gpu_init\": " + num(t_gpu) + ", \"gpu_setup\": " +
                   num(t_setup) + ", \"solve\": " + num(t_solve) + ", \"write_out\": " +

@@ -156,9 +156,12 @@ hbj_status hbj_system_get_x(hbj_system* sys, float* x);
 #define HBJ_ITER_KERNELS 3
 /* Bit-identical SpMV streaming variants a system can use: 0 = dword loads,
  * gathers from HBM; 1 = 16-byte loads, gathers from HBM; 2 = tile-local
- * column lists in LDS, 512 entries per wave pass (default); 3 = tile-local,
- * 1024 entries per pass. */
-#define HBJ_SPMV_VARIANTS 4
+ * column lists in LDS, 512 entries per wave pass; 3 = tile-local, 1024
+ * entries per pass; 4 = sliced (default): each lane streams its own row's
+ * values step by step and takes the column positions from the row's runs of
+ * consecutive columns, no per-entry index (systems whose rows have at most 8
+ * such runs, e.g. FEA node blocks; for other systems the same as 2). */
+#define HBJ_SPMV_VARIANTS 5
 
 typedef struct {
   uint32_t n, nnz, n_blocks, n_tiles;
@@ -170,6 +173,10 @@ typedef struct {
   uint64_t spmv_layout_bytes; /* one SpMV in the tile-local layout: 6 bytes per
                                  entry of a local tile, 8 per other entry, the
                                  column lists, row_ptr, x once and q */
+  uint64_t spmv_sliced_bytes; /* one SpMV in the sliced layout (variant 4): the
+                                 padded values, the rows' runs, the column lists,
+                                 the tile and slice headers, row_ptr and q; 0 when
+                                 the system has no sliced layout */
 } hbj_system_info;
 
 /* Algorithm of a system's plain solves (hbj_solve, hbj_system_solve_resident,

@@ -693,8 +693,11 @@ struct hbj_system {
   uint16_t* lidx = nullptr;   // tile-local SpMV tables (build_tile_local)
   TileHdr* thdr = nullptr;
   uint32_t* tcols = nullptr;
+  float* sval = nullptr;      // sliced SpMV layout (build_sliced; kFlagSliced)
+  uint32_t* runs = nullptr;
+  SliceHdr* shdr = nullptr;
   uint32_t local_tiles = 0;
-  uint64_t local_columns = 0, spmv_layout_bytes = 0;
+  uint64_t local_columns = 0, spmv_layout_bytes = 0, spmv_sliced_bytes = 0;
   int spmv_variant = -1;      // chosen variant (-1: the engine's default)
   uint64_t bytes = 0;
   Engine eng;
@@ -717,9 +720,10 @@ struct hbj_system {
   char* ccg_block = nullptr;
 
   // The pieces of the main block for a system of n rows, nnz entries,
-  // n_starts block starts and n_tcols tile-local columns.
+  // n_starts block starts, n_tcols tile-local columns and a sliced layout of
+  // n_sval values and n_runs runs (0 when there is none).
   void layout(BlockLayout& L, uint32_t n_, uint32_t nnz_, uint32_t n_starts, uint64_t n_tcols,
-              bool with_ir) {
+              uint64_t n_sval, uint64_t n_runs, bool with_ir) {
     const uint32_t nt = n_ ? (n_ + kTile - 1) / kTile : 1u;
     const uint64_t nb_ = n_starts ? n_starts - 1 : 0;
     L.add(&rp, n_ + 1ull);
@@ -736,6 +740,9 @@ struct hbj_system {
     L.add(&lidx, nnz_ + 8ull);
     L.add(&thdr, nt);
     L.add(&tcols, n_tcols);
+    L.add(&sval, n_sval);
+    L.add(&runs, n_runs);
+    L.add(&shdr, n_sval ? (uint64_t)nt * kWaves : 0u);
     if (with_ir) layout_ir(L, n_);
   }
   void layout_ir(BlockLayout& L, uint32_t n_) {
@@ -768,6 +775,9 @@ struct hbj_system {
     d.lidx = lidx;
     d.thdr = thdr;
     d.tcols = tcols;
+    d.sval = sval;
+    d.runs = runs;
+    d.shdr = shdr;
     d.st = nullptr;
     d.n = n;
     d.nb = nb;

@@ -51,6 +51,21 @@ struct alignas(32) TileHdr {
   uint32_t pad;
 };
 
+// Sliced SpMV layout (variant 4) of a system whose every tile has a column
+// list and whose rows' entries form at most kMaxRuns runs of consecutive
+// list positions (a FEA matrix's dense node blocks): per 64-row wave of a
+// tile, the values step-major -- value of step s of the wave's row l at
+// sval[voff + 4 (64 (s / 4) + l) + s % 4], nq steps of 4, zero padded -- and
+// per row its runs at runs[roff + 64 k + l], k < nr: (end position of run k
+// in the row) | (list position of the run's first entry minus its position
+// in the row) << 16, unused runs 0xFFFF.  Entry s of a row is list position
+// s + delta(first run whose end exceeds s); the row's length is the end of
+// its last run.
+constexpr uint32_t kMaxRuns = 8;
+struct SliceHdr {
+  uint32_t voff, roff, nq, nr;
+};
+
 enum Phase : int32_t {
   kPhInit = 0,      // r0 = b - A x0, z0 = M^-1 r0, p0 = z0, rz_old = r0.z0
   kPhIter = 1,      // PCG iteration
@@ -70,6 +85,7 @@ constexpr uint32_t kFlagIr = 2u;        // iterative refinement enabled
 constexpr uint32_t kFlagDot64 = 4u;     // inner dots accumulated in FP64
 constexpr uint32_t kFlagBlocks6 = 8u;   // block b is rows [6b, min(6b + 6, n)): block
                                         // ranges and starts are arithmetic
+constexpr uint32_t kFlagSliced = 16u;   // the sliced SpMV layout (variant 4) exists
 
 struct alignas(128) SysState {
   // FP32 PCG scalars, the reference's mailbox slots (compute.rs:271-277)
@@ -115,6 +131,9 @@ struct SysDesc {
   const uint16_t* lidx;      // nnz (+8 zero padding): entry's index in its tile's column list
   const TileHdr* thdr;       // per tile: column list and wave entry ranges
   const uint32_t* tcols;     // concatenated sorted distinct column lists of the tiles
+  const float* sval;         // sliced layout (kFlagSliced): values,
+  const uint32_t* runs;      // per-row runs,
+  const SliceHdr* shdr;      // and per wave of each tile its offsets and sizes
   SysState* st;
   uint32_t n, nb, n_tiles, flags;
 };

@@ -30,9 +30,10 @@ struct LaunchSet {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // kernel timing (optional)
 };
 // SpMV variants: 0 dword loads with HBM gathers, 1 16-byte loads with HBM
-// gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass.
-constexpr uint32_t kSpmvVariantCount = 4;
-constexpr uint32_t kSpmvDefault = 2;  // tile-local columns, 512 entries per pass
+// gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass,
+// 4 the sliced layout (systems that have one; others as 2).
+constexpr uint32_t kSpmvVariantCount = 5;
+constexpr uint32_t kSpmvDefault = 4;  // sliced where the system has the layout, else as 2
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
@@ -68,6 +69,13 @@ void launch_op_update_scalars(float* slots, uint32_t a, uint32_t b, uint32_t c, 
 void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
                       std::vector<uint16_t>& lidx, std::vector<uint2>& tloc,
                       std::vector<uint32_t>& tcols);
+
+// Sliced SpMV layout (variant 4) from the tile-local tables; false when the
+// system does not qualify (outputs cleared).
+bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
+                  const std::vector<uint16_t>& lidx, const std::vector<uint2>& tloc,
+                  std::vector<float>& sval, std::vector<uint32_t>& runs,
+                  std::vector<SliceHdr>& shdr);
 
 // thread-local error text (hbj_last_error)
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));

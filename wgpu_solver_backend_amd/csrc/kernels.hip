@@ -657,6 +657,59 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
   return sum;
 }
 
+// Sliced form (variant 4, layout in hbj_device.hpp): lane l sums its own
+// row, wave row0 + l, step by step in CSR order, sum = sum + v_s x_c(s) from
+// 0.0, each product rounded once -- the reference's row sum.  The values
+// stream step-major (16-byte loads, consecutive lanes read consecutive 16
+// bytes); an entry's position in the tile's column list comes from the
+// row's runs, so no per-entry index is read; x comes from the tile's LDS
+// copy s_v (stage() fills it while the first loads are in flight).  Steps
+// past the row's length (padding) are not added.
+template <typename Stage>
+__device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t tile, const float* s_v,
+                                                  Stage stage) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t w = uni((uint32_t)(threadIdx.x >> 6));
+  const SliceHdr h = uni_copy(d.shdr[tile * (uint32_t)kWaves + w]);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(d.sval + h.voff, h.nq * 1024u);
+  const __amdgpu_buffer_rsrc_t rr = buf_rsrc(d.runs + h.roff, h.nr * 256u);
+  uint32_t run[kMaxRuns];
+#pragma unroll
+  for (uint32_t k = 0; k < kMaxRuns; ++k) {
+    const uint32_t x = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * 64u + lane) * 4u), 0, 0);
+    run[k] = k < h.nr ? x : 0xFFFFu;
+  }
+  f32x4v a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(rv, (int)(lane * 16u), 0, kNt));
+  stage();
+  uint32_t bnd[kMaxRuns];
+  int32_t dl[kMaxRuns];
+  uint32_t len = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kMaxRuns; ++k) {
+    bnd[k] = run[k] & 0xFFFFu;
+    dl[k] = (int32_t)(int16_t)(run[k] >> 16);
+    len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
+  }
+  float sum = 0.0f;
+  for (uint32_t q = 0; q < h.nq; ++q) {
+    const f32x4v v = a;
+    a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                       rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kNt));
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+      const uint32_t st = 4u * q + i;
+      int32_t delta = dl[kMaxRuns - 1];
+#pragma unroll
+      for (int k = (int)kMaxRuns - 2; k >= 0; --k) delta = st < bnd[k] ? dl[k] : delta;
+      const bool in = st < len;
+      const float x = s_v[in ? (uint32_t)((int32_t)st + delta) : 0u];
+      const float pr = v[i] * x;
+      sum = in ? sum + pr : sum;
+    }
+  }
+  return sum;
+}
+
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
 // Run by the whole workgroup.  Loads are unconditional (positions past m
 // re-read the list's first entry).
@@ -694,6 +747,7 @@ template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<2> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<3> { static constexpr int kSlice = 1024; };
+template <> struct SpmvCfg<4> { static constexpr int kSlice = 512; };  // sliced; else as 2
 
 // --------------------------------------------------- block-Jacobi tile body
 
@@ -875,7 +929,7 @@ template <int V>
 struct SpmvLds {
   static constexpr int kSliceF = SpmvCfg<V>::kSlice > kPer * 64 ? SpmvCfg<V>::kSlice : kPer * 64;
   static constexpr bool kLocal = V >= 2;
-  static constexpr int K = V == 2 ? 2 : 4;  // groups per lane and pass (local)
+  static constexpr int K = V == 3 ? 4 : 2;  // groups per lane and pass (local)
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
 };
 
@@ -934,7 +988,9 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
     float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
     float q;
     if constexpr (kLocal) {
-      if (hdr.cnt != kLocalOff) {
+      if (V == 4 && (d.flags & kFlagSliced)) {
+        q = spmv_wave_sliced(d, tr.tile, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
+      } else if (hdr.cnt != kLocalOff) {
         // the wave's entries from the header (selects, not an index: a
         // runtime index into the struct would place it in scratch memory)
         const uint32_t w = uni(wave);
@@ -1633,7 +1689,9 @@ __global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
   float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
   float q;
   if constexpr (kLocal) {
-    if (hdr.cnt != kLocalOff) {
+    if (V == 4 && (d.flags & kFlagSliced)) {
+      q = spmv_wave_sliced(d, tr.tile, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
+    } else if (hdr.cnt != kLocalOff) {
       const uint32_t w = uni(wave);
       const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
       const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
@@ -2125,6 +2183,7 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 1: launch_spmv_v<1>(ls, s); break;
     case 2: launch_spmv_v<2>(ls, s); break;
     case 3: launch_spmv_v<3>(ls, s); break;
+    case 4: launch_spmv_v<4>(ls, s); break;
     default: launch_spmv_v<0>(ls, s); break;
   }
 }
@@ -2184,6 +2243,7 @@ void launch_ccg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 1: launch_ccg_spmv_v<1>(ls, s); break;
     case 2: launch_ccg_spmv_v<2>(ls, s); break;
     case 3: launch_ccg_spmv_v<3>(ls, s); break;
+    case 4: launch_ccg_spmv_v<4>(ls, s); break;
     default: launch_ccg_spmv_v<0>(ls, s); break;
   }
 }

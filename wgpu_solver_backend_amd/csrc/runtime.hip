@@ -362,6 +362,10 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   std::vector<uint2> tloc_h;
   std::vector<uint32_t> tcols_h;
   build_tile_local(n, row_ptr, col_idx, lidx_h, tloc_h, tcols_h);
+  std::vector<float> sval_h;
+  std::vector<uint32_t> runs_h;
+  std::vector<SliceHdr> shdr_h;
+  const bool sliced = build_sliced(n, row_ptr, values, lidx_h, tloc_h, sval_h, runs_h, shdr_h);
   auto* s = new hbj_system();
   s->ctx = ctx;
   s->n = n;
@@ -389,6 +393,12 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
     if (block_starts[b] != 6ull * b) blocks6 = false;
   }
   if (blocks6) s->flags |= kFlagBlocks6;
+  if (sliced) {
+    s->flags |= kFlagSliced;
+    s->spmv_sliced_bytes = 4ull * sval_h.size() + 4ull * runs_h.size() +
+                           (sizeof(SliceHdr) * kWaves + sizeof(TileHdr)) * s->n_tiles +
+                           4ull * s->local_columns + 8ull * n;
+  }
   auto fail = [&](hbj_status st) {
     s->release();
     delete s;
@@ -401,7 +411,8 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   }
   const uint32_t nt = s->n_tiles ? s->n_tiles : 1;
   BlockLayout L;
-  s->layout(L, n, nnz, n_starts, tcols_h.size(), with_ir);
+  s->layout(L, n, nnz, n_starts, tcols_h.size(), sliced ? sval_h.size() : 0u,
+            sliced ? runs_h.size() : 0u, with_ir);
   if (arena) {
     if (!arena->take(L.size, &s->arena_off)) {
       set_error("device arena: no free range of %llu bytes", (unsigned long long)L.size);
@@ -439,6 +450,11 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
     up(s->thdr, th.data(), sizeof(TileHdr) * th.size());
   }
   up(s->tcols, tcols_h.data(), sizeof(uint32_t) * tcols_h.size());
+  if (sliced) {
+    up(s->sval, sval_h.data(), sizeof(float) * sval_h.size());
+    up(s->runs, runs_h.data(), sizeof(uint32_t) * runs_h.size());
+    up(s->shdr, shdr_h.data(), sizeof(SliceHdr) * shdr_h.size());
+  }
   {
     // per tile: the first block touching rows [256t, 256t+256) and how many do
     std::vector<uint2> tb(nt, make_uint2(0u, 0u));
@@ -526,6 +542,7 @@ hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
   out->device_bytes = s->bytes;
   out->spmv_csr_bytes = 8ull * s->nnz + 12ull * s->n + 4;
   out->spmv_layout_bytes = s->spmv_layout_bytes;
+  out->spmv_sliced_bytes = s->spmv_sliced_bytes;
   return HBJ_OK;
 }
 

@@ -304,14 +304,17 @@ void worker_main(hbj_server* S) {
 }
 
 // The block a request's system takes in the arena, with its tile-local
-// column lists at their largest (min(nnz, kLocalMax) per tile): an upper
-// bound of the block system_create carves once the lists are built.
+// column lists and its sliced layout at their largest (min(nnz, kLocalMax)
+// columns per tile; build_sliced's padding bound and kMaxRuns runs per row):
+// an upper bound of the block system_create carves once they are built.
 uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
   const uint64_t nt = n ? (n + kTile - 1) / kTile : 1u;
   const uint64_t tcols = std::min<uint64_t>(nnz ? nnz : 1u, (uint64_t)kLocalMax * nt);
+  const uint64_t sval = (uint64_t)nnz + nnz / 4 + 1024ull * nt;
+  const uint64_t runs = (uint64_t)kMaxRuns * kTile * nt;
   hbj_system probe;
   BlockLayout L;
-  probe.layout(L, n, nnz, n_starts, tcols, true);
+  probe.layout(L, n, nnz, n_starts, tcols, sval, runs, true);
   return (L.size + 255) & ~255ull;
 }
 

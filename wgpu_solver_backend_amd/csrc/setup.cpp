@@ -176,6 +176,93 @@ void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
   });
 }
 
+// Sliced SpMV layout (variant 4, layout in hbj_device.hpp) from the tile-local
+// tables.  False, with the outputs cleared, when the system does not
+// qualify: a tile without a column list, a row with more than kMaxRuns runs
+// or more than 16,384 entries, or padding that would take the values past
+// 1.25 nnz + 1,024 per tile.
+bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
+                  const std::vector<uint16_t>& lidx, const std::vector<uint2>& tloc,
+                  std::vector<float>& sval, std::vector<uint32_t>& runs,
+                  std::vector<SliceHdr>& shdr) {
+  const uint32_t nt = (n + kTile - 1) / kTile, ns = nt * (uint32_t)kWaves;
+  sval.clear();
+  runs.clear();
+  shdr.assign(ns, SliceHdr{0u, 0u, 0u, 0u});
+  for (uint32_t t = 0; t < nt; ++t)
+    if (tloc[t].y == kLocalOff) return false;
+  // a row's runs: maximal stretches of entries whose list positions step by one
+  auto row_runs = [&](uint32_t r, uint32_t* out) -> uint32_t {
+    const uint32_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
+    uint32_t k = 0;
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t pos = e - e0;
+      if (e == e0 || lidx[e] != (uint32_t)lidx[e - 1] + 1u) {
+        if (k == kMaxRuns) return kMaxRuns + 1u;
+        const int32_t dlt = (int32_t)lidx[e] - (int32_t)pos;
+        out[k++] = (uint32_t)(uint16_t)(int16_t)dlt << 16;
+      }
+      out[k - 1] = (out[k - 1] & 0xFFFF0000u) | (pos + 1u);  // end of the current run
+    }
+    return k;
+  };
+  std::vector<uint8_t> bad(nt, 0);
+  parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
+    uint32_t tmp[kMaxRuns];
+    for (uint32_t t = t0; t < t1; ++t) {
+      for (uint32_t w = 0; w < (uint32_t)kWaves; ++w) {
+        const uint32_t r0 = t * kTile + 64u * w;
+        uint32_t maxlen = 0, maxruns = 0;
+        for (uint32_t l = 0; l < 64u && r0 + l < n; ++l) {
+          const uint32_t r = r0 + l, len = row_ptr[r + 1] - row_ptr[r];
+          const uint32_t k = row_runs(r, tmp);
+          if (len > 16384u || k > kMaxRuns) bad[t] = 1;
+          maxlen = len > maxlen ? len : maxlen;
+          maxruns = k > maxruns ? k : maxruns;
+        }
+        SliceHdr& h = shdr[t * kWaves + w];
+        h.nq = (maxlen + 3u) / 4u;
+        h.nr = maxruns;
+      }
+    }
+  });
+  uint64_t vtot = 0, rtot = 0;
+  for (uint32_t i = 0; i < ns; ++i) {
+    if (bad[i / kWaves]) {
+      shdr.clear();
+      return false;
+    }
+    shdr[i].voff = (uint32_t)vtot;
+    shdr[i].roff = (uint32_t)rtot;
+    vtot += 256ull * shdr[i].nq;
+    rtot += 64ull * shdr[i].nr;
+  }
+  const uint64_t nnz = n ? row_ptr[n] : 0u;
+  if (vtot > nnz + nnz / 4 + 1024ull * nt || vtot >= (1ull << 30) || rtot >= (1ull << 30)) {
+    shdr.clear();
+    return false;
+  }
+  sval.assign(vtot ? vtot : 1u, 0.0f);
+  runs.assign(rtot ? rtot : 1u, 0xFFFFu);
+  parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
+    uint32_t tmp[kMaxRuns];
+    for (uint32_t t = t0; t < t1; ++t) {
+      for (uint32_t w = 0; w < (uint32_t)kWaves; ++w) {
+        const SliceHdr& h = shdr[t * kWaves + w];
+        const uint32_t r0 = t * kTile + 64u * w;
+        for (uint32_t l = 0; l < 64u && r0 + l < n; ++l) {
+          const uint32_t r = r0 + l, e0 = row_ptr[r], len = row_ptr[r + 1] - e0;
+          for (uint32_t s = 0; s < len; ++s)
+            sval[h.voff + 4u * (64u * (s / 4u) + l) + s % 4u] = val[e0 + s];
+          const uint32_t k = row_runs(r, tmp);
+          for (uint32_t j = 0; j < k; ++j) runs[h.roff + 64u * j + l] = tmp[j];
+        }
+      }
+    }
+  });
+  return true;
+}
+
 }  // namespace hbj
 
 using namespace hbj;

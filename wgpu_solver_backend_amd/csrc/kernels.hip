@@ -657,14 +657,43 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
   return sum;
 }
 
+// Steps of a sliced row (below) with the run search unrolled over the
+// slice's R = nr runs (a wave-uniform count): entry st's list position is
+// st + delta of the first run whose end exceeds st.  Rows with fewer runs
+// hold end 0xFFFF in their unused slots; steps from the row's length on
+// (padding) are not added.
+template <uint32_t R>
+__device__ __forceinline__ float sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t nq, f32x4v a,
+                                              const uint32_t (&bnd)[kMaxRuns],
+                                              const int32_t (&dl)[kMaxRuns], uint32_t len,
+                                              const float* s_v, uint32_t lane) {
+  float sum = 0.0f;
+  for (uint32_t q = 0; q < nq; ++q) {
+    const f32x4v v = a;
+    a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                       rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kNt));
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+      const uint32_t st = 4u * q + i;
+      int32_t delta = dl[R - 1];
+#pragma unroll
+      for (int k = (int)R - 2; k >= 0; --k) delta = st < bnd[k] ? dl[k] : delta;
+      const bool in = st < len;
+      const float x = s_v[in ? (uint32_t)((int32_t)st + delta) : 0u];
+      const float pr = v[i] * x;
+      sum = in ? sum + pr : sum;
+    }
+  }
+  return sum;
+}
+
 // Sliced form (variant 4, layout in hbj_device.hpp): lane l sums its own
 // row, wave row0 + l, step by step in CSR order, sum = sum + v_s x_c(s) from
 // 0.0, each product rounded once -- the reference's row sum.  The values
 // stream step-major (16-byte loads, consecutive lanes read consecutive 16
 // bytes); an entry's position in the tile's column list comes from the
 // row's runs, so no per-entry index is read; x comes from the tile's LDS
-// copy s_v (stage() fills it while the first loads are in flight).  Steps
-// past the row's length (padding) are not added.
+// copy s_v (stage() fills it while the first loads are in flight).
 template <typename Stage>
 __device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t tile, const float* s_v,
                                                   Stage stage) {
@@ -690,24 +719,17 @@ __device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t til
     dl[k] = (int32_t)(int16_t)(run[k] >> 16);
     len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
   }
-  float sum = 0.0f;
-  for (uint32_t q = 0; q < h.nq; ++q) {
-    const f32x4v v = a;
-    a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
-                                       rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kNt));
-#pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-      const uint32_t st = 4u * q + i;
-      int32_t delta = dl[kMaxRuns - 1];
-#pragma unroll
-      for (int k = (int)kMaxRuns - 2; k >= 0; --k) delta = st < bnd[k] ? dl[k] : delta;
-      const bool in = st < len;
-      const float x = s_v[in ? (uint32_t)((int32_t)st + delta) : 0u];
-      const float pr = v[i] * x;
-      sum = in ? sum + pr : sum;
-    }
+  switch (h.nr) {
+    case 0:
+    case 1: return sliced_steps<1>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 2: return sliced_steps<2>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 3: return sliced_steps<3>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 4: return sliced_steps<4>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 5: return sliced_steps<5>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 6: return sliced_steps<6>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 7: return sliced_steps<7>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    default: return sliced_steps<kMaxRuns>(rv, h.nq, a, bnd, dl, len, s_v, lane);
   }
-  return sum;
 }
 
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.

@@ -657,33 +657,52 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
   return sum;
 }
 
-// Steps of a sliced row (below) with the run search unrolled over the
-// slice's R = nr runs (a wave-uniform count): entry st's list position is
-// st + delta of the first run whose end exceeds st.  Rows with fewer runs
-// hold end 0xFFFF in their unused slots; steps from the row's length on
-// (padding) are not added.
-template <uint32_t R>
-__device__ __forceinline__ float sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t nq, f32x4v a,
-                                              const uint32_t (&bnd)[kMaxRuns],
-                                              const int32_t (&dl)[kMaxRuns], uint32_t len,
-                                              const float* s_v, uint32_t lane) {
-  float sum = 0.0f;
-  for (uint32_t q = 0; q < nq; ++q) {
+// Quads [q0, q1) of a sliced row (below), with the run search unrolled
+// over the slice's R = nr runs (a wave-uniform count): entry st's list
+// position is st + delta of the first run whose end exceeds st; dl4 holds
+// 4 delta, so 4 st + dl4 is the byte offset into s_v.  Rows with fewer runs
+// hold end 0xFFFF in their unused slots.  kGuard: some lane may reach its
+// row's end inside these quads, so steps from len on (padding) are not
+// added; without it every step is below every lane's length.  `a` holds quad
+// q0's values on entry and quad q1's on return.
+template <uint32_t R, bool kGuard>
+__device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t q0, uint32_t q1,
+                                             f32x4v& a, const uint32_t (&bnd)[kMaxRuns],
+                                             const int32_t (&dl4)[kMaxRuns], uint32_t len,
+                                             const float* s_v, uint32_t lane, float& sum) {
+  const char* s_b = reinterpret_cast<const char*>(s_v);
+  for (uint32_t q = q0; q < q1; ++q) {
     const f32x4v v = a;
     a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
                                        rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kNt));
 #pragma unroll
     for (uint32_t i = 0; i < 4u; ++i) {
       const uint32_t st = 4u * q + i;
-      int32_t delta = dl[R - 1];
+      int32_t d4 = dl4[R - 1];
 #pragma unroll
-      for (int k = (int)R - 2; k >= 0; --k) delta = st < bnd[k] ? dl[k] : delta;
-      const bool in = st < len;
-      const float x = s_v[in ? (uint32_t)((int32_t)st + delta) : 0u];
-      const float pr = v[i] * x;
-      sum = in ? sum + pr : sum;
+      for (int k = (int)R - 2; k >= 0; --k) d4 = st < bnd[k] ? dl4[k] : d4;
+      const uint32_t off = 4u * st + (uint32_t)d4;
+      if constexpr (kGuard) {
+        const bool in = st < len;
+        const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
+        const float pr = v[i] * x;
+        sum = in ? sum + pr : sum;
+      } else {
+        const float x = *reinterpret_cast<const float*>(s_b + off);
+        sum = sum + v[i] * x;
+      }
     }
   }
+}
+
+template <uint32_t R>
+__device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t qa, uint32_t nq,
+                                            f32x4v a, const uint32_t (&bnd)[kMaxRuns],
+                                            const int32_t (&dl4)[kMaxRuns], uint32_t len,
+                                            const float* s_v, uint32_t lane) {
+  float sum = 0.0f;
+  sliced_steps<R, false>(rv, 0u, qa, a, bnd, dl4, len, s_v, lane, sum);
+  sliced_steps<R, true>(rv, qa, nq, a, bnd, dl4, len, s_v, lane, sum);
   return sum;
 }
 
@@ -711,24 +730,33 @@ __device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t til
   f32x4v a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(rv, (int)(lane * 16u), 0, kNt));
   stage();
   uint32_t bnd[kMaxRuns];
-  int32_t dl[kMaxRuns];
+  int32_t dl4[kMaxRuns];
   uint32_t len = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kMaxRuns; ++k) {
     bnd[k] = run[k] & 0xFFFFu;
-    dl[k] = (int32_t)(int16_t)(run[k] >> 16);
+    dl4[k] = 4 * (int32_t)(int16_t)(run[k] >> 16);
     len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
   }
+  // the slice's shortest row: quads below it need no padding guard
+  uint32_t ml = len;
+  ml = min(ml, lane_down_u32<32, true>(ml));
+  ml = min(ml, lane_down_u32<16, true>(ml));
+  ml = min(ml, lane_down_u32<8, true>(ml));
+  ml = min(ml, lane_down_u32<4, true>(ml));
+  ml = min(ml, lane_down_u32<2, true>(ml));
+  ml = min(ml, lane_down_u32<1, true>(ml));
+  const uint32_t qa = min(uni(ml) / 4u, h.nq);
   switch (h.nr) {
     case 0:
-    case 1: return sliced_steps<1>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    case 2: return sliced_steps<2>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    case 3: return sliced_steps<3>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    case 4: return sliced_steps<4>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    case 5: return sliced_steps<5>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    case 6: return sliced_steps<6>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    case 7: return sliced_steps<7>(rv, h.nq, a, bnd, dl, len, s_v, lane);
-    default: return sliced_steps<kMaxRuns>(rv, h.nq, a, bnd, dl, len, s_v, lane);
+    case 1: return sliced_row<1>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 2: return sliced_row<2>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 3: return sliced_row<3>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 4: return sliced_row<4>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 5: return sliced_row<5>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 6: return sliced_row<6>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 7: return sliced_row<7>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    default: return sliced_row<kMaxRuns>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
   }
 }
 

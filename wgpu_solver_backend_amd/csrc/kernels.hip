@@ -712,30 +712,50 @@ __device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t 
 // stream step-major (16-byte loads, consecutive lanes read consecutive 16
 // bytes); an entry's position in the tile's column list comes from the
 // row's runs, so no per-entry index is read; x comes from the tile's LDS
-// copy s_v (stage() fills it while the first loads are in flight).
-template <typename Stage>
-__device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t tile, const float* s_v,
-                                                  Stage stage) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t w = uni((uint32_t)(threadIdx.x >> 6));
-  const SliceHdr h = uni_copy(d.shdr[tile * (uint32_t)kWaves + w]);
-  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(d.sval + h.voff, h.nq * 1024u);
-  const __amdgpu_buffer_rsrc_t rr = buf_rsrc(d.runs + h.roff, h.nr * 256u);
+// copy s_v.
+//
+// The matrix's part does not depend on the system's state, so it can be
+// issued in two steps around the state's load: sliced_hdr (the wave's slice
+// header, a scalar load) and sliced_issue (the row's runs and first
+// values); sliced_rows then stages the vector and streams the rest.
+struct SlicedPre {
+  __amdgpu_buffer_rsrc_t rv;
+  uint32_t nq, nr;
   uint32_t run[kMaxRuns];
+  f32x4v a;
+};
+
+__device__ __forceinline__ SliceHdr sliced_hdr(const SysDesc& d, uint32_t tile) {
+  const uint32_t w = uni((uint32_t)(threadIdx.x >> 6));
+  return uni_copy(d.shdr[tile * (uint32_t)kWaves + w]);
+}
+
+__device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h, SlicedPre& p) {
+  const uint32_t lane = threadIdx.x & 63u;
+  p.nq = h.nq;
+  p.nr = h.nr;
+  p.rv = buf_rsrc(d.sval + h.voff, h.nq * 1024u);
+  const __amdgpu_buffer_rsrc_t rr = buf_rsrc(d.runs + h.roff, h.nr * 256u);
 #pragma unroll
-  for (uint32_t k = 0; k < kMaxRuns; ++k) {
-    const uint32_t x = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * 64u + lane) * 4u), 0, 0);
-    run[k] = k < h.nr ? x : 0xFFFFu;
-  }
-  f32x4v a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(rv, (int)(lane * 16u), 0, kNt));
+  for (uint32_t k = 0; k < kMaxRuns; ++k)
+    p.run[k] = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * 64u + lane) * 4u), 0, 0);
+  p.a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(p.rv, (int)(lane * 16u), 0, kNt));
+}
+
+// stage() fills s_v (the tile's LDS copy of the vector) while the loads of
+// sliced_issue are in flight.
+template <typename Stage>
+__device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_v, Stage stage) {
+  const uint32_t lane = threadIdx.x & 63u;
   stage();
   uint32_t bnd[kMaxRuns];
   int32_t dl4[kMaxRuns];
   uint32_t len = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kMaxRuns; ++k) {
-    bnd[k] = run[k] & 0xFFFFu;
-    dl4[k] = 4 * (int32_t)(int16_t)(run[k] >> 16);
+    const uint32_t run = k < p.nr ? p.run[k] : 0xFFFFu;
+    bnd[k] = run & 0xFFFFu;
+    dl4[k] = 4 * (int32_t)(int16_t)(run >> 16);
     len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
   }
   // the slice's shortest row: quads below it need no padding guard
@@ -746,18 +766,27 @@ __device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t til
   ml = min(ml, lane_down_u32<4, true>(ml));
   ml = min(ml, lane_down_u32<2, true>(ml));
   ml = min(ml, lane_down_u32<1, true>(ml));
-  const uint32_t qa = min(uni(ml) / 4u, h.nq);
-  switch (h.nr) {
+  const uint32_t qa = min(uni(ml) / 4u, p.nq);
+  switch (p.nr) {
     case 0:
-    case 1: return sliced_row<1>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    case 2: return sliced_row<2>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    case 3: return sliced_row<3>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    case 4: return sliced_row<4>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    case 5: return sliced_row<5>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    case 6: return sliced_row<6>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    case 7: return sliced_row<7>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
-    default: return sliced_row<kMaxRuns>(rv, qa, h.nq, a, bnd, dl4, len, s_v, lane);
+    case 1: return sliced_row<1>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 2: return sliced_row<2>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 3: return sliced_row<3>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 4: return sliced_row<4>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 5: return sliced_row<5>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 6: return sliced_row<6>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 7: return sliced_row<7>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    default: return sliced_row<kMaxRuns>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
   }
+}
+
+// All three steps in one, after the state was read.
+template <typename Stage>
+__device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t tile, const float* s_v,
+                                                  Stage stage) {
+  SlicedPre p;
+  sliced_issue(d, sliced_hdr(d, tile), p);
+  return sliced_rows(p, s_v, stage);
 }
 
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
@@ -983,18 +1012,110 @@ struct SpmvLds {
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
 };
 
+// Refinement residual of one tile: r64 = b64 - A x64 in FP64 (FP32 matrix
+// entries are exact in FP64) and the tile's FP64 partials of |r64|^2 and
+// |b64|^2.
+template <int V>
+__device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& tr, uint32_t mb,
+                                              uint32_t me, double* s_prod) {
+  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+  double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
+  double* s_rb = s_prod + kTile;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
+  const bool own = t < tr.nrows;
+  const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
+  double* slice = s_prod + wave * (kSliceF / 2);
+  const double ax = spmv_wave<double, kPer / 2>(
+      mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows, slice);
+  __syncthreads();
+  double rr = 0.0, bb = 0.0;
+  if (own) {
+    const double bi = d.b64[row];
+    const double ri = bi - ax;
+    d.r64[row] = ri;
+    rr = ri * ri;
+    bb = bi * bi;
+  }
+  tree256x2<double>(rr, bb, s_ra, s_rb);
+  if (t == 0) {
+    d.part64[3u * d.n_tiles + tr.tile] = rr;
+    d.part64[4u * d.n_tiles + tr.tile] = bb;
+  }
+}
+
+// q of the tile's rows stored, and the tile's dot partial: dot(b,b) at init
+// (compute.rs:235-242), dot(p,Ap) per iteration (:353-360).
+template <typename VecAt>
+__device__ __forceinline__ void spmv_tail(const SysDesc& d, const TileRef& tr, float q, bool init,
+                                          VecAt vec_at, double* s_prod) {
+  double* s_ra = s_prod;
+  double* s_rb = s_prod + kTile;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t;
+  const bool own = t < tr.nrows;
+  __syncthreads();  // all waves are done with their slices
+  float w = 0.0f;
+  double w64 = 0.0;
+  if (own) {
+    bst(d.q, row, q);
+    const float a = init ? bld(d.b, row) : vec_at(row);
+    const float c = init ? a : q;
+    w = a * c;
+    w64 = (double)a * (double)c;
+  }
+  if ((d.flags & kFlagDot64) && !init) {
+    const double pd = tree256<double>(w64, s_rb);
+    if (t == 0) d.part64[tr.tile] = pd;
+  } else {
+    const float pd = tree256<float>(w, reinterpret_cast<float*>(s_ra));
+    if (t == 0) d.part[tr.tile] = pd;
+  }
+}
+
+// One tile of a system with the sliced layout (variant 4).  The tile and
+// slice headers are read with the state, and the rows' runs and first
+// values are issued before the state is tested: none of them depends on it.
+// A path of its own, so that these registers are not live in the other
+// variants' code.
+__device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileRef& tr,
+                                                 double* s_prod, float* s_v) {
+  const SysDesc& d = tr.d;
+  const SysState* st = d.st + ta.par * ta.st_stride;
+  const TileHdr hdr = uni_copy(d.thdr[tr.tile]);
+  const SliceHdr sh = sliced_hdr(d, tr.tile);
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  SlicedPre pre;
+  sliced_issue(d, sh, pre);
+  if (status != kRunning) return;
+  if (phase == kPhIrResid) {
+    uint32_t mb, me;
+    lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
+    spmv_ir_resid<4>(d, tr, mb, me, s_prod);
+    return;
+  }
+  if (phase != kPhInit && phase != kPhIter) return;
+  const bool init = phase == kPhInit;
+  // the multiplied vector: x0 at init (compute.rs:294), p per iteration
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.p, d.n * 4u);
+  auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
+  const float q = sliced_rows(pre, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
+  spmv_tail(d, tr, q, init, vec_at, s_prod);
+}
+
 // One tile of k_pcg_spmv.
 template <int V>
 __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr, double* s_prod,
                                           float* s_v) {
   constexpr int kSliceF = SpmvLds<V>::kSliceF;
   constexpr bool kLocal = SpmvLds<V>::kLocal;
-  double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
-  double* s_rb = s_prod + kTile;
   const SysDesc& d = tr.d;
+  if constexpr (V == 4) {
+    if (d.flags & kFlagSliced) {
+      spmv_tile_sliced(ta, tr, s_prod, s_v);
+      return;
+    }
+  }
   const SysState* st = d.st + ta.par * ta.st_stride;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
-  const bool own = t < tr.nrows;
+  const uint32_t t = threadIdx.x, wave = t >> 6;
   // the tile header (tile-local variants) goes out with the state's loads
   TileHdr hdr{};
   if constexpr (kLocal) hdr = uni_copy(d.thdr[tr.tile]);
@@ -1004,78 +1125,38 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
   if (status != kRunning) return;
 
   if (phase == kPhIrResid) {
-    // r64 = b64 - A x64 in FP64 (FP32 matrix entries are exact in FP64)
-    const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
-    double* slice = s_prod + wave * (kSliceF / 2);
-    const double ax = spmv_wave<double, kPer / 2>(
-        mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows,
-        slice);
-    __syncthreads();
-    double rr = 0.0, bb = 0.0;
-    if (own) {
-      const double bi = d.b64[row];
-      const double ri = bi - ax;
-      d.r64[row] = ri;
-      rr = ri * ri;
-      bb = bi * bi;
-    }
-    tree256x2<double>(rr, bb, s_ra, s_rb);
-    if (t == 0) {
-      d.part64[3u * d.n_tiles + tr.tile] = rr;
-      d.part64[4u * d.n_tiles + tr.tile] = bb;
-    }
+    spmv_ir_resid<V>(d, tr, mb, me, s_prod);
     return;
   }
   if (phase != kPhInit && phase != kPhIter) return;
   // (the refinement residual above returns on its own: joining it with this
   // path before the entry stream starts made the compiler wait for every
   // outstanding load there, the row bounds included)
-  {
-    const bool init = phase == kPhInit;
-    // the multiplied vector: x0 at init (compute.rs:294), p per iteration
-    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.p, d.n * 4u);
-    auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
-    float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
-    float q;
-    if constexpr (kLocal) {
-      if (V == 4 && (d.flags & kFlagSliced)) {
-        q = spmv_wave_sliced(d, tr.tile, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
-      } else if (hdr.cnt != kLocalOff) {
-        // the wave's entries from the header (selects, not an index: a
-        // runtime index into the struct would place it in scratch memory)
-        const uint32_t w = uni(wave);
-        const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
-        const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
-        q = spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [&]() {
-          stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
-        });
-      } else {
-        q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
-      }
-    } else if constexpr (V == 1) {
-      q = spmv_wave_vec<float, 4>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+  const bool init = phase == kPhInit;
+  // the multiplied vector: x0 at init (compute.rs:294), p per iteration
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.p, d.n * 4u);
+  auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
+  float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
+  float q;
+  if constexpr (kLocal) {
+    if (hdr.cnt != kLocalOff) {
+      // the wave's entries from the header (selects, not an index: a
+      // runtime index into the struct would place it in scratch memory)
+      const uint32_t w = uni(wave);
+      const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
+      const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
+      q = spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [&]() {
+        stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
+      });
     } else {
       q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
     }
-    __syncthreads();  // all waves are done with their slices
-    // dot(b,b) at init (compute.rs:235-242), dot(p,Ap) per iteration (:353-360)
-    float w = 0.0f;
-    double w64 = 0.0;
-    if (own) {
-      bst(d.q, row, q);
-      const float a = init ? bld(d.b, row) : vec_at(row);
-      const float c = init ? a : q;
-      w = a * c;
-      w64 = (double)a * (double)c;
-    }
-    if ((d.flags & kFlagDot64) && !init) {
-      const double pd = tree256<double>(w64, s_rb);
-      if (t == 0) d.part64[tr.tile] = pd;
-    } else {
-      const float pd = tree256<float>(w, reinterpret_cast<float*>(s_ra));
-      if (t == 0) d.part[tr.tile] = pd;
-    }
+  } else if constexpr (V == 1) {
+    q = spmv_wave_vec<float, 4>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+  } else {
+    q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
   }
+  spmv_tail(d, tr, q, init, vec_at, s_prod);
 }
 
 // One workgroup per table entry.  (A persistent grid in which every

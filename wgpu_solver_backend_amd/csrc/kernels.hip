@@ -1044,10 +1044,10 @@ __device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& t
 }
 
 // q of the tile's rows stored, and the tile's dot partial: dot(b,b) at init
-// (compute.rs:235-242), dot(p,Ap) per iteration (:353-360).
-template <typename VecAt>
+// (compute.rs:235-242), dot(p,Ap) per iteration (:353-360); a = b[row] at
+// init, p[row] per iteration (read by the caller).
 __device__ __forceinline__ void spmv_tail(const SysDesc& d, const TileRef& tr, float q, bool init,
-                                          VecAt vec_at, double* s_prod) {
+                                          float a, double* s_prod) {
   double* s_ra = s_prod;
   double* s_rb = s_prod + kTile;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
@@ -1057,7 +1057,6 @@ __device__ __forceinline__ void spmv_tail(const SysDesc& d, const TileRef& tr, f
   double w64 = 0.0;
   if (own) {
     bst(d.q, row, q);
-    const float a = init ? bld(d.b, row) : vec_at(row);
     const float c = init ? a : q;
     w = a * c;
     w64 = (double)a * (double)c;
@@ -1097,8 +1096,11 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   // the multiplied vector: x0 at init (compute.rs:294), p per iteration
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.p, d.n * 4u);
   auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
+  // the row's own b or p for the dot, issued before the stream
+  const uint32_t rowc = threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0;
+  const float a = init ? bld(d.b, rowc) : vec_at(rowc);
   const float q = sliced_rows(pre, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
-  spmv_tail(d, tr, q, init, vec_at, s_prod);
+  spmv_tail(d, tr, q, init, a, s_prod);
 }
 
 // One tile of k_pcg_spmv.
@@ -1156,7 +1158,8 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
   } else {
     q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
   }
-  spmv_tail(d, tr, q, init, vec_at, s_prod);
+  const uint32_t row = tr.row0 + threadIdx.x;
+  spmv_tail(d, tr, q, init, threadIdx.x < tr.nrows ? (init ? bld(d.b, row) : vec_at(row)) : 0.0f, s_prod);
 }
 
 // One workgroup per table entry.  (A persistent grid in which every
@@ -1661,6 +1664,23 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   // workgroups that will only need beta (below) do not fetch the r.r partials
   const bool maybe_beta_only = !kDot64 && !writer && !(d.flags & kFlagIr);
   const bool pre_a = pre && !maybe_beta_only;
+  // the rows' z and p for an iteration's p-update, loaded once beta is
+  // known (rows past the system read row 0).  Issuing them before the state
+  // and the partials measured no faster, and costs 16 VGPRs.
+  constexpr uint32_t kT = kFinishTiles;
+  float zv[kT], pv[kT];
+  uint32_t rw[kT];
+  auto load_rows = [&]() {
+#pragma unroll
+    for (uint32_t k = 0; k < kT; ++k) {
+      const uint32_t tile = tr.tile + k * step;
+      const uint32_t row = tile * (uint32_t)kTile + t;
+      rw[k] = (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
+      const uint32_t rc = rw[k] != 0xFFFFFFFFu ? row : 0u;
+      zv[k] = bld(d.z, rc);
+      pv[k] = bld(d.p, rc);
+    }
+  };
   if (pre_a) level_fetch<TP>(part + nt, nt, va);
   if (pre) level_fetch<TP>(part + 2u * nt, nt, vb);
   if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(in)[t];
@@ -1690,18 +1710,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
     const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
     const float rz_old = s_ns.rz_old;
     const float beta = rz_old != 0.0f ? (float)b / rz_old : 0.0f;  // pcg_transition's beta
-    constexpr uint32_t kT = kFinishTiles;
-    float zv[kT], pv[kT];
-    uint32_t rw[kT];
-#pragma unroll
-    for (uint32_t k = 0; k < kT; ++k) {
-      const uint32_t tile = tr.tile + k * step;
-      const uint32_t row = tile * (uint32_t)kTile + t;
-      rw[k] = (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
-      const uint32_t rc = rw[k] != 0xFFFFFFFFu ? row : 0u;
-      zv[k] = bld(d.z, rc);
-      pv[k] = bld(d.p, rc);
-    }
+    load_rows();
 #pragma unroll
     for (uint32_t k = 0; k < kT; ++k) {
       if (rw[k] == 0xFFFFFFFFu) continue;
@@ -1732,19 +1741,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   }
   if (nstatus != kRunning) return;
   if (nphase == kPhIter) {
-    // all of the workgroup's loads first (rows past the system read row 0)
-    constexpr uint32_t kT = kFinishTiles;
-    float zv[kT], pv[kT];
-    uint32_t rw[kT];
-#pragma unroll
-    for (uint32_t k = 0; k < kT; ++k) {
-      const uint32_t tile = tr.tile + k * step;
-      const uint32_t row = tile * (uint32_t)kTile + t;
-      rw[k] = (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
-      const uint32_t rc = rw[k] != 0xFFFFFFFFu ? row : 0u;
-      zv[k] = bld(d.z, rc);
-      pv[k] = bld(d.p, rc);
-    }
+    load_rows();
 #pragma unroll
     for (uint32_t k = 0; k < kT; ++k) {
       if (rw[k] == 0xFFFFFFFFu) continue;
@@ -1756,7 +1753,6 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
       }
     }
   } else if (nphase == kPhIrResid) {
-    constexpr uint32_t kT = kFinishTiles;
     for (uint32_t k = 0; k < kT; ++k) {
       const uint32_t tile = tr.tile + k * step;
       const uint32_t row = tile * (uint32_t)kTile + t;

@@ -780,15 +780,6 @@ __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_
   }
 }
 
-// All three steps in one, after the state was read.
-template <typename Stage>
-__device__ __forceinline__ float spmv_wave_sliced(const SysDesc& d, uint32_t tile, const float* s_v,
-                                                  Stage stage) {
-  SlicedPre p;
-  sliced_issue(d, sliced_hdr(d, tile), p);
-  return sliced_rows(p, s_v, stage);
-}
-
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
 // Run by the whole workgroup.  Loads are unconditional (positions past m
 // re-read the list's first entry).
@@ -1785,54 +1776,15 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
 // Vectors: u in SysDesc.z, w in SysDesc.q; r and s double-buffered by
 // parity (a tile reads its neighbours' rows for straddling blocks).
 
-template <int V, bool kOne>
-__global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
-  __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
-  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
-  constexpr int kSliceF = SpmvLds<V>::kSliceF;
-  constexpr bool kLocal = SpmvLds<V>::kLocal;
-  TileRef tr;
-  if (!get_tile<kOne>(ta, tr)) return;
+// w = A u of the tile's rows stored, and the tile's partials: b.b at init,
+// else (r,u), (w,u) and (r,r).
+__device__ __forceinline__ void ccg_spmv_tail(const SysDesc& d, const TileRef& tr, float q, bool init,
+                                              float ri, float ui, double* s_prod) {
   float* s_ga = reinterpret_cast<float*>(s_prod);  // tile reductions reuse the slices
   float* s_gb = s_ga + kTile;
   float* s_gc = s_gb + kTile;
-  const SysDesc& d = tr.d;
-  const SysState* st = d.st + ta.par * ta.st_stride;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
-  const uint32_t rowc = own ? row : tr.row0;
-  TileHdr hdr{};
-  if constexpr (kLocal) hdr = uni_copy(d.thdr[tr.tile]);
-  uint32_t mb, me;
-  lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
-  const float* __restrict__ r_cur = ta.par ? d.r[1] : d.r[0];
-  const float ri = bld(r_cur, rowc), ui = bld(d.z, rowc);
-  const int32_t status = uni(st->status), phase = uni(st->phase);
-  if (status != kRunning) return;
-  if (phase != kPhCcgInit && phase != kPhCcg) return;
-  const bool init = phase == kPhCcgInit;
-  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.z, d.n * 4u);
-  auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
-  float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
-  float q;
-  if constexpr (kLocal) {
-    if (V == 4 && (d.flags & kFlagSliced)) {
-      q = spmv_wave_sliced(d, tr.tile, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
-    } else if (hdr.cnt != kLocalOff) {
-      const uint32_t w = uni(wave);
-      const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
-      const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
-      q = spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [&]() {
-        stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
-      });
-    } else {
-      q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
-    }
-  } else if constexpr (V == 1) {
-    q = spmv_wave_vec<float, 4>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
-  } else {
-    q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
-  }
   __syncthreads();  // all waves are done with their slices
   if (own) bst(d.q, row, q);
   if (init) {
@@ -1854,6 +1806,83 @@ __global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
     d.part[nt + tr.tile] = dl;
     d.part[2u * nt + tr.tile] = rr;
   }
+}
+
+// One tile of k_ccg_spmv on a system with the sliced layout: as
+// spmv_tile_sliced, the headers go out with the state and the matrix's
+// first loads before the state is tested, on a path of its own.
+__device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRef& tr, double* s_prod,
+                                                float* s_v) {
+  const SysDesc& d = tr.d;
+  const SysState* st = d.st + ta.par * ta.st_stride;
+  const uint32_t t = threadIdx.x;
+  const uint32_t rowc = t < tr.nrows ? tr.row0 + t : tr.row0;
+  const TileHdr hdr = uni_copy(d.thdr[tr.tile]);
+  const SliceHdr sh = sliced_hdr(d, tr.tile);
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  SlicedPre pre;
+  sliced_issue(d, sh, pre);
+  const float* __restrict__ r_cur = ta.par ? d.r[1] : d.r[0];
+  const float ri = bld(r_cur, rowc), ui = bld(d.z, rowc);
+  if (status != kRunning) return;
+  if (phase != kPhCcgInit && phase != kPhCcg) return;
+  const bool init = phase == kPhCcgInit;
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.z, d.n * 4u);
+  auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
+  const float q = sliced_rows(pre, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
+  ccg_spmv_tail(d, tr, q, init, ri, ui, s_prod);
+}
+
+template <int V, bool kOne>
+__global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
+  __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+  constexpr bool kLocal = SpmvLds<V>::kLocal;
+  TileRef tr;
+  if (!get_tile<kOne>(ta, tr)) return;
+  const SysDesc& d = tr.d;
+  if constexpr (V == 4) {
+    if (d.flags & kFlagSliced) {
+      ccg_tile_sliced(ta, tr, s_prod, s_v);
+      return;
+    }
+  }
+  const SysState* st = d.st + ta.par * ta.st_stride;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
+  const bool own = t < tr.nrows;
+  const uint32_t rowc = own ? row : tr.row0;
+  TileHdr hdr{};
+  if constexpr (kLocal) hdr = uni_copy(d.thdr[tr.tile]);
+  uint32_t mb, me;
+  lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
+  const float* __restrict__ r_cur = ta.par ? d.r[1] : d.r[0];
+  const float ri = bld(r_cur, rowc), ui = bld(d.z, rowc);
+  const int32_t status = uni(st->status), phase = uni(st->phase);
+  if (status != kRunning) return;
+  if (phase != kPhCcgInit && phase != kPhCcg) return;
+  const bool init = phase == kPhCcgInit;
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.z, d.n * 4u);
+  auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
+  float* slice = reinterpret_cast<float*>(s_prod) + wave * kSliceF;
+  float q;
+  if constexpr (kLocal) {
+    if (hdr.cnt != kLocalOff) {
+      const uint32_t w = uni(wave);
+      const uint32_t wb = w == 0u ? hdr.eb[0] : w == 1u ? hdr.eb[1] : w == 2u ? hdr.eb[2] : hdr.eb[3];
+      const uint32_t we = w == 0u ? hdr.eb[1] : w == 1u ? hdr.eb[2] : w == 2u ? hdr.eb[3] : hdr.eb[4];
+      q = spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [&]() {
+        stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
+      });
+    } else {
+      q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+    }
+  } else if constexpr (V == 1) {
+    q = spmv_wave_vec<float, 4>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+  } else {
+    q = spmv_wave<float, kPer>(mb, me, d.col, d.val, vec_at, tr.nrows, slice);
+  }
+  ccg_spmv_tail(d, tr, q, init, ri, ui, s_prod);
 }
 
 // kUT consecutive tiles of the system per workgroup (single system: tiles

@@ -201,15 +201,21 @@ def test_server_waits_for_a_free_range(ctx, hbj):
     print("server arena: sides 10 / 12 / 11, E over 3 decades", flush=True)
     p = hbj.IrParams(target_rel_res=1e-8)
     Ls = [hbj.lattice(N, seed=200 + i, e_log10_half_range=1.5) for i, N in enumerate((10, 12, 11))]
-    alone = []
+    alone, dev = [], []
     for L in Ls:
         s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
         alone.append(s.solve_ir(L["rhs"].astype(np.float64), None, p))
+        dev.append(s.info()["device_bytes"])  # its block and its refinement block
         s.close()
     assert alone[0][1].iterations < alone[1][1].iterations  # A finishes first
+    # admission reserves the upper bound request_bytes; the arena places the
+    # actual block (the same pieces as a standalone system's two blocks)
     a, b, c = (hbj.Server.request_bytes(L["n"], L["nnz"], L["block_starts"].size) for L in Ls)
-    assert a < c
-    budget = a + b + c - a // 2  # tail after A and B: c - a/2 < c
+    A, B, C = dev
+    assert A <= a and B <= b and C <= c and A < C
+    budget = A + B + C - A // 2  # free ranges once A is done: A's block and a tail of C - A/2
+    assert a + b <= budget < a + b + c  # A and B admitted at once, C only after A
+    assert b + c <= budget  # ... and then admitted, but placed only once B is done
     srv = hbj.Server(ctx, budget, max_active=4, chunk_iters=8)
     tickets = [srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
                           L["rhs"].astype(np.float64), p) for L in Ls]

@@ -60,10 +60,14 @@ struct alignas(32) TileHdr {
 // in the row) | (list position of the run's first entry minus its position
 // in the row) << 16, unused runs 0xFFFF.  Entry s of a row is list position
 // s + delta(first run whose end exceeds s); the row's length is the end of
-// its last run.
+// its last run.  A full wave whose rows of each group of 6 (rows 6g .. 6g+5,
+// a node's dofs) have identical runs stores them once per group
+// (kRunsPerGroup set in nr): runs[roff + nbs k + (row0 + l) / 6 - row0 / 6]
+// with nbs = (row0 + 63) / 6 - row0 / 6 + 1 groups.
 constexpr uint32_t kMaxRuns = 8;
+constexpr uint32_t kRunsPerGroup = 0x100u;
 struct SliceHdr {
-  uint32_t voff, roff, nq, nr;
+  uint32_t voff, roff, nq, nr;  // nr: run count | kRunsPerGroup
 };
 
 enum Phase : int32_t {

@@ -730,15 +730,21 @@ __device__ __forceinline__ SliceHdr sliced_hdr(const SysDesc& d, uint32_t tile) 
   return uni_copy(d.shdr[tile * (uint32_t)kWaves + w]);
 }
 
-__device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h, SlicedPre& p) {
+// row0: the wave's first row (its runs may be stored once per group of 6).
+__device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h, uint32_t row0,
+                                             SlicedPre& p) {
   const uint32_t lane = threadIdx.x & 63u;
+  const bool grp = (h.nr & kRunsPerGroup) != 0u;
   p.nq = h.nq;
-  p.nr = h.nr;
+  p.nr = h.nr & ~kRunsPerGroup;
   p.rv = buf_rsrc(d.sval + h.voff, h.nq * 1024u);
-  const __amdgpu_buffer_rsrc_t rr = buf_rsrc(d.runs + h.roff, h.nr * 256u);
+  const uint32_t g0 = row0 / 6u;
+  const uint32_t stride = grp ? (row0 + 63u) / 6u - g0 + 1u : 64u;
+  const uint32_t slot = grp ? (row0 + lane) / 6u - g0 : lane;
+  const __amdgpu_buffer_rsrc_t rr = buf_rsrc(d.runs + h.roff, p.nr * stride * 4u);
 #pragma unroll
   for (uint32_t k = 0; k < kMaxRuns; ++k)
-    p.run[k] = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * 64u + lane) * 4u), 0, 0);
+    p.run[k] = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * stride + slot) * 4u), 0, 0);
   p.a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(p.rv, (int)(lane * 16u), 0, kNt));
 }
 
@@ -1074,7 +1080,7 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   const SliceHdr sh = sliced_hdr(d, tr.tile);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   SlicedPre pre;
-  sliced_issue(d, sh, pre);
+  sliced_issue(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
   if (status != kRunning) return;
   if (phase == kPhIrResid) {
     uint32_t mb, me;
@@ -1821,7 +1827,7 @@ __device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRe
   const SliceHdr sh = sliced_hdr(d, tr.tile);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   SlicedPre pre;
-  sliced_issue(d, sh, pre);
+  sliced_issue(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
   const float* __restrict__ r_cur = ta.par ? d.r[1] : d.r[0];
   const float ri = bld(r_cur, rowc), ui = bld(d.z, rowc);
   if (status != kRunning) return;

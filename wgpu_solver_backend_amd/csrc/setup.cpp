@@ -208,24 +208,36 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
   };
   std::vector<uint8_t> bad(nt, 0);
   parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
-    uint32_t tmp[kMaxRuns];
+    uint32_t tmp[kMaxRuns], prev[kMaxRuns];
     for (uint32_t t = t0; t < t1; ++t) {
       for (uint32_t w = 0; w < (uint32_t)kWaves; ++w) {
         const uint32_t r0 = t * kTile + 64u * w;
-        uint32_t maxlen = 0, maxruns = 0;
+        uint32_t maxlen = 0, maxruns = 0, kprev = 0;
+        bool group = r0 + 64u <= n;  // a full wave whose groups of 6 share runs
         for (uint32_t l = 0; l < 64u && r0 + l < n; ++l) {
           const uint32_t r = r0 + l, len = row_ptr[r + 1] - row_ptr[r];
           const uint32_t k = row_runs(r, tmp);
-          if (len > 16384u || k > kMaxRuns) bad[t] = 1;
+          if (len > 16384u || k > kMaxRuns) {
+            bad[t] = 1;
+            group = false;
+            continue;
+          }
+          if (l > 0 && r % 6u != 0u &&
+              (k != kprev || !std::equal(tmp, tmp + k, prev)))
+            group = false;
+          std::copy(tmp, tmp + k, prev);
+          kprev = k;
           maxlen = len > maxlen ? len : maxlen;
           maxruns = k > maxruns ? k : maxruns;
         }
         SliceHdr& h = shdr[t * kWaves + w];
         h.nq = (maxlen + 3u) / 4u;
-        h.nr = maxruns;
+        h.nr = maxruns | (group ? kRunsPerGroup : 0u);
       }
     }
   });
+  // groups of 6 a full wave's rows fall in
+  auto groups = [](uint32_t r0) { return (r0 + 63u) / 6u - r0 / 6u + 1u; };
   uint64_t vtot = 0, rtot = 0;
   for (uint32_t i = 0; i < ns; ++i) {
     if (bad[i / kWaves]) {
@@ -235,7 +247,8 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
     shdr[i].voff = (uint32_t)vtot;
     shdr[i].roff = (uint32_t)rtot;
     vtot += 256ull * shdr[i].nq;
-    rtot += 64ull * shdr[i].nr;
+    const uint32_t nr = shdr[i].nr & ~kRunsPerGroup;
+    rtot += (shdr[i].nr & kRunsPerGroup) ? (uint64_t)groups(i * 64u) * nr : 64ull * nr;
   }
   const uint64_t nnz = n ? row_ptr[n] : 0u;
   if (vtot > nnz + nnz / 4 + 1024ull * nt || vtot >= (1ull << 30) || rtot >= (1ull << 30)) {
@@ -255,7 +268,12 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
           for (uint32_t s = 0; s < len; ++s)
             sval[h.voff + 4u * (64u * (s / 4u) + l) + s % 4u] = val[e0 + s];
           const uint32_t k = row_runs(r, tmp);
-          for (uint32_t j = 0; j < k; ++j) runs[h.roff + 64u * j + l] = tmp[j];
+          if (h.nr & kRunsPerGroup) {
+            const uint32_t nbs = groups(r0), g = r / 6u - r0 / 6u;
+            for (uint32_t j = 0; j < k; ++j) runs[h.roff + nbs * j + g] = tmp[j];
+          } else {
+            for (uint32_t j = 0; j < k; ++j) runs[h.roff + 64u * j + l] = tmp[j];
+          }
         }
       }
     }

@@ -693,6 +693,7 @@ struct hbj_system {
   uint16_t* lidx = nullptr;   // tile-local SpMV tables (build_tile_local)
   TileHdr* thdr = nullptr;
   uint32_t* tcols = nullptr;
+  uint32_t* tgrp = nullptr;
   float* sval = nullptr;      // sliced SpMV layout (build_sliced; kFlagSliced)
   uint32_t* runs = nullptr;
   SliceHdr* shdr = nullptr;
@@ -721,9 +722,9 @@ struct hbj_system {
 
   // The pieces of the main block for a system of n rows, nnz entries,
   // n_starts block starts, n_tcols tile-local columns and a sliced layout of
-  // n_sval values and n_runs runs (0 when there is none).
+  // n_sval values, n_runs runs and n_tgrp column groups (0 when there is none).
   void layout(BlockLayout& L, uint32_t n_, uint32_t nnz_, uint32_t n_starts, uint64_t n_tcols,
-              uint64_t n_sval, uint64_t n_runs, bool with_ir) {
+              uint64_t n_sval, uint64_t n_runs, uint64_t n_tgrp, bool with_ir) {
     const uint32_t nt = n_ ? (n_ + kTile - 1) / kTile : 1u;
     const uint64_t nb_ = n_starts ? n_starts - 1 : 0;
     L.add(&rp, n_ + 1ull);
@@ -743,6 +744,7 @@ struct hbj_system {
     L.add(&sval, n_sval);
     L.add(&runs, n_runs);
     L.add(&shdr, n_sval ? (uint64_t)nt * kWaves : 0u);
+    L.add(&tgrp, n_tgrp);
     if (with_ir) layout_ir(L, n_);
   }
   void layout_ir(BlockLayout& L, uint32_t n_) {
@@ -775,6 +777,7 @@ struct hbj_system {
     d.lidx = lidx;
     d.thdr = thdr;
     d.tcols = tcols;
+    d.tgrp = tgrp;
     d.sval = sval;
     d.runs = runs;
     d.shdr = shdr;

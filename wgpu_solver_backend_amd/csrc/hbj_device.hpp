@@ -48,7 +48,9 @@ struct alignas(32) TileHdr {
   uint32_t off;    // offset of the tile's column list in tcols
   uint32_t cnt;    // its length, or kLocalOff
   uint32_t eb[5];  // row_ptr at the tile's rows 0, 64, 128, 192 and its end (clamped)
-  uint32_t pad;
+  uint32_t goff;   // 1 + offset of the list's groups in tgrp, or 0: a list made of
+                   // whole groups of 6 columns (6g .. 6g+5) is also kept as its
+                   // groups g, which the sliced SpMV stages from
 };
 
 // Sliced SpMV layout (variant 4) of a system whose every tile has a column
@@ -135,6 +137,7 @@ struct SysDesc {
   const uint16_t* lidx;      // nnz (+8 zero padding): entry's index in its tile's column list
   const TileHdr* thdr;       // per tile: column list and wave entry ranges
   const uint32_t* tcols;     // concatenated sorted distinct column lists of the tiles
+  const uint32_t* tgrp;      // the lists made of whole groups of 6, as groups (TileHdr.goff)
   const float* sval;         // sliced layout (kFlagSliced): values,
   const uint32_t* runs;      // per-row runs,
   const SliceHdr* shdr;      // and per wave of each tile its offsets and sizes

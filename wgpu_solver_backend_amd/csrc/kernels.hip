@@ -813,6 +813,30 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
   __syncthreads();
 }
 
+// stage_local for a list made of whole groups of 6 columns, read from its
+// groups: entry i is column 6 grp[goff + i / 6] + i % 6 (m a multiple of 6).
+template <typename G>
+__device__ __forceinline__ void stage_groups(const uint32_t* __restrict__ grp, uint32_t goff,
+                                             uint32_t m, G g, float* s_v) {
+  constexpr int kSteps = (int)(kLocalMax / kTile);
+  const uint32_t t = threadIdx.x;
+  uint32_t c[kSteps];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const uint32_t i = t + (uint32_t)k * kTile, ic = i < m ? i : 0u, j = ic / 6u;
+    c[k] = 6u * grp[goff + j] + (ic - 6u * j);
+  }
+  float x[kSteps];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) x[k] = g(c[k]);
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const uint32_t i = t + (uint32_t)k * kTile;
+    if (i < m) s_v[i] = x[k];
+  }
+  __syncthreads();
+}
+
 // Streaming variants of the FP32 SpMV (all bit-identical; the autotuner
 // picks one per system): 0 = dword loads, 512 entries per pass, gathers from
 // HBM; 1 = 16-byte loads, 1024, gathers from HBM; 2 = tile-local columns,
@@ -1096,7 +1120,10 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   // the row's own b or p for the dot, issued before the stream
   const uint32_t rowc = threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0;
   const float a = init ? bld(d.b, rowc) : vec_at(rowc);
-  const float q = sliced_rows(pre, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
+  const float q = sliced_rows(pre, s_v, [&]() {
+    if (hdr.goff) stage_groups(d.tgrp, hdr.goff - 1u, hdr.cnt, vec_at, s_v);
+    else stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
+  });
   spmv_tail(d, tr, q, init, a, s_prod);
 }
 
@@ -1835,7 +1862,10 @@ __device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRe
   const bool init = phase == kPhCcgInit;
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.z, d.n * 4u);
   auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
-  const float q = sliced_rows(pre, s_v, [&]() { stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v); });
+  const float q = sliced_rows(pre, s_v, [&]() {
+    if (hdr.goff) stage_groups(d.tgrp, hdr.goff - 1u, hdr.cnt, vec_at, s_v);
+    else stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
+  });
   ccg_spmv_tail(d, tr, q, init, ri, ui, s_prod);
 }
 

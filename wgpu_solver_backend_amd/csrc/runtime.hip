@@ -366,6 +366,25 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   std::vector<uint32_t> runs_h;
   std::vector<SliceHdr> shdr_h;
   const bool sliced = build_sliced(n, row_ptr, values, lidx_h, tloc_h, sval_h, runs_h, shdr_h);
+  // column lists made of whole groups of 6 columns, also kept as their groups
+  // (staged from by the sliced SpMV): goff_h[t] = 1 + offset in tgrp_h, or 0
+  std::vector<uint32_t> tgrp_h, goff_h(tloc_h.size(), 0u);
+  uint64_t staged_columns = 0;  // list entries the sliced SpMV reads
+  if (sliced) {
+    for (uint32_t t = 0; t < (uint32_t)tloc_h.size(); ++t) {
+      const uint32_t off = tloc_h[t].x, cnt = tloc_h[t].y;
+      bool whole = cnt > 0 && cnt % 6u == 0;
+      for (uint32_t i = 0; whole && i < cnt; i += 6)
+        whole = tcols_h[off + i] % 6u == 0 && tcols_h[off + i + 5] == tcols_h[off + i] + 5u;
+      if (whole) {
+        goff_h[t] = (uint32_t)tgrp_h.size() + 1u;
+        for (uint32_t i = 0; i < cnt; i += 6) tgrp_h.push_back(tcols_h[off + i] / 6u);
+        staged_columns += cnt / 6u;
+      } else {
+        staged_columns += cnt;
+      }
+    }
+  }
   auto* s = new hbj_system();
   s->ctx = ctx;
   s->n = n;
@@ -397,7 +416,7 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
     s->flags |= kFlagSliced;
     s->spmv_sliced_bytes = 4ull * sval_h.size() + 4ull * runs_h.size() +
                            (sizeof(SliceHdr) * kWaves + sizeof(TileHdr)) * s->n_tiles +
-                           4ull * s->local_columns + 8ull * n;
+                           4ull * staged_columns + 8ull * n;
   }
   auto fail = [&](hbj_status st) {
     s->release();
@@ -412,7 +431,7 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   const uint32_t nt = s->n_tiles ? s->n_tiles : 1;
   BlockLayout L;
   s->layout(L, n, nnz, n_starts, tcols_h.size(), sliced ? sval_h.size() : 0u,
-            sliced ? runs_h.size() : 0u, with_ir);
+            sliced ? runs_h.size() : 0u, tgrp_h.size(), with_ir);
   if (arena) {
     if (!arena->take(L.size, &s->arena_off)) {
       set_error("device arena: no free range of %llu bytes", (unsigned long long)L.size);
@@ -445,7 +464,7 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
       h.cnt = tloc_h[t].y;
       for (uint32_t k = 0; k < 4; ++k) h.eb[k] = row_ptr[std::min<uint32_t>(r0 + 64u * k, r1)];
       h.eb[4] = row_ptr[r1];
-      h.pad = 0;
+      h.goff = goff_h[t];
     }
     up(s->thdr, th.data(), sizeof(TileHdr) * th.size());
   }
@@ -454,6 +473,7 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
     up(s->sval, sval_h.data(), sizeof(float) * sval_h.size());
     up(s->runs, runs_h.data(), sizeof(uint32_t) * runs_h.size());
     up(s->shdr, shdr_h.data(), sizeof(SliceHdr) * shdr_h.size());
+    up(s->tgrp, tgrp_h.data(), sizeof(uint32_t) * tgrp_h.size());
   }
   {
     // per tile: the first block touching rows [256t, 256t+256) and how many do

@@ -312,9 +312,10 @@ uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
   const uint64_t tcols = std::min<uint64_t>(nnz ? nnz : 1u, (uint64_t)kLocalMax * nt);
   const uint64_t sval = (uint64_t)nnz + nnz / 4 + 1024ull * nt;
   const uint64_t runs = (uint64_t)kMaxRuns * kTile * nt;
+  const uint64_t tgrp = tcols / 6u + nt;
   hbj_system probe;
   BlockLayout L;
-  probe.layout(L, n, nnz, n_starts, tcols, sval, runs, true);
+  probe.layout(L, n, nnz, n_starts, tcols, sval, runs, tgrp, true);
   return (L.size + 255) & ~255ull;
 }
 

@@ -191,7 +191,7 @@ def bench_plain(args):
                      "algorithmic_bytes": b1,
                      "note": "achieved = CSR algorithmic bytes (8 nnz + 12 n + 4) / kernel time. "
                              "The kernel's own layout streams fewer bytes than CSR accounting "
-                             "(spmv.layout_bytes: the sliced layout reads about 4.7 bytes per "
+                             "(spmv.layout_bytes: the sliced layout reads about 4.6 bytes per "
                              "entry, the tile-local one 6), so frac is not the bandwidth drawn: "
                              "that is spmv.layout_gbps, and traffic / time from the counters"},
     }

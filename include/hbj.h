@@ -174,9 +174,12 @@ typedef struct {
                                  entry of a local tile, 8 per other entry, the
                                  column lists, row_ptr, x once and q */
   uint64_t spmv_sliced_bytes; /* one SpMV in the sliced layout (variant 4): the
-                                 padded values, the rows' runs, the column lists,
-                                 the tile and slice headers, row_ptr and q; 0 when
-                                 the system has no sliced layout */
+                                 padded values, the rows' runs (once per group of
+                                 6 rows where the group shares them), the column
+                                 lists (as groups of 6 columns where they are made
+                                 of whole groups), the tile and slice headers, the
+                                 rows' own x and q; 0 when the system has no
+                                 sliced layout */
 } hbj_system_info;
 
 /* Algorithm of a system's plain solves (hbj_solve, hbj_system_solve_resident,

@@ -405,3 +405,49 @@ def test_tile_and_block_edges_bitwise(ctx, hbj, oracle, n):
             if sts == 0:
                 assert np.array_equal(bits(x), bits(xref)), (alg, v)
     sys_.close()
+
+
+def _insert_zero(rp, ci, va, r, c):
+    """CSR with an explicit zero entry added at (r, c) (columns kept sorted)."""
+    e0, e1 = int(rp[r]), int(rp[r + 1])
+    pos = e0 + int(np.searchsorted(ci[e0:e1], c))
+    assert pos == e1 or ci[pos] != c
+    rp2 = rp.copy()
+    rp2[r + 1:] += 1
+    return rp2, np.insert(ci, pos, np.uint32(c)), np.insert(va, pos, F(0.0))
+
+
+def test_sliced_layout_mixed_groups_bitwise(ctx, hbj, oracle):
+    """Sliced layout with both storage forms in one system: a lattice's node rows
+    share their runs and its tiles' column lists are whole groups of 6 columns,
+    stored once per group; an explicit zero at (3000, 0) gives one row runs of
+    its own and its tile a list that is not whole groups, one at (6000, 6012)
+    changes a row's runs only.  A is unchanged, so the oracle's x is the
+    lattice's; every variant and both algorithms stay bit-identical to it."""
+    print("lattice N=12 with two explicit zeros: mixed sliced storage", flush=True)
+    L = hbj.lattice(12, seed=3)
+    rp, ci, va = L["row_ptr"], L["col_idx"], L["values"]
+    for r, c in ((3000, 0), (6000, 6012)):
+        rp, ci, va = _insert_zero(rp, ci, va, r, c)
+    bs, b = L["block_starts"], L["rhs"]
+    n = L["n"]
+    x0 = np.zeros(n, F)
+    base = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], bs)
+    base_bytes = base.info()["spmv_sliced_bytes"]
+    base.close()
+    sys_ = hbj.System(ctx, rp, ci, va, bs)
+    mixed_bytes = sys_.info()["spmv_sliced_bytes"]
+    assert 0 < base_bytes < mixed_bytes  # both systems sliced; the mixed one partly per row
+    st, it, xo, _ = oracle_solve(oracle, rp, ci, va, bs, b, x0, 5000, 1e-6, 0.0)
+    st_lu, lu = oracle.build_lu_blocks_6(n, rp, ci, va, bs)
+    st_c, it_c, xc = oracle.ccg(rp, ci, va, lu, bs, b, x0, 5000, 1e-6, 0.0)
+    assert st == 0 and st_c == 0
+    for alg, its, xref in ((hbj.ALG_PCG, it, xo), (hbj.ALG_PCG_ONE_REDUCTION, it_c, xc)):
+        sys_.set_algorithm(alg)
+        for v in range(hbj.SPMV_VARIANTS):
+            print(f"  algorithm {alg}, SpMV variant {v}", flush=True)
+            sys_.set_spmv_variant(v)
+            x, res = sys_.solve(b, x0, 5000, 1e-6, 0.0)
+            assert res.status == 0 and res.iterations == its, (alg, v)
+            assert np.array_equal(bits(x), bits(xref)), (alg, v)
+    sys_.close()

@@ -143,7 +143,7 @@ def bench_plain(args):
     b1, b2, b3 = (kernel_bytes_one_reduction if one_red else kernel_bytes)(n, nnz, nb)
     gbps = lambda b, us: b / (us * 1e-6) / 1e9 if us > 0 else None
     spmv_gbps = b1 / (avg_us[0] * 1e-6) / 1e9
-    layout_bytes = info["spmv_sliced_bytes"] if best == 4 and info["spmv_sliced_bytes"] else \
+    layout_bytes = info["spmv_sliced_bytes"] if best in (4, 5) and info["spmv_sliced_bytes"] else \
         info["spmv_layout_bytes"]
     layout_gbps = layout_bytes / (avg_us[0] * 1e-6) / 1e9
     iter_us = float(avg_us.sum())

@@ -160,8 +160,11 @@ hbj_status hbj_system_get_x(hbj_system* sys, float* x);
  * entries per pass; 4 = sliced (default): each lane streams its own row's
  * values step by step and takes the column positions from the row's runs of
  * consecutive columns, no per-entry index (systems whose rows have at most 8
- * such runs, e.g. FEA node blocks; for other systems the same as 2). */
-#define HBJ_SPMV_VARIANTS 5
+ * such runs, e.g. FEA node blocks; for other systems the same as 2), values
+ * read non-temporally; 5 = 4 with the values read through the caches, so a
+ * layout that fits the GPU's last-level cache stays there between SpMVs
+ * (faster for such systems, slower for larger ones: the autotuner decides). */
+#define HBJ_SPMV_VARIANTS 6
 
 typedef struct {
   uint32_t n, nnz, n_blocks, n_tiles;

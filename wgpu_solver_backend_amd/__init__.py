@@ -40,7 +40,7 @@ CLI_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "hbj_cli"
 ITER_KERNELS = 3
 ITER_KERNEL_NAMES = ("spmv", "update", "finish")
 # bit-identical SpMV variants (HBJ_SPMV_VARIANTS)
-SPMV_VARIANTS = 5
+SPMV_VARIANTS = 6
 # hbj_system_set_algorithm
 ALG_PCG = 0
 ALG_PCG_ONE_REDUCTION = 1

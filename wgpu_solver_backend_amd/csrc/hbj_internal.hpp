@@ -32,7 +32,7 @@ struct LaunchSet {
 // SpMV variants: 0 dword loads with HBM gathers, 1 16-byte loads with HBM
 // gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass,
 // 4 the sliced layout (systems that have one; others as 2).
-constexpr uint32_t kSpmvVariantCount = 5;
+constexpr uint32_t kSpmvVariantCount = 6;
 constexpr uint32_t kSpmvDefault = 4;  // sliced where the system has the layout, else as 2
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);

@@ -665,7 +665,7 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
 // row's end inside these quads, so steps from len on (padding) are not
 // added; without it every step is below every lane's length.  `a` holds quad
 // q0's values on entry and quad q1's on return.
-template <uint32_t R, bool kGuard>
+template <uint32_t R, bool kGuard, int kAux>
 __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t q0, uint32_t q1,
                                              f32x4v& a, const uint32_t (&bnd)[kMaxRuns],
                                              const int32_t (&dl4)[kMaxRuns], uint32_t len,
@@ -674,7 +674,7 @@ __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t
   for (uint32_t q = q0; q < q1; ++q) {
     const f32x4v v = a;
     a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
-                                       rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kNt));
+                                       rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kAux));
 #pragma unroll
     for (uint32_t i = 0; i < 4u; ++i) {
       const uint32_t st = 4u * q + i;
@@ -695,14 +695,14 @@ __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t
   }
 }
 
-template <uint32_t R>
+template <uint32_t R, int kAux>
 __device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t qa, uint32_t nq,
                                             f32x4v a, const uint32_t (&bnd)[kMaxRuns],
                                             const int32_t (&dl4)[kMaxRuns], uint32_t len,
                                             const float* s_v, uint32_t lane) {
   float sum = 0.0f;
-  sliced_steps<R, false>(rv, 0u, qa, a, bnd, dl4, len, s_v, lane, sum);
-  sliced_steps<R, true>(rv, qa, nq, a, bnd, dl4, len, s_v, lane, sum);
+  sliced_steps<R, false, kAux>(rv, 0u, qa, a, bnd, dl4, len, s_v, lane, sum);
+  sliced_steps<R, true, kAux>(rv, qa, nq, a, bnd, dl4, len, s_v, lane, sum);
   return sum;
 }
 
@@ -731,6 +731,10 @@ __device__ __forceinline__ SliceHdr sliced_hdr(const SysDesc& d, uint32_t tile) 
 }
 
 // row0: the wave's first row (its runs may be stored once per group of 6).
+// kAux: cache policy of the value loads -- non-temporal (variant 4), or the
+// default, which keeps a layout that fits the GPU's last-level cache there
+// from one SpMV to the next (variant 5).
+template <int kAux>
 __device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h, uint32_t row0,
                                              SlicedPre& p) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -745,12 +749,12 @@ __device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h
 #pragma unroll
   for (uint32_t k = 0; k < kMaxRuns; ++k)
     p.run[k] = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * stride + slot) * 4u), 0, 0);
-  p.a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(p.rv, (int)(lane * 16u), 0, kNt));
+  p.a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(p.rv, (int)(lane * 16u), 0, kAux));
 }
 
 // stage() fills s_v (the tile's LDS copy of the vector) while the loads of
 // sliced_issue are in flight.
-template <typename Stage>
+template <int kAux, typename Stage>
 __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_v, Stage stage) {
   const uint32_t lane = threadIdx.x & 63u;
   stage();
@@ -775,14 +779,14 @@ __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_
   const uint32_t qa = min(uni(ml) / 4u, p.nq);
   switch (p.nr) {
     case 0:
-    case 1: return sliced_row<1>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 2: return sliced_row<2>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 3: return sliced_row<3>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 4: return sliced_row<4>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 5: return sliced_row<5>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 6: return sliced_row<6>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 7: return sliced_row<7>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    default: return sliced_row<kMaxRuns>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 1: return sliced_row<1, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 2: return sliced_row<2, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 3: return sliced_row<3, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 4: return sliced_row<4, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 5: return sliced_row<5, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 6: return sliced_row<6, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 7: return sliced_row<7, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    default: return sliced_row<kMaxRuns, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
   }
 }
 
@@ -841,13 +845,16 @@ __device__ __forceinline__ void stage_groups(const uint32_t* __restrict__ grp, u
 // picks one per system): 0 = dword loads, 512 entries per pass, gathers from
 // HBM; 1 = 16-byte loads, 1024, gathers from HBM; 2 = tile-local columns,
 // 512; 3 = tile-local, 1024 (2 and 3 use variant 0 on tiles without a local
-// column list).  kSlice = LDS floats per wave.
+// column list); 4 = sliced, values read non-temporally; 5 = sliced, values
+// read through the caches (4 and 5 run 2 on systems without the sliced
+// layout).  kSlice = LDS floats per wave.
 template <int V> struct SpmvCfg;
 template <> struct SpmvCfg<0> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<1> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<2> { static constexpr int kSlice = 512; };
 template <> struct SpmvCfg<3> { static constexpr int kSlice = 1024; };
 template <> struct SpmvCfg<4> { static constexpr int kSlice = 512; };  // sliced; else as 2
+template <> struct SpmvCfg<5> { static constexpr int kSlice = 512; };  // sliced, cached; else as 2
 
 // --------------------------------------------------- block-Jacobi tile body
 
@@ -1096,6 +1103,7 @@ __device__ __forceinline__ void spmv_tail(const SysDesc& d, const TileRef& tr, f
 // values are issued before the state is tested: none of them depends on it.
 // A path of its own, so that these registers are not live in the other
 // variants' code.
+template <int kAux>
 __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileRef& tr,
                                                  double* s_prod, float* s_v) {
   const SysDesc& d = tr.d;
@@ -1104,7 +1112,7 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   const SliceHdr sh = sliced_hdr(d, tr.tile);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   SlicedPre pre;
-  sliced_issue(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
+  sliced_issue<kAux>(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
   if (status != kRunning) return;
   if (phase == kPhIrResid) {
     uint32_t mb, me;
@@ -1120,7 +1128,7 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   // the row's own b or p for the dot, issued before the stream
   const uint32_t rowc = threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0;
   const float a = init ? bld(d.b, rowc) : vec_at(rowc);
-  const float q = sliced_rows(pre, s_v, [&]() {
+  const float q = sliced_rows<kAux>(pre, s_v, [&]() {
     if (hdr.goff) stage_groups(d.tgrp, hdr.goff - 1u, hdr.cnt, vec_at, s_v);
     else stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
   });
@@ -1134,9 +1142,9 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
   constexpr int kSliceF = SpmvLds<V>::kSliceF;
   constexpr bool kLocal = SpmvLds<V>::kLocal;
   const SysDesc& d = tr.d;
-  if constexpr (V == 4) {
+  if constexpr (V >= 4) {
     if (d.flags & kFlagSliced) {
-      spmv_tile_sliced(ta, tr, s_prod, s_v);
+      spmv_tile_sliced<V == 4 ? kNt : 0>(ta, tr, s_prod, s_v);
       return;
     }
   }
@@ -1844,6 +1852,7 @@ __device__ __forceinline__ void ccg_spmv_tail(const SysDesc& d, const TileRef& t
 // One tile of k_ccg_spmv on a system with the sliced layout: as
 // spmv_tile_sliced, the headers go out with the state and the matrix's
 // first loads before the state is tested, on a path of its own.
+template <int kAux>
 __device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRef& tr, double* s_prod,
                                                 float* s_v) {
   const SysDesc& d = tr.d;
@@ -1854,7 +1863,7 @@ __device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRe
   const SliceHdr sh = sliced_hdr(d, tr.tile);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   SlicedPre pre;
-  sliced_issue(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
+  sliced_issue<kAux>(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
   const float* __restrict__ r_cur = ta.par ? d.r[1] : d.r[0];
   const float ri = bld(r_cur, rowc), ui = bld(d.z, rowc);
   if (status != kRunning) return;
@@ -1862,7 +1871,7 @@ __device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRe
   const bool init = phase == kPhCcgInit;
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(init ? d.x : d.z, d.n * 4u);
   auto vec_at = [&](uint32_t c) -> float { return buf_gather<float>(rv, c); };
-  const float q = sliced_rows(pre, s_v, [&]() {
+  const float q = sliced_rows<kAux>(pre, s_v, [&]() {
     if (hdr.goff) stage_groups(d.tgrp, hdr.goff - 1u, hdr.cnt, vec_at, s_v);
     else stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
   });
@@ -1878,9 +1887,9 @@ __global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
   TileRef tr;
   if (!get_tile<kOne>(ta, tr)) return;
   const SysDesc& d = tr.d;
-  if constexpr (V == 4) {
+  if constexpr (V >= 4) {
     if (d.flags & kFlagSliced) {
-      ccg_tile_sliced(ta, tr, s_prod, s_v);
+      ccg_tile_sliced<V == 4 ? kNt : 0>(ta, tr, s_prod, s_v);
       return;
     }
   }
@@ -2376,6 +2385,7 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 2: launch_spmv_v<2>(ls, s); break;
     case 3: launch_spmv_v<3>(ls, s); break;
     case 4: launch_spmv_v<4>(ls, s); break;
+    case 5: launch_spmv_v<5>(ls, s); break;
     default: launch_spmv_v<0>(ls, s); break;
   }
 }
@@ -2436,6 +2446,7 @@ void launch_ccg_spmv(const LaunchSet& ls, hipStream_t s) {
     case 2: launch_ccg_spmv_v<2>(ls, s); break;
     case 3: launch_ccg_spmv_v<3>(ls, s); break;
     case 4: launch_ccg_spmv_v<4>(ls, s); break;
+    case 5: launch_ccg_spmv_v<5>(ls, s); break;
     default: launch_ccg_spmv_v<0>(ls, s); break;
   }
 }

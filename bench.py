@@ -193,7 +193,9 @@ def bench_plain(args):
                              "The kernel's own layout streams fewer bytes than CSR accounting "
                              "(spmv.layout_bytes: the sliced layout reads about 4.6 bytes per "
                              "entry, the tile-local one 6), so frac is not the bandwidth drawn: "
-                             "that is spmv.layout_gbps, and traffic / time from the counters"},
+                             "that is spmv.layout_gbps, and traffic / time from the counters; "
+                             "variant 5 reads the layout through the caches, and a layout that fits "
+                             "the 256 MB last-level cache is partly served from there"},
     }
     if not args.no_cpu:
         log("cpu baseline (oracle on host cores)")

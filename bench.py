@@ -228,6 +228,10 @@ def bench_ir(args):
     n, nnz = L["n"], L["nnz"]
     ctx = H.GpuContext.create(0)
     system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    # the SpMV variant, timed on this system's FP32 iterations
+    system.set_rhs(L["rhs"], np.zeros(n, np.float32))
+    best, tune_us = system.autotune(40)
+    log(f"autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 2).tolist()}")
     b64 = L["rhs"].astype(np.float64)
     system.set_rhs64(b64)
     params = H.IrParams(max_outer=40, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-10)
@@ -253,6 +257,7 @@ def bench_ir(args):
                        "n": n, "nnz": nnz, "outer": int(res.outer_iterations),
                        "inner_iterations": int(res.iterations)},
             "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99))},
+            "spmv": {"variant": best, "autotune_us_per_iter": [float(x) for x in tune_us]},
             "true_rel_residual_fp64_device": res.rel_res_true_fp64,
             "true_rel_residual_fp64_host": host_res,
             "plain_fp32_true_rel_residual_fp64": r32.rel_res_true_fp64,

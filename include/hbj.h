@@ -116,6 +116,19 @@ hbj_status hbj_op_dot(hbj_ctx* ctx, uint32_t n, const float* a, const float* b, 
 /* SpmvExecutor::create + encode_copy_x_from + encode_spmv (spmv_exec.rs:38-147): y = A x */
 hbj_status hbj_op_spmv(hbj_ctx* ctx, uint32_t n_rows, uint32_t nnz, const uint32_t* row_ptr,
                        const uint32_t* col_idx, const float* values, const float* x, float* y);
+/* No counterpart in the reference: hbj_op_dot for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  a, b: n floats and out: one float, all
+ * in device memory.  Enqueued on `stream` (a hipStream_t; NULL = the
+ * context's stream) without host synchronisation and without any allocation:
+ * the per-256-element partials live in a fixed device array of the library
+ * (1 MiB per GPU, held until the process ends), so n is limited to
+ * HBJ_DOT_NOALLOC_MAX_N (larger n: HBJ_E_INVALID).  All calls on one GPU
+ * share that array: calls that overlap on one GPU race, the caller
+ * serialises them.  *out receives the same bits as hbj_op_dot. */
+#define HBJ_DOT_NOALLOC_MAX_N (1u << 26)
+hbj_status hbj_op_dot_noalloc(hbj_ctx* ctx, uint32_t n, const float* a, const float* b,
+                              float* out, void* stream);
 /* BlockJacobiExecutor::create + encode_apply (block_jacobi_exec.rs:48-124): z = M^-1 r.
  * z is in/out: rows no block writes (past the 6th row of an oversize block) keep their value. */
 hbj_status hbj_op_block_jacobi(hbj_ctx* ctx, uint32_t n, const float* lu,

@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhbj.so")
 CLI = os.path.join(PKG, "hbj_cli")
 
-SOURCES = ["kernels.hip", "runtime.hip", "server.hip", "setup.cpp", "lattice.cpp", "caseio.cpp"]
+SOURCES = ["kernels.hip", "runtime.hip", "server.hip", "noalloc.hip", "setup.cpp", "lattice.cpp", "caseio.cpp"]
 
 # -ffp-contract=off: the FP32 path restates the reference's operation order
 # (no fused multiply-adds), IEEE division is the hipcc default.

@@ -71,6 +71,7 @@ _SIGS = {
     "hbj_op_axpy": (c_i32, [c_void_p, c_u32, c_f32, f32p, f32p]),
     "hbj_op_scale": (c_i32, [c_void_p, c_u32, c_f32, f32p]),
     "hbj_op_dot": (c_i32, [c_void_p, c_u32, f32p, f32p, P(c_f32)]),
+    "hbj_op_dot_noalloc": (c_i32, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hbj_op_spmv": (c_i32, [c_void_p, c_u32, c_u32, u32p, u32p, f32p, f32p, f32p]),
     "hbj_op_block_jacobi": (c_i32, [c_void_p, c_u32, f32p, u32p, c_u32, f32p, f32p]),
     "hbj_op_update_scalars": (c_i32, [c_void_p, f32p, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32]),

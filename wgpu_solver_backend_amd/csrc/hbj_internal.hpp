@@ -77,6 +77,11 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
                   std::vector<float>& sval, std::vector<uint32_t>& runs,
                   std::vector<SliceHdr>& shdr);
 
+// The no-alloc dot's device scratch (noalloc.hip): noalloc_init resolves
+// its address on `device` (current) once; noalloc_scratch returns it.
+hbj_status noalloc_init(int device);
+float* noalloc_scratch(int device);
+
 // thread-local error text (hbj_last_error)
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 void clear_error();

@@ -95,6 +95,13 @@ hbj_status hbj_ctx_create(int device, hbj_ctx** out) {
     delete c;
     return HBJ_E_DEVICE;
   }
+  // the device address of the no-alloc dot's scratch array (once per device)
+  const hbj_status ns = noalloc_init(device);
+  if (ns != HBJ_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ns;
+  }
   *out = c;
   return HBJ_OK;
 }

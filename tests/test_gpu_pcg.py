@@ -174,6 +174,12 @@ def test_full_size_lattice_50_bitwise(ctx, hbj, oracle):
                                  L["rhs"], x0, 5000, 1e-6, 0.0)
     assert res.status == st == 0 and res.iterations == it
     assert np.array_equal(bits(x), bits(xo))
+    # variant 5 (the sliced layout read through the caches: what the autotuner
+    # picks for this system) on the same oracle x
+    sys_.set_spmv_variant(5)
+    x5, res5 = sys_.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+    print(f"variant 5: {res5.iterations} its, {res5.solve_ms:.1f} ms", flush=True)
+    assert res5.iterations == it and np.array_equal(bits(x5), bits(xo))
     sys_.close()
 
 

@@ -1,20 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark: block-Jacobi PCG solves on one MI355X.
+"""Benchmark: block-Jacobi PCG solves on one MI355X.  Prints one JSON line.
 
-Default workload (one "step" = one full solve): plain FP32 block-Jacobi PCG
-on the 50^3 beam lattice (750,000 unknowns, 30,352,200 nnz, 6x6 blocks,
-tol 1e-6, max 5000 iterations), matrix, right-hand side and x0 resident in
-HBM before the timed region.  Prints one JSON line.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 50] [--no-cpu]
+Default workload (the headline): 256 independent beam-lattice systems, sides
+log-uniform in [4, 24] nodes (384 .. 82,944 unknowns each), FP32 block-Jacobi
+PCG inside FP64 iterative refinement to a 1e-8 relative residual, all
+systems advanced together.  One step = one solve of all 256 systems from
+x0 = 0, matrices and right-hand sides resident in HBM.  `value` is systems
+solved per second; latency_ms holds each system's time to its FP64-verified
+solution within its step.  The same line carries:
+  * serving: a seeded Poisson stream of lattices with sides log-uniform in
+    [6, 40], FP64 refinement to 1e-8, at the configured 200 requests/s and at
+    0.7x the completed rate measured there (p50 / p99 latency, offered and
+    completed rates);
+  * the plain FP32 PCG on the 50^3 lattice (750,000 unknowns, 30,352,200
+    nnz, tol 1e-6): solves/s, and the SpMV roofline -- the kernel's mean
+    duration (HIP events, sampled in an untimed pass after the timed solves)
+    against the bytes its layout streams, and a 64^3 point whose layout does
+    not fit the 256 MB last-level cache (HBM alone);
+  * cpu_baseline: the same batch workload on the host cores (FP32 PCG
+    restatement of the reference inside FP64 refinement), on a bounded sample.
 
-Other workloads (extra lines, not the headline): --workload ir | batch | serve.
+Other workloads (one line each): --workload plain | ir | batch | serve.
 """
 import argparse
 import json
 import os
 import statistics
 import sys
+import threading
 import time
 
 import numpy as np
@@ -30,9 +45,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Host threads for the CPU baselines: the cores this process may run on,
+    capped by OMP_NUM_THREADS (the GPU box grants a job a 16-core share and
+    says so there), and the machine's core count for the record."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(affinity, cap) if cap > 0 else affinity
+    return max(threads, 1), {"nproc": os.cpu_count(), "affinity": affinity,
+                             "omp_num_threads": cap or None}
+
+
 def kernel_bytes(n, nnz, nb):
-    """Algorithmic HBM bytes per launch of the PCG kernels: the SpMV in CSR
-    accounting (8 bytes per entry, row_ptr, p once, q), the update (x rw, p,
+    """Algorithmic HBM bytes per launch of the PCG kernels in CSR accounting:
+    the SpMV (8 bytes per entry, row_ptr, p once, q), the update (x rw, p,
     r_in, q, r_out, z, the 6x6 factors and block starts) and the finish
     (p rw, z; its partials and state are a few KB)."""
     spmv = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * n
@@ -50,6 +79,17 @@ def kernel_bytes_one_reduction(n, nnz, nb):
     return spmv, update, 0
 
 
+def spmv_stream_bytes(info, variant):
+    """Bytes one launch of the chosen SpMV variant streams: the sliced layout
+    (variants 4 and 5 on a system that has it), the tile-local layout
+    (variants 2, 3, and 4/5 without a sliced layout) or CSR (0, 1)."""
+    if variant in (4, 5) and info["spmv_sliced_bytes"]:
+        return info["spmv_sliced_bytes"], "sliced"
+    if variant >= 2:
+        return info["spmv_layout_bytes"], "tile-local"
+    return info["spmv_csr_bytes"], "csr"
+
+
 def traffic_from_profiles(workload_key):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}.json")
     if os.path.exists(path):
@@ -58,165 +98,469 @@ def traffic_from_profiles(workload_key):
     return None
 
 
-def cpu_baseline(L, iters_gpu, sample_iters, sample_iters_1core):
-    """The oracle (FP32 restatement of the reference loop) on the host cores,
-    on a bounded sample: `sample_iters` PCG iterations of the same system on
-    up to 16 threads (and `sample_iters_1core` on one), scaled to solves/s
-    with the GPU's iteration count."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_lib
-    oracle_lib.build()
-    st, lu = oracle_lib.build_lu_blocks_6(L["n"], L["row_ptr"], L["col_idx"], L["values"],
-                                          L["block_starts"])
-    x0 = np.zeros(L["n"], np.float32)
-
-    def sample(threads, iters):
-        oracle_lib.set_threads(threads)
-        t0 = time.perf_counter()
-        _, it, _, _ = oracle_lib.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"],
-                                     L["rhs"], x0, iters, 1e-6, 0.0)
-        dt = time.perf_counter() - t0
-        per_iter = dt / max(it, 1)
-        return 1.0 / (per_iter * (iters_gpu + 1)), it, dt, per_iter
-
-    side = round((L["n"] / 6) ** (1.0 / 3.0))
-    threads = min(16, os.cpu_count() or 1)
-    v, it, dt, per_iter = sample(threads, sample_iters)
-    v1, it1, dt1, per_iter1 = sample(1, sample_iters_1core)
-    if it >= iters_gpu:  # the whole solve ran: no scaling
-        v = 1.0 / dt
-        what = f"the whole solve of the same {side}^3 system ({it} iterations) in {dt:.1f} s"
-    else:
-        what = (f"{it} PCG iterations of the same {side}^3 system in {dt:.1f} s "
-                f"({per_iter * 1e3:.2f} ms/iteration), scaled to {iters_gpu} iterations")
-    return {"value": v, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": what,
-            "single_core": {"value": v1, "unit": "solves/s", "cores": 1,
-                            "sample": f"{it1} iterations in {dt1:.1f} s ({per_iter1 * 1e3:.2f} ms/iteration)"}}
+def lattice_sides(seed, count, lo, hi):
+    rng = np.random.default_rng(seed)
+    return [int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(count)]
 
 
-def bench_plain(args):
-    import wgpu_solver_backend_amd as H
-    N = args.N
-    log(f"generating {N}^3 lattice")
+def host_rel_res(L, x):
+    import scipy.sparse as sp
+    n = L["n"]
+    A = sp.csr_matrix((L["values"].astype(np.float64), L["col_idx"], L["row_ptr"]), shape=(n, n))
+    b = L["rhs"].astype(np.float64)
+    return float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v, np.float64), q))
+
+
+# ----------------------------------------------------------------- batch
+
+def run_batch(H, ctx, args, steps, warmup):
+    """256 lattices (sides log-uniform [4,24], seeds 1000+i) solved together to
+    an FP64 1e-8 relative residual; per-system time to solution; host FP64
+    verification of every result of the last step; the per-system path on the
+    same systems."""
+    sides = lattice_sides(0, args.batch, 4, 24)
+    log(f"batch: generating {len(sides)} lattices")
+    Ls = [H.lattice(N, seed=1000 + i) for i, N in enumerate(sides)]
+    systems = [H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"]) for L in Ls]
+    for L, s in zip(Ls, systems):
+        s.set_rhs64(L["rhs"].astype(np.float64))
+    params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
+    batch = H.Batch(ctx, systems)
+    best, tune_us = batch.autotune(20, ir=True)
+    log(f"batch autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 1).tolist()}")
+    for i in range(warmup):
+        st, _ = batch.solve_ir(params)
+        if st != 0:
+            raise RuntimeError(f"batch warmup status {st}: {H._lib.last_error()}")
+    lat, step_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        st, results = batch.solve_ir(params)
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        if st != 0:
+            raise RuntimeError(f"batch status {st}: {H._lib.last_error()}")
+        lat.extend(r.solve_ms for r in results)
+    elapsed = time.perf_counter() - t0
+    value = steps * len(systems) / elapsed
+    log(f"batch: {steps} steps of {len(systems)} systems in {elapsed:.3f} s = {value:.1f} solves/s")
+    # FP64 verification on the host of every system's last result
+    dev_res = [r.rel_res_true_fp64 for r in results]
+    host_res = [host_rel_res(L, s.get_x64()) for L, s in zip(Ls, systems)]
+    iters = [r.iterations for r in results]
+    outer = [r.outer_iterations for r in results]
+    # the per-system path: each system solved alone, one after another
+    t1 = time.perf_counter()
+    same = True
+    for L, s, r in zip(Ls, systems, results):
+        _, ra = s.solve_ir(L["rhs"].astype(np.float64), None, params)
+        same &= ra.iterations == r.iterations
+    t_alone = time.perf_counter() - t1
+    alone_sps = len(systems) / t_alone
+    ns = [L["n"] for L in Ls]
+    out = {
+        "value": value, "elapsed": elapsed, "step_ms": step_ms, "latency_ms": lat,
+        "best": best, "tune_us": tune_us, "Ls": Ls, "params": params, "sides": sides,
+        "summary": {
+            "systems": len(systems), "unknowns_min": min(ns), "unknowns_max": max(ns),
+            "unknowns_sum": int(sum(ns)), "nnz_sum": int(sum(L["nnz"] for L in Ls)),
+            "inner_iterations_mean": float(np.mean(iters)), "outer_iterations_max": int(max(outer)),
+            "worst_true_rel_residual_fp64_device": float(max(dev_res)),
+            "worst_true_rel_residual_fp64_host": float(max(host_res)),
+            "all_verified_1e-8": bool(max(host_res) <= 1e-8 * 1.001),
+            "per_system_path_solves_per_s": alone_sps,
+            "speedup_vs_per_system": value / alone_sps,
+            "iterations_identical_alone_vs_batched": bool(same),
+            "spmv_variant": best, "autotune_us_per_iter": [float(u) for u in tune_us],
+        },
+    }
+    batch.close()
+    for s in systems:
+        s.close()
+    return out
+
+
+# ----------------------------------------------------------------- serving
+
+def serve_stream(H, ctx, reqs, rate, seed, params, budget_gb, max_active):
+    """Submits `reqs` with exponential gaps (mean 1/rate) from this thread while
+    another thread waits for them; returns latencies, statuses, timing."""
+    count = len(reqs)
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
+    srv = H.Server(ctx, int(budget_gb * (1 << 30)), max_active=max_active, chunk_iters=8)
+    tickets = [None] * count
+    lat = [None] * count
+    status = [None] * count
+    dev_res = [None] * count
+    iters = [(0, 0)] * count
+    t_done = [0.0]
+
+    def waiter():
+        for i in range(count):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            st, _, res, ms = srv.wait(tickets[i])
+            lat[i], status[i], dev_res[i] = ms, st, res.rel_res_true_fp64
+            iters[i] = (res.iterations, res.outer_iterations)
+        t_done[0] = time.perf_counter()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, L in enumerate(reqs):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets[i] = srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                L["rhs"].astype(np.float64), params)
+    t_submitted = time.perf_counter()
+    th.join()
+    elapsed = t_done[0] - t0
+    stats = srv.stats()
+    srv.close()
+    ok = sum(1 for s in status if s == 0)
+    # bytes the requests' iterations stream in CSR accounting: per inner PCG
+    # iteration 8 nnz + 76.7 n (SpMV, dots, updates, block solves fused), per
+    # FP64 residual step 8 nnz + 28 n + 4
+    moved = sum(it * (8 * L["nnz"] + 76.7 * L["n"]) + (outer + 1) * (8 * L["nnz"] + 28 * L["n"] + 4)
+                for (it, outer), L in zip(iters, reqs))
+    return {"offered_rate": count / (t_submitted - t0), "target_rate": rate,
+            "completed_rate": ok / elapsed, "requests": count, "completed_ok": ok,
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": float(max(lat))},
+            "worst_true_rel_residual_fp64_device": float(max(r for r in dev_res if r is not None)),
+            "inner_iterations_total": int(sum(i for i, _ in iters)),
+            "streamed_gbps_csr_accounting": moved / elapsed / 1e9, "elapsed_s": elapsed,
+            "server": {k: stats[k] for k in ("peak_active", "peak_device_bytes", "chunks",
+                                             "table_changes", "arena_waits", "admit_ms", "table_ms")}}
+
+
+def run_serving(H, ctx, args):
+    sides = lattice_sides(0, args.requests, 6, 40)
+    log(f"serve: generating {len(sides)} requests")
+    reqs = [H.lattice(N, seed=2000 + i) for i, N in enumerate(sides)]
+    params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
+    at_target = serve_stream(H, ctx, reqs, args.rate, 1, params, args.budget_gb, args.max_active)
+    log(f"serve at {args.rate:g} req/s: completed {at_target['completed_rate']:.1f}/s, "
+        f"p50 {at_target['latency_ms']['p50']:.0f} ms, p99 {at_target['latency_ms']['p99']:.0f} ms")
+    capacity = min(at_target["completed_rate"], args.rate)
+    sustained_rate = 0.7 * capacity
+    sustained = serve_stream(H, ctx, reqs, sustained_rate, 2, params, args.budget_gb, args.max_active)
+    log(f"serve at {sustained_rate:.1f} req/s: completed {sustained['completed_rate']:.1f}/s, "
+        f"p50 {sustained['latency_ms']['p50']:.0f} ms, p99 {sustained['latency_ms']['p99']:.0f} ms")
+    ns = [L["n"] for L in reqs]
+    return {"workload": (f"serving: Poisson arrivals (seeded), {len(reqs)} requests, lattice sides "
+                         f"log-uniform [6,40] ({min(ns):,}..{max(ns):,} unknowns), FP64 refinement to "
+                         f"1e-8 (FP32 inner rel tol {args.inner_tol:g}), HBM budget {args.budget_gb:g} "
+                         f"GiB, <= {args.max_active} resident"),
+            "at_target_rate": at_target, "at_0.7x_capacity": sustained}
+
+
+# ----------------------------------------------------------------- plain 50^3
+
+def run_plain(H, ctx, N, steps, warmup, alg="pcg", sample_every=4):
+    """Plain FP32 PCG on the N^3 lattice: timed solves on the production path
+    (graph replay, no kernel timing), then one untimed solve with HIP-event
+    kernel timing for the per-kernel durations."""
+    log(f"plain: generating {N}^3 lattice")
     L = H.lattice(N, seed=0)
     n, nnz, nb = L["n"], L["nnz"], L["n"] // 6
-    ctx = H.GpuContext.create(0)
-    log(ctx.describe())
     system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
-    one_red = args.alg == "one-reduction"
+    one_red = alg == "one-reduction"
     if one_red:
         system.set_algorithm(H.ALG_PCG_ONE_REDUCTION)
     system.set_rhs(L["rhs"], np.zeros(n, np.float32))
     max_iter, tol = 5000, 1e-6
     best, tune_us = system.autotune(40)
     info = system.info()
-    log(f"autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 2).tolist()}; "
-        f"{info['local_tiles']}/{info['n_tiles']} tiles gather from LDS")
+    log(f"plain {N}^3 autotune: SpMV variant {best}; us/iteration {np.round(tune_us, 2).tolist()}")
+    for _ in range(warmup):
+        system.solve_resident(max_iter, tol, 0.0)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        res = system.solve_resident(max_iter, tol, 0.0)
+        lat.append((time.perf_counter() - ts) * 1e3)
+    elapsed = time.perf_counter() - t0
+    # untimed pass: kernel durations from HIP events (every k-th iteration)
+    system.set_kernel_timing(sample_every)
+    system.solve_resident(max_iter, tol, 0.0)
+    avg_us, cnt = system.kernel_timing()
+    system.set_kernel_timing(0)
+    b1, b2, b3 = (kernel_bytes_one_reduction if one_red else kernel_bytes)(n, nnz, nb)
+    sb, layout = spmv_stream_bytes(info, best)
+    gbps = lambda b, us: b / (us * 1e-6) / 1e9 if us > 0 else None
+    out = {"L": L, "value": steps / elapsed, "elapsed": elapsed, "lat": lat, "res": res,
+           "best": best, "tune_us": tune_us, "info": info, "avg_us": avg_us, "cnt": cnt,
+           "n": n, "nnz": nnz,
+           "spmv": {"variant": best, "layout": layout, "stream_bytes": sb,
+                    "us": float(avg_us[0]), "gbps": gbps(sb, avg_us[0]),
+                    "csr_bytes": b1, "csr_gbps": gbps(b1, avg_us[0]),
+                    "autotune_us_per_iter": [float(x) for x in tune_us],
+                    "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"]},
+           "iteration": {"kernel_us": {"spmv": float(avg_us[0]), "update": float(avg_us[1]),
+                                       "finish": float(avg_us[2])},
+                         "kernel_gbps_csr_accounting": {"spmv": gbps(b1, avg_us[0]),
+                                                        "update": gbps(b2, avg_us[1]),
+                                                        "finish": gbps(b3, avg_us[2])},
+                         "us": float(avg_us.sum()),
+                         "wall_us": elapsed / steps / int(res.iterations + 1) * 1e6,
+                         "samples": [int(c) for c in cnt]}}
+    system.close()
+    return out
+
+
+def run_hbm_only(H, ctx, N=64, iters=300):
+    """The SpMV on a lattice whose layout exceeds the 256 MB last-level cache:
+    the autotuner's variant, HIP-event durations over `iters` iterations."""
+    log(f"hbm-only: generating {N}^3 lattice")
+    L = H.lattice(N, seed=0)
+    s = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    s.set_rhs(L["rhs"], np.zeros(L["n"], np.float32))
+    best, tune_us = s.autotune(20)
+    info = s.info()
+    s.set_kernel_timing(2)
+    s.solve_resident(iters, 0.0, 0.0, raise_on_error=False)  # runs `iters` iterations
+    avg_us, cnt = s.kernel_timing()
+    s.close()
+    sb, layout = spmv_stream_bytes(info, best)
+    g = sb / (avg_us[0] * 1e-6) / 1e9
+    log(f"hbm-only {N}^3: variant {best}, SpMV {avg_us[0]:.2f} us, {layout} {sb / 1e6:.1f} MB, {g:.0f} GB/s")
+    return {"lattice": f"{N}^3", "n": L["n"], "nnz": L["nnz"], "variant": best, "layout": layout,
+            "stream_bytes": sb, "us": float(avg_us[0]), "achieved": g, "frac": g / HBM_PEAK_GBPS,
+            "csr_bytes": info["spmv_csr_bytes"], "samples": int(cnt[0]),
+            "autotune_us_per_iter": [float(x) for x in tune_us]}
+
+
+# ----------------------------------------------------------------- CPU baselines
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib
+    oracle_lib.build()
+    return oracle_lib
+
+
+def cpu_baseline_batch(Ls, inner_tol, budget_s):
+    """The batch workload on the host cores: for each system in order (FP32 LU
+    build excluded, as on the GPU), FP64 refinement to 1e-8 around the
+    oracle's FP32 restatement of the reference PCG (compute.rs:189-495) on the
+    scaled residual; stops after the system during which `budget_s` elapsed."""
+    O = _oracle()
+    threads, cores = host_threads()
+    O.set_threads(threads)
+    done, t_solve, t_start = 0, 0.0, time.perf_counter()
+    for L in Ls:
+        st, lu = O.build_lu_blocks_6(L["n"], L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+        b = L["rhs"].astype(np.float64)
+        t0 = time.perf_counter()
+        x = np.zeros(L["n"])
+        bn = np.linalg.norm(b)
+        for _ in range(30):
+            r = b - O_spmv64(O, L, x)
+            rn = np.linalg.norm(r)
+            if rn <= 1e-8 * bn:
+                break
+            b32 = (r / rn).astype(np.float32)
+            _, _, d, _ = O.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"], b32,
+                               np.zeros(L["n"], np.float32), 20000, inner_tol, 0.0)
+            x += d.astype(np.float64) * rn
+        t_solve += time.perf_counter() - t0
+        done += 1
+        if time.perf_counter() - t_start > budget_s:
+            break
+    return {"value": done / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+            "host": cores,
+            "sample": (f"the first {done} of the {len(Ls)} systems of the same batch, each refined in "
+                       f"FP64 to 1e-8 around the oracle's FP32 PCG (inner rel tol {inner_tol:g}), one "
+                       f"after another, {t_solve:.1f} s of solves on {threads} threads")}
+
+
+def O_spmv64(O, L, x):
+    y = np.zeros(max(L["n"], 1))
+    O.lib().or_spmv64(L["n"], L["row_ptr"], L["col_idx"], L["values"], np.ascontiguousarray(x), y)
+    return y[: L["n"]]
+
+
+def cpu_baseline_plain(L, iters_gpu, sample_iters, sample_iters_1core):
+    """The oracle (FP32 restatement of the reference loop) on the host cores:
+    `sample_iters` PCG iterations of the same system on all granted threads
+    (the whole solve when it converges within them) and `sample_iters_1core`
+    on one, scaled to solves/s with the GPU's iteration count."""
+    O = _oracle()
+    st, lu = O.build_lu_blocks_6(L["n"], L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    x0 = np.zeros(L["n"], np.float32)
+
+    def sample(threads, iters):
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        _, it, _, _ = O.pcg(L["row_ptr"], L["col_idx"], L["values"], lu, L["block_starts"],
+                            L["rhs"], x0, iters, 1e-6, 0.0)
+        dt = time.perf_counter() - t0
+        per_iter = dt / max(it, 1)
+        return 1.0 / (per_iter * (iters_gpu + 1)), it, dt, per_iter
+
+    threads, cores = host_threads()
+    side = round((L["n"] / 6) ** (1.0 / 3.0))
+    v, it, dt, per_iter = sample(threads, sample_iters)
+    v1, it1, dt1, per_iter1 = sample(1, sample_iters_1core)
+    if it >= iters_gpu:
+        v = 1.0 / dt
+        what = f"the whole solve of the same {side}^3 system ({it} iterations) in {dt:.1f} s"
+    else:
+        what = (f"{it} PCG iterations of the same {side}^3 system in {dt:.1f} s "
+                f"({per_iter * 1e3:.2f} ms/iteration), scaled to {iters_gpu} iterations")
+    return {"value": v, "unit": "solves/s", "cores": threads, "kind": "port", "host": cores,
+            "sample": what,
+            "single_core": {"value": v1, "unit": "solves/s", "cores": 1,
+                            "sample": f"{it1} iterations in {dt1:.1f} s ({per_iter1 * 1e3:.2f} ms/iteration)"}}
+
+
+# ----------------------------------------------------------------- workloads
+
+def roofline_from_plain(P, N, hbm_only=None):
+    sp_ = P["spmv"]
+    traffic = traffic_from_profiles(f"plain_N{N}")
+    t_bytes = ((traffic or {}).get("k_pcg_spmv_bytes_by_variant") or {}).get(str(P["best"]))
+    r = {"bound": "hbm", "kernel": "k_pcg_spmv", "workload": f"plain FP32 PCG SpMV, {N}^3 lattice",
+         "achieved": sp_["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+         "frac": sp_["gbps"] / HBM_PEAK_GBPS, "traffic": t_bytes,
+         "algorithmic_bytes": sp_["stream_bytes"], "kernel_us": sp_["us"],
+         "csr_accounting": {"bytes": sp_["csr_bytes"], "gbps": sp_["csr_gbps"],
+                            "frac": sp_["csr_gbps"] / HBM_PEAK_GBPS},
+         "note": (f"achieved = the bytes SpMV variant {P['best']} streams per launch ({sp_['layout']} "
+                  "layout: values, runs, column lists, headers, the rows' own p and q) / its mean "
+                  "duration from HIP events on its own stream; csr_accounting divides the "
+                  "reference's CSR bytes (8 nnz + 12 n + 4) by the same time; traffic = FETCH_SIZE + "
+                  "WRITE_SIZE per launch from rocprofv3 (profiles/), which cannot separate "
+                  "last-level-cache hits from HBM reads")}
+    if hbm_only:
+        r["hbm_only"] = hbm_only
+    return r
+
+
+def bench_headline(args):
+    import wgpu_solver_backend_amd as H
+    ctx = H.GpuContext.create(0)
+    log(ctx.describe())
     copy_bytes = 256 << 20
     copy_us = ctx.stream_copy_us(copy_bytes, 10)
     copy_gbps = 2 * copy_bytes / (copy_us * 1e-6) / 1e9
-    log(f"stream copy {copy_bytes >> 20} MiB: {copy_us:.1f} us, {copy_gbps:.0f} GB/s")
-    if args.profile_iters:
-        # short counter-collection run: a few full-size iterations, no JSON headline
-        for _ in range(max(args.steps, 1)):
-            r = system.solve_resident(args.profile_iters, tol, 0.0, raise_on_error=False)
-            log(f"profile run: status {r.status}, {r.iterations} its")
-        print(json.dumps({"profile_run": True, "iterations_per_solve": args.profile_iters,
-                          "copy_bytes": copy_bytes, "copy_us": copy_us}), flush=True)
-        return None
-    for i in range(args.warmup):
-        res = system.solve_resident(max_iter, tol, 0.0)
-        log(f"warmup {i}: {res.iterations} its, {res.solve_ms:.1f} ms")
-    system.set_kernel_timing(args.sample_every)
-    lat = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        res = system.solve_resident(max_iter, tol, 0.0)  # returns after the device finished
-        lat.append((time.perf_counter() - ts) * 1e3)
-    elapsed = time.perf_counter() - t0
-    avg_us, cnt = system.kernel_timing()
-    system.set_kernel_timing(0)
-    log(f"timed: {args.steps} solves in {elapsed:.3f} s; {res.iterations} its; "
-        f"kernel avg us {avg_us.round(2).tolist()} ({cnt.tolist()} samples)")
-    b1, b2, b3 = (kernel_bytes_one_reduction if one_red else kernel_bytes)(n, nnz, nb)
-    gbps = lambda b, us: b / (us * 1e-6) / 1e9 if us > 0 else None
-    spmv_gbps = b1 / (avg_us[0] * 1e-6) / 1e9
-    layout_bytes = info["spmv_sliced_bytes"] if best in (4, 5) and info["spmv_sliced_bytes"] else \
-        info["spmv_layout_bytes"]
-    layout_gbps = layout_bytes / (avg_us[0] * 1e-6) / 1e9
-    iter_us = float(avg_us.sum())
-    iter_bytes = b1 + b2 + b3
-    key = f"plain_N{N}"
-    traffic = traffic_from_profiles(key)
+    B = run_batch(H, ctx, args, args.steps, args.warmup)
+    S = run_serving(H, ctx, args) if not args.no_serve else None
+    P = run_plain(H, ctx, 50, args.plain_steps, 1)
+    HB = run_hbm_only(H, ctx, 64) if not args.no_hbm_only else None
+    bsum = B["summary"]
     out = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": B["value"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": B["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"plain FP32 block-Jacobi PCG, {N}^3 beam lattice ({n:,} unknowns, "
-                                f"{nnz:,} nnz, 6x6 blocks), tol {tol:g}, max {max_iter}, x0 = 0"
-                                + (", one-reduction (Chronopoulos-Gear) PCG: two launches per "
-                                   "iteration, not bit-identical to the reference" if one_red else "")),
-                   "n": n, "nnz": nnz, "iterations": int(res.iterations), "seed": 0,
-                   "algorithm": "one-reduction" if one_red else "pcg"},
-        "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
-                       "max": max(lat)},
-        "true_rel_residual_fp64": res.rel_res_true_fp64,
-        "spmv_gbps": spmv_gbps,
+        "config": {"workload": (f"batched: {bsum['systems']} beam lattices, sides log-uniform [4,24] "
+                                f"({bsum['unknowns_min']:,}..{bsum['unknowns_max']:,} unknowns, sum "
+                                f"{bsum['unknowns_sum']:,}), FP32 block-Jacobi PCG (inner rel tol "
+                                f"{args.inner_tol:g}) in FP64 refinement to a 1e-8 relative residual, "
+                                f"all systems advanced together; one step = one solve of all "
+                                f"{bsum['systems']} from x0 = 0"),
+                   "systems": bsum["systems"], "target_rel_residual": 1e-8, "seed": 0},
+        "latency_ms": {"p50": pct(B["latency_ms"], 50), "p99": pct(B["latency_ms"], 99),
+                       "max": float(max(B["latency_ms"])),
+                       "what": "each system's time from its step's start to the status poll that "
+                               "showed its FP64-verified solution"},
+        "verification": {k: bsum[k] for k in ("worst_true_rel_residual_fp64_device",
+                                              "worst_true_rel_residual_fp64_host", "all_verified_1e-8")},
+        "batch": bsum,
+        "serving": S,
+        "plain_fp32_50": {"workload": "plain FP32 block-Jacobi PCG, 50^3 beam lattice (750,000 unknowns, "
+                                      "30,352,200 nnz, 6x6 blocks), tol 1e-6 (recurrence), max 5000, x0 = 0",
+                          "solves_per_s": P["value"], "iterations": int(P["res"].iterations),
+                          "latency_ms": {"p50": statistics.median(P["lat"]), "max": max(P["lat"])},
+                          "true_rel_residual_fp64": P["res"].rel_res_true_fp64,
+                          "spmv": P["spmv"], "iteration": P["iteration"]},
         "stream_copy_gbps": copy_gbps,
-        "iteration": {"us": iter_us, "wall_us": elapsed / args.steps / int(res.iterations + 1) * 1e6,
-                      "bytes": iter_bytes, "gbps": iter_bytes / (iter_us * 1e-6) / 1e9,
-                      "kernel_us": {"spmv": avg_us[0], "update": avg_us[1],
-                                    "finish": avg_us[2]},
-                      "kernel_gbps": {"spmv": spmv_gbps, "update": gbps(b2, avg_us[1]),
-                                      "finish": gbps(b3, avg_us[2])}},
-        "spmv": {"variant": best, "autotune_us_per_iter": [float(x) for x in tune_us],
-                 "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"],
-                 "layout_bytes": layout_bytes, "layout_gbps": layout_gbps},
-        "roofline": {"bound": "hbm", "kernel": "k_ccg_spmv" if one_red else "k_pcg_spmv",
-                     "achieved": spmv_gbps,
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": spmv_gbps / HBM_PEAK_GBPS,
-                     "traffic": None if one_red else
-                     ((traffic or {}).get("k_pcg_spmv_bytes_by_variant") or {}).get(str(best)),
-                     "algorithmic_bytes": b1,
-                     "note": "achieved = CSR algorithmic bytes (8 nnz + 12 n + 4) / kernel time. "
-                             "The kernel's own layout streams fewer bytes than CSR accounting "
-                             "(spmv.layout_bytes: the sliced layout reads about 4.6 bytes per "
-                             "entry, the tile-local one 6), so frac is not the bandwidth drawn: "
-                             "that is spmv.layout_gbps, and traffic / time from the counters; "
-                             "variant 5 reads the layout through the caches, and a layout that fits "
-                             "the 256 MB last-level cache is partly served from there"},
+        "roofline": roofline_from_plain(P, 50, HB),
     }
     if not args.no_cpu:
-        log("cpu baseline (oracle on host cores)")
-        out["cpu_baseline"] = cpu_baseline(L, int(res.iterations), args.cpu_iters, args.cpu_iters_1core)
+        log("cpu baseline: the batch workload on the host cores")
+        out["cpu_baseline"] = cpu_baseline_batch(B["Ls"], args.inner_tol, args.cpu_budget_s)
+        log("cpu baseline: plain 50^3")
+        out["plain_fp32_50"]["cpu_baseline"] = cpu_baseline_plain(
+            P["L"], int(P["res"].iterations), args.cpu_iters, args.cpu_iters_1core)
     else:
         out["cpu_baseline"] = None
-    system.close()
     ctx.close()
     return out
 
 
-def _lattice_sides(rng, count, lo, hi):
-    return [int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(count)]
+def bench_plain(args):
+    import wgpu_solver_backend_amd as H
+    ctx = H.GpuContext.create(0)
+    P = run_plain(H, ctx, args.N, args.steps, args.warmup, args.alg)
+    one_red = args.alg == "one-reduction"
+    out = {"metric": f"solves/s (plain FP32 PCG, recurrence rel tol 1e-6); SpMV HBM GB/s vs peak",
+           "value": P["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": P["elapsed"] / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": (f"plain FP32 block-Jacobi PCG, {args.N}^3 beam lattice ({P['n']:,} "
+                                   f"unknowns, {P['nnz']:,} nnz, 6x6 blocks), tol 1e-6, max 5000, x0 = 0"
+                                   + (", one-reduction (Chronopoulos-Gear) PCG" if one_red else "")),
+                      "iterations": int(P["res"].iterations), "algorithm": args.alg},
+           "latency_ms": {"p50": statistics.median(P["lat"]), "p99": pct(P["lat"], 99)},
+           "true_rel_residual_fp64": P["res"].rel_res_true_fp64,
+           "spmv": P["spmv"], "iteration": P["iteration"],
+           "roofline": roofline_from_plain(P, args.N)}
+    if one_red:
+        out["roofline"]["kernel"] = "k_ccg_spmv"
+        out["roofline"]["traffic"] = None
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_plain(P["L"], int(P["res"].iterations), args.cpu_iters,
+                                                 args.cpu_iters_1core)
+    ctx.close()
+    return out
 
 
-def _host_rel_res(L, x):
-    import scipy.sparse as sp
-    n = L["n"]
-    A = sp.csr_matrix((L["values"].astype(np.float64), L["col_idx"], L["row_ptr"]), shape=(n, n))
-    b = L["rhs"].astype(np.float64)
-    return float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+def bench_profile(args):
+    """Counter runs: `--profile-iters` PCG iterations per solve of the plain
+    N^3 system with SpMV variant `--variant` (or the autotuner's), no JSON headline."""
+    import wgpu_solver_backend_amd as H
+    ctx = H.GpuContext.create(0)
+    L = H.lattice(args.N, seed=0)
+    s = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    s.set_rhs(L["rhs"], np.zeros(L["n"], np.float32))
+    if args.variant is None:
+        best, _ = s.autotune(20)
+    else:
+        best = args.variant
+        s.set_spmv_variant(best)
+    copy_bytes = 256 << 20
+    copy_us = ctx.stream_copy_us(copy_bytes, 10)
+    for _ in range(max(args.steps, 1)):
+        r = s.solve_resident(args.profile_iters, 1e-6, 0.0, raise_on_error=False)
+        log(f"profile run: status {r.status}, {r.iterations} its")
+    info = s.info()
+    sb, layout = spmv_stream_bytes(info, best)
+    print(json.dumps({"profile_run": True, "N": args.N, "variant": best, "layout": layout,
+                      "stream_bytes": sb, "csr_bytes": info["spmv_csr_bytes"],
+                      "iterations_per_solve": args.profile_iters, "copy_bytes": copy_bytes,
+                      "copy_us": copy_us}), flush=True)
+    s.close()
+    ctx.close()
+    return None
 
 
 def bench_ir(args):
@@ -228,7 +572,6 @@ def bench_ir(args):
     n, nnz = L["n"], L["nnz"]
     ctx = H.GpuContext.create(0)
     system = H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
-    # the SpMV variant, timed on this system's FP32 iterations
     system.set_rhs(L["rhs"], np.zeros(n, np.float32))
     best, tune_us = system.autotune(40)
     log(f"autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 2).tolist()}")
@@ -244,11 +587,11 @@ def bench_ir(args):
         x, res = system.solve_ir(b64, None, params)
         lat.append((time.perf_counter() - ts) * 1e3)
     elapsed = time.perf_counter() - t0
-    host_res = _host_rel_res(L, x)
-    # plain FP32 PCG on the same system for comparison (its FP64 true residual)
+    host_res = host_rel_res(L, x)
     system.set_rhs(L["rhs"])
     r32 = system.solve_resident(20000, 1e-6, 0.0, raise_on_error=False)
-    return {"metric": METRIC, "value": args.steps / elapsed, "unit": "solves/s", "n_gpus": 1,
+    return {"metric": "solves/s at a 1e-10 FP64-verified relative residual (mixed precision)",
+            "value": args.steps / elapsed, "unit": "solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
             "data": "synthetic",
@@ -256,7 +599,7 @@ def bench_ir(args):
                                    f"(inner rel tol {args.inner_tol:g}) in FP64 refinement to 1e-10",
                        "n": n, "nnz": nnz, "outer": int(res.outer_iterations),
                        "inner_iterations": int(res.iterations)},
-            "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99))},
+            "latency_ms": {"p50": statistics.median(lat), "p99": pct(lat, 99)},
             "spmv": {"variant": best, "autotune_us_per_iter": [float(x) for x in tune_us]},
             "true_rel_residual_fp64_device": res.rel_res_true_fp64,
             "true_rel_residual_fp64_host": host_res,
@@ -265,115 +608,33 @@ def bench_ir(args):
 
 
 def bench_batch(args):
-    """256 independent lattices (sides log-uniform in [4,24]) solved together,
-    FP32 inner + FP64 refinement to 1e-8, against the same systems solved one
-    at a time through the per-system path."""
     import wgpu_solver_backend_amd as H
-    rng = np.random.default_rng(0)
-    sides = _lattice_sides(rng, args.batch, 4, 24)
     ctx = H.GpuContext.create(0)
-    Ls = [H.lattice(N, seed=1000 + i) for i, N in enumerate(sides)]
-    systems = [H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"]) for L in Ls]
-    for L, s in zip(Ls, systems):
-        s.set_rhs64(L["rhs"].astype(np.float64))
-    params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
-    batch = H.Batch(ctx, systems)
-    best, tune_us = batch.autotune(20, ir=True)
-    log(f"batch autotune: SpMV variant {best}; us/iteration per variant {np.round(tune_us, 1).tolist()}")
-    for _ in range(args.warmup):
-        batch.solve_ir(params)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st, results = batch.solve_ir(params)
-    elapsed = time.perf_counter() - t0
-    batched_sps = args.steps * len(systems) / elapsed
-    # per-system path on the same systems (one solve each)
-    t1 = time.perf_counter()
-    alone = []
-    for L, s in zip(Ls, systems):
-        x, r = s.solve_ir(L["rhs"].astype(np.float64), None, params)
-        alone.append(r)
-    t_alone = time.perf_counter() - t1
-    alone_sps = len(systems) / t_alone
-    worst = max(r.rel_res_true_fp64 for r in results)
-    same = all(a.iterations == r.iterations for a, r in zip(alone, results))
-    ns = [L["n"] for L in Ls]
-    return {"metric": METRIC, "value": batched_sps, "unit": "solves/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-            "data": "synthetic",
-            "config": {"workload": f"batched: {len(systems)} beam lattices, sides log-uniform [4,24] "
-                                   f"({min(ns):,}..{max(ns):,} unknowns, sum {sum(ns):,}), FP32 inner "
-                                   f"(rel tol {args.inner_tol:g}) + FP64 refinement to 1e-8, one tile "
-                                   f"table, graph-replayed chunks",
-                       "systems": len(systems)},
-            "spmv": {"variant": best, "autotune_us_per_iter": [float(u) for u in tune_us]},
-            "per_system_path_solves_per_s": alone_sps, "speedup_vs_per_system": batched_sps / alone_sps,
-            "worst_true_rel_residual_fp64": worst, "iterations_identical_alone_vs_batched": same,
-            "status_all_ok": st == 0}
+    B = run_batch(H, ctx, args, args.steps, args.warmup)
+    bsum = B["summary"]
+    out = {"metric": METRIC, "value": B["value"], "unit": "solves/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": B["elapsed"] / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic",
+           "config": {"workload": f"batched: {bsum['systems']} beam lattices, sides log-uniform [4,24], "
+                                  f"FP64 refinement to 1e-8 (inner rel tol {args.inner_tol:g})"},
+           "latency_ms": {"p50": pct(B["latency_ms"], 50), "p99": pct(B["latency_ms"], 99)},
+           "batch": bsum}
+    ctx.close()
+    return out
 
 
 def bench_serve(args):
-    """Poisson request stream (seeded, target 200 req/s) of lattices with sides
-    log-uniform in [6,40], IR to 1e-8, continuous batching under an HBM budget."""
     import wgpu_solver_backend_amd as H
-    import threading
-    rng = np.random.default_rng(0)
-    count = args.requests
-    sides = _lattice_sides(rng, count, 6, 40)
-    gaps = rng.exponential(1.0 / args.rate, count)
-    log(f"serve: generating {count} requests")
-    reqs = [H.lattice(N, seed=2000 + i) for i, N in enumerate(sides)]
     ctx = H.GpuContext.create(0)
-    params = H.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-8)
-    srv = H.Server(ctx, int(args.budget_gb * (1 << 30)), max_active=args.max_active, chunk_iters=8)
-    tickets = [None] * count
-    lat = [None] * count
-    status = [None] * count
-    iters = [(0, 0)] * count
-
-    def waiter():
-        for i in range(count):
-            while tickets[i] is None:
-                time.sleep(0.0005)
-            st, x, res, ms = srv.wait(tickets[i])
-            lat[i], status[i] = ms, st
-            iters[i] = (res.iterations, res.outer_iterations)
-
-    th = threading.Thread(target=waiter)
-    th.start()
-    t0 = time.perf_counter()
-    t_next = t0
-    for i, L in enumerate(reqs):
-        t_next += gaps[i]
-        d = t_next - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        tickets[i] = srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
-                                L["rhs"].astype(np.float64), params)
-    th.join()
-    elapsed = time.perf_counter() - t0
-    stats = srv.stats()
-    srv.close()
-    ok = sum(1 for s in status if s == 0)
-    # bytes the requests' iterations stream in CSR accounting: per inner PCG
-    # iteration 8 nnz + 76.7 n (SpMV, dots, updates, block solves fused), per
-    # FP64 residual step 8 nnz + 28 n + 4
-    moved = sum(it * (8 * L["nnz"] + 76.7 * L["n"]) + (outer + 1) * (8 * L["nnz"] + 28 * L["n"] + 4)
-                for (it, outer), L in zip(iters, reqs))
-    return {"metric": METRIC, "value": ok / elapsed, "unit": "solves/s", "n_gpus": 1,
-            "steps": count, "warmup": 0, "ms_per_step": elapsed / count * 1e3,
+    S = run_serving(H, ctx, args)
+    a = S["at_target_rate"]
+    ctx.close()
+    return {"metric": METRIC, "value": a["completed_rate"], "unit": "solves/s", "n_gpus": 1,
+            "steps": a["requests"], "warmup": 0, "ms_per_step": a["elapsed_s"] / a["requests"] * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-            "data": "synthetic",
-            "config": {"workload": f"serving: Poisson arrivals at {args.rate:g} req/s (seed 0), {count} "
-                                   f"requests, lattice sides log-uniform [6,40], FP64-verified 1e-8 (FP32 inner "
-                                   f"rel tol {args.inner_tol:g}), "
-                                   f"HBM budget {args.budget_gb:g} GiB, <= {args.max_active} resident"},
-            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                           "max": float(max(lat))},
-            "offered_rate": count / (t_next - t0), "completed_ok": ok,
-            "inner_iterations_total": int(sum(i for i, _ in iters)),
-            "streamed_gbps": moved / elapsed / 1e9, "server": stats}
+            "data": "synthetic", "config": {"workload": S["workload"]},
+            "latency_ms": a["latency_ms"], "serving": S}
 
 
 def main():
@@ -382,27 +643,30 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=50)
-    ap.add_argument("--sample-every", type=int, default=32,
-                    help="HIP-event timing of every k-th PCG iteration's kernels (inside the "
-                         "timed solves; each sample costs a few microseconds)")
-    ap.add_argument("--cpu-iters", type=int, default=5000,
-                    help="most oracle PCG iterations timed on up to 16 host threads (the default "
-                         "lets the 50^3 solve run to convergence: the whole solve is timed)")
-    ap.add_argument("--cpu-iters-1core", type=int, default=800,
-                    help="oracle PCG iterations timed on one host thread (scaled)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="plain", choices=["plain", "ir", "batch", "serve"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "plain", "ir", "batch", "serve", "profile"])
     ap.add_argument("--alg", default="pcg", choices=["pcg", "one-reduction"],
-                    help="plain workload: the reference's PCG (default, bit-identical to the "
-                         "reference) or the one-reduction PCG (two launches per iteration)")
+                    help="plain workload: the reference's PCG (default) or the one-reduction PCG")
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="headline: timed plain 50^3 solves")
+    ap.add_argument("--cpu-iters", type=int, default=5000,
+                    help="most oracle PCG iterations of the plain system timed on the granted host "
+                         "threads (the default lets the 50^3 solve converge: the whole solve is timed)")
+    ap.add_argument("--cpu-iters-1core", type=int, default=400,
+                    help="oracle PCG iterations of the plain system timed on one host thread (scaled)")
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0,
+                    help="host seconds of the batch workload's CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-hbm-only", action="store_true")
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="inner FP32 PCG relative tolerance of the refinement loop (default: 1e-4 "
-                         "for --workload ir, 1e-3 for batch and serve; the fastest in a sweep over "
-                         "1e-2..1e-5 on one MI355X)")
+                         "for --workload ir, 1e-3 for the 1e-8 workloads)")
     ap.add_argument("--profile-iters", type=int, default=0,
-                    help="counter runs: solves capped at this many iterations, no headline line")
+                    help="counter runs: plain solves capped at this many iterations, no headline line")
+    ap.add_argument("--variant", type=int, default=None, help="profile: SpMV variant (default: autotuned)")
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--max-active", type=int, default=256)
@@ -415,7 +679,11 @@ def main():
         sys.exit(2)
     if args.inner_tol is None:
         args.inner_tol = 1e-4 if args.workload == "ir" else 1e-3
-    out = {"plain": bench_plain, "ir": bench_ir, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
+    if args.profile_iters:
+        args.workload = "profile"
+    fn = {"headline": bench_headline, "plain": bench_plain, "ir": bench_ir, "batch": bench_batch,
+          "serve": bench_serve, "profile": bench_profile}[args.workload]
+    out = fn(args)
     if out is not None:
         print(json.dumps(out), flush=True)
 

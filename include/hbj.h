@@ -54,7 +54,10 @@ typedef struct hbj_result {
   float reserved0;
   double rel_res_recurrence;   /* sqrt(rr / b_norm2) */
   double rel_res_true_fp64;    /* ||b - A x||_2 / ||b||_2 evaluated in FP64 (NaN if not evaluated) */
-  double solve_ms;             /* host wall time of the call's device work */
+  double solve_ms;             /* host wall time of the call's device work; for a
+                                  batch's systems: from the start of the batch's
+                                  device work to the status poll (every chunk of
+                                  iterations) that first showed this system done */
 } hbj_result;
 
 /* Mixed-precision (iterative refinement) parameters. */

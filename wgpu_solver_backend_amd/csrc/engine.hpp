@@ -85,6 +85,17 @@ inline uint32_t env_u32(const char* name, uint32_t dflt) {
   return (uint32_t)std::strtoul(v, nullptr, 10);
 }
 
+// A rocprofiler-sdk tool (rocprofv3 and the like) is loaded into the process:
+// the SDK loads the libraries named by ROCP_TOOL_LIBRARIES.  Replaying a
+// captured graph under such a tool's kernel tracing crashed on the host
+// inside hipGraphLaunch (a read of an unmapped address by the runtime's
+// intercepted dispatch path; the same graphs replay correctly without the
+// tool), so the engine launches the kernels directly while one is attached.
+inline bool profiler_tool_attached() {
+  const char* v = std::getenv("ROCP_TOOL_LIBRARIES");
+  return v && *v;
+}
+
 }  // namespace hbj_detail
 using namespace hbj_detail;
 
@@ -147,7 +158,7 @@ struct Engine {
     n_tiles = ntiles;
     chunk = env_u32("HBJ_CHUNK", 16);
     chunk = chunk < 2 ? 2u : (chunk & ~1u);  // even: the state parity returns to 0
-    use_graph = env_u32("HBJ_GRAPH", 1) != 0;
+    use_graph = env_u32("HBJ_GRAPH", 1) != 0 && !profiler_tool_attached();
     remap = env_u32("HBJ_XCD_REMAP", 1);
     update_groups = env_u32("HBJ_UPDATE_GROUPS", 1);
     spmv_variant = env_u32("HBJ_SPMV_VARIANT", kSpmvDefault) % kSpmvVariantCount;
@@ -416,12 +427,22 @@ struct Engine {
     return true;
   }
 
+  // Host time (ms from the start of the last run's device work) of the
+  // status poll that first showed each system's final status.
+  std::vector<double> done_ms;
+  void note_done(const SysState* polled, double t) {
+    for (uint32_t i = 0; i < n_sys; ++i)
+      if (done_ms[i] < 0.0 && polled[i].status != kRunning) done_ms[i] = t;
+  }
+
   // Runs until every system has a final status.  `init` holds n_sys states;
   // final states are returned in `out`.
   hbj_status run(const SysState* init, SysState* out, double* ms) {
+    done_ms.assign(n_sys, -1.0);
     HIPCHK(hipSetDevice(device));
     if (n_tiles == 0 || all_done(init, n_sys)) {
       std::memcpy(out, init, sizeof(SysState) * n_sys);
+      done_ms.assign(n_sys, 0.0);
       if (ms) *ms = 0.0;
       return HBJ_OK;
     }
@@ -465,6 +486,7 @@ struct Engine {
       if (c > 0) {
         const int prev = pool ^ 1;
         HIPCHK(hipEventSynchronize(ev[prev]));
+        note_done(h_poll[prev], now_ms() - t0);
         if (timing_every) CHK(harvest_timing(prev));
         if (all_done(h_poll[prev], n_sys)) {
           final_pool = pool;
@@ -481,7 +503,11 @@ struct Engine {
     }
     HIPCHK(hipEventSynchronize(ev[final_pool]));
     if (timing_every) CHK(harvest_timing(final_pool));
-    if (ms) *ms = now_ms() - t0;
+    const double t_end = now_ms() - t0;
+    note_done(h_poll[final_pool], t_end);
+    for (double& d : done_ms)
+      if (d < 0.0) d = t_end;
+    if (ms) *ms = t_end;
     std::memcpy(out, h_poll[final_pool], sizeof(SysState) * n_sys);
     if (timing_every) commit_timing(init, out, n_sys);
     return HBJ_OK;

@@ -883,7 +883,7 @@ hbj_status hbj_batch_solve(hbj_batch* b, uint32_t max_iter, float rel_tol, float
   CHK(b->eng.run(init.data(), fin.data(), &ms));
   hbj_status worst = HBJ_OK;
   for (uint32_t i = 0; i < S; ++i) {
-    fill_result(fin[i], false, ms, &results[i]);
+    fill_result(fin[i], false, b->eng.done_ms[i], &results[i]);
     hbj_system* s = b->sys[i];
     if (fin[i].status == HBJ_OK && s->n) {
       if (fin[i].b_norm2 == 0.0f) results[i].rel_res_true_fp64 = 0.0;
@@ -922,7 +922,7 @@ hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* 
   CHK(b->eng.run(init.data(), fin.data(), &ms));
   hbj_status worst = HBJ_OK;
   for (uint32_t i = 0; i < S; ++i) {
-    fill_result(fin[i], true, ms, &results[i]);
+    fill_result(fin[i], true, b->eng.done_ms[i], &results[i]);
     if (fin[i].status != HBJ_OK && worst == HBJ_OK) {
       worst = (hbj_status)fin[i].status;
       status_message(fin[i].status, p->max_inner);

@@ -236,8 +236,11 @@ def serve_stream(H, ctx, reqs, rate, seed, params, budget_gb, max_active):
             "worst_true_rel_residual_fp64_device": float(max(r for r in dev_res if r is not None)),
             "inner_iterations_total": int(sum(i for i, _ in iters)),
             "streamed_gbps_csr_accounting": moved / elapsed / 1e9, "elapsed_s": elapsed,
-            "server": {k: stats[k] for k in ("peak_active", "peak_device_bytes", "chunks",
-                                             "table_changes", "arena_waits", "admit_ms", "table_ms")}}
+            "server": dict({k: stats[k] for k in ("peak_active", "peak_device_bytes", "chunks",
+                                                  "chunks_cached_spmv", "table_changes",
+                                                  "arena_waits", "admit_ms", "table_ms",
+                                                  "upload_bytes")},
+                           admit_ms_per_request=stats["admit_ms"] / max(count, 1))}
 
 
 def run_serving(H, ctx, args):

@@ -301,16 +301,26 @@ typedef struct hbj_server_stats {
                               (the budget allowed them, no free range was long enough) */
   double admit_ms;         /* host time spent making requests resident (summed) */
   double table_ms;         /* worker time spent on table rebuilds (summed) */
+  uint64_t upload_bytes;   /* bytes copied host -> device for admitted requests */
+  uint64_t chunks_cached_spmv; /* chunks whose SpMV read the values through the caches
+                                  (variant 5: the resident layouts fit the last-level
+                                  cache); the others read them non-temporally (4) */
 } hbj_server_stats;
 
 /* Continuous-batching request loop: a host thread admits queued requests as
  * HBM budget and slots free up, uploads them while others iterate, advances
  * all resident requests together, and retires each as it converges; the
  * slot table changes between chunks of iterations without waiting for the
- * device. */
+ * device.  A request's system is built on the host (tables, SpMV layout, LU
+ * blocks) and copied in one asynchronous upload from pinned staging; a
+ * system with the sliced SpMV layout keeps only that layout on the device
+ * (no CSR arrays). */
 hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_server** out);
 void hbj_server_destroy(hbj_server* srv);
-/* Copies the request; returns a ticket.  b64 in FP64; solved with IR (params). */
+/* Copies the request; returns a ticket.  b64 in FP64; solved with IR (params).
+ * HBJ_E_REJECTED at once when no block for these sizes could fit the budget;
+ * a request whose exact block (hbj_server_request_bytes) exceeds the budget
+ * completes with HBJ_E_REJECTED. */
 hbj_status hbj_server_submit(hbj_server* srv, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
                              const uint32_t* col_idx, const float* values,
                              const uint32_t* block_starts, uint32_t n_starts, const double* b64,
@@ -320,10 +330,13 @@ hbj_status hbj_server_submit(hbj_server* srv, uint32_t n, uint32_t nnz, const ui
 hbj_status hbj_server_wait(hbj_server* srv, uint64_t ticket, double* x, hbj_result* result,
                            double* latency_ms);
 hbj_status hbj_server_stats_get(hbj_server* srv, hbj_server_stats* stats);
-/* Device bytes a request of these sizes reserves from the budget (its block
- * in the server's arena, tile-local column lists counted at their largest).
- * No reference counterpart: sizing of budgets. */
-uint64_t hbj_server_request_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts);
+/* Device bytes a request reserves from the budget: the exact block its
+ * system takes in the server's arena (the host tables are built to find it;
+ * 0 and an error text for an invalid matrix or a zero pivot).  No reference
+ * counterpart: sizing of budgets. */
+uint64_t hbj_server_request_bytes(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                                  const uint32_t* col_idx, const float* values,
+                                  const uint32_t* block_starts, uint32_t n_starts);
 
 /* Calibration: `reps` launches of a plain streaming copy of `bytes` (read
  * bytes + write bytes per launch); *avg_us = mean launch duration (HIP events).

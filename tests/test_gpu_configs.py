@@ -106,6 +106,26 @@ def test_ir_1e10_error_vs_fp64_reference(ctx, hbj, oracle):
     s.close()
 
 
+def test_ir_residual_paths_bitwise(ctx, hbj):
+    """The FP64 refinement residual reads the CSR arrays under SpMV variants
+    0-3 and the sliced layout under 4 and 5: the same products in the same
+    order, so every variant refines to the same bits."""
+    L = hbj.lattice(12, seed=9, e_log10_half_range=1.5)
+    s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    assert s.info()["spmv_sliced_bytes"] > 0
+    p = hbj.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=1e-4, target_rel_res=1e-10)
+    ref = None
+    for v in range(hbj.SPMV_VARIANTS):
+        s.set_spmv_variant(v)
+        x, res = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+        assert res.status == 0
+        if ref is None:
+            ref = (x, res.iterations, res.outer_iterations)
+        assert (res.iterations, res.outer_iterations) == ref[1:], v
+        assert np.array_equal(bits64(x), bits64(ref[0])), v
+    s.close()
+
+
 @pytest.mark.slow
 def test_config4_batch_256(ctx, hbj, oracle):
     """256 lattices, sides log-uniform in [4, 24] (384 .. 82,944 unknowns),

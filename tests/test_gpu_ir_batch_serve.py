@@ -201,19 +201,17 @@ def test_server_waits_for_a_free_range(ctx, hbj):
     print("server arena: sides 10 / 12 / 11, E over 3 decades", flush=True)
     p = hbj.IrParams(target_rel_res=1e-8)
     Ls = [hbj.lattice(N, seed=200 + i, e_log10_half_range=1.5) for i, N in enumerate((10, 12, 11))]
-    alone, dev = [], []
+    alone = []
     for L in Ls:
         s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
         alone.append(s.solve_ir(L["rhs"].astype(np.float64), None, p))
-        dev.append(s.info()["device_bytes"])  # its block and its refinement block
         s.close()
     assert alone[0][1].iterations < alone[1][1].iterations  # A finishes first
-    # admission reserves the upper bound request_bytes; the arena places the
-    # actual block (the same pieces as a standalone system's two blocks)
-    a, b, c = (hbj.Server.request_bytes(L["n"], L["nnz"], L["block_starts"].size) for L in Ls)
-    A, B, C = dev
-    assert A <= a and B <= b and C <= c and A < C
-    budget = A + B + C - A // 2  # free ranges once A is done: A's block and a tail of C - A/2
+    # admission reserves each request's exact block, and the arena places it
+    a, b, c = (hbj.Server.request_bytes(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+               for L in Ls)
+    assert a < c
+    budget = a + b + c - a // 2  # free ranges once A is done: A's block and a tail of c - a/2
     assert a + b <= budget < a + b + c  # A and B admitted at once, C only after A
     assert b + c <= budget  # ... and then admitted, but placed only once B is done
     srv = hbj.Server(ctx, budget, max_active=4, chunk_iters=8)
@@ -226,4 +224,28 @@ def test_server_waits_for_a_free_range(ctx, hbj):
     stats = srv.stats()
     assert stats["arena_waits"] >= 1
     assert stats["peak_device_bytes"] <= budget
+    assert stats["upload_bytes"] > 0
+    srv.close()
+
+
+def test_server_keeps_only_the_sliced_layout(ctx, hbj):
+    """A served lattice system keeps its sliced SpMV layout and no CSR arrays
+    on the device: its block is under 60% of the same system created for
+    standalone solves (CSR, entry indices, both layouts), and a request whose
+    exact block exceeds the budget completes with REJECTED."""
+    L = hbj.lattice(10, seed=7)
+    s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    s.set_rhs64(L["rhs"].astype(np.float64))  # the refinement block too
+    standalone = s.info()["device_bytes"]
+    s.close()
+    served = hbj.Server.request_bytes(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    print(f"10^3 lattice: served block {served} B, standalone {standalone} B", flush=True)
+    assert served < 0.6 * standalone
+    # above the budget only by its exact size: admitted as far as the bound, rejected later
+    srv = hbj.Server(ctx, served - 4096, max_active=2)
+    t = srv.submit(L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                   L["rhs"].astype(np.float64))
+    st, _, _, _ = srv.wait(t, n=L["n"])
+    assert st == 9 and "above the admission budget" in hbj._lib.last_error()
+    assert srv.stats()["rejected"] == 1
     srv.close()

@@ -479,9 +479,16 @@ class Server:
         return st, (x[:n] if x is not None else None), res, lat.value
 
     @staticmethod
-    def request_bytes(n, nnz, n_starts):
-        """Device bytes a request of these sizes reserves from the budget."""
-        return int(_lib.lib().hbj_server_request_bytes(int(n), int(nnz), int(n_starts)))
+    def request_bytes(row_ptr, col_idx, values, block_starts):
+        """Device bytes the request's system reserves from the budget (its exact
+        block in the server's arena)."""
+        rp, ci, va, bs = _u32(row_ptr), _u32(col_idx), _f32(values), _u32(block_starts)
+        ci1, va1 = _nz(ci, np.uint32), _nz(va, np.float32)  # kept alive across the call
+        nb = _lib.lib().hbj_server_request_bytes(rp.size - 1, ci.size, rp, ci1.ctypes.data,
+                                                 va1.ctypes.data, bs, bs.size)
+        if nb == 0:
+            raise SolverError(1, _lib.last_error())
+        return int(nb)
 
     def stats(self):
         s = hbj_server_stats()

@@ -42,7 +42,8 @@ class hbj_server_stats(ctypes.Structure):
     _fields_ = [("submitted", c_u64), ("completed", c_u64), ("rejected", c_u64), ("active", c_u64),
                 ("queued", c_u64), ("device_bytes_in_use", c_u64), ("peak_device_bytes", c_u64),
                 ("peak_active", c_u64), ("chunks", c_u64), ("table_changes", c_u64),
-                ("arena_waits", c_u64), ("admit_ms", c_f64), ("table_ms", c_f64)]
+                ("arena_waits", c_u64), ("admit_ms", c_f64), ("table_ms", c_f64),
+                ("upload_bytes", c_u64), ("chunks_cached_spmv", c_u64)]
 
 
 class hbj_system_info(ctypes.Structure):
@@ -103,7 +104,7 @@ _SIGS = {
                                   P(hbj_ir_params), P(c_u64)]),
     "hbj_server_wait": (c_i32, [c_void_p, c_u64, c_void_p, P(hbj_result), P(c_f64)]),
     "hbj_server_stats_get": (c_i32, [c_void_p, P(hbj_server_stats)]),
-    "hbj_server_request_bytes": (c_u64, [c_u32, c_u32, c_u32]),
+    "hbj_server_request_bytes": (c_u64, [c_u32, c_u32, u32p, c_void_p, c_void_p, u32p, c_u32]),
     "hbj_case_load": (c_i32, [c_char_p, P(c_void_p)]),
     "hbj_case_free": (None, [c_void_p]),
     "hbj_case_view": (c_i32, [c_void_p, P(c_u32), P(c_u32), P(c_u32)] + [P(c_void_p)] * 6),

@@ -699,6 +699,36 @@ struct BlockLayout {
 
 }  // namespace hbj
 
+namespace hbj {
+
+// What system creation derives on the host from a validated CSR matrix,
+// before it touches the device: the SpMV tables (tile-local lists and entry
+// indices, the sliced layout where the system qualifies), the tile headers,
+// the per-tile block ranges, and -- for a lean system -- the LU blocks built
+// on the host.  A lean system (the server's) keeps on the device only what
+// its kernels read: with a sliced layout, no CSR arrays and no per-entry
+// indices (the SpMV, its FP64 refinement residual and the block solves
+// never read them when the sliced layout runs).
+struct SystemImage {
+  uint32_t n = 0, nnz = 0, nb = 0, n_tiles = 0, n_starts = 0, flags = 0;
+  bool lean = false;  // a sliced system keeps no CSR / lidx on the device
+  std::vector<uint16_t> lidx;
+  std::vector<uint2> tloc;
+  std::vector<uint32_t> tcols, tgrp;
+  std::vector<float> sval;
+  std::vector<uint32_t> runs;
+  std::vector<SliceHdr> shdr;
+  std::vector<TileHdr> thdr;
+  std::vector<uint2> tile_blk;
+  std::vector<float> lu;  // lean systems: host-built LU blocks (36 per block)
+  uint32_t local_tiles = 0;
+  uint64_t local_columns = 0, spmv_layout_bytes = 0, spmv_sliced_bytes = 0;
+  bool sliced() const { return (flags & kFlagSliced) != 0u; }
+  bool no_csr() const { return lean && sliced(); }
+};
+
+}  // namespace hbj
+
 // ------------------------------------------------------------------ system
 
 struct hbj_system {
@@ -725,6 +755,8 @@ struct hbj_system {
   SliceHdr* shdr = nullptr;
   uint32_t local_tiles = 0;
   uint64_t local_columns = 0, spmv_layout_bytes = 0, spmv_sliced_bytes = 0;
+  bool lean = false;          // no CSR arrays or entry indices on the device (a served
+                              // sliced system: only SpMV variants 4 and 5 may run it)
   int spmv_variant = -1;      // chosen variant (-1: the engine's default)
   uint64_t bytes = 0;
   Engine eng;
@@ -775,6 +807,34 @@ struct hbj_system {
   }
   void layout_ir(BlockLayout& L, uint32_t n_) {
     for (double** v : {&x64, &x0_64, &b64, &r64}) L.add(v, n_);
+    L.add(&b32, n_);
+  }
+
+  // The block of a lean system without CSR (SystemImage::no_csr()), with the
+  // refinement buffers: the pieces copied from the host first, up to
+  // *data_end, then the ones that start at zero (vectors, partials).  The
+  // SpMV never reads row_ptr, col, val or lidx of such a system.
+  void layout_lean(BlockLayout& L, const SystemImage& im, uint64_t* data_end) {
+    const uint32_t n_ = im.n;
+    const uint32_t nt = n_ ? (n_ + kTile - 1) / kTile : 1u;
+    const uint64_t nb_ = im.n_starts ? im.n_starts - 1 : 0;
+    L.add(&bs, (uint64_t)im.n_starts);
+    L.add(&lu, 36 * (nb_ ? nb_ : 1));
+    L.add(&tile_blk, nt);
+    L.add(&thdr, nt);
+    L.add(&tcols, im.tcols.size());
+    L.add(&tgrp, im.tgrp.size());
+    L.add(&sval, im.sval.size());
+    L.add(&runs, im.runs.size());
+    L.add(&shdr, im.shdr.size());
+    L.add(&b64, n_);
+    *data_end = (L.size + 255) & ~255ull;
+    for (float** v : {&x, &x0, &b, &p, &q, &z, &ra, &rb}) L.add(v, n_);
+    L.add(&part, 3ull * nt);
+    L.add(&part64, 5ull * nt);
+    L.add(&resid2, 2ull * nt);
+    L.add(&d_desc1, 1);
+    for (double** v : {&x64, &x0_64, &r64}) L.add(v, n_);
     L.add(&b32, n_);
   }
 
@@ -877,6 +937,10 @@ struct hbj_system {
 
   // ||b - A x|| / ||b|| in FP64 for the FP32 solution in `x` (plain mode).
   hbj_status true_residual(hipStream_t s, double* out) {
+    if (lean) {
+      set_error("true residual: the system keeps no CSR arrays on the device");
+      return HBJ_E_INVALID;
+    }
     if (!h_part2) CHK(halloc(&h_part2, 2ull * n_tiles));
     launch_true_resid(d_desc1, x, resid2, n_tiles, s);
     HIPCHK(hipGetLastError());
@@ -910,10 +974,28 @@ struct hbj_system {
 };
 
 namespace hbj {
-// hbj_system_create with its options: with_ir puts the FP64 refinement
-// buffers into the system's block (zeroed); arena (optional) supplies the
-// block instead of hipMalloc.
+// The host part of system creation for a validated CSR matrix (no HIP
+// calls).  lean: build the LU blocks on the host and mark a sliced system
+// as keeping no CSR on the device.  Fails only with the LU builder's zero
+// pivot (lean) or a system above the per-system size limit.
+hbj_status prepare_image(uint32_t n, uint32_t nnz, const uint32_t* row_ptr, const uint32_t* col_idx,
+                         const float* values, const uint32_t* block_starts, uint32_t n_starts,
+                         bool lean, SystemImage& im);
+// Device bytes of a lean system's block with the refinement buffers.
+uint64_t lean_block_bytes(const SystemImage& im);
+// hbj_system_create with its option: with_ir puts the FP64 refinement
+// buffers into the system's block (zeroed).
 hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
                          const uint32_t* col_idx, const float* values, const uint32_t* block_starts,
-                         uint32_t n_starts, bool with_ir, DeviceArena* arena, hbj_system** out);
+                         uint32_t n_starts, bool with_ir, hbj_system** out);
+// A lean system (with the refinement buffers) in a block of `arena` at
+// `off` (of lean_block_bytes(im) bytes): the block is written by copies
+// enqueued on `stream` from `staging` (pinned, at least the data part of the
+// block, free until `done` completes), CSR only when the system is not
+// sliced; `done` is recorded after the copies; *copied receives the bytes
+// copied from the host.  No host wait.
+hbj_status system_place(hbj_ctx* ctx, const SystemImage& im, const uint32_t* row_ptr,
+                        const uint32_t* col_idx, const float* values, const uint32_t* block_starts,
+                        const double* b64, DeviceArena* arena, uint64_t off, char* staging,
+                        hipStream_t stream, hipEvent_t done, hbj_system** out, uint64_t* copied);
 }  // namespace hbj

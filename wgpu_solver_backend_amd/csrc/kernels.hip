@@ -1040,21 +1040,18 @@ struct SpmvLds {
   static_assert(kWaves * kSliceF / 2 >= 2 * kTile, "reduction scratch aliases the slices");
 };
 
-// Refinement residual of one tile: r64 = b64 - A x64 in FP64 (FP32 matrix
-// entries are exact in FP64) and the tile's FP64 partials of |r64|^2 and
-// |b64|^2.
-template <int V>
-__device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& tr, uint32_t mb,
-                                              uint32_t me, double* s_prod) {
-  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+// Refinement residual of one tile: r64 = b64 - A x64 in FP64 (each FP32
+// entry times an FP64 x rounded once, summed from 0.0 in the row's entry
+// order) and the tile's FP64 partials of |r64|^2 and |b64|^2.  ax: the
+// thread's row of A x64, from the CSR arrays (spmv_ir_resid) or from the
+// sliced layout (spmv_ir_resid_sliced) -- the same products added in the
+// same order, so the same bits.
+__device__ __forceinline__ void ir_resid_tail(const SysDesc& d, const TileRef& tr, double ax,
+                                              double* s_prod) {
   double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
   double* s_rb = s_prod + kTile;
-  const uint32_t t = threadIdx.x, row = tr.row0 + t, wave = t >> 6;
+  const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
-  const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
-  double* slice = s_prod + wave * (kSliceF / 2);
-  const double ax = spmv_wave<double, kPer / 2>(
-      mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows, slice);
   __syncthreads();
   double rr = 0.0, bb = 0.0;
   if (own) {
@@ -1069,6 +1066,60 @@ __device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& t
     d.part64[3u * d.n_tiles + tr.tile] = rr;
     d.part64[4u * d.n_tiles + tr.tile] = bb;
   }
+}
+
+template <int V>
+__device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& tr, uint32_t mb,
+                                              uint32_t me, double* s_prod) {
+  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+  const uint32_t wave = threadIdx.x >> 6;
+  const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
+  double* slice = s_prod + wave * (kSliceF / 2);
+  const double ax = spmv_wave<double, kPer / 2>(
+      mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows, slice);
+  ir_resid_tail(d, tr, ax, s_prod);
+}
+
+// The refinement residual of a tile with the sliced layout, which needs no
+// CSR arrays: lane l walks its row's steps in entry order; step st's value
+// is at sval step-major, its column-list position is st + delta of the
+// first run ending after st, and the column comes from the tile's list (or
+// its groups of 6).  x64 is gathered from HBM: this runs once per
+// refinement step, not per PCG iteration.  Lanes without a row (past the
+// system, or rows without entries) have no runs and sum nothing.
+__device__ __forceinline__ void spmv_ir_resid_sliced(const SysDesc& d, const TileRef& tr,
+                                                     const TileHdr& hdr, const SlicedPre& p,
+                                                     double* s_prod) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t bnd[kMaxRuns];
+  int32_t dl[kMaxRuns];
+  uint32_t len = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kMaxRuns; ++k) {
+    const uint32_t run = k < p.nr ? p.run[k] : 0xFFFFu;
+    bnd[k] = run & 0xFFFFu;
+    dl[k] = (int32_t)(int16_t)(run >> 16);
+    len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
+  }
+  const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
+  double ax = 0.0;
+  for (uint32_t st = 0; st < len; ++st) {
+    const float v = __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(p.rv, (int)(((st / 4u) * 64u + lane) * 16u + (st % 4u) * 4u), 0, 0));
+    int32_t dd = dl[kMaxRuns - 1];
+#pragma unroll
+    for (int k = (int)kMaxRuns - 2; k >= 0; --k) dd = st < bnd[k] ? dl[k] : dd;
+    const uint32_t pos = (uint32_t)((int32_t)st + dd);
+    uint32_t c;
+    if (hdr.goff) {
+      const uint32_t j = pos / 6u;
+      c = 6u * d.tgrp[hdr.goff - 1u + j] + (pos - 6u * j);
+    } else {
+      c = d.tcols[hdr.off + pos];
+    }
+    ax = ax + (double)v * buf_gather<double>(rx64, c);
+  }
+  ir_resid_tail(d, tr, ax, s_prod);
 }
 
 // q of the tile's rows stored, and the tile's dot partial: dot(b,b) at init
@@ -1115,9 +1166,7 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   sliced_issue<kAux>(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
   if (status != kRunning) return;
   if (phase == kPhIrResid) {
-    uint32_t mb, me;
-    lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
-    spmv_ir_resid<4>(d, tr, mb, me, s_prod);
+    spmv_ir_resid_sliced(d, tr, hdr, pre, s_prod);
     return;
   }
   if (phase != kPhInit && phase != kPhIter) return;

@@ -342,16 +342,140 @@ hbj_status hbj_system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint3
                              const uint32_t* col_idx, const float* values,
                              const uint32_t* block_starts, uint32_t n_starts, hbj_system** out) {
   return hbj::system_create(ctx, n, nnz, row_ptr, col_idx, values, block_starts, n_starts, false,
-                            nullptr, out);
+                            out);
 }
 
 }  // extern "C"
 
 namespace hbj {
 
+hbj_status prepare_image(uint32_t n, uint32_t nnz, const uint32_t* row_ptr, const uint32_t* col_idx,
+                         const float* values, const uint32_t* block_starts, uint32_t n_starts,
+                         bool lean, SystemImage& im) {
+  if ((uint64_t)n > (uint64_t)kMaxTiles * kTile) {
+    set_error("hbj_system_create: n = %u exceeds the per-system limit of %llu rows", n,
+              (unsigned long long)kMaxTiles * kTile);
+    return HBJ_E_INVALID;
+  }
+  im = SystemImage{};
+  im.n = n;
+  im.nnz = nnz;
+  im.n_starts = n_starts;
+  im.nb = n_starts - 1;
+  im.n_tiles = (n + kTile - 1) / kTile;
+  im.lean = lean;
+  const uint32_t nb = im.nb, nt = im.n_tiles ? im.n_tiles : 1u;
+  build_tile_local(n, row_ptr, col_idx, im.lidx, im.tloc, im.tcols);
+  const bool sliced = build_sliced(n, row_ptr, values, im.lidx, im.tloc, im.sval, im.runs, im.shdr);
+  // column lists made of whole groups of 6 columns, also kept as their groups
+  // (staged from by the sliced SpMV): goff[t] = 1 + offset in tgrp, or 0
+  std::vector<uint32_t> goff(im.tloc.size(), 0u);
+  uint64_t staged_columns = 0;  // list entries the sliced SpMV reads
+  if (sliced) {
+    for (uint32_t t = 0; t < (uint32_t)im.tloc.size(); ++t) {
+      const uint32_t off = im.tloc[t].x, cnt = im.tloc[t].y;
+      bool whole = cnt > 0 && cnt % 6u == 0;
+      for (uint32_t i = 0; whole && i < cnt; i += 6)
+        whole = im.tcols[off + i] % 6u == 0 && im.tcols[off + i + 5] == im.tcols[off + i] + 5u;
+      if (whole) {
+        goff[t] = (uint32_t)im.tgrp.size() + 1u;
+        for (uint32_t i = 0; i < cnt; i += 6) im.tgrp.push_back(im.tcols[off + i] / 6u);
+        staged_columns += cnt / 6u;
+      } else {
+        staged_columns += cnt;
+      }
+    }
+  }
+  {
+    uint64_t csr_entries = 0;
+    for (uint32_t t = 0; t < (uint32_t)im.tloc.size(); ++t) {
+      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
+      const uint64_t e = row_ptr[r1] - row_ptr[r0];
+      if (im.tloc[t].y == kLocalOff) {
+        csr_entries += e;
+      } else {
+        im.local_tiles += 1;
+        im.local_columns += im.tloc[t].y;
+        im.spmv_layout_bytes += 6 * e + 4ull * im.tloc[t].y + sizeof(TileHdr);
+      }
+    }
+    im.spmv_layout_bytes += 8 * csr_entries + 12ull * n + 4;
+  }
+  bool blocks6 = nb > 0 && block_starts[nb] == n && n <= 6ull * nb && n > 6ull * (nb - 1);
+  for (uint32_t b = 0; b < nb; ++b) {
+    if (block_starts[b + 1] - block_starts[b] > (uint32_t)kBlock) im.flags |= kFlagOversize;
+    if (block_starts[b] != 6ull * b) blocks6 = false;
+  }
+  if (blocks6) im.flags |= kFlagBlocks6;
+  if (sliced) {
+    im.flags |= kFlagSliced;
+    im.spmv_sliced_bytes = 4ull * im.sval.size() + 4ull * im.runs.size() +
+                           (sizeof(SliceHdr) * kWaves + sizeof(TileHdr)) * im.n_tiles +
+                           4ull * staged_columns + 8ull * n;
+  }
+  im.thdr.resize(im.tloc.size());
+  for (uint32_t t = 0; t < (uint32_t)im.tloc.size(); ++t) {
+    const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
+    TileHdr& h = im.thdr[t];
+    h.off = im.tloc[t].x;
+    h.cnt = im.tloc[t].y;
+    for (uint32_t k = 0; k < 4; ++k) h.eb[k] = row_ptr[std::min<uint32_t>(r0 + 64u * k, r1)];
+    h.eb[4] = row_ptr[r1];
+    h.goff = goff[t];
+  }
+  // per tile: the first block touching rows [256t, 256t+256) and how many do
+  im.tile_blk.assign(nt, make_uint2(0u, 0u));
+  for (uint32_t t = 0; t < im.n_tiles; ++t) {
+    const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n) - 1;
+    const uint32_t bf = (uint32_t)(std::upper_bound(block_starts, block_starts + n_starts, r0) - block_starts) - 1;
+    const uint32_t bl = (uint32_t)(std::upper_bound(block_starts, block_starts + n_starts, r1) - block_starts) - 1;
+    im.tile_blk[t] = make_uint2(bf, bl - bf + 1);
+  }
+  if (lean) {
+    // the LU blocks on the host (compute.rs:84-181; the device builder's
+    // bits): a lean system may keep no CSR on the device to build them from
+    im.lu.assign(36ull * (nb ? nb : 1u), 0.0f);
+    const hbj_status st = hbj_build_lu_blocks_6(n, row_ptr, n + 1, col_idx, nnz, values, nnz,
+                                                block_starts, n_starts, im.lu.data());
+    if (st != HBJ_OK) return st;
+    if (im.no_csr()) std::vector<uint16_t>().swap(im.lidx);  // host-only from here on
+  }
+  return HBJ_OK;
+}
+
+uint64_t lean_block_bytes(const SystemImage& im) {
+  hbj_system probe;
+  BlockLayout L;
+  if (im.no_csr()) {
+    uint64_t data_end = 0;
+    probe.layout_lean(L, im, &data_end);
+  } else {
+    probe.layout(L, im.n, im.nnz, im.n_starts, im.tcols.size(), im.sliced() ? im.sval.size() : 0u,
+                 im.sliced() ? im.runs.size() : 0u, im.tgrp.size(), true);
+  }
+  return (L.size + 255) & ~255ull;
+}
+
+// The fields of a new system from its image (no device memory yet).
+static hbj_system* new_system(hbj_ctx* ctx, const SystemImage& im) {
+  auto* s = new hbj_system();
+  s->ctx = ctx;
+  s->n = im.n;
+  s->nnz = im.nnz;
+  s->nb = im.nb;
+  s->n_tiles = im.n_tiles;
+  s->flags = im.flags;
+  s->local_tiles = im.local_tiles;
+  s->local_columns = im.local_columns;
+  s->spmv_layout_bytes = im.spmv_layout_bytes;
+  s->spmv_sliced_bytes = im.spmv_sliced_bytes;
+  s->lean = im.no_csr();
+  return s;
+}
+
 hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
                          const uint32_t* col_idx, const float* values, const uint32_t* block_starts,
-                         uint32_t n_starts, bool with_ir, DeviceArena* arena, hbj_system** out) {
+                         uint32_t n_starts, bool with_ir, hbj_system** out) {
   clear_error();
   if (!ctx || !out || (nnz && !values)) {
     set_error("hbj_system_create: bad argument");
@@ -359,72 +483,10 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   }
   *out = nullptr;
   CHK(validate_case(n, nnz, row_ptr, col_idx, block_starts, n_starts));
-  if ((uint64_t)n > (uint64_t)kMaxTiles * kTile) {
-    set_error("hbj_system_create: n = %u exceeds the per-system limit of %llu rows", n,
-              (unsigned long long)kMaxTiles * kTile);
-    return HBJ_E_INVALID;
-  }
-  const uint32_t nb = n_starts - 1;
-  std::vector<uint16_t> lidx_h;
-  std::vector<uint2> tloc_h;
-  std::vector<uint32_t> tcols_h;
-  build_tile_local(n, row_ptr, col_idx, lidx_h, tloc_h, tcols_h);
-  std::vector<float> sval_h;
-  std::vector<uint32_t> runs_h;
-  std::vector<SliceHdr> shdr_h;
-  const bool sliced = build_sliced(n, row_ptr, values, lidx_h, tloc_h, sval_h, runs_h, shdr_h);
-  // column lists made of whole groups of 6 columns, also kept as their groups
-  // (staged from by the sliced SpMV): goff_h[t] = 1 + offset in tgrp_h, or 0
-  std::vector<uint32_t> tgrp_h, goff_h(tloc_h.size(), 0u);
-  uint64_t staged_columns = 0;  // list entries the sliced SpMV reads
-  if (sliced) {
-    for (uint32_t t = 0; t < (uint32_t)tloc_h.size(); ++t) {
-      const uint32_t off = tloc_h[t].x, cnt = tloc_h[t].y;
-      bool whole = cnt > 0 && cnt % 6u == 0;
-      for (uint32_t i = 0; whole && i < cnt; i += 6)
-        whole = tcols_h[off + i] % 6u == 0 && tcols_h[off + i + 5] == tcols_h[off + i] + 5u;
-      if (whole) {
-        goff_h[t] = (uint32_t)tgrp_h.size() + 1u;
-        for (uint32_t i = 0; i < cnt; i += 6) tgrp_h.push_back(tcols_h[off + i] / 6u);
-        staged_columns += cnt / 6u;
-      } else {
-        staged_columns += cnt;
-      }
-    }
-  }
-  auto* s = new hbj_system();
-  s->ctx = ctx;
-  s->n = n;
-  {
-    uint64_t csr_entries = 0;
-    for (uint32_t t = 0; t < (uint32_t)tloc_h.size(); ++t) {
-      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
-      const uint64_t e = row_ptr[r1] - row_ptr[r0];
-      if (tloc_h[t].y == kLocalOff) {
-        csr_entries += e;
-      } else {
-        s->local_tiles += 1;
-        s->local_columns += tloc_h[t].y;
-        s->spmv_layout_bytes += 6 * e + 4ull * tloc_h[t].y + sizeof(TileHdr);
-      }
-    }
-    s->spmv_layout_bytes += 8 * csr_entries + 12ull * n + 4;
-  }
-  s->nnz = nnz;
-  s->nb = nb;
-  s->n_tiles = (n + kTile - 1) / kTile;
-  bool blocks6 = nb > 0 && block_starts[nb] == n && n <= 6ull * nb && n > 6ull * (nb - 1);
-  for (uint32_t b = 0; b < nb; ++b) {
-    if (block_starts[b + 1] - block_starts[b] > (uint32_t)kBlock) s->flags |= kFlagOversize;
-    if (block_starts[b] != 6ull * b) blocks6 = false;
-  }
-  if (blocks6) s->flags |= kFlagBlocks6;
-  if (sliced) {
-    s->flags |= kFlagSliced;
-    s->spmv_sliced_bytes = 4ull * sval_h.size() + 4ull * runs_h.size() +
-                           (sizeof(SliceHdr) * kWaves + sizeof(TileHdr)) * s->n_tiles +
-                           4ull * staged_columns + 8ull * n;
-  }
+  SystemImage im;
+  CHK(prepare_image(n, nnz, row_ptr, col_idx, values, block_starts, n_starts, false, im));
+  const uint32_t nb = im.nb;
+  hbj_system* s = new_system(ctx, im);
   auto fail = [&](hbj_status st) {
     s->release();
     delete s;
@@ -437,16 +499,9 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   }
   const uint32_t nt = s->n_tiles ? s->n_tiles : 1;
   BlockLayout L;
-  s->layout(L, n, nnz, n_starts, tcols_h.size(), sliced ? sval_h.size() : 0u,
-            sliced ? runs_h.size() : 0u, tgrp_h.size(), with_ir);
-  if (arena) {
-    if (!arena->take(L.size, &s->arena_off)) {
-      set_error("device arena: no free range of %llu bytes", (unsigned long long)L.size);
-      return fail(HBJ_E_OOM);
-    }
-    s->arena = arena;
-    s->block = arena->base + s->arena_off;
-  } else {
+  s->layout(L, n, nnz, n_starts, im.tcols.size(), im.sliced() ? im.sval.size() : 0u,
+            im.sliced() ? im.runs.size() : 0u, im.tgrp.size(), with_ir);
+  {
     hbj_status st = dalloc(&s->block, L.size);
     if (st != HBJ_OK) return fail(st);
   }
@@ -461,38 +516,16 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
   up(s->col, col_idx, 4ull * nnz);
   up(s->val, values, 4ull * nnz);
   up(s->bs, block_starts, 4ull * n_starts);
-  up(s->lidx, lidx_h.data(), sizeof(uint16_t) * lidx_h.size());
-  {
-    std::vector<TileHdr> th(tloc_h.size());
-    for (uint32_t t = 0; t < (uint32_t)tloc_h.size(); ++t) {
-      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n);
-      TileHdr& h = th[t];
-      h.off = tloc_h[t].x;
-      h.cnt = tloc_h[t].y;
-      for (uint32_t k = 0; k < 4; ++k) h.eb[k] = row_ptr[std::min<uint32_t>(r0 + 64u * k, r1)];
-      h.eb[4] = row_ptr[r1];
-      h.goff = goff_h[t];
-    }
-    up(s->thdr, th.data(), sizeof(TileHdr) * th.size());
+  up(s->lidx, im.lidx.data(), sizeof(uint16_t) * im.lidx.size());
+  up(s->thdr, im.thdr.data(), sizeof(TileHdr) * im.thdr.size());
+  up(s->tcols, im.tcols.data(), sizeof(uint32_t) * im.tcols.size());
+  if (im.sliced()) {
+    up(s->sval, im.sval.data(), sizeof(float) * im.sval.size());
+    up(s->runs, im.runs.data(), sizeof(uint32_t) * im.runs.size());
+    up(s->shdr, im.shdr.data(), sizeof(SliceHdr) * im.shdr.size());
+    up(s->tgrp, im.tgrp.data(), sizeof(uint32_t) * im.tgrp.size());
   }
-  up(s->tcols, tcols_h.data(), sizeof(uint32_t) * tcols_h.size());
-  if (sliced) {
-    up(s->sval, sval_h.data(), sizeof(float) * sval_h.size());
-    up(s->runs, runs_h.data(), sizeof(uint32_t) * runs_h.size());
-    up(s->shdr, shdr_h.data(), sizeof(SliceHdr) * shdr_h.size());
-    up(s->tgrp, tgrp_h.data(), sizeof(uint32_t) * tgrp_h.size());
-  }
-  {
-    // per tile: the first block touching rows [256t, 256t+256) and how many do
-    std::vector<uint2> tb(nt, make_uint2(0u, 0u));
-    for (uint32_t t = 0; t < s->n_tiles; ++t) {
-      const uint32_t r0 = t * kTile, r1 = std::min<uint32_t>(r0 + kTile, n) - 1;
-      const uint32_t bf = (uint32_t)(std::upper_bound(block_starts, block_starts + n_starts, r0) - block_starts) - 1;
-      const uint32_t bl = (uint32_t)(std::upper_bound(block_starts, block_starts + n_starts, r1) - block_starts) - 1;
-      tb[t] = make_uint2(bf, bl - bf + 1);
-    }
-    up(s->tile_blk, tb.data(), sizeof(uint2) * nt);
-  }
+  up(s->tile_blk, im.tile_blk.data(), sizeof(uint2) * nt);
   // LU blocks on the device from the uploaded CSR (compute.rs:84-181; the
   // block starts were validated above, so the only error left is a zero
   // pivot, reported for the first failing block as the host builder does)
@@ -529,6 +562,75 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
     return fail(HBJ_E_DEVICE);
   }
   *out = s;
+  return HBJ_OK;
+}
+
+hbj_status system_place(hbj_ctx* ctx, const SystemImage& im, const uint32_t* row_ptr,
+                        const uint32_t* col_idx, const float* values, const uint32_t* block_starts,
+                        const double* b64, DeviceArena* arena, uint64_t off, char* staging,
+                        hipStream_t stream, hipEvent_t done, hbj_system** out, uint64_t* copied) {
+  *out = nullptr;
+  *copied = 0;
+  hbj_system* s = new_system(ctx, im);
+  const uint32_t n = im.n, nnz = im.nnz, nt = im.n_tiles ? im.n_tiles : 1u;
+  BlockLayout L;
+  uint64_t data_end = 0;
+  if (im.no_csr()) {
+    s->layout_lean(L, im, &data_end);
+  } else {
+    s->layout(L, n, nnz, im.n_starts, im.tcols.size(), im.sliced() ? im.sval.size() : 0u,
+              im.sliced() ? im.runs.size() : 0u, im.tgrp.size(), true);
+    data_end = L.size;  // the whole block goes through the staging image
+    std::memset(staging, 0, L.size);
+  }
+  s->arena = arena;
+  s->arena_off = off;
+  s->block = arena->base + off;
+  s->block_bytes = L.size;
+  s->bytes = L.size;
+  // the pieces' offsets in the staging image: the block's own layout
+  L.assign(staging);
+  auto put = [&](void* dst, const void* src, size_t bytes) {
+    if (dst && bytes) std::memcpy(dst, src, bytes);
+  };
+  put(s->bs, block_starts, 4ull * im.n_starts);
+  put(s->lu, im.lu.data(), 4ull * 36u * im.nb);
+  put(s->tile_blk, im.tile_blk.data(), sizeof(uint2) * nt);
+  put(s->thdr, im.thdr.data(), sizeof(TileHdr) * im.thdr.size());
+  put(s->tcols, im.tcols.data(), sizeof(uint32_t) * im.tcols.size());
+  if (im.sliced()) {
+    put(s->sval, im.sval.data(), sizeof(float) * im.sval.size());
+    put(s->runs, im.runs.data(), sizeof(uint32_t) * im.runs.size());
+    put(s->shdr, im.shdr.data(), sizeof(SliceHdr) * im.shdr.size());
+    put(s->tgrp, im.tgrp.data(), sizeof(uint32_t) * im.tgrp.size());
+  }
+  put(s->b64, b64, 8ull * n);
+  if (!im.no_csr()) {  // the CSR path keeps the arrays (and their zero padding)
+    put(s->rp, row_ptr, 4ull * (n + 1));
+    put(s->col, col_idx, 4ull * nnz);
+    put(s->val, values, 4ull * nnz);
+    put(s->lidx, im.lidx.data(), sizeof(uint16_t) * im.lidx.size());
+  }
+  // the device block: the same offsets from its base
+  L.assign(s->block);
+  if (!im.no_csr()) {  // the FP64 check's descriptor (device pointers), in the image
+    const SysDesc d1 = s->desc(false);
+    std::memcpy(staging + (reinterpret_cast<char*>(s->d_desc1) - s->block), &d1, sizeof d1);
+  }
+  hipError_t e = hipMemcpyAsync(s->block, staging, data_end, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && L.size > data_end)
+    e = hipMemsetAsync(s->block + data_end, 0, L.size - data_end, stream);
+  if (e == hipSuccess) e = hipEventRecord(done, stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("upload failed: %s", hipGetErrorString(e));
+    s->arena = nullptr;  // the caller returns the block
+    s->block = nullptr;
+    delete s;
+    return HBJ_E_DEVICE;
+  }
+  *out = s;
+  *copied = data_end;
   return HBJ_OK;
 }
 

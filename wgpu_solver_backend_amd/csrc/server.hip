@@ -1,15 +1,25 @@
 // server.hip -- continuous-batching request loop (hbj_server_*).
 //
 // Two host threads share one device:
-//   admitter  takes queued requests in arrival order while the HBM budget
-//             and the slot count allow, and turns each into a resident
-//             system (validation, tile tables, uploads, LU blocks, FP64 b)
-//             in a block of the server's device arena -- this host work
-//             overlaps the iterations already running on the GPU;
+//   admitter  takes queued requests in arrival order.  For each it first
+//             builds the system's host image (tile tables, the sliced SpMV
+//             layout, the LU blocks, on the host), which fixes the exact
+//             size of its device block; then, once the HBM budget and the
+//             slot count allow, places the block in the server's device
+//             arena and uploads it: the image goes into pinned staging
+//             (two buffers, alternating), one asynchronous copy moves it on
+//             the server's upload stream and an event marks its end.  A
+//             system with the sliced layout keeps only that layout on the
+//             device (no CSR arrays, no per-entry indices).  This host work
+//             and the copies overlap the iterations running on the GPU;
 //   worker    owns the slot table.  It enqueues chunks of PCG iterations
 //             (one launch of each kernel covers every resident request) and
 //             always has the next chunk queued before it waits for the
-//             previous one's status poll.  Arrivals and retirements change
+//             previous one's status poll.  A new request's slot writes wait
+//             on its upload event in the engine's stream (not on the host).
+//             The SpMV reads the values through the caches (variant 5) while
+//             the resident systems' layouts fit the last-level cache, and
+//             non-temporally (variant 4) otherwise.  Arrivals and retirements change
 //             the table between two enqueues, without waiting for the
 //             device: the slot and tile writes are copies on the engine's
 //             stream, so they land between the chunk in flight and the next.
@@ -36,9 +46,10 @@ struct Request {
   std::vector<float> val;
   std::vector<double> b64;
   hbj_ir_params params{};
-  uint64_t est_bytes = 0;
+  uint64_t est_bytes = 0;      // the exact device block, once the image is built
   double t_submit = 0, t_done = 0;
   hbj_system* sys = nullptr;
+  hipEvent_t up_ev = nullptr;  // the end of the system's upload
   int slot = -1;
   bool done = false;
   int32_t status = HBJ_OK;
@@ -69,7 +80,23 @@ struct hbj_server {
   bool failed = false;            // the worker stopped on a device failure
   std::thread admitter, worker;
   Engine eng;
+  // uploads: the admitter's stream, pinned staging buffers (each free once
+  // its event has completed) and the buffers they outgrew (freed at the end:
+  // hipHostFree waits for all device work)
+  hipStream_t up_stream = nullptr;
+  char* stg[2] = {nullptr, nullptr};
+  uint64_t stg_cap[2] = {0, 0};
+  hipEvent_t stg_ev[2] = {nullptr, nullptr};
+  bool stg_used[2] = {false, false};
+  int stg_next = 0;
+  std::vector<char*> stg_old;
 };
+
+// The SpMV reads the values through the caches (variant 5) while the
+// resident systems' sliced layouts total at most this many bytes (the
+// last-level cache holds 256 MB; the vectors the iterations stream need the
+// rest), non-temporally (variant 4) above it.
+constexpr uint64_t kCachedLayoutBytes = 160ull << 20;
 
 namespace {
 
@@ -82,52 +109,125 @@ void finish(hbj_server* S, Request* r, int32_t status, const std::string& err) {
   S->cv_done.notify_all();
 }
 
+// A pinned staging buffer of at least `bytes` that no upload still reads.
+hbj_status staging_take(hbj_server* S, uint64_t bytes, char** out, int* slot) {
+  const int k = S->stg_next;
+  S->stg_next ^= 1;
+  if (S->stg_used[k]) HIPCHK(hipEventSynchronize(S->stg_ev[k]));
+  if (S->stg_cap[k] < bytes) {
+    if (S->stg[k]) S->stg_old.push_back(S->stg[k]);
+    S->stg[k] = nullptr;
+    S->stg_cap[k] = 0;
+    const uint64_t cap = bytes + bytes / 4 + (1ull << 20);
+    CHK(halloc(&S->stg[k], cap));
+    S->stg_cap[k] = cap;
+  }
+  *out = S->stg[k];
+  *slot = k;
+  return HBJ_OK;
+}
+
 void admitter_main(hbj_server* S) {
   (void)hipSetDevice(S->ctx->device);
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
-    S->cv_admit.wait(lk, [&] {
-      if (S->failed || (S->stop && S->queue.empty())) return true;
-      if (S->queue.empty()) return false;
-      if (S->arena_blocked && S->arena_gives == S->blocked_at) return false;
-      const Request* h = S->queue.front();
-      return S->resident < S->cfg.max_active && S->bytes_in_use + h->est_bytes <= S->cfg.hbm_budget_bytes;
-    });
+    S->cv_admit.wait(lk, [&] { return S->failed || S->stop || !S->queue.empty(); });
     if (S->failed || S->queue.empty()) break;  // device failure, or stop requested and nothing left
     Request* r = S->queue.front();
     S->queue.pop_front();
-    S->bytes_in_use += r->est_bytes;
-    S->resident += 1;
-    S->arena_blocked = false;
     lk.unlock();
+    // the host image: tables, sliced layout, LU blocks (no device memory)
     const double t0 = now_ms();
-    hbj_system* sys = nullptr;
-    hbj_status s = system_create(S->ctx, r->n, r->nnz, r->rp.data(), r->col.data(), r->val.data(),
-                                 r->bs.data(), r->n_starts, true, &S->arena, &sys);
-    if (s == HBJ_OK) s = hbj_system_set_rhs64(sys, r->b64.data(), nullptr);
+    SystemImage im;
+    hbj_status s = prepare_image(r->n, r->nnz, r->rp.data(), r->col.data(), r->val.data(),
+                                 r->bs.data(), r->n_starts, true, im);
     std::string err = s == HBJ_OK ? "" : hbj_last_error();
-    const double t1 = now_ms();
+    const uint64_t need = s == HBJ_OK ? lean_block_bytes(im) : 0u;
     lk.lock();
-    S->st.admit_ms += t1 - t0;
+    S->st.admit_ms += now_ms() - t0;
     if (s != HBJ_OK) {
-      if (sys) hbj_system_destroy(sys);
-      S->bytes_in_use -= r->est_bytes;
-      S->resident -= 1;
-      if (s == HBJ_E_OOM && !sys && S->resident > 0) {
-        // the arena has no free range this large right now (its free space
-        // is split): retry once a resident system has given its block back
-        S->queue.push_front(r);
-        S->arena_blocked = true;
-        S->blocked_at = S->arena_gives;
-        S->st.arena_waits += 1;
-        continue;
+      finish(S, r, s, err);
+      continue;
+    }
+    if (need > S->cfg.hbm_budget_bytes) {
+      S->st.rejected += 1;
+      char msg[200];
+      snprintf(msg, sizeof msg, "request needs %llu device bytes, above the admission budget of %llu",
+               (unsigned long long)need, (unsigned long long)S->cfg.hbm_budget_bytes);
+      finish(S, r, HBJ_E_REJECTED, msg);
+      continue;
+    }
+    r->est_bytes = need;
+    // budget, a slot, and after a failed placement a block given back
+    uint64_t off = 0;
+    bool placed = false;
+    while (!S->failed) {
+      S->cv_admit.wait(lk, [&] {
+        if (S->failed) return true;
+        if (S->arena_blocked && S->arena_gives == S->blocked_at) return false;
+        return S->resident < S->cfg.max_active && S->bytes_in_use + need <= S->cfg.hbm_budget_bytes;
+      });
+      if (S->failed) break;
+      S->arena_blocked = false;
+      if (S->arena.take(need, &off)) {
+        placed = true;
+        break;
       }
+      // the arena's free space is split: wait until a resident system gives
+      // its block back (there is one: an empty arena holds the whole budget)
+      S->arena_blocked = true;
+      S->blocked_at = S->arena_gives;
+      S->st.arena_waits += 1;
+    }
+    if (!placed) {
+      finish(S, r, HBJ_E_DEVICE, "server stopped on a device failure");
+      break;
+    }
+    S->bytes_in_use += need;
+    S->resident += 1;
+    lk.unlock();
+    // upload: the image into pinned staging, one asynchronous copy
+    const double t1 = now_ms();
+    hbj_system* sys = nullptr;
+    hipEvent_t ev = nullptr;
+    char* stg = nullptr;
+    int slot = 0;
+    uint64_t copied = 0;
+    s = staging_take(S, need, &stg, &slot);
+    if (s == HBJ_OK && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipEventCreate failed");
+      s = HBJ_E_DEVICE;
+    }
+    if (s == HBJ_OK)
+      s = system_place(S->ctx, im, r->rp.data(), r->col.data(), r->val.data(), r->bs.data(),
+                       r->b64.data(), &S->arena, off, stg, S->up_stream, ev, &sys, &copied);
+    if (s == HBJ_OK && hipEventRecord(S->stg_ev[slot], S->up_stream) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipEventRecord failed");
+      s = HBJ_E_DEVICE;
+    }
+    if (s == HBJ_OK) S->stg_used[slot] = true;
+    err = s == HBJ_OK ? "" : hbj_last_error();
+    lk.lock();
+    S->st.admit_ms += now_ms() - t1;
+    if (s != HBJ_OK) {
+      if (sys) {
+        sys->arena = nullptr;
+        delete sys;
+      }
+      if (ev) (void)hipEventDestroy(ev);
+      S->arena.give(off, need);
+      S->bytes_in_use -= need;
+      S->resident -= 1;
       finish(S, r, s, err);
       S->cv_admit.notify_all();
       continue;
     }
-    // host copies are no longer needed once the system is resident
+    S->st.upload_bytes += copied;
+    // host copies are no longer needed once the image is staged
     r->sys = sys;
+    r->up_ev = ev;
     std::vector<uint32_t>().swap(r->rp);
     std::vector<uint32_t>().swap(r->col);
     std::vector<uint32_t>().swap(r->bs);
@@ -164,6 +264,8 @@ hbj_status worker_loop(hbj_server* S) {
   int cur = 0;
   uint32_t active = 0;
   bool table_dirty = false;
+  uint64_t cached_chunks = 0;
+  E.set_spmv_variant(4u);
 
   // retire every slot whose polled status is final
   auto harvest = [&](const SysState* polled, uint64_t chunk_id) -> hbj_status {
@@ -183,6 +285,8 @@ hbj_status worker_loop(hbj_server* S) {
       // the chunks still queued no longer write this system (its status is final)
       CHK(hbj_system_get_x64(r->sys, r->x.data()));
       graves.push_back(Grave{r->sys, r->est_bytes, next_chunk - 1});
+      if (r->up_ev) (void)hipEventDestroy(r->up_ev);  // long complete: the system ran
+      r->up_ev = nullptr;
       std::lock_guard<std::mutex> g(S->mu);
       r->res = res;
       r->res.solve_ms = now_ms() - r->t_submit;
@@ -246,14 +350,23 @@ hbj_status worker_loop(hbj_server* S) {
         slots[slot] = r;
         since[slot] = next_chunk;
         r->slot = (int)slot;
+        // the slot's writes and every later chunk come after the upload
+        HIPCHK(hipStreamWaitEvent(E.stream, r->up_ev, 0));
         CHK(E.stage_slot(cur, slot, r->sys->desc(true), r->sys->init_state(true, 0, 0, 0, &r->params)));
         ++active;
       }
       std::vector<uint2> tiles;
+      uint64_t layouts = 0;
       for (uint32_t i = 0; i < cap; ++i)
-        if (slots[i])
+        if (slots[i]) {
           for (uint32_t t = 0; t < slots[i]->sys->n_tiles; ++t) tiles.push_back(make_uint2(i, t));
+          layouts += slots[i]->sys->spmv_sliced_bytes ? slots[i]->sys->spmv_sliced_bytes
+                                                      : slots[i]->sys->spmv_layout_bytes;
+        }
       CHK(E.stage_tiles(cur, tiles));
+      // the resident layouts' bytes pick the SpMV's cache policy (variants 4
+      // and 5 are the only ones a system without CSR arrays can run)
+      E.set_spmv_variant(layouts <= kCachedLayoutBytes ? 5u : 4u);
       table_dirty = false;
       staged = true;
       std::lock_guard<std::mutex> g(S->mu);
@@ -268,6 +381,7 @@ hbj_status worker_loop(hbj_server* S) {
       CHK(E.enqueue_poll(T, cur));
       enq = cur;
       enq_id = next_chunk++;
+      if (E.spmv_variant == 5u) cached_chunks += 1;
     } else if (staged) {
       CHK(E.sync_stream());  // nothing follows the staged copies: let them land
     }
@@ -285,6 +399,7 @@ hbj_status worker_loop(hbj_server* S) {
     std::lock_guard<std::mutex> g(S->mu);
     S->st.active = active;
     if (enq >= 0) S->st.chunks += 1;
+    S->st.chunks_cached_spmv = cached_chunks;
   }
   return HBJ_OK;
 }
@@ -303,20 +418,28 @@ void worker_main(hbj_server* S) {
   }
 }
 
-// The block a request's system takes in the arena, with its tile-local
-// column lists and its sliced layout at their largest (min(nnz, kLocalMax)
-// columns per tile; build_sliced's padding bound and kMaxRuns runs per row):
-// an upper bound of the block system_create carves once they are built.
-uint64_t estimate_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
-  const uint64_t nt = n ? (n + kTile - 1) / kTile : 1u;
-  const uint64_t tcols = std::min<uint64_t>(nnz ? nnz : 1u, (uint64_t)kLocalMax * nt);
-  const uint64_t sval = (uint64_t)nnz + nnz / 4 + 1024ull * nt;
-  const uint64_t runs = (uint64_t)kMaxRuns * kTile * nt;
-  const uint64_t tgrp = tcols / 6u + nt;
-  hbj_system probe;
-  BlockLayout L;
-  probe.layout(L, n, nnz, n_starts, tcols, sval, runs, tgrp, true);
-  return (L.size + 255) & ~255ull;
+// The upload stream, staging buffers and their events (after the stream's
+// copies are done).
+void release_uploads(hbj_server* S) {
+  if (S->up_stream) (void)hipStreamSynchronize(S->up_stream);
+  for (int k = 0; k < 2; ++k) {
+    hfree(S->stg[k]);
+    if (S->stg_ev[k]) (void)hipEventDestroy(S->stg_ev[k]);
+    S->stg_ev[k] = nullptr;
+  }
+  for (char*& p : S->stg_old) hfree(p);
+  S->stg_old.clear();
+  if (S->up_stream) (void)hipStreamDestroy(S->up_stream);
+  S->up_stream = nullptr;
+}
+
+// Device bytes no request of these sizes can go below (its values once,
+// its LU blocks and block starts, the right-hand side and the solver's
+// vectors): requests above the budget by this much are rejected at once,
+// the others once their exact block is known.
+uint64_t min_request_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
+  const uint64_t nb = n_starts ? n_starts - 1u : 0u;
+  return 4ull * nnz + 76ull * n + 144ull * nb + 4ull * n_starts;
 }
 
 }  // namespace
@@ -354,7 +477,18 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   }
   // the resident systems' device memory: one allocation of the budget
   if (s == HBJ_OK) s = S->arena.init(cfg->hbm_budget_bytes);
+  // uploads: a stream of their own and the staging buffers' events
+  if (s == HBJ_OK) {
+    e = hipStreamCreateWithFlags(&S->up_stream, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&S->stg_ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      set_error("hbj_server_create: %s", hipGetErrorString(e));
+      s = HBJ_E_DEVICE;
+    }
+  }
   if (s != HBJ_OK) {
+    release_uploads(S);
     S->eng.destroy();
     S->arena.destroy();
     delete S;
@@ -383,8 +517,10 @@ void hbj_server_destroy(hbj_server* S) {
   (void)hipSetDevice(S->ctx->device);
   for (auto& kv : S->reqs) {
     if (kv.second->sys) hbj_system_destroy(kv.second->sys);
+    if (kv.second->up_ev) (void)hipEventDestroy(kv.second->up_ev);
     delete kv.second;
   }
+  release_uploads(S);
   S->eng.destroy();
   S->arena.destroy();
   delete S;
@@ -409,17 +545,17 @@ hbj_status hbj_server_submit(hbj_server* S, uint32_t n, uint32_t nnz, const uint
   r->n = n;
   r->nnz = nnz;
   r->n_starts = n_starts;
-  r->est_bytes = estimate_bytes(n, nnz, n_starts);
+  const uint64_t min_bytes = min_request_bytes(n, nnz, n_starts);
   std::lock_guard<std::mutex> g(S->mu);
   if (S->stop) {
     delete r;
     set_error("hbj_server_submit: server is shutting down");
     return HBJ_E_INVALID;
   }
-  if (r->est_bytes > S->cfg.hbm_budget_bytes) {
+  if (min_bytes > S->cfg.hbm_budget_bytes) {
     S->st.rejected += 1;
-    set_error("request needs %llu device bytes, above the admission budget of %llu",
-              (unsigned long long)r->est_bytes, (unsigned long long)S->cfg.hbm_budget_bytes);
+    set_error("request needs at least %llu device bytes, above the admission budget of %llu",
+              (unsigned long long)min_bytes, (unsigned long long)S->cfg.hbm_budget_bytes);
     delete r;
     return HBJ_E_REJECTED;
   }
@@ -465,8 +601,19 @@ hbj_status hbj_server_wait(hbj_server* S, uint64_t ticket, double* x, hbj_result
   return (hbj_status)st;
 }
 
-uint64_t hbj_server_request_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
-  return estimate_bytes(n, nnz, n_starts);
+uint64_t hbj_server_request_bytes(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                                  const uint32_t* col_idx, const float* values,
+                                  const uint32_t* block_starts, uint32_t n_starts) {
+  clear_error();
+  if (!row_ptr || !block_starts || (nnz && (!col_idx || !values))) {
+    set_error("hbj_server_request_bytes: bad argument");
+    return 0;
+  }
+  if (validate_case(n, nnz, row_ptr, col_idx, block_starts, n_starts) != HBJ_OK) return 0;
+  SystemImage im;
+  if (prepare_image(n, nnz, row_ptr, col_idx, values, block_starts, n_starts, true, im) != HBJ_OK)
+    return 0;
+  return lean_block_bytes(im);
 }
 
 hbj_status hbj_server_stats_get(hbj_server* S, hbj_server_stats* out) {

@@ -384,7 +384,7 @@ hbj_status hbj_build_lu_blocks_6(uint32_t n, const uint32_t* row_ptr, uint32_t r
       std::memcpy(lu_out + 36ull * blk, mat, sizeof mat);
     }
   };
-  unsigned nt = nb >= 65536 ? std::thread::hardware_concurrency() : 1u;
+  unsigned nt = nb >= 8192 ? std::thread::hardware_concurrency() : 1u;
   if (nt == 0) nt = 1;
   if (nt > 16) nt = 16;
   std::vector<int64_t> fb(nt, -1);

@@ -154,13 +154,14 @@ def run_batch(H, ctx, args, steps, warmup):
     iters = [r.iterations for r in results]
     outer = [r.outer_iterations for r in results]
     # the per-system path: each system solved alone, one after another
-    t1 = time.perf_counter()
-    same = True
-    for L, s, r in zip(Ls, systems, results):
-        _, ra = s.solve_ir(L["rhs"].astype(np.float64), None, params)
-        same &= ra.iterations == r.iterations
-    t_alone = time.perf_counter() - t1
-    alone_sps = len(systems) / t_alone
+    same, alone_sps = None, None
+    if not args.skip_per_system:
+        t1 = time.perf_counter()
+        same = True
+        for L, s, r in zip(Ls, systems, results):
+            _, ra = s.solve_ir(L["rhs"].astype(np.float64), None, params)
+            same &= ra.iterations == r.iterations
+        alone_sps = len(systems) / (time.perf_counter() - t1)
     ns = [L["n"] for L in Ls]
     out = {
         "value": value, "elapsed": elapsed, "step_ms": step_ms, "latency_ms": lat,
@@ -173,8 +174,8 @@ def run_batch(H, ctx, args, steps, warmup):
             "worst_true_rel_residual_fp64_host": float(max(host_res)),
             "all_verified_1e-8": bool(max(host_res) <= 1e-8 * 1.001),
             "per_system_path_solves_per_s": alone_sps,
-            "speedup_vs_per_system": value / alone_sps,
-            "iterations_identical_alone_vs_batched": bool(same),
+            "speedup_vs_per_system": value / alone_sps if alone_sps else None,
+            "iterations_identical_alone_vs_batched": same,
             "spmv_variant": best, "autotune_us_per_iter": [float(u) for u in tune_us],
         },
     }
@@ -298,11 +299,18 @@ def run_plain(H, ctx, N, steps, warmup, alg="pcg", sample_every=4):
     system.set_kernel_timing(0)
     b1, b2, b3 = (kernel_bytes_one_reduction if one_red else kernel_bytes)(n, nnz, nb)
     sb, layout = spmv_stream_bytes(info, best)
+    two = info["launches_per_iteration"] == 2 and not one_red
+    if two:
+        # the SpMV launch also finishes the previous iteration: it reads z and
+        # writes the new p for its own rows (p_prev replaces the own p it reads)
+        b1, sb, b3 = b1 + 8 * n, sb + 8 * n, 0
     gbps = lambda b, us: b / (us * 1e-6) / 1e9 if us > 0 else None
     out = {"L": L, "value": steps / elapsed, "elapsed": elapsed, "lat": lat, "res": res,
            "best": best, "tune_us": tune_us, "info": info, "avg_us": avg_us, "cnt": cnt,
            "n": n, "nnz": nnz,
            "spmv": {"variant": best, "layout": layout, "stream_bytes": sb,
+                    "kernel": "k_ccg_spmv" if one_red else ("k_pcg_fspmv" if two else "k_pcg_spmv"),
+                    "launches_per_iteration": info["launches_per_iteration"],
                     "us": float(avg_us[0]), "gbps": gbps(sb, avg_us[0]),
                     "csr_bytes": b1, "csr_gbps": gbps(b1, avg_us[0]),
                     "autotune_us_per_iter": [float(x) for x in tune_us],
@@ -430,15 +438,16 @@ def cpu_baseline_plain(L, iters_gpu, sample_iters, sample_iters_1core):
 def roofline_from_plain(P, N, hbm_only=None):
     sp_ = P["spmv"]
     traffic = traffic_from_profiles(f"plain_N{N}")
-    t_bytes = ((traffic or {}).get("k_pcg_spmv_bytes_by_variant") or {}).get(str(P["best"]))
-    r = {"bound": "hbm", "kernel": "k_pcg_spmv", "workload": f"plain FP32 PCG SpMV, {N}^3 lattice",
+    t_bytes = ((traffic or {}).get(f"{sp_['kernel']}_bytes_by_variant") or {}).get(str(P["best"]))
+    r = {"bound": "hbm", "kernel": sp_["kernel"], "workload": f"plain FP32 PCG SpMV, {N}^3 lattice",
          "achieved": sp_["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
          "frac": sp_["gbps"] / HBM_PEAK_GBPS, "traffic": t_bytes,
          "algorithmic_bytes": sp_["stream_bytes"], "kernel_us": sp_["us"],
          "csr_accounting": {"bytes": sp_["csr_bytes"], "gbps": sp_["csr_gbps"],
                             "frac": sp_["csr_gbps"] / HBM_PEAK_GBPS},
          "note": (f"achieved = the bytes SpMV variant {P['best']} streams per launch ({sp_['layout']} "
-                  "layout: values, runs, column lists, headers, the rows' own p and q) / its mean "
+                  "layout: values, runs, column lists, headers, the rows' own p and q; with two "
+                  "launches per iteration also the rows' z and new p) / its mean "
                   "duration from HIP events on its own stream; csr_accounting divides the "
                   "reference's CSR bytes (8 nnz + 12 n + 4) by the same time; traffic = FETCH_SIZE + "
                   "WRITE_SIZE per launch from rocprofv3 (profiles/), which cannot separate "
@@ -528,7 +537,6 @@ def bench_plain(args):
            "spmv": P["spmv"], "iteration": P["iteration"],
            "roofline": roofline_from_plain(P, args.N)}
     if one_red:
-        out["roofline"]["kernel"] = "k_ccg_spmv"
         out["roofline"]["traffic"] = None
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_plain(P["L"], int(P["res"].iterations), args.cpu_iters,
@@ -669,6 +677,8 @@ def main():
                     help="counter runs: plain solves capped at this many iterations, no headline line")
     ap.add_argument("--variant", type=int, default=None, help="profile: SpMV variant (default: autotuned)")
     ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--skip-per-system", action="store_true",
+                    help="batch: do not also solve each system alone (profiled runs)")
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--budget-gb", type=float, default=64.0)

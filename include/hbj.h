@@ -199,6 +199,12 @@ typedef struct {
                                  of whole groups), the tile and slice headers, the
                                  rows' own x and q; 0 when the system has no
                                  sliced layout */
+  uint32_t launches_per_iteration; /* kernel launches per PCG iteration of its
+                                      solves: 2 (the previous iteration's finish
+                                      folded into the SpMV; the default), 3 (SpMV,
+                                      update, finish: HBJ_FUSED=0), 2 for the
+                                      one-reduction PCG */
+  uint32_t reserved0;
 } hbj_system_info;
 
 /* Algorithm of a system's plain solves (hbj_solve, hbj_system_solve_resident,

@@ -50,7 +50,8 @@ class hbj_system_info(ctypes.Structure):
     _fields_ = [("n", c_u32), ("nnz", c_u32), ("n_blocks", c_u32), ("n_tiles", c_u32),
                 ("local_tiles", c_u32), ("spmv_variant", c_u32), ("local_columns", c_u64),
                 ("device_bytes", c_u64), ("spmv_csr_bytes", c_u64), ("spmv_layout_bytes", c_u64),
-                ("spmv_sliced_bytes", c_u64)]
+                ("spmv_sliced_bytes", c_u64), ("launches_per_iteration", c_u32),
+                ("reserved0", c_u32)]
 
 
 class hbj_compare_stats(ctypes.Structure):

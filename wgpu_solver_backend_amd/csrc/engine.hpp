@@ -91,6 +91,14 @@ inline uint32_t env_u32(const char* name, uint32_t dflt) {
 // inside hipGraphLaunch (a read of an unmapped address by the runtime's
 // intercepted dispatch path; the same graphs replay correctly without the
 // tool), so the engine launches the kernels directly while one is attached.
+// The two-launch PCG iteration (k_pcg_fspmv + k_pcg_update), the default;
+// HBJ_FUSED=0 selects the three-launch one (k_pcg_spmv, k_pcg_update,
+// k_pcg_finish).  Same arithmetic, same bits.  Read once per process.
+inline bool fused_mode() {
+  static const bool f = env_u32("HBJ_FUSED", 1) != 0;
+  return f;
+}
+
 inline bool profiler_tool_attached() {
   const char* v = std::getenv("ROCP_TOOL_LIBRARIES");
   return v && *v;
@@ -293,14 +301,21 @@ struct Engine {
     }
   }
 
-  // one PCG iteration (parity par) of every system in the table: SpMV,
-  // update, finish (one-reduction PCG: SpMV, update)
+  // one PCG iteration (parity par) of every system in the table: the
+  // previous finish with this SpMV, update (three-launch form: SpMV,
+  // update, finish; one-reduction PCG: SpMV, update)
+  bool fused = fused_mode();
   void launch_iter(uint32_t par) {
     LaunchSet l = ls();
     l.par = par;
     if (alg == 1) {
       launch_ccg_spmv(l, stream);
       launch_ccg_update(l, stream);
+      return;
+    }
+    if (fused) {
+      launch_pcg_fspmv(l, stream);
+      launch_pcg_update(l, stream);
       return;
     }
     launch_pcg_spmv(l, stream);
@@ -356,6 +371,9 @@ struct Engine {
           if (alg == 1) {  // two kernels: the third sample stays empty
             timed(0, launch_ccg_spmv);
             timed(1, launch_ccg_update);
+          } else if (fused) {
+            timed(0, launch_pcg_fspmv);
+            timed(1, launch_pcg_update);
           } else {
             timed(0, launch_pcg_spmv);
             timed(1, launch_pcg_update);
@@ -387,8 +405,8 @@ struct Engine {
       std::array<double, kKernels + 1> smp;
       smp[0] = (double)tev_trip[pool][u];
       for (int k = 0; k < kKernels; ++k) {
-        smp[k + 1] = -1.0;  // no launch of this kernel (one-reduction PCG: no third)
-        if (alg == 1 && k == 2) continue;
+        smp[k + 1] = -1.0;  // no launch of this kernel (two-launch iterations: no third)
+        if ((alg == 1 || fused) && k == 2) continue;
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e[2 * k], e[2 * k + 1]));
         smp[k + 1] = ms * 1000.0;
@@ -451,9 +469,9 @@ struct Engine {
     // below never polls past it, whatever the device reports.
     uint64_t max_trips = 0;
     for (uint32_t i = 0; i < n_sys; ++i) {
-      const uint64_t t = init[i].phase == kPhIrResid
-                             ? (uint64_t)(init[i].max_outer + 1) * (init[i].max_iter + 3) + 4
-                             : (uint64_t)init[i].max_iter + 3;
+      const bool ir = init[i].phase == kPhIrResid || init[i].phase == kPhIrStart;
+      const uint64_t t = ir ? (uint64_t)(init[i].max_outer + 1) * (init[i].max_iter + 4) + 4
+                            : (uint64_t)init[i].max_iter + 4;
       max_trips = t > max_trips ? t : max_trips;
     }
     const uint64_t max_chunks = max_trips / chunk + 3;
@@ -738,11 +756,12 @@ struct hbj_system {
   uint32_t *rp = nullptr, *col = nullptr, *bs = nullptr;
   float *val = nullptr, *lu = nullptr;
   float *x = nullptr, *x0 = nullptr, *b = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+  float* pn = nullptr;  // p of odd launch sets (the two-launch iteration)
   float *ra = nullptr, *rb = nullptr;
   float* part = nullptr;      // 3 * n_tiles per-tile FP32 partials
   double* part64 = nullptr;  // 5 * n_tiles per-tile FP64 partials
   double* resid2 = nullptr;  // 2 * n_tiles FP64 partials of the residual check
-  double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr;
+  double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr, *x64b = nullptr;
   float* b32 = nullptr;
   double* h_part2 = nullptr;  // pinned, 2 * n_tiles (allocated by the first residual check)
   uint2* tile_blk = nullptr;  // per tile: first block touching it, count
@@ -790,7 +809,7 @@ struct hbj_system {
     L.add(&val, nnz_ + 4ull);
     L.add(&bs, (uint64_t)n_starts);
     L.add(&lu, 36 * (nb_ ? nb_ : 1));
-    for (float** v : {&x, &x0, &b, &p, &q, &z, &ra, &rb}) L.add(v, n_);
+    for (float** v : {&x, &x0, &b, &p, &pn, &q, &z, &ra, &rb}) L.add(v, n_);
     L.add(&part, 3ull * nt);
     L.add(&part64, 5ull * nt);
     L.add(&resid2, 2ull * nt);
@@ -806,7 +825,7 @@ struct hbj_system {
     if (with_ir) layout_ir(L, n_);
   }
   void layout_ir(BlockLayout& L, uint32_t n_) {
-    for (double** v : {&x64, &x0_64, &b64, &r64}) L.add(v, n_);
+    for (double** v : {&x64, &x0_64, &b64, &r64, &x64b}) L.add(v, n_);
     L.add(&b32, n_);
   }
 
@@ -829,12 +848,12 @@ struct hbj_system {
     L.add(&shdr, im.shdr.size());
     L.add(&b64, n_);
     *data_end = (L.size + 255) & ~255ull;
-    for (float** v : {&x, &x0, &b, &p, &q, &z, &ra, &rb}) L.add(v, n_);
+    for (float** v : {&x, &x0, &b, &p, &pn, &q, &z, &ra, &rb}) L.add(v, n_);
     L.add(&part, 3ull * nt);
     L.add(&part64, 5ull * nt);
     L.add(&resid2, 2ull * nt);
     L.add(&d_desc1, 1);
-    for (double** v : {&x64, &x0_64, &r64}) L.add(v, n_);
+    for (double** v : {&x64, &x0_64, &r64, &x64b}) L.add(v, n_);
     L.add(&b32, n_);
   }
 
@@ -850,9 +869,11 @@ struct hbj_system {
     d.z = z;
     d.b = ir ? b32 : b;
     d.p = p;
+    d.pn = fused_mode() ? pn : p;
     d.part = part;
     d.part64 = part64;
     d.x64 = x64;
+    d.x64b = x64b;
     d.b64 = b64;
     d.r64 = r64;
     d.r[0] = ra;
@@ -871,7 +892,7 @@ struct hbj_system {
     d.n = n;
     d.nb = nb;
     d.n_tiles = n_tiles;
-    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u);
+    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u) | (fused_mode() ? kFlagTwoLaunch : 0u);
     return d;
   }
 
@@ -881,7 +902,8 @@ struct hbj_system {
     SysState s;
     std::memset(&s, 0, sizeof s);
     s.status = kRunning;
-    s.phase = ir ? kPhIrResid : (ccg ? kPhCcgInit : kPhInit);
+    const bool f = fused_mode();
+    s.phase = ir ? (f ? kPhIrStart : kPhIrResid) : (ccg ? kPhCcgInit : (f ? kPhStart : kPhInit));
     s.max_iter = ir ? ip->max_inner : max_iter;
     s.rel_tol2 = ir ? ip->inner_rel_tol * ip->inner_rel_tol : rel_tol * rel_tol;  // compute.rs:256
     s.abs_tol2 = ir ? 0.0f : abs_tol * abs_tol;                                    // compute.rs:257

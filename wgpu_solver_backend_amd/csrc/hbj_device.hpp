@@ -83,6 +83,10 @@ enum Phase : int32_t {
   kPhCcgInit = 3,   // q = A x0, ||b||^2; then r0 = b - q, u0 = M^-1 r0, p = s = 0
   kPhCcg = 4,       // w = A u with (r,u), (w,u), (r,r); then alpha, beta, the
                     // stop test, p, s, x, r and u = M^-1 r
+  // The two-launch iteration (the finish of one launch set folded into the
+  // next set's SpMV): the states before a launch set's first SpMV
+  kPhStart = 5,     // q = A x0 and b.b pending (then kPhInit)
+  kPhIrStart = 6,   // IR: the FP64 residual pending (then kPhIrResid)
 };
 
 // SysDesc.flags
@@ -92,6 +96,9 @@ constexpr uint32_t kFlagDot64 = 4u;     // inner dots accumulated in FP64
 constexpr uint32_t kFlagBlocks6 = 8u;   // block b is rows [6b, min(6b + 6, n)): block
                                         // ranges and starts are arithmetic
 constexpr uint32_t kFlagSliced = 16u;   // the sliced SpMV layout (variant 4) exists
+constexpr uint32_t kFlagTwoLaunch = 32u; // the two-launch iteration (k_pcg_fspmv + k_pcg_update):
+                                        // the refinement residual folds the correction on the
+                                        // fly into x64b, which k_pcg_update copies to x64
 
 struct alignas(128) SysState {
   // FP32 PCG scalars, the reference's mailbox slots (compute.rs:271-277)
@@ -120,7 +127,8 @@ struct SysDesc {
   const uint32_t* bstart;    // nb+1
   const float* lu;           // 36*nb, row-major packed LU per block
   float* x;                  // solution (FP32 PCG) / correction (IR inner)
-  float* p;
+  float* p;                  // search direction (the two-launch iteration: of even
+  float* pn;                 // launch sets; pn: of odd ones -- otherwise pn == p)
   float* q;                  // A p
   float* z;
   float* b;                  // right-hand side of the FP32 PCG
@@ -128,6 +136,8 @@ struct SysDesc {
   double* part64;            // 5 * n_tiles FP64 partials (FP64-accumulated dots; the
                              // FP64 residual's r.r and b.b at 3 and 4 * n_tiles)
   double* x64;               // IR solution
+  double* x64b;              // IR, two-launch iteration: x64 with the inner solve's
+                             // correction folded in, written by the residual SpMV
   const double* b64;         // IR right-hand side
   double* r64;               // IR residual
   float* r[2];               // r, double-buffered: a launch set of parity par reads

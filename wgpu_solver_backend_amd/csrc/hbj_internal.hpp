@@ -38,6 +38,8 @@ constexpr uint32_t kSpmvDefault = 4;  // sliced where the system has the layout,
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_update(const LaunchSet& ls, hipStream_t s);
 void launch_pcg_finish(const LaunchSet& ls, hipStream_t s);
+// the two-launch iteration: the previous set's finish and this set's SpMV
+void launch_pcg_fspmv(const LaunchSet& ls, hipStream_t s);
 // one-reduction PCG (Chronopoulos-Gear): SpMV with its three partials, update
 void launch_ccg_spmv(const LaunchSet& ls, hipStream_t s);
 void launch_ccg_update(const LaunchSet& ls, hipStream_t s);

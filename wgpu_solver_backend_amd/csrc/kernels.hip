@@ -271,8 +271,8 @@ __device__ __forceinline__ void reduce_fetched3(const T (&va)[4][4], const T (&v
 // PCG system) and otherwise to `scratch` (global, ceil(len/256) values),
 // which a second level reduces into lds.  Returns the value in every thread.
 template <typename T, bool kLean = false>
-__device__ T reduce_partials(const T* __restrict__ part, uint32_t len, T* lds,
-                             T* scratch = nullptr) {
+__device__ __forceinline__ T reduce_partials(const T* __restrict__ part, uint32_t len, T* lds,
+                                             T* scratch = nullptr) {
   if (len <= 1u) return part[0];
   uint32_t out_len = (len + 255u) / 256u;
   if (out_len > 256u) {
@@ -333,6 +333,17 @@ __device__ __forceinline__ T uni_copy(const T& src) {
 #pragma unroll
   for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = uni(in[i]);
   return out;
+}
+
+// The search direction of a launch set of parity par in the two-launch
+// iteration: d.p for even sets, d.pn for odd ones (the same buffer in the
+// three-launch form).  Both pointers are moved to scalar registers before
+// the select: a select between the addresses of the two fields makes the
+// compiler place the whole descriptor in scratch memory.
+__device__ __forceinline__ float* p_of(const SysDesc& d, uint32_t par) {
+  float* const even = uni(d.p);
+  float* const odd = uni(d.pn);
+  return par ? odd : even;
 }
 
 struct TileRef {
@@ -1047,12 +1058,14 @@ struct SpmvLds {
 // sliced layout (spmv_ir_resid_sliced) -- the same products added in the
 // same order, so the same bits.
 __device__ __forceinline__ void ir_resid_tail(const SysDesc& d, const TileRef& tr, double ax,
-                                              double* s_prod) {
+                                              double xown, double* s_prod) {
   double* s_ra = s_prod;  // tile reductions reuse the slices after a barrier
   double* s_rb = s_prod + kTile;
   const uint32_t t = threadIdx.x, row = tr.row0 + t;
   const bool own = t < tr.nrows;
   __syncthreads();
+  // the two-launch iteration: the row's x64 with the correction folded in
+  if (own && (d.flags & kFlagTwoLaunch)) bst(d.x64b, row, xown);
   double rr = 0.0, bb = 0.0;
   if (own) {
     const double bi = d.b64[row];
@@ -1068,16 +1081,34 @@ __device__ __forceinline__ void ir_resid_tail(const SysDesc& d, const TileRef& t
   }
 }
 
+// x64 at column c as the residual sees it: with fold (the two-launch
+// iteration after an inner solve), x64 + x32 * inv_scale -- the fold
+// k_pcg_finish applies in the three-launch form, the same operations.
+struct X64Eff {
+  __amdgpu_buffer_rsrc_t r64, r32;
+  double inv_scale;
+  bool fold;
+  __device__ __forceinline__ double operator()(uint32_t c) const {
+    const double v = buf_gather<double>(r64, c);
+    const float xs = buf_gather<float>(r32, c);
+    return fold ? v + (double)xs * inv_scale : v;
+  }
+};
+__device__ __forceinline__ X64Eff x64_eff(const SysDesc& d, bool fold, double inv_scale) {
+  return X64Eff{buf_rsrc(d.x64, d.n * 8u), buf_rsrc(d.x, d.n * 4u), inv_scale, fold};
+}
+
 template <int V>
 __device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& tr, uint32_t mb,
-                                              uint32_t me, double* s_prod) {
+                                              uint32_t me, double* s_prod, bool fold = false,
+                                              double inv_scale = 0.0) {
   constexpr int kSliceF = SpmvLds<V>::kSliceF;
   const uint32_t wave = threadIdx.x >> 6;
-  const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
+  const X64Eff xg = x64_eff(d, fold, inv_scale);
   double* slice = s_prod + wave * (kSliceF / 2);
-  const double ax = spmv_wave<double, kPer / 2>(
-      mb, me, d.col, d.val, [&](uint32_t c) { return buf_gather<double>(rx64, c); }, tr.nrows, slice);
-  ir_resid_tail(d, tr, ax, s_prod);
+  const double xown = xg(threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0);
+  const double ax = spmv_wave<double, kPer / 2>(mb, me, d.col, d.val, xg, tr.nrows, slice);
+  ir_resid_tail(d, tr, ax, xown, s_prod);
 }
 
 // The refinement residual of a tile with the sliced layout, which needs no
@@ -1089,7 +1120,8 @@ __device__ __forceinline__ void spmv_ir_resid(const SysDesc& d, const TileRef& t
 // system, or rows without entries) have no runs and sum nothing.
 __device__ __forceinline__ void spmv_ir_resid_sliced(const SysDesc& d, const TileRef& tr,
                                                      const TileHdr& hdr, const SlicedPre& p,
-                                                     double* s_prod) {
+                                                     double* s_prod, bool fold = false,
+                                                     double inv_scale = 0.0) {
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t bnd[kMaxRuns];
   int32_t dl[kMaxRuns];
@@ -1101,7 +1133,8 @@ __device__ __forceinline__ void spmv_ir_resid_sliced(const SysDesc& d, const Til
     dl[k] = (int32_t)(int16_t)(run >> 16);
     len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
   }
-  const __amdgpu_buffer_rsrc_t rx64 = buf_rsrc(d.x64, d.n * 8u);
+  const X64Eff xg = x64_eff(d, fold, inv_scale);
+  const double xown = xg(threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0);
   double ax = 0.0;
   for (uint32_t st = 0; st < len; ++st) {
     const float v = __builtin_bit_cast(
@@ -1117,9 +1150,9 @@ __device__ __forceinline__ void spmv_ir_resid_sliced(const SysDesc& d, const Til
     } else {
       c = d.tcols[hdr.off + pos];
     }
-    ax = ax + (double)v * buf_gather<double>(rx64, c);
+    ax = ax + (double)v * xg(c);
   }
-  ir_resid_tail(d, tr, ax, s_prod);
+  ir_resid_tail(d, tr, ax, xown, s_prod);
 }
 
 // q of the tile's rows stored, and the tile's dot partial: dot(b,b) at init
@@ -1315,7 +1348,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
   float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
   const float ri_prev = bld(r_in, rowc), rh_prev = bld(r_in, hrowc);
-  const float pi = bld(d.p, rowc);
+  const float pi = bld(p_of(d, ta.par), rowc);
   // blocks touching the tile, their row ranges and factors: arithmetic with
   // 6-row blocks, so the factor loads go out now; otherwise after the table
   // lookup below
@@ -1338,6 +1371,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrResid) return;
   const bool iter = phase == kPhIter, restart = phase == kPhIrResid;
+  // the two-launch iteration: the residual left the folded x64 in x64b
+  if (restart && (d.flags & kFlagTwoLaunch) && own) bst(d.x64, row, bld(d.x64b, row));
 
   // alpha (pcg_update_scalars.wgsl:59-62, 70-71), ||b||^2 at init, or the
   // refinement restart's scale
@@ -1468,6 +1503,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
   const SysDesc& d = tr.d;
   const uint32_t nt = d.n_tiles, t = threadIdx.x, tile0 = tr.tile;
   SysState* st = d.st + ta.par * ta.st_stride;
+  const float* __restrict__ p_cur = ta.par ? d.pn : d.p;
   const float* __restrict__ r_in = ta.par ? d.r[1] : d.r[0];
   float* __restrict__ r_out = ta.par ? d.r[0] : d.r[1];
   const bool b6 = (d.flags & kFlagBlocks6) != 0u;
@@ -1496,7 +1532,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
     L.zi_old = (d.flags & kFlagOversize) ? bld(d.z, L.rowc) : 0.0f;
     L.ri_prev = bld(r_in, L.rowc);
     L.rh_prev = bld(r_in, L.hrowc);
-    L.pi = bld(d.p, L.rowc);
+    L.pi = bld(p_cur, L.rowc);
     L.tb = make_uint2(0u, 0u);
     L.bs_s = L.bs_e = 0;
     if (b6) {
@@ -1525,6 +1561,14 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
   if (status != kRunning) return;
   if (phase != kPhInit && phase != kPhIter && phase != kPhIrResid) return;
   const bool iter = phase == kPhIter, restart = phase == kPhIrResid;
+  // the two-launch iteration: the residual left the folded x64 in x64b
+  if (restart && (d.flags & kFlagTwoLaunch)) {
+    for (uint32_t u = 0; u < kUT; ++u) {
+      const uint32_t tile = tile0 + u;  // the workgroup's consecutive tiles
+      const uint32_t row = tile * (uint32_t)kTile + t;
+      if (tile < nt && row < d.n) bst(d.x64, row, bld(d.x64b, row));
+    }
+  }
   float alpha = 0.0f;
   double scale = 0.0;
   if (iter) {
@@ -1841,6 +1885,251 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
         bst(d.x64, row, bld(d.x64, row) + (double)bld(d.x, row) * inv_scale);
     }
   }
+}
+
+// ----------------------------------------- the two-launch iteration
+//
+// k_pcg_fspmv folds k_pcg_finish into the next launch set's SpMV, so that
+// an iteration is two launches, k_pcg_fspmv then k_pcg_update.  Per tile:
+//   1. the previous set's finish, as k_pcg_finish does it: the state (copy
+//      par ^ 1, the one the previous k_pcg_update used), the r.z reduction
+//      (r.r too in the state writer and with refinement), beta, the
+//      breakdown checks, the stop test and the transition; the system's
+//      tile-0 workgroup writes the next state into copy par (a final state
+//      into both copies);
+//   2. this set's SpMV q = A p, with p = (p_prev * beta) + z (two
+//      roundings; p = z on the first iteration) evaluated where it is used:
+//      at the tile's listed columns when the vector is staged (or per
+//      gathered entry), and for the tile's own rows, which the tile stores
+//      into p[par].  p is double-buffered by launch-set parity (d.p for even
+//      sets, d.pn for odd ones), since other tiles still read p_prev at
+//      these rows; every evaluation applies the same operations to the same
+//      values, so all copies of p_k carry the same bits.
+// kPhStart / kPhIrStart mark a set whose SpMV has no finish before it (the
+// first SpMV of a solve, q = A x0; a refinement residual).  When an inner
+// solve of the refinement stops, the next set computes the residual from
+// x64 + x32 * inv_scale, evaluated per gathered column (the three-launch
+// form's fold, the same operations), and stores the tile's folded rows into
+// x64b, which the following k_pcg_update copies to x64 (no tile rewrites a
+// value another tile of the same launch reads).
+
+// The thread's row of A v for the tile, v given by g(column): the sliced
+// layout (its loads issued by the caller), or the tile-local / HBM-gather
+// forms.  Every thread of the workgroup calls it (staging holds barriers).
+template <int V, typename G>
+__device__ __forceinline__ float spmv_rows(const SysDesc& d, const TileRef& tr, bool sliced,
+                                           const TileHdr& hdr, const SlicedPre& pre, G g,
+                                           double* s_prod, float* s_v) {
+  constexpr int kSliceF = SpmvLds<V>::kSliceF;
+  constexpr bool kLocal = SpmvLds<V>::kLocal;
+  // the header's fields as values (a lambda holding the header's address
+  // would place it in scratch memory)
+  const uint32_t h_off = hdr.off, h_cnt = hdr.cnt, h_goff = hdr.goff;
+  const uint32_t* __restrict__ tgrp = d.tgrp;
+  const uint32_t* __restrict__ tcols = d.tcols;
+  if constexpr (V >= 4) {
+    if (sliced)
+      return sliced_rows<V == 4 ? kNt : 0>(pre, s_v, [=]() {
+        if (h_goff) stage_groups(tgrp, h_goff - 1u, h_cnt, g, s_v);
+        else stage_local(tcols, h_off, h_cnt, g, s_v);
+      });
+  }
+  uint32_t mb, me;
+  lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
+  float* slice = reinterpret_cast<float*>(s_prod) + (threadIdx.x >> 6) * kSliceF;
+  if constexpr (kLocal) {
+    if (h_cnt != kLocalOff) {
+      // the wave's entry range: the bounds as separate scalars first (a
+      // select over the header's array becomes an indexed scratch load)
+      const uint32_t w = uni((uint32_t)(threadIdx.x >> 6));
+      const uint32_t e0 = uni(hdr.eb[0]), e1 = uni(hdr.eb[1]), e2 = uni(hdr.eb[2]);
+      const uint32_t e3 = uni(hdr.eb[3]), e4 = uni(hdr.eb[4]);
+      const uint32_t wb = w == 0u ? e0 : w == 1u ? e1 : w == 2u ? e2 : e3;
+      const uint32_t we = w == 0u ? e1 : w == 1u ? e2 : w == 2u ? e3 : e4;
+      return spmv_wave_local<SpmvLds<V>::K>(wb, we, mb, me, d.lidx, d.val, s_v, slice, [=]() {
+        stage_local(tcols, h_off, h_cnt, g, s_v);
+      });
+    }
+    return spmv_wave<float, kPer>(mb, me, d.col, d.val, g, tr.nrows, slice);
+  } else if constexpr (V == 1) {
+    return spmv_wave_vec<float, 4>(mb, me, d.col, d.val, g, tr.nrows, slice);
+  } else {
+    return spmv_wave<float, kPer>(mb, me, d.col, d.val, g, tr.nrows, slice);
+  }
+}
+
+// One tile of k_pcg_fspmv; kSliced: the system has the sliced layout (its
+// loads are issued unconditionally, so their registers and resources stay
+// out of the other systems' path).
+template <int V, bool kOne, bool kDot64, bool kSliced>
+__device__ __forceinline__ void fspmv_tile(const TileArgs& ta, const TileRef& tr, double* s_prod,
+                                           float* s_v, SysState& s_ns) {
+  constexpr uint32_t kWords = sizeof(SysState) / 4;
+  const SysDesc& d = tr.d;
+  const uint32_t t = threadIdx.x, nt = d.n_tiles;
+  SysState* in = d.st + (ta.par ^ 1u) * ta.st_stride;
+  SysState* out = d.st + ta.par * ta.st_stride;
+  const bool writer = tr.tile == 0;
+  // the matrix's loads that do not depend on the state go out first
+  constexpr bool sliced = kSliced;
+  TileHdr hdr{};
+  SlicedPre pre;
+  if constexpr (SpmvLds<V>::kLocal) hdr = uni_copy(d.thdr[tr.tile]);
+  if constexpr (kSliced) {
+    const SliceHdr sh = sliced_hdr(d, tr.tile);
+    sliced_issue<V == 4 ? kNt : 0>(d, sh, tr.row0 + 64u * uni((uint32_t)(t >> 6)), pre);
+  }
+  // the previous update's r.z partials, prefetched with the state (FP64
+  // ones only for a single system: in a table their 32 registers cost more
+  // rounds of workgroups than the load round they save)
+  using TP = typename std::conditional<kDot64, double, float>::type;
+  const TP* part = kDot64 ? reinterpret_cast<const TP*>(d.part64) : reinterpret_cast<const TP*>(d.part);
+  const bool prefetch = (kOne || !kDot64) && nt > 1u && nt <= kFetchMax;
+  TP vb[4][4];
+  if (prefetch) level_fetch<TP>(part + 2u * nt, nt, vb);
+  if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(in)[t];
+  __syncthreads();
+  const int32_t status = uni(s_ns.status), phase = uni(s_ns.phase);
+  auto put_state = [&](bool both) {
+    if (writer && t < kWords) {
+      const uint32_t w = reinterpret_cast<const uint32_t*>(&s_ns)[t];
+      reinterpret_cast<uint32_t*>(out)[t] = w;
+      if (both) reinterpret_cast<uint32_t*>(in)[t] = w;
+    }
+  };
+  if (status != kRunning) {  // finished: carry the state to the other copy
+    put_state(false);
+    return;
+  }
+  // 0: SpMV of x0 (init), 1: SpMV of p, 2: the FP64 residual
+  int mode = -1;
+  float beta = 0.0f;
+  bool first = false;
+  const bool fold = s_ns.outer >= 1u;  // residual: an inner solve's correction to fold in
+  const double fold_scale = s_ns.inv_scale;
+  double* s_a = s_prod;
+  double* s_b = s_prod + kTile;
+  if (phase == kPhStart || phase == kPhIrStart) {
+    mode = phase == kPhStart ? 0 : 2;
+    if (t == 0) {
+      s_ns.phase = phase == kPhStart ? kPhInit : kPhIrResid;
+      s_ns.sets += 1u;
+    }
+    __syncthreads();
+    put_state(false);
+  } else if (phase == kPhInit || phase == kPhIter || phase == kPhIrResid) {
+    int32_t dec = kRunning;
+    double rr64 = 0.0, bb64 = 0.0;
+    if (phase == kPhIrResid) {
+      dec = ir_decide(d, s_ns.target2, s_ns.outer, s_ns.max_outer, s_a, s_b, rr64, bb64);
+      __syncthreads();  // s_a / s_b are reused below
+    }
+    const bool reduce = !(phase == kPhInit && s_ns.b_norm2 == 0.0f) &&
+                        !(phase == kPhIrResid && dec != kRunning);
+    // Within an iteration only the state writer needs r.r (the stop test):
+    // the other workgroups take beta from r.z alone and run the SpMV even in
+    // the set that stops the system or its inner solve (harmless: nothing
+    // reads this q or p then; a stopped inner solve's correction is folded
+    // in by the next residual).
+    const bool beta_only = !writer && phase == kPhIter;
+    if (reduce && beta_only) {
+      TP* lb = reinterpret_cast<TP*>(s_b);
+      const TP b = prefetch ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
+      // pcg_transition's beta
+      if constexpr (kDot64) {
+        const double rz64_old = s_ns.rz64_old;
+        beta = rz64_old != 0.0 ? (float)(b / rz64_old) : 0.0f;
+      } else {
+        const float rz_old = s_ns.rz_old;
+        beta = rz_old != 0.0f ? b / rz_old : 0.0f;
+      }
+      mode = 1;
+      __syncthreads();  // the slices alias the reduction's LDS
+    } else {
+      float rr = 0.0f, rz = 0.0f;
+      double rz64 = 0.0;
+      if (reduce) {
+        TP* la = reinterpret_cast<TP*>(s_a);
+        TP* lb = reinterpret_cast<TP*>(s_b);
+        const TP a = reduce_partials<TP>(part + nt, nt, la);
+        const TP b = prefetch ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
+        rr = (float)a;
+        rz = (float)b;
+        if constexpr (kDot64) rz64 = b;
+      }
+      if (t == 0) {
+        pcg_transition(s_ns, d.flags, rr, rz, rz64, dec, rr64, bb64);
+        // an inner solve stopped: the next set computes the residual (with
+        // the correction folded in)
+        if (phase == kPhIter && s_ns.status == kRunning && s_ns.phase == kPhIrResid)
+          s_ns.phase = kPhIrStart;
+      }
+      __syncthreads();
+      const int32_t nstatus = uni(s_ns.status), nphase = uni(s_ns.phase);
+      put_state(nstatus != kRunning);
+      if (nstatus != kRunning) return;
+      beta = s_ns.beta;
+      if (nphase == kPhIter) {
+        mode = 1;
+        first = uni(s_ns.iter) == 0u;
+      }
+    }
+  }
+  if (mode < 0) return;
+  const uint32_t row = tr.row0 + t, rowc = t < tr.nrows ? row : tr.row0;
+  const bool own = t < tr.nrows;
+  if (mode == 2) {
+    if constexpr (kSliced) {
+      spmv_ir_resid_sliced(d, tr, hdr, pre, s_prod, fold, fold_scale);
+      return;
+    }
+    uint32_t mb, me;
+    lane_bounds(d.row_ptr, tr.row0, tr.nrows, mb, me);
+    spmv_ir_resid<V>(d, tr, mb, me, s_prod, fold, fold_scale);
+    return;
+  }
+  if (mode == 0) {  // compute.rs:294 (q = A x0) and ||b||^2 (:235-242)
+    const __amdgpu_buffer_rsrc_t rx = buf_rsrc(d.x, d.n * 4u);
+    auto xg = [&](uint32_t c) -> float { return buf_gather<float>(rx, c); };
+    const float a = bld(d.b, rowc);
+    const float q = spmv_rows<V>(d, tr, sliced, hdr, pre, xg, s_prod, s_v);
+    spmv_tail(d, tr, q, true, a, s_prod);
+    return;
+  }
+  // p_k = (p_{k-1} * beta) + z (scale_from_scalar_results.wgsl:37, axpy.wgsl:41
+  // with alpha = 1), or z on the first iteration (compute.rs:313-314)
+  float* const pcur = p_of(d, ta.par);
+  const __amdgpu_buffer_rsrc_t rpp = buf_rsrc(p_of(d, ta.par ^ 1u), d.n * 4u);
+  const __amdgpu_buffer_rsrc_t rz = buf_rsrc(d.z, d.n * 4u);
+  auto pg = [&](uint32_t c) -> float {
+    const float zc = buf_gather<float>(rz, c);
+    const float pc = buf_gather<float>(rpp, c);
+    const float pb = pc * beta;
+    return first ? zc : pb + zc;
+  };
+  const float a = pg(rowc);
+  if (own) bst(pcur, row, a);
+  const float q = spmv_rows<V>(d, tr, sliced, hdr, pre, pg, s_prod, s_v);
+  spmv_tail(d, tr, q, false, a, s_prod);
+}
+
+template <int V, bool kOne, bool kDot64>
+__global__ __launch_bounds__(kTile) void k_pcg_fspmv(const TileArgs ta) {
+  __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  __shared__ SysState s_ns;
+  static_assert(sizeof(SysState) / 4 <= (uint32_t)kTile && sizeof(SysState) % 4 == 0,
+                "state copy by dwords");
+  static_assert(kWaves * SpmvLds<V>::kSliceF / 2 >= 2 * kTile, "finish reductions alias the slices");
+  TileRef tr;
+  if (!get_tile<kOne>(ta, tr)) return;
+  if constexpr (V >= 4) {
+    if (tr.d.flags & kFlagSliced) {
+      fspmv_tile<V, kOne, kDot64, true>(ta, tr, s_prod, s_v, s_ns);
+      return;
+    }
+  }
+  fspmv_tile<V, kOne, kDot64, false>(ta, tr, s_prod, s_v, s_ns);
 }
 
 // ------------------------------------------ one-reduction PCG (optional)
@@ -2480,6 +2769,29 @@ void launch_pcg_finish(const LaunchSet& ls, hipStream_t s) {
   const TileArgs a = tile_args(ls);
   if (ls.dot64) launch_finish_t<true>(ls, a, s);
   else launch_finish_t<false>(ls, a, s);
+}
+
+template <int V>
+static void launch_fspmv_v(const LaunchSet& ls, hipStream_t s) {
+  const TileArgs a = tile_args(ls);
+  const dim3 g(ls.n_tiles), b(kTile);
+  if (ls.one) {
+    if (ls.dot64) launch(ls, k_pcg_fspmv<V, true, true>, g, b, s, a);
+    else launch(ls, k_pcg_fspmv<V, true, false>, g, b, s, a);
+  } else {
+    if (ls.dot64) launch(ls, k_pcg_fspmv<V, false, true>, g, b, s, a);
+    else launch(ls, k_pcg_fspmv<V, false, false>, g, b, s, a);
+  }
+}
+void launch_pcg_fspmv(const LaunchSet& ls, hipStream_t s) {
+  switch (ls.spmv_variant) {
+    case 1: launch_fspmv_v<1>(ls, s); break;
+    case 2: launch_fspmv_v<2>(ls, s); break;
+    case 3: launch_fspmv_v<3>(ls, s); break;
+    case 4: launch_fspmv_v<4>(ls, s); break;
+    case 5: launch_fspmv_v<5>(ls, s); break;
+    default: launch_fspmv_v<0>(ls, s); break;
+  }
 }
 
 template <int V>

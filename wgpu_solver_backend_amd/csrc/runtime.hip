@@ -672,6 +672,7 @@ hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
   out->spmv_csr_bytes = 8ull * s->nnz + 12ull * s->n + 4;
   out->spmv_layout_bytes = s->spmv_layout_bytes;
   out->spmv_sliced_bytes = s->spmv_sliced_bytes;
+  out->launches_per_iteration = (s->alg == 1 || fused_mode()) ? 2u : 3u;
   return HBJ_OK;
 }
 

@@ -462,3 +462,67 @@ def test_sliced_layout_mixed_groups_bitwise(ctx, hbj, oracle):
             assert res.status == 0 and res.iterations == its, (alg, v)
             assert np.array_equal(bits(x), bits(xref)), (alg, v)
     sys_.close()
+
+
+def test_three_launch_form_above_prefetch_limit(ctx, hbj, oracle):
+    """A system of more than 4,096 tiles runs the three-launch iteration (its
+    r.z partials exceed what a workgroup prefetches); a smaller one the
+    two-launch iteration.  Both give the oracle's bits.  The large system is
+    a chain of 6x6 node blocks, diagonally dominant, so the oracle's solve is
+    short: 183,000 nodes, 1,098,000 rows (4,290 tiles)."""
+    nodes = 183_000
+    n = 6 * nodes
+    rng = np.random.default_rng(21)
+    # per node: diagonal block (symmetric, dominant) and the coupling block to
+    # the next node (its transpose couples back)
+    D = rng.uniform(-0.1, 0.1, (nodes, 6, 6)).astype(np.float32)
+    D = (D + D.transpose(0, 2, 1)) * np.float32(0.5) + np.eye(6, dtype=np.float32) * np.float32(4.0)
+    C = rng.uniform(-0.2, 0.2, (nodes - 1, 6, 6)).astype(np.float32)
+    rows = []
+    # CSR row by row: [C_{g-1}^T (row d)] [D_g (row d)] [C_g (row d)], columns ascending
+    counts = np.full(nodes, 18, np.int64)
+    counts[0] -= 6
+    counts[-1] -= 6
+    row_len = np.repeat(counts, 6)
+    row_ptr = np.zeros(n + 1, np.uint32)
+    row_ptr[1:] = np.cumsum(row_len)
+    nnz = int(row_ptr[-1])
+    col = np.empty(nnz, np.uint32)
+    val = np.empty(nnz, np.float32)
+    g = np.arange(nodes)
+    for d in range(6):
+        r = 6 * g + d
+        pos = row_ptr[r].astype(np.int64)
+        has_prev = g > 0
+        k = pos.copy()
+        for j in range(6):  # lower block: (C_{g-1}^T)[d, j] = C_{g-1}[j, d]
+            sel = has_prev
+            col[k[sel]] = 6 * (g[sel] - 1) + j
+            val[k[sel]] = C[g[sel] - 1, j, d]
+            k[sel] += 1
+        for j in range(6):
+            col[k] = 6 * g + j
+            val[k] = D[g, d, j]
+            k += 1
+        has_next = g < nodes - 1
+        for j in range(6):
+            sel = has_next
+            col[k[sel]] = 6 * (g[sel] + 1) + j
+            val[k[sel]] = C[g[sel], d, j]
+            k[sel] += 1
+    bs = np.arange(0, n + 1, 6, dtype=np.uint32)
+    b = rng.uniform(-1, 1, n).astype(F)
+    x0 = np.zeros(n, F)
+    s = hbj.System(ctx, row_ptr, col, val, bs)
+    info = s.info()
+    assert info["n_tiles"] > 4096 and info["launches_per_iteration"] == 3
+    x, res = s.solve(b, x0, 500, 1e-6, 0.0)
+    st, it, xo, _ = oracle_solve(oracle, row_ptr, col, val, bs, b, x0, 500, 1e-6, 0.0)
+    print(f"chain n={n}: {res.iterations} iterations", flush=True)
+    assert res.status == st == 0 and res.iterations == it
+    assert np.array_equal(bits(x), bits(xo))
+    s.close()
+    small = hbj.lattice(8, seed=1)
+    s2 = hbj.System(ctx, small["row_ptr"], small["col_idx"], small["values"], small["block_starts"])
+    assert s2.info()["launches_per_iteration"] == 2
+    s2.close()

@@ -91,9 +91,10 @@ inline uint32_t env_u32(const char* name, uint32_t dflt) {
 // inside hipGraphLaunch (a read of an unmapped address by the runtime's
 // intercepted dispatch path; the same graphs replay correctly without the
 // tool), so the engine launches the kernels directly while one is attached.
-// The two-launch PCG iteration (k_pcg_fspmv + k_pcg_update), the default;
-// HBJ_FUSED=0 selects the three-launch one (k_pcg_spmv, k_pcg_update,
-// k_pcg_finish).  Same arithmetic, same bits.  Read once per process.
+// The two-launch PCG iteration (k_pcg_fspmv + k_pcg_update), the default
+// for tables whose systems have at most kFetchMaxTiles tiles; HBJ_FUSED=0
+// selects the three-launch one (k_pcg_spmv, k_pcg_update, k_pcg_finish)
+// everywhere.  Same arithmetic, same bits.  Read once per process.
 inline bool fused_mode() {
   static const bool f = env_u32("HBJ_FUSED", 1) != 0;
   return f;
@@ -857,7 +858,14 @@ struct hbj_system {
     L.add(&b32, n_);
   }
 
-  SysDesc desc(bool ir) const {
+  // Whether this system's solves use the two-launch iteration: the r.z
+  // partials every SpMV workgroup reduces are prefetched with the state up
+  // to kFetchMax tiles; past that the reduction waits behind the state, and
+  // the three-launch form (its finish kernel reduces in a quarter as many
+  // workgroups) measured faster (64^3 lattice: 89.5 vs 92.6 us per iteration).
+  bool two_launch() const { return fused_mode() && n_tiles <= kFetchMaxTiles; }
+
+  SysDesc desc(bool ir, bool fused) const {
     SysDesc d{};
     d.row_ptr = rp;
     d.col = col;
@@ -869,7 +877,7 @@ struct hbj_system {
     d.z = z;
     d.b = ir ? b32 : b;
     d.p = p;
-    d.pn = fused_mode() ? pn : p;
+    d.pn = fused ? pn : p;
     d.part = part;
     d.part64 = part64;
     d.x64 = x64;
@@ -892,17 +900,17 @@ struct hbj_system {
     d.n = n;
     d.nb = nb;
     d.n_tiles = n_tiles;
-    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u) | (fused_mode() ? kFlagTwoLaunch : 0u);
+    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u) | (fused ? kFlagTwoLaunch : 0u);
     return d;
   }
 
   // ccg: start the one-reduction PCG (plain solves only)
   SysState init_state(bool ir, uint32_t max_iter, float rel_tol, float abs_tol,
-                      const hbj_ir_params* ip, bool ccg = false) const {
+                      const hbj_ir_params* ip, bool ccg, bool fused) const {
     SysState s;
     std::memset(&s, 0, sizeof s);
     s.status = kRunning;
-    const bool f = fused_mode();
+    const bool f = fused;
     s.phase = ir ? (f ? kPhIrStart : kPhIrResid) : (ccg ? kPhCcgInit : (f ? kPhStart : kPhInit));
     s.max_iter = ir ? ip->max_inner : max_iter;
     s.rel_tol2 = ir ? ip->inner_rel_tol * ip->inner_rel_tol : rel_tol * rel_tol;  // compute.rs:256
@@ -944,6 +952,7 @@ struct hbj_system {
   hbj_status ensure_engine(int mode) {
     if (!eng_ready) {
       CHK(eng.init(ctx->device, 1, n_tiles));
+      eng.fused = two_launch();
       eng_ready = true;
     }
     eng.set_alg(mode == 0 ? alg : 0u);
@@ -951,7 +960,7 @@ struct hbj_system {
     if (eng_mode != mode) {
       std::vector<uint2> tiles(n_tiles);
       for (uint32_t t = 0; t < n_tiles; ++t) tiles[t] = make_uint2(0u, t);
-      CHK(eng.set_table({desc(mode == 1)}, tiles));
+      CHK(eng.set_table({desc(mode == 1, eng.fused)}, tiles));
       eng_mode = mode;
     }
     return HBJ_OK;

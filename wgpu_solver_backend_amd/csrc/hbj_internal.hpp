@@ -33,6 +33,9 @@ struct LaunchSet {
 // gathers, 2 / 3 tile-local columns in LDS with 512 / 1024 entries per pass,
 // 4 the sliced layout (systems that have one; others as 2).
 constexpr uint32_t kSpmvVariantCount = 6;
+// Tile partials a kernel can prefetch into registers before it reads a
+// system's state (the kernels' level_fetch holds up to this many).
+constexpr uint32_t kFetchMaxTiles = 4096;
 constexpr uint32_t kSpmvDefault = 4;  // sliced where the system has the layout, else as 2
 
 void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s);

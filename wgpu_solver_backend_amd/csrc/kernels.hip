@@ -180,6 +180,7 @@ __device__ __forceinline__ void level_pass(const T* __restrict__ src, uint32_t l
 // reads anything else and reduce them afterwards (reduce_fetched).  Same
 // chunks and trees as level_pass; the result is reduce_partials' result.
 constexpr uint32_t kFetchMax = 4096;
+static_assert(kFetchMax == kFetchMaxTiles, "the host's prefetch limit is the kernels'");
 template <typename T>
 __device__ __forceinline__ void level_fetch(const T* __restrict__ src, uint32_t len, T (&v)[4][4]) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;

@@ -554,7 +554,7 @@ hbj_status system_create(hbj_ctx* ctx, uint32_t n, uint32_t nnz, const uint32_t*
     for (double* v : {s->x64, s->x0_64, s->b64})
       if (e == hipSuccess) e = hipMemset(v, 0, 8ull * n);
   if (e == hipSuccess) {
-    SysDesc d1 = s->desc(false);
+    SysDesc d1 = s->desc(false, false);
     e = hipMemcpy(s->d_desc1, &d1, sizeof d1, hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) {
@@ -614,7 +614,7 @@ hbj_status system_place(hbj_ctx* ctx, const SystemImage& im, const uint32_t* row
   // the device block: the same offsets from its base
   L.assign(s->block);
   if (!im.no_csr()) {  // the FP64 check's descriptor (device pointers), in the image
-    const SysDesc d1 = s->desc(false);
+    const SysDesc d1 = s->desc(false, false);
     std::memcpy(staging + (reinterpret_cast<char*>(s->d_desc1) - s->block), &d1, sizeof d1);
   }
   hipError_t e = hipMemcpyAsync(s->block, staging, data_end, hipMemcpyHostToDevice, stream);
@@ -672,7 +672,7 @@ hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
   out->spmv_csr_bytes = 8ull * s->nnz + 12ull * s->n + 4;
   out->spmv_layout_bytes = s->spmv_layout_bytes;
   out->spmv_sliced_bytes = s->spmv_sliced_bytes;
-  out->launches_per_iteration = (s->alg == 1 || fused_mode()) ? 2u : 3u;
+  out->launches_per_iteration = (s->alg == 1 || s->two_launch()) ? 2u : 3u;
   return HBJ_OK;
 }
 
@@ -722,7 +722,7 @@ hbj_status hbj_system_autotune(hbj_system* s, uint32_t iters, uint32_t* best, do
       HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, s->eng.stream));
       if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, s->eng.stream));
       // tolerances 0: runs `iters` iterations unless the residual is exactly 0
-      SysState init = s->init_state(false, iters, 0.0f, 0.0f, nullptr, s->alg == 1), fin;
+      SysState init = s->init_state(false, iters, 0.0f, 0.0f, nullptr, s->alg == 1, s->eng.fused), fin;
       double ms = 0;
       const hbj_status st = s->eng.run(&init, &fin, &ms);
       if (st != HBJ_OK) {
@@ -801,7 +801,7 @@ hbj_status hbj_system_solve_resident(hbj_system* s, uint32_t max_iter, float rel
   hipStream_t st = s->eng.stream;
   HIPCHK(hipMemcpyAsync(s->x, s->x0, 4ull * s->n, hipMemcpyDeviceToDevice, st));
   if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
-  SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr, s->alg == 1), fin;
+  SysState init = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr, s->alg == 1, s->eng.fused), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
   fill_result(fin, false, ms, res);
@@ -870,7 +870,7 @@ static hbj_status solve_ir_resident(hbj_system* s, const hbj_ir_params* p, hbj_r
   hipStream_t st = s->eng.stream;
   HIPCHK(hipMemcpyAsync(s->x64, s->x0_64, 8ull * s->n, hipMemcpyDeviceToDevice, st));
   if (s->flags & kFlagOversize) HIPCHK(hipMemsetAsync(s->z, 0, 4ull * s->n, st));
-  SysState init = s->init_state(true, 0, 0, 0, p), fin;
+  SysState init = s->init_state(true, 0, 0, 0, p, false, s->eng.fused), fin;
   double ms = 0;
   CHK(s->eng.run(&init, &fin, &ms));
   fill_result(fin, true, ms, res);
@@ -913,6 +913,10 @@ struct hbj_batch {
   hbj_status ensure(int m) {
     if (!ready) {
       CHK(eng.init(ctx->device, (uint32_t)sys.size(), n_tiles ? n_tiles : 1));
+      // the two-launch iteration when every system's partials fit the prefetch
+      bool fused = fused_mode();
+      for (const hbj_system* s : sys) fused = fused && s->two_launch();
+      eng.fused = fused;
       ready = true;
     }
     if (mode != m) {
@@ -920,7 +924,7 @@ struct hbj_batch {
       std::vector<uint2> tiles;
       tiles.reserve(n_tiles);
       for (uint32_t i = 0; i < sys.size(); ++i) {
-        descs.push_back(sys[i]->desc(m == 1));
+        descs.push_back(sys[i]->desc(m == 1, eng.fused));
         for (uint32_t t = 0; t < sys[i]->n_tiles; ++t) tiles.push_back(make_uint2(i, t));
       }
       if (tiles.empty()) tiles.push_back(make_uint2(0, 0));
@@ -974,7 +978,7 @@ hbj_status hbj_batch_solve(hbj_batch* b, uint32_t max_iter, float rel_tol, float
   std::vector<SysState> init(S), fin(S);
   for (uint32_t i = 0; i < S; ++i) {
     hbj_system* s = b->sys[i];
-    init[i] = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr);
+    init[i] = s->init_state(false, max_iter, rel_tol, abs_tol, nullptr, false, b->eng.fused);
     if (s->n == 0) {
       init[i].status = HBJ_OK;
       continue;
@@ -1013,7 +1017,7 @@ hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* 
   std::vector<SysState> init(S), fin(S);
   for (uint32_t i = 0; i < S; ++i) {
     hbj_system* s = b->sys[i];
-    init[i] = s->init_state(true, 0, 0, 0, p);
+    init[i] = s->init_state(true, 0, 0, 0, p, false, b->eng.fused);
     if (s->n == 0) {
       init[i].status = HBJ_OK;
       continue;
@@ -1063,7 +1067,8 @@ hbj_status hbj_batch_autotune(hbj_batch* b, uint32_t iters, int ir, uint32_t* be
     for (int rep = 0; rep < 4; ++rep) {  // the first run captures the graph; best of 3
       for (uint32_t i = 0; i < S; ++i) {
         hbj_system* s = b->sys[i];
-        init[i] = ir ? s->init_state(true, 0, 0, 0, &p) : s->init_state(false, iters, 0.0f, 0.0f, nullptr);
+        init[i] = ir ? s->init_state(true, 0, 0, 0, &p, false, b->eng.fused)
+                     : s->init_state(false, iters, 0.0f, 0.0f, nullptr, false, b->eng.fused);
         if (s->n == 0) {
           init[i].status = HBJ_OK;
           continue;

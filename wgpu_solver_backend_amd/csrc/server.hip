@@ -352,7 +352,8 @@ hbj_status worker_loop(hbj_server* S) {
         r->slot = (int)slot;
         // the slot's writes and every later chunk come after the upload
         HIPCHK(hipStreamWaitEvent(E.stream, r->up_ev, 0));
-        CHK(E.stage_slot(cur, slot, r->sys->desc(true), r->sys->init_state(true, 0, 0, 0, &r->params)));
+        CHK(E.stage_slot(cur, slot, r->sys->desc(true, E.fused),
+                         r->sys->init_state(true, 0, 0, 0, &r->params, false, E.fused)));
         ++active;
       }
       std::vector<uint2> tiles;

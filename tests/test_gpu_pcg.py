@@ -526,3 +526,38 @@ def test_three_launch_form_above_prefetch_limit(ctx, hbj, oracle):
     s2 = hbj.System(ctx, small["row_ptr"], small["col_idx"], small["values"], small["block_starts"])
     assert s2.info()["launches_per_iteration"] == 2
     s2.close()
+
+
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_repeated_solves_multi_level_reductions(ctx, hbj, oracle, monkeypatch, fused):
+    """A system whose dot partials take two levels of the reduction tree
+    (24^3 lattice: 324 tiles): repeated solves on the same system, under
+    several SpMV variants, in the two-launch (HBJ_FUSED=1) and the
+    three-launch (HBJ_FUSED=0) iteration, give the oracle's bits every time
+    (nothing a solve leaves behind -- state copies, partials, p of either
+    parity -- reaches the next one); so do two refinement solves in a row."""
+    monkeypatch.setenv("HBJ_FUSED", fused)
+    print(f"24^3 lattice, repeated solves, HBJ_FUSED={fused}", flush=True)
+    L = hbj.lattice(24, seed=2)
+    n = L["n"]
+    x0 = np.zeros(n, F)
+    st, it, xo, _ = oracle_solve(oracle, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"],
+                                 L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert st == 0
+    s = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    assert s.info()["n_tiles"] > 256
+    assert s.info()["launches_per_iteration"] == (2 if fused == "1" else 3)
+    for v in (5, 5, 2, 4):
+        s.set_spmv_variant(v)
+        x, res = s.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+        assert res.status == 0 and res.iterations == it, v
+        assert np.array_equal(bits(x), bits(xo)), v
+    p = hbj.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=1e-4, target_rel_res=1e-9)
+    x1, r1 = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+    x2, r2 = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+    assert r1.status == r2.status == 0
+    assert (r1.iterations, r1.outer_iterations) == (r2.iterations, r2.outer_iterations)
+    assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64))
+    x, res = s.solve(L["rhs"], x0, 5000, 1e-6, 0.0)
+    assert res.iterations == it and np.array_equal(bits(x), bits(xo))
+    s.close()

@@ -94,11 +94,9 @@ inline uint32_t env_u32(const char* name, uint32_t dflt) {
 // The two-launch PCG iteration (k_pcg_fspmv + k_pcg_update), the default
 // for tables whose systems have at most kFetchMaxTiles tiles; HBJ_FUSED=0
 // selects the three-launch one (k_pcg_spmv, k_pcg_update, k_pcg_finish)
-// everywhere.  Same arithmetic, same bits.  Read once per process.
-inline bool fused_mode() {
-  static const bool f = env_u32("HBJ_FUSED", 1) != 0;
-  return f;
-}
+// everywhere.  Same arithmetic, same bits.  Read when a system's engine is
+// set up (a later change applies to engines set up after it).
+inline bool fused_mode() { return env_u32("HBJ_FUSED", 1) != 0; }
 
 inline bool profiler_tool_attached() {
   const char* v = std::getenv("ROCP_TOOL_LIBRARIES");

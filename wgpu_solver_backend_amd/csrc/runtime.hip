@@ -672,7 +672,8 @@ hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
   out->spmv_csr_bytes = 8ull * s->nnz + 12ull * s->n + 4;
   out->spmv_layout_bytes = s->spmv_layout_bytes;
   out->spmv_sliced_bytes = s->spmv_sliced_bytes;
-  out->launches_per_iteration = (s->alg == 1 || s->two_launch()) ? 2u : 3u;
+  const bool two = s->eng_ready ? s->eng.fused : s->two_launch();
+  out->launches_per_iteration = (s->alg == 1 || two) ? 2u : 3u;
   return HBJ_OK;
 }
 

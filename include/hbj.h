@@ -168,6 +168,14 @@ hbj_status hbj_system_solve_resident(hbj_system* sys, uint32_t max_iter, float r
                                      float abs_tol, hbj_result* result);
 hbj_status hbj_system_get_x(hbj_system* sys, float* x);
 
+/* SpmvExecutor::encode_copy_x_from + encode_spmv (spmv_exec.rs:122-147) on a
+ * created system: y = A x (x, y: n floats on the host) through the SpMV of
+ * the system's PCG iteration -- its resident layout and chosen variant --, so
+ * y has the bits every iteration's A p has (each row summed in entry order
+ * from 0.0).  Uses the system's x and q vectors: call between solves, not
+ * during one. */
+hbj_status hbj_system_spmv(hbj_system* sys, const float* x, float* y);
+
 /* Launches per PCG iteration. */
 #define HBJ_ITER_KERNELS 3
 /* Bit-identical SpMV streaming variants a system can use: 0 = dword loads,
@@ -204,7 +212,11 @@ typedef struct {
                                       folded into the SpMV; the default), 3 (SpMV,
                                       update, finish: HBJ_FUSED=0), 2 for the
                                       one-reduction PCG */
-  uint32_t reserved0;
+  uint32_t sliced_nz_waves; /* 64-row waves of the sliced layout that store
+                               only their rows' nonzero values (rows of at most
+                               64 entries with many explicit zeros, e.g. dense
+                               FEA node blocks; HBJ_SLICED_NZ=0 at creation
+                               keeps every entry) */
 } hbj_system_info;
 
 /* Algorithm of a system's plain solves (hbj_solve, hbj_system_solve_resident,

@@ -172,8 +172,11 @@ def test_server_mixed_stream_identical_to_alone(ctx, hbj):
 
 
 def test_server_rejects_and_reports_failures(ctx, hbj):
+    """A request whose device-byte lower bound (its nonzero values, LU blocks
+    and vectors: 9.5 MB for the 22^3 lattice) exceeds the 8 MiB budget is
+    rejected at submit; a zero pivot fails its own request only."""
     srv = hbj.Server(ctx, 8 << 20, max_active=4)
-    big = hbj.lattice(20)
+    big = hbj.lattice(22)
     with pytest.raises(hbj.SolverError) as e:
         srv.submit(big["row_ptr"], big["col_idx"], big["values"], big["block_starts"],
                    big["rhs"].astype(np.float64))

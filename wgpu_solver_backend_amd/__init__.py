@@ -364,6 +364,15 @@ class System:
         _check(_lib.lib().hbj_system_get_x(self._h, x))
         return x[: self.n]
 
+    def spmv(self, x):
+        """y = A x through the system's own SpMV (resident layout, chosen
+        variant): SpmvExecutor::encode_spmv, spmv_exec.rs:122-147."""
+        xx = np.zeros(max(self.n, 1), np.float32)
+        xx[: self.n] = _f32(x)
+        y = np.zeros(max(self.n, 1), np.float32)
+        _check(_lib.lib().hbj_system_spmv(self._h, xx, y))
+        return y[: self.n]
+
     def solve_ir(self, b64, x0_64=None, params=None, raise_on_error=True):
         p = (params or IrParams()).c()
         x = np.zeros(self.n, np.float64) if x0_64 is None else _f64(x0_64).copy()

@@ -51,7 +51,7 @@ class hbj_system_info(ctypes.Structure):
                 ("local_tiles", c_u32), ("spmv_variant", c_u32), ("local_columns", c_u64),
                 ("device_bytes", c_u64), ("spmv_csr_bytes", c_u64), ("spmv_layout_bytes", c_u64),
                 ("spmv_sliced_bytes", c_u64), ("launches_per_iteration", c_u32),
-                ("reserved0", c_u32)]
+                ("sliced_nz_waves", c_u32)]
 
 
 class hbj_compare_stats(ctypes.Structure):
@@ -89,6 +89,7 @@ _SIGS = {
     "hbj_system_set_rhs": (c_i32, [c_void_p, f32p, c_void_p]),
     "hbj_system_solve_resident": (c_i32, [c_void_p, c_u32, c_f32, c_f32, P(hbj_result)]),
     "hbj_system_get_x": (c_i32, [c_void_p, f32p]),
+    "hbj_system_spmv": (c_i32, [c_void_p, f32p, f32p]),
     "hbj_system_set_kernel_timing": (c_i32, [c_void_p, c_u32]),
     "hbj_system_kernel_timing": (c_i32, [c_void_p, f64p, np.ctypeslib.ndpointer(dtype=np.uint64)]),
     "hbj_solve_ir": (c_i32, [c_void_p, f64p, f64p, P(hbj_ir_params), P(hbj_result)]),

@@ -738,7 +738,7 @@ struct SystemImage {
   std::vector<TileHdr> thdr;
   std::vector<uint2> tile_blk;
   std::vector<float> lu;  // lean systems: host-built LU blocks (36 per block)
-  uint32_t local_tiles = 0;
+  uint32_t local_tiles = 0, sliced_nz_waves = 0;
   uint64_t local_columns = 0, spmv_layout_bytes = 0, spmv_sliced_bytes = 0;
   bool sliced() const { return (flags & kFlagSliced) != 0u; }
   bool no_csr() const { return lean && sliced(); }
@@ -771,7 +771,7 @@ struct hbj_system {
   float* sval = nullptr;      // sliced SpMV layout (build_sliced; kFlagSliced)
   uint32_t* runs = nullptr;
   SliceHdr* shdr = nullptr;
-  uint32_t local_tiles = 0;
+  uint32_t local_tiles = 0, sliced_nz_waves = 0;
   uint64_t local_columns = 0, spmv_layout_bytes = 0, spmv_sliced_bytes = 0;
   bool lean = false;          // no CSR arrays or entry indices on the device (a served
                               // sliced system: only SpMV variants 4 and 5 may run it)

@@ -59,17 +59,35 @@ struct alignas(32) TileHdr {
 // tile, the values step-major -- value of step s of the wave's row l at
 // sval[voff + 4 (64 (s / 4) + l) + s % 4], nq steps of 4, zero padded -- and
 // per row its runs at runs[roff + 64 k + l], k < nr: (end position of run k
-// in the row) | (list position of the run's first entry minus its position
-// in the row) << 16, unused runs 0xFFFF.  Entry s of a row is list position
-// s + delta(first run whose end exceeds s); the row's length is the end of
-// its last run.  A full wave whose rows of each group of 6 (rows 6g .. 6g+5,
-// a node's dofs) have identical runs stores them once per group
-// (kRunsPerGroup set in nr): runs[roff + nbs k + (row0 + l) / 6 - row0 / 6]
-// with nbs = (row0 + 63) / 6 - row0 / 6 + 1 groups.
+// in the row) << 16 | (4 delta as a 16-bit two's complement number), delta
+// = list position of the run's first entry minus its position in the row;
+// unused runs kRunUnused.  Entry s of a row is list position s + delta(first
+// run whose end exceeds s) -- the first run k with (s << 16 | 0xFFFF) <
+// runs[k] -- and the row's length is the end of its last run.  A full wave
+// whose rows of each group of 6 (rows 6g .. 6g+5, a node's dofs) have
+// identical runs stores them once per group (kRunsPerGroup set in nr):
+// runs[roff + nbs k + (row0 + l) / 6 - row0 / 6] with nbs = (row0 + 63) / 6 -
+// row0 / 6 + 1 groups.
+//
+// A wave whose rows have at most 64 entries each and many explicit zeros
+// (kSliceNz set in nr) stores only its rows' nonzero values: sval[voff + l]
+// and sval[voff + 64 + l] hold (as bits) the low and high words of row l's
+// mask of stored entries (bit s: entry s of the row is not +-0), and the
+// stored values follow step-major from voff + 128, nq steps of 4 counting
+// stored entries.  The t-th stored value of a row is its entry at the
+// position of the mask's t-th set bit.  Skipping an entry whose value is +-0
+// leaves the row sum's bits unchanged while the multiplied vector is finite
+// there (the sum starts at +0.0 and is never -0, so adding a +-0 product is
+// exact); the SpMV checks the staged vector and restores the NaN such an
+// entry would produce when it is not.
 constexpr uint32_t kMaxRuns = 8;
+constexpr uint32_t kRunUnused = 0xFFFF0000u;
+constexpr uint32_t kSlicedMaxLen = 8192;  // entries per row (4 delta fits 16 bits)
 constexpr uint32_t kRunsPerGroup = 0x100u;
+constexpr uint32_t kSliceNz = 0x200u;
+constexpr uint32_t kNzMaxLen = 64;  // entries per row a mask covers
 struct SliceHdr {
-  uint32_t voff, roff, nq, nr;  // nr: run count | kRunsPerGroup
+  uint32_t voff, roff, nq, nr;  // nr: run count | kRunsPerGroup | kSliceNz
 };
 
 enum Phase : int32_t {

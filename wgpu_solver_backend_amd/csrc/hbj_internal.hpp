@@ -76,10 +76,11 @@ void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
                       std::vector<uint32_t>& tcols);
 
 // Sliced SpMV layout (variant 4) from the tile-local tables; false when the
-// system does not qualify (outputs cleared).
+// system does not qualify (outputs cleared).  allow_nz: waves may store only
+// their nonzero values (kSliceNz).
 bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
                   const std::vector<uint16_t>& lidx, const std::vector<uint2>& tloc,
-                  std::vector<float>& sval, std::vector<uint32_t>& runs,
+                  bool allow_nz, std::vector<float>& sval, std::vector<uint32_t>& runs,
                   std::vector<SliceHdr>& shdr);
 
 // The no-alloc dot's device scratch (noalloc.hip): noalloc_init resolves

@@ -669,11 +669,22 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
   return sum;
 }
 
-// Quads [q0, q1) of a sliced row (below), with the run search unrolled
-// over the slice's R = nr runs (a wave-uniform count): entry st's list
-// position is st + delta of the first run whose end exceeds st; dl4 holds
-// 4 delta, so 4 st + dl4 is the byte offset into s_v.  Rows with fewer runs
-// hold end 0xFFFF in their unused slots.  kGuard: some lane may reach its
+// Byte offset into s_v of entry st of a row with runs pk (packed as in
+// hbj_device.hpp), the search unrolled over the slice's R = nr runs (a
+// wave-uniform count): the first run k with (st << 16 | 0xFFFF) < pk[k]
+// holds 4 delta in its low 16 bits, and the entry's list position is st +
+// delta.  Rows with fewer runs hold kRunUnused in their unused slots.
+template <uint32_t R>
+__device__ __forceinline__ uint32_t run_off(uint32_t st, const uint32_t (&pk)[kMaxRuns]) {
+  const uint32_t key = (st << 16) | 0xFFFFu;
+  uint32_t sel = pk[R - 1];
+#pragma unroll
+  for (int k = (int)R - 2; k >= 0; --k) sel = key < pk[k] ? pk[k] : sel;
+  return 4u * st + (uint32_t)(int32_t)(int16_t)(sel & 0xFFFFu);
+}
+
+// Quads [q0, q1) of a sliced row (below); entry st is at run_off<R>(st).
+// kGuard: some lane may reach its
 // row's end inside these quads, so steps from len on (padding) are not
 // added; without it every step is below every lane's length.  `a` holds quad
 // q0's values on entry and quad q1's on return.  (Loading 2-4 quads ahead
@@ -681,9 +692,9 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
 // SIMD, 46.4 -> 50.0 us per 50^3 iteration at 3.)
 template <uint32_t R, bool kGuard, int kAux>
 __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t q0, uint32_t q1,
-                                             f32x4v& a, const uint32_t (&bnd)[kMaxRuns],
-                                             const int32_t (&dl4)[kMaxRuns], uint32_t len,
-                                             const float* s_v, uint32_t lane, float& sum) {
+                                             f32x4v& a, const uint32_t (&pk)[kMaxRuns],
+                                             uint32_t len, const float* s_v, uint32_t lane,
+                                             float& sum) {
   const char* s_b = reinterpret_cast<const char*>(s_v);
   for (uint32_t q = q0; q < q1; ++q) {
     const f32x4v v = a;
@@ -692,10 +703,7 @@ __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t
 #pragma unroll
     for (uint32_t i = 0; i < 4u; ++i) {
       const uint32_t st = 4u * q + i;
-      int32_t d4 = dl4[R - 1];
-#pragma unroll
-      for (int k = (int)R - 2; k >= 0; --k) d4 = st < bnd[k] ? dl4[k] : d4;
-      const uint32_t off = 4u * st + (uint32_t)d4;
+      const uint32_t off = run_off<R>(st, pk);
       if constexpr (kGuard) {
         const bool in = st < len;
         const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
@@ -711,13 +719,79 @@ __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t
 
 template <uint32_t R, int kAux>
 __device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t qa, uint32_t nq,
-                                            f32x4v a, const uint32_t (&bnd)[kMaxRuns],
-                                            const int32_t (&dl4)[kMaxRuns], uint32_t len,
-                                            const float* s_v, uint32_t lane) {
+                                            f32x4v a, const uint32_t (&pk)[kMaxRuns],
+                                            uint32_t len, const float* s_v, uint32_t lane) {
   float sum = 0.0f;
-  sliced_steps<R, false, kAux>(rv, 0u, qa, a, bnd, dl4, len, s_v, lane, sum);
-  sliced_steps<R, true, kAux>(rv, qa, nq, a, bnd, dl4, len, s_v, lane, sum);
+  sliced_steps<R, false, kAux>(rv, 0u, qa, a, pk, len, s_v, lane, sum);
+  sliced_steps<R, true, kAux>(rv, qa, nq, a, pk, len, s_v, lane, sum);
   return sum;
+}
+
+// Quads [q0, q1) of the stored values of a kSliceNz row: stored value t is
+// the row's entry st, the lowest set bit left in its mask (lo: entries
+// 0-31, hi: 32-63), whose list position comes from the runs as above.  The
+// products of the stored entries are added in entry order from 0.0, as the
+// full row's; its +-0 entries are left out (see nz_poisoned).  kGuard: steps
+// from the row's stored count cnt on are padding.
+template <uint32_t R, bool kGuard, int kAux>
+__device__ __forceinline__ void nz_steps(__amdgpu_buffer_rsrc_t rv, uint32_t q0, uint32_t q1,
+                                         f32x4v& a, const uint32_t (&pk)[kMaxRuns], uint32_t cnt,
+                                         uint32_t& lo, uint32_t& hi, const float* s_v,
+                                         uint32_t lane, float& sum) {
+  const char* s_b = reinterpret_cast<const char*>(s_v);
+  for (uint32_t q = q0; q < q1; ++q) {
+    const f32x4v v = a;
+    a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                       rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kAux));
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+      const bool in_lo = lo != 0u;
+      const uint32_t w = in_lo ? lo : hi;
+      const uint32_t st = (uint32_t)__builtin_ctzg(w, 32) + (in_lo ? 0u : 32u);
+      const uint32_t w1 = w & (w - 1u);
+      lo = in_lo ? w1 : lo;
+      hi = in_lo ? hi : w1;
+      const uint32_t off = run_off<R>(st, pk);
+      if constexpr (kGuard) {
+        const bool in = 4u * q + i < cnt;
+        const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
+        const float pr = v[i] * x;
+        sum = in ? sum + pr : sum;
+      } else {
+        const float x = *reinterpret_cast<const float*>(s_b + off);
+        sum = sum + v[i] * x;
+      }
+    }
+  }
+}
+
+template <uint32_t R, int kAux>
+__device__ __forceinline__ float nz_row(__amdgpu_buffer_rsrc_t rv, uint32_t qa, uint32_t nq,
+                                        f32x4v a, const uint32_t (&pk)[kMaxRuns], uint32_t cnt,
+                                        uint32_t lo, uint32_t hi, const float* s_v,
+                                        uint32_t lane) {
+  float sum = 0.0f;
+  nz_steps<R, false, kAux>(rv, 0u, qa, a, pk, cnt, lo, hi, s_v, lane, sum);
+  nz_steps<R, true, kAux>(rv, qa, nq, a, pk, cnt, lo, hi, s_v, lane, sum);
+  return sum;
+}
+
+// Whether a +-0 entry that a kSliceNz row leaves out meets a vector value
+// that is not finite: its product, and then the full row's sum, would be
+// NaN.  Runs only when the tile's staging found such a value (never on a
+// converging solve); the masks are read again rather than kept in registers.
+__device__ __forceinline__ bool nz_poisoned(__amdgpu_buffer_rsrc_t rm, const uint32_t (&pk)[kMaxRuns],
+                                            uint32_t len, const float* s_v, uint32_t lane) {
+  const char* s_b = reinterpret_cast<const char*>(s_v);
+  const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)(lane * 4u), 0, 0);
+  const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)((64u + lane) * 4u), 0, 0);
+  bool bad = false;
+  for (uint32_t st = 0; st < len && st < kNzMaxLen; ++st) {
+    if ((((st < 32u ? lo : hi) >> (st & 31u)) & 1u) != 0u) continue;
+    const float x = *reinterpret_cast<const float*>(s_b + run_off<kMaxRuns>(st, pk));
+    bad = bad || !__builtin_isfinite(x);
+  }
+  return bad;
 }
 
 // Sliced form (variant 4, layout in hbj_device.hpp): lane l sums its own
@@ -733,11 +807,22 @@ __device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t 
 // header, a scalar load) and sliced_issue (the row's runs and first
 // values); sliced_rows then stages the vector and streams the rest.
 struct SlicedPre {
-  __amdgpu_buffer_rsrc_t rv;
-  uint32_t nq, nr;
+  __amdgpu_buffer_rsrc_t rv, rm;  // values; masks (kSliceNz waves, else empty)
+  uint32_t nq, nr, nz;
   uint32_t run[kMaxRuns];
   f32x4v a;
 };
+
+// Row l's mask of stored entries in a kSliceNz wave, low and high words (0
+// elsewhere: the resource is empty there).  Loaded after the state, with the
+// staging: before it, two more registers held across the state's wait cost
+// the fused SpMV a wave per SIMD.
+template <int kAux>
+__device__ __forceinline__ uint2 nz_mask(const SlicedPre& p) {
+  const uint32_t lane = threadIdx.x & 63u;
+  return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(p.rm, (int)(lane * 4u), 0, kAux),
+                    __builtin_amdgcn_raw_buffer_load_b32(p.rm, (int)((64u + lane) * 4u), 0, kAux));
+}
 
 __device__ __forceinline__ SliceHdr sliced_hdr(const SysDesc& d, uint32_t tile) {
   const uint32_t w = uni((uint32_t)(threadIdx.x >> 6));
@@ -753,9 +838,12 @@ __device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h
                                              SlicedPre& p) {
   const uint32_t lane = threadIdx.x & 63u;
   const bool grp = (h.nr & kRunsPerGroup) != 0u;
+  const bool nz = (h.nr & kSliceNz) != 0u;
   p.nq = h.nq;
-  p.nr = h.nr & ~kRunsPerGroup;
-  p.rv = buf_rsrc(d.sval + h.voff, h.nq * 1024u);
+  p.nr = h.nr & ~(kRunsPerGroup | kSliceNz);
+  p.nz = nz ? 1u : 0u;
+  p.rm = buf_rsrc(d.sval + h.voff, nz ? 512u : 0u);
+  p.rv = buf_rsrc(d.sval + h.voff + (nz ? 128u : 0u), h.nq * 1024u);
   const uint32_t g0 = row0 / 6u;
   const uint32_t stride = grp ? (row0 + 63u) / 6u - g0 + 1u : 64u;
   const uint32_t slot = grp ? (row0 + lane) / 6u - g0 : lane;
@@ -771,16 +859,15 @@ __device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h
 template <int kAux, typename Stage>
 __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_v, Stage stage) {
   const uint32_t lane = threadIdx.x & 63u;
+  const uint2 m = nz_mask<kAux>(p);
   stage();
-  uint32_t bnd[kMaxRuns];
-  int32_t dl4[kMaxRuns];
+  uint32_t pk[kMaxRuns];
   uint32_t len = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kMaxRuns; ++k) {
-    const uint32_t run = k < p.nr ? p.run[k] : 0xFFFFu;
-    bnd[k] = run & 0xFFFFu;
-    dl4[k] = 4 * (int32_t)(int16_t)(run >> 16);
-    len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
+    pk[k] = k < p.nr ? p.run[k] : kRunUnused;
+    const uint32_t end = pk[k] >> 16;
+    len = (end != 0xFFFFu && end > len) ? end : len;
   }
   // the slice's shortest row: quads below it need no padding guard
   uint32_t ml = len;
@@ -790,18 +877,57 @@ __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_
   ml = min(ml, lane_down_u32<4, true>(ml));
   ml = min(ml, lane_down_u32<2, true>(ml));
   ml = min(ml, lane_down_u32<1, true>(ml));
+  if (p.nz) {
+    const uint32_t lo = m.x, hi = m.y;
+    const uint32_t cnt = (uint32_t)__builtin_popcount(lo) + (uint32_t)__builtin_popcount(hi);
+    uint32_t mc = cnt;
+    mc = min(mc, lane_down_u32<32, true>(mc));
+    mc = min(mc, lane_down_u32<16, true>(mc));
+    mc = min(mc, lane_down_u32<8, true>(mc));
+    mc = min(mc, lane_down_u32<4, true>(mc));
+    mc = min(mc, lane_down_u32<2, true>(mc));
+    mc = min(mc, lane_down_u32<1, true>(mc));
+    const uint32_t qz = min(uni(mc) / 4u, p.nq);
+    float sum;
+    switch (p.nr) {
+      case 0:
+      case 1: sum = nz_row<1, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      case 2: sum = nz_row<2, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      case 3: sum = nz_row<3, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      case 4: sum = nz_row<4, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      case 5: sum = nz_row<5, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      case 6: sum = nz_row<6, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      case 7: sum = nz_row<7, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
+      default: sum = nz_row<kMaxRuns, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane);
+    }
+    // the staging's flags: some staged value is not finite (workgroup-uniform)
+    const uint32_t* fl = reinterpret_cast<const uint32_t*>(s_v + kLocalMax);
+    if (uni(fl[0] | fl[1] | fl[2] | fl[3]) != 0u && nz_poisoned(p.rm, pk, len, s_v, lane))
+      sum = __builtin_nanf("");
+    return sum;
+  }
   const uint32_t qa = min(uni(ml) / 4u, p.nq);
   switch (p.nr) {
     case 0:
-    case 1: return sliced_row<1, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 2: return sliced_row<2, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 3: return sliced_row<3, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 4: return sliced_row<4, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 5: return sliced_row<5, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 6: return sliced_row<6, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    case 7: return sliced_row<7, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
-    default: return sliced_row<kMaxRuns, kAux>(p.rv, qa, p.nq, p.a, bnd, dl4, len, s_v, lane);
+    case 1: return sliced_row<1, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 2: return sliced_row<2, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 3: return sliced_row<3, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 4: return sliced_row<4, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 5: return sliced_row<5, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 6: return sliced_row<6, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 7: return sliced_row<7, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    default: return sliced_row<kMaxRuns, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
   }
+}
+
+// Each wave's flag after the staging barrier, at s_v[kLocalMax + wave] (the
+// LDS copy of the vector is declared with kWaves words beyond kLocalMax): 1
+// when a value it staged is not finite (a kSliceNz row's left-out zeros
+// then need nz_poisoned), else 0.
+__device__ __forceinline__ void flag_staged(bool bad, float* s_v) {
+  const bool any = __ballot(bad) != 0ull;
+  if ((threadIdx.x & 63u) == 0u)
+    reinterpret_cast<uint32_t*>(s_v + kLocalMax)[threadIdx.x >> 6] = any ? 1u : 0u;
 }
 
 // s_v[i] = g(tcols[off + i]) for i < m (m <= kLocalMax), then a barrier.
@@ -812,6 +938,7 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
                                             uint32_t m, G g, float* s_v) {
   constexpr int kSteps = (int)(kLocalMax / kTile);
   const uint32_t t = threadIdx.x;
+  bool bad = false;
   if (m > 0u) {
     uint32_t c[kSteps];
 #pragma unroll
@@ -826,8 +953,10 @@ __device__ __forceinline__ void stage_local(const uint32_t* __restrict__ tcols, 
     for (int k = 0; k < kSteps; ++k) {
       const uint32_t i = t + (uint32_t)k * kTile;
       if (i < m) s_v[i] = x[k];
+      bad = bad || (i < m && !__builtin_isfinite(x[k]));
     }
   }
+  flag_staged(bad, s_v);
   __syncthreads();
 }
 
@@ -847,11 +976,14 @@ __device__ __forceinline__ void stage_groups(const uint32_t* __restrict__ grp, u
   float x[kSteps];
 #pragma unroll
   for (int k = 0; k < kSteps; ++k) x[k] = g(c[k]);
+  bool bad = false;
 #pragma unroll
   for (int k = 0; k < kSteps; ++k) {
     const uint32_t i = t + (uint32_t)k * kTile;
     if (i < m) s_v[i] = x[k];
+    bad = bad || (i < m && !__builtin_isfinite(x[k]));
   }
+  flag_staged(bad, s_v);
   __syncthreads();
 }
 
@@ -1126,26 +1258,28 @@ __device__ __forceinline__ void spmv_ir_resid_sliced(const SysDesc& d, const Til
                                                      double* s_prod, bool fold = false,
                                                      double inv_scale = 0.0) {
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t bnd[kMaxRuns];
-  int32_t dl[kMaxRuns];
+  uint32_t pk[kMaxRuns];
   uint32_t len = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kMaxRuns; ++k) {
-    const uint32_t run = k < p.nr ? p.run[k] : 0xFFFFu;
-    bnd[k] = run & 0xFFFFu;
-    dl[k] = (int32_t)(int16_t)(run >> 16);
-    len = (bnd[k] != 0xFFFFu && bnd[k] > len) ? bnd[k] : len;
+    pk[k] = k < p.nr ? p.run[k] : kRunUnused;
+    const uint32_t end = pk[k] >> 16;
+    len = (end != 0xFFFFu && end > len) ? end : len;
   }
   const X64Eff xg = x64_eff(d, fold, inv_scale);
   const double xown = xg(threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0);
   double ax = 0.0;
+  const uint2 m = nz_mask<0>(p);
+  uint32_t j = 0;  // kSliceNz: stored values so far; the row's +-0 entries are
+                   // added as 0.0 (the full row's operations, NaN included)
   for (uint32_t st = 0; st < len; ++st) {
-    const float v = __builtin_bit_cast(
-        float, __builtin_amdgcn_raw_buffer_load_b32(p.rv, (int)(((st / 4u) * 64u + lane) * 16u + (st % 4u) * 4u), 0, 0));
-    int32_t dd = dl[kMaxRuns - 1];
-#pragma unroll
-    for (int k = (int)kMaxRuns - 2; k >= 0; --k) dd = st < bnd[k] ? dl[k] : dd;
-    const uint32_t pos = (uint32_t)((int32_t)st + dd);
+    const bool stored = !p.nz || (((st < 32u ? m.x : m.y) >> (st & 31u)) & 1u) != 0u;
+    const uint32_t e = p.nz ? j : st;
+    j += stored ? 1u : 0u;
+    const float vs = __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(p.rv, (int)(((e / 4u) * 64u + lane) * 16u + (e % 4u) * 4u), 0, 0));
+    const float v = stored ? vs : 0.0f;
+    const uint32_t pos = run_off<kMaxRuns>(st, pk) / 4u;
     uint32_t c;
     if (hdr.goff) {
       const uint32_t j = pos / 6u;
@@ -1288,7 +1422,7 @@ __device__ __forceinline__ void spmv_tile(const TileArgs& ta, const TileRef& tr,
 template <int V, bool kOne>
 __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
-  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax + kWaves : 1];  // tile-local copy of the vector
   TileRef tr;
   if (!get_tile<kOne>(ta, tr)) return;
   spmv_tile<V>(ta, tr, s_prod, s_v);
@@ -2128,7 +2262,7 @@ __device__ __forceinline__ void fspmv_tile(const TileArgs& ta, const TileRef& tr
 template <int V, bool kOne, bool kDot64>
 __global__ __launch_bounds__(kTile) void k_pcg_fspmv(const TileArgs ta) {
   __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
-  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax + kWaves : 1];  // tile-local copy of the vector
   __shared__ SysState s_ns;
   static_assert(sizeof(SysState) / 4 <= (uint32_t)kTile && sizeof(SysState) % 4 == 0,
                 "state copy by dwords");
@@ -2231,7 +2365,7 @@ __device__ __forceinline__ void ccg_tile_sliced(const TileArgs& ta, const TileRe
 template <int V, bool kOne>
 __global__ __launch_bounds__(kTile) void k_ccg_spmv(const TileArgs ta) {
   __shared__ __align__(16) double s_prod[kWaves * SpmvLds<V>::kSliceF / 2];
-  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax : 1];  // tile-local copy of the vector
+  __shared__ float s_v[SpmvLds<V>::kLocal ? kLocalMax + kWaves : 1];  // tile-local copy of the vector
   constexpr int kSliceF = SpmvLds<V>::kSliceF;
   constexpr bool kLocal = SpmvLds<V>::kLocal;
   TileRef tr;

@@ -366,7 +366,8 @@ hbj_status prepare_image(uint32_t n, uint32_t nnz, const uint32_t* row_ptr, cons
   im.lean = lean;
   const uint32_t nb = im.nb, nt = im.n_tiles ? im.n_tiles : 1u;
   build_tile_local(n, row_ptr, col_idx, im.lidx, im.tloc, im.tcols);
-  const bool sliced = build_sliced(n, row_ptr, values, im.lidx, im.tloc, im.sval, im.runs, im.shdr);
+  const bool sliced = build_sliced(n, row_ptr, values, im.lidx, im.tloc,
+                                   env_u32("HBJ_SLICED_NZ", 1) != 0, im.sval, im.runs, im.shdr);
   // column lists made of whole groups of 6 columns, also kept as their groups
   // (staged from by the sliced SpMV): goff[t] = 1 + offset in tgrp, or 0
   std::vector<uint32_t> goff(im.tloc.size(), 0u);
@@ -409,6 +410,7 @@ hbj_status prepare_image(uint32_t n, uint32_t nnz, const uint32_t* row_ptr, cons
   if (blocks6) im.flags |= kFlagBlocks6;
   if (sliced) {
     im.flags |= kFlagSliced;
+    for (const SliceHdr& h : im.shdr) im.sliced_nz_waves += (h.nr & kSliceNz) ? 1u : 0u;
     im.spmv_sliced_bytes = 4ull * im.sval.size() + 4ull * im.runs.size() +
                            (sizeof(SliceHdr) * kWaves + sizeof(TileHdr)) * im.n_tiles +
                            4ull * staged_columns + 8ull * n;
@@ -469,6 +471,7 @@ static hbj_system* new_system(hbj_ctx* ctx, const SystemImage& im) {
   s->local_columns = im.local_columns;
   s->spmv_layout_bytes = im.spmv_layout_bytes;
   s->spmv_sliced_bytes = im.spmv_sliced_bytes;
+  s->sliced_nz_waves = im.sliced_nz_waves;
   s->lean = im.no_csr();
   return s;
 }
@@ -674,6 +677,7 @@ hbj_status hbj_system_get_info(const hbj_system* s, hbj_system_info* out) {
   out->spmv_sliced_bytes = s->spmv_sliced_bytes;
   const bool two = s->eng_ready ? s->eng.fused : s->two_launch();
   out->launches_per_iteration = (s->alg == 1 || two) ? 2u : 3u;
+  out->sliced_nz_waves = s->sliced_nz_waves;
   return HBJ_OK;
 }
 
@@ -763,6 +767,30 @@ hbj_status hbj_system_get_x(hbj_system* s, float* x) {
   if (s->n == 0) return HBJ_OK;
   HIPCHK(hipSetDevice(s->ctx->device));
   HIPCHK(hipMemcpy(x, s->x, 4ull * s->n, hipMemcpyDeviceToHost));
+  return HBJ_OK;
+}
+
+// spmv_exec.rs:122-147 on the system's resident matrix: the iteration's SpMV
+// kernel (three-launch form) in its initial phase, q = A x with x as given.
+hbj_status hbj_system_spmv(hbj_system* s, const float* x, float* y) {
+  clear_error();
+  if (!s || (s->n && (!x || !y))) { set_error("hbj_system_spmv: bad argument"); return HBJ_E_INVALID; }
+  if (s->n == 0) return HBJ_OK;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  CHK(s->ensure_engine(0));
+  Engine& E = s->eng;
+  hipStream_t st = E.stream;
+  HIPCHK(hipMemcpyAsync(s->x, x, 4ull * s->n, hipMemcpyHostToDevice, st));
+  std::memset(E.h_init, 0, sizeof(SysState));
+  E.h_init[0].status = kRunning;
+  E.h_init[0].phase = kPhInit;
+  HIPCHK(hipMemcpyAsync(E.d_states, E.h_init, sizeof(SysState), hipMemcpyHostToDevice, st));
+  LaunchSet l = E.ls();
+  l.par = 0;
+  launch_pcg_spmv(l, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, s->q, 4ull * s->n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return HBJ_OK;
 }
 

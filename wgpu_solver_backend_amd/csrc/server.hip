@@ -434,13 +434,16 @@ void release_uploads(hbj_server* S) {
   S->up_stream = nullptr;
 }
 
-// Device bytes no request of these sizes can go below (its values once,
-// its LU blocks and block starts, the right-hand side and the solver's
-// vectors): requests above the budget by this much are rejected at once,
-// the others once their exact block is known.
-uint64_t min_request_bytes(uint32_t n, uint32_t nnz, uint32_t n_starts) {
+// Device bytes no request of these sizes can go below (its nonzero values
+// once -- every layout stores those, a sliced one may leave out the +-0
+// entries --, its LU blocks and block starts, the right-hand side and the
+// solver's vectors): requests above the budget by this much are rejected at
+// once, the others once their exact block is known.
+uint64_t min_request_bytes(uint32_t n, uint32_t nnz, const float* values, uint32_t n_starts) {
   const uint64_t nb = n_starts ? n_starts - 1u : 0u;
-  return 4ull * nnz + 76ull * n + 144ull * nb + 4ull * n_starts;
+  uint64_t stored = 0;
+  for (uint32_t e = 0; e < nnz; ++e) stored += values[e] != 0.0f ? 1u : 0u;
+  return 4ull * stored + 76ull * n + 144ull * nb + 4ull * n_starts;
 }
 
 }  // namespace
@@ -546,7 +549,7 @@ hbj_status hbj_server_submit(hbj_server* S, uint32_t n, uint32_t nnz, const uint
   r->n = n;
   r->nnz = nnz;
   r->n_starts = n_starts;
-  const uint64_t min_bytes = min_request_bytes(n, nnz, n_starts);
+  const uint64_t min_bytes = min_request_bytes(n, nnz, values, n_starts);
   std::lock_guard<std::mutex> g(S->mu);
   if (S->stop) {
     delete r;

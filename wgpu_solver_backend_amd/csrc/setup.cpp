@@ -176,14 +176,24 @@ void build_tile_local(uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
   });
 }
 
+// A value a kSliceNz wave stores: anything but +0.0 and -0.0 (bits, so that
+// denormals count as nonzero whatever the host's floating-point mode).
+static inline bool nonzero(float v) {
+  uint32_t b;
+  std::memcpy(&b, &v, 4);
+  return (b & 0x7FFFFFFFu) != 0u;
+}
+
 // Sliced SpMV layout (variant 4, layout in hbj_device.hpp) from the tile-local
 // tables.  False, with the outputs cleared, when the system does not
 // qualify: a tile without a column list, a row with more than kMaxRuns runs
-// or more than 16,384 entries, or padding that would take the values past
-// 1.25 nnz + 1,024 per tile.
+// or more than kSlicedMaxLen entries, or padding that would take the values past
+// 1.25 nnz + 1,024 per tile.  With allow_nz, a wave whose rows have at most
+// kNzMaxLen entries stores only their nonzero values (kSliceNz) when that
+// takes at most 80% of the bytes of all of them.
 bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
                   const std::vector<uint16_t>& lidx, const std::vector<uint2>& tloc,
-                  std::vector<float>& sval, std::vector<uint32_t>& runs,
+                  bool allow_nz, std::vector<float>& sval, std::vector<uint32_t>& runs,
                   std::vector<SliceHdr>& shdr) {
   const uint32_t nt = (n + kTile - 1) / kTile, ns = nt * (uint32_t)kWaves;
   sval.clear();
@@ -191,7 +201,9 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
   shdr.assign(ns, SliceHdr{0u, 0u, 0u, 0u});
   for (uint32_t t = 0; t < nt; ++t)
     if (tloc[t].y == kLocalOff) return false;
-  // a row's runs: maximal stretches of entries whose list positions step by one
+  // a row's runs: maximal stretches of entries whose list positions step by
+  // one, packed (end << 16) | (4 delta as 16 bits); |4 delta| < 2^15 since
+  // rows have at most kSlicedMaxLen entries and lists at most kLocalMax
   auto row_runs = [&](uint32_t r, uint32_t* out) -> uint32_t {
     const uint32_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
     uint32_t k = 0;
@@ -200,11 +212,17 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
       if (e == e0 || lidx[e] != (uint32_t)lidx[e - 1] + 1u) {
         if (k == kMaxRuns) return kMaxRuns + 1u;
         const int32_t dlt = (int32_t)lidx[e] - (int32_t)pos;
-        out[k++] = (uint32_t)(uint16_t)(int16_t)dlt << 16;
+        out[k++] = (uint32_t)(uint16_t)(int16_t)(4 * dlt);
       }
-      out[k - 1] = (out[k - 1] & 0xFFFF0000u) | (pos + 1u);  // end of the current run
+      out[k - 1] = (out[k - 1] & 0xFFFFu) | ((pos + 1u) << 16);  // end of the current run
     }
     return k;
+  };
+  // entries of row r whose value is not +-0 (the ones a kSliceNz wave stores)
+  auto stored_count = [&](uint32_t r) -> uint32_t {
+    uint32_t c = 0;
+    for (uint32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) c += nonzero(val[e]) ? 1u : 0u;
+    return c;
   };
   std::vector<uint8_t> bad(nt, 0);
   parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
@@ -212,12 +230,12 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
     for (uint32_t t = t0; t < t1; ++t) {
       for (uint32_t w = 0; w < (uint32_t)kWaves; ++w) {
         const uint32_t r0 = t * kTile + 64u * w;
-        uint32_t maxlen = 0, maxruns = 0, kprev = 0;
+        uint32_t maxlen = 0, maxruns = 0, kprev = 0, maxnz = 0;
         bool group = r0 + 64u <= n;  // a full wave whose groups of 6 share runs
         for (uint32_t l = 0; l < 64u && r0 + l < n; ++l) {
           const uint32_t r = r0 + l, len = row_ptr[r + 1] - row_ptr[r];
           const uint32_t k = row_runs(r, tmp);
-          if (len > 16384u || k > kMaxRuns) {
+          if (len > kSlicedMaxLen || k > kMaxRuns) {
             bad[t] = 1;
             group = false;
             continue;
@@ -229,10 +247,16 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
           kprev = k;
           maxlen = len > maxlen ? len : maxlen;
           maxruns = k > maxruns ? k : maxruns;
+          const uint32_t nzc = stored_count(r);
+          maxnz = nzc > maxnz ? nzc : maxnz;
         }
         SliceHdr& h = shdr[t * kWaves + w];
-        h.nq = (maxlen + 3u) / 4u;
-        h.nr = maxruns | (group ? kRunsPerGroup : 0u);
+        const uint32_t nq = (maxlen + 3u) / 4u, nq_nz = (maxnz + 3u) / 4u;
+        // values only: 1,024 bytes per step of 4, plus 512 bytes of masks
+        const bool nz = allow_nz && maxlen > 0 && maxlen <= kNzMaxLen &&
+                        5u * (1024u * nq_nz + 512u) <= 4u * 1024u * nq;
+        h.nq = nz ? nq_nz : nq;
+        h.nr = maxruns | (group ? kRunsPerGroup : 0u) | (nz ? kSliceNz : 0u);
       }
     }
   });
@@ -246,8 +270,8 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
     }
     shdr[i].voff = (uint32_t)vtot;
     shdr[i].roff = (uint32_t)rtot;
-    vtot += 256ull * shdr[i].nq;
-    const uint32_t nr = shdr[i].nr & ~kRunsPerGroup;
+    vtot += 256ull * shdr[i].nq + ((shdr[i].nr & kSliceNz) ? 128u : 0u);
+    const uint32_t nr = shdr[i].nr & ~(kRunsPerGroup | kSliceNz);
     rtot += (shdr[i].nr & kRunsPerGroup) ? (uint64_t)groups(i * 64u) * nr : 64ull * nr;
   }
   const uint64_t nnz = n ? row_ptr[n] : 0u;
@@ -256,7 +280,7 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
     return false;
   }
   sval.assign(vtot ? vtot : 1u, 0.0f);
-  runs.assign(rtot ? rtot : 1u, 0xFFFFu);
+  runs.assign(rtot ? rtot : 1u, kRunUnused);
   parallel_ranges(nt, 64, [&](uint32_t t0, uint32_t t1) {
     uint32_t tmp[kMaxRuns];
     for (uint32_t t = t0; t < t1; ++t) {
@@ -265,8 +289,22 @@ bool build_sliced(uint32_t n, const uint32_t* row_ptr, const float* val,
         const uint32_t r0 = t * kTile + 64u * w;
         for (uint32_t l = 0; l < 64u && r0 + l < n; ++l) {
           const uint32_t r = r0 + l, e0 = row_ptr[r], len = row_ptr[r + 1] - e0;
-          for (uint32_t s = 0; s < len; ++s)
-            sval[h.voff + 4u * (64u * (s / 4u) + l) + s % 4u] = val[e0 + s];
+          if (h.nr & kSliceNz) {
+            uint64_t mask = 0;
+            uint32_t j = 0;
+            for (uint32_t s = 0; s < len; ++s) {
+              if (!nonzero(val[e0 + s])) continue;
+              mask |= 1ull << s;
+              sval[h.voff + 128u + 4u * (64u * (j / 4u) + l) + j % 4u] = val[e0 + s];
+              ++j;
+            }
+            const uint32_t mw[2] = {(uint32_t)mask, (uint32_t)(mask >> 32)};
+            std::memcpy(&sval[h.voff + l], &mw[0], 4);
+            std::memcpy(&sval[h.voff + 64u + l], &mw[1], 4);
+          } else {
+            for (uint32_t s = 0; s < len; ++s)
+              sval[h.voff + 4u * (64u * (s / 4u) + l) + s % 4u] = val[e0 + s];
+          }
           const uint32_t k = row_runs(r, tmp);
           if (h.nr & kRunsPerGroup) {
             const uint32_t nbs = groups(r0), g = r / 6u - r0 / 6u;

@@ -333,7 +333,8 @@ def run_plain(H, ctx, N, steps, warmup, alg="pcg", sample_every=4):
                     "us": float(avg_us[0]), "gbps": gbps(sb, avg_us[0]),
                     "csr_bytes": b1, "csr_gbps": gbps(b1, avg_us[0]),
                     "autotune_us_per_iter": [float(x) for x in tune_us],
-                    "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"]},
+                    "local_tiles": info["local_tiles"], "n_tiles": info["n_tiles"],
+                    "nonzero_only_waves": info["sliced_nz_waves"], "waves": 4 * info["n_tiles"]},
            "iteration": {"kernel_us": {"spmv": float(avg_us[0]), "update": float(avg_us[1]),
                                        "finish": float(avg_us[2])},
                          "kernel_gbps_csr_accounting": {"spmv": gbps(b1, avg_us[0]),
@@ -346,8 +347,9 @@ def run_plain(H, ctx, N, steps, warmup, alg="pcg", sample_every=4):
     return out
 
 
-def run_hbm_only(H, ctx, N=64, iters=300):
-    """The SpMV on a lattice whose layout exceeds the 256 MB last-level cache:
+def run_hbm_only(H, ctx, N=100, iters=200):
+    """The SpMV on a lattice whose layout exceeds the 256 MB last-level cache
+    (100^3: about 595 MB with its nonzero-only waves, 6,000,000 unknowns):
     the autotuner's variant, HIP-event durations over `iters` iterations."""
     log(f"hbm-only: generating {N}^3 lattice")
     L = H.lattice(N, seed=0)
@@ -371,6 +373,7 @@ def run_hbm_only(H, ctx, N=64, iters=300):
     return {"lattice": f"{N}^3", "n": L["n"], "nnz": L["nnz"], "kernel": kernel, "variant": best,
             "layout": layout, "stream_bytes": sb, "us": float(avg_us[0]), "achieved": g,
             "frac": g / HBM_PEAK_GBPS, "traffic": t_bytes, "csr_bytes": info["spmv_csr_bytes"],
+            "nonzero_only_waves": info["sliced_nz_waves"], "waves": 4 * info["n_tiles"],
             "samples": int(cnt[0]), "autotune_us_per_iter": [float(x) for x in tune_us]}
 
 
@@ -471,7 +474,9 @@ def roofline_from_plain(P, N, hbm_only=None):
          "csr_accounting": {"bytes": sp_["csr_bytes"], "gbps": sp_["csr_gbps"],
                             "frac": sp_["csr_gbps"] / HBM_PEAK_GBPS},
          "note": (f"achieved = the bytes SpMV variant {P['best']} streams per launch ({sp_['layout']} "
-                  "layout: values, runs, column lists, headers, the rows' own p and q; with two "
+                  "layout: values -- in waves of rows with explicit zeros only the nonzero values "
+                  "and each row's 8-byte mask of stored entries --, runs, column lists, headers, the "
+                  "rows' own p and q; with two "
                   "launches per iteration also the rows' z and new p) / its mean "
                   "duration from HIP events on its own stream; csr_accounting divides the "
                   "reference's CSR bytes (8 nnz + 12 n + 4) by the same time; traffic = FETCH_SIZE + "
@@ -492,7 +497,7 @@ def bench_headline(args):
     B = run_batch(H, ctx, args, args.steps, args.warmup)
     S = run_serving(H, ctx, args) if not args.no_serve else None
     P = run_plain(H, ctx, 50, args.plain_steps, 1)
-    HB = run_hbm_only(H, ctx, 64) if not args.no_hbm_only else None
+    HB = run_hbm_only(H, ctx, args.hbm_N) if not args.no_hbm_only else None
     bsum = B["summary"]
     out = {
         "metric": METRIC,
@@ -695,6 +700,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-hbm-only", action="store_true")
+    ap.add_argument("--hbm-N", type=int, default=100,
+                    help="lattice side of the HBM-only SpMV point (layout past the last-level cache)")
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="inner FP32 PCG relative tolerance of the refinement loop (default: 1e-4 "
                          "for --workload ir, 1e-3 for the 1e-8 workloads)")

@@ -669,6 +669,19 @@ __device__ __forceinline__ float spmv_wave_local(uint32_t wb, uint32_t we, uint3
   return sum;
 }
 
+// What sliced_issue loads for a wave of a sliced tile before the state is
+// read: resources, sizes and flags from the slice header, the row's runs,
+// and its first quad of values (a) -- its first kPreQuads with kQ = kPreQuads
+// (ax holds quads 1 ..).
+constexpr int kPreQuads = 4;
+struct SlicedPre {
+  __amdgpu_buffer_rsrc_t rv, rm;  // values; masks (kSliceNz waves, else empty)
+  uint32_t nq, nr, nz;
+  uint32_t run[kMaxRuns];
+  f32x4v a;
+  f32x4v ax[kPreQuads - 1];
+};
+
 // Byte offset into s_v of entry st of a row with runs pk (packed as in
 // hbj_device.hpp), the search unrolled over the slice's R = nr runs (a
 // wave-uniform count): the first run k with (st << 16 | 0xFFFF) < pk[k]
@@ -690,40 +703,64 @@ __device__ __forceinline__ uint32_t run_off(uint32_t st, const uint32_t (&pk)[kM
 // q0's values on entry and quad q1's on return.  (Loading 2-4 quads ahead
 // instead measured slower in the fused SpMV: the registers cost a wave per
 // SIMD, 46.4 -> 50.0 us per 50^3 iteration at 3.)
+template <uint32_t R, bool kGuard>
+__device__ __forceinline__ void sliced_quad(const f32x4v v, uint32_t q,
+                                            const uint32_t (&pk)[kMaxRuns], uint32_t len,
+                                            const float* s_v, float& sum) {
+  const char* s_b = reinterpret_cast<const char*>(s_v);
+#pragma unroll
+  for (uint32_t i = 0; i < 4u; ++i) {
+    const uint32_t st = 4u * q + i;
+    const uint32_t off = run_off<R>(st, pk);
+    if constexpr (kGuard) {
+      const bool in = st < len;
+      const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
+      const float pr = v[i] * x;
+      sum = in ? sum + pr : sum;
+    } else {
+      const float x = *reinterpret_cast<const float*>(s_b + off);
+      sum = sum + v[i] * x;
+    }
+  }
+}
+
 template <uint32_t R, bool kGuard, int kAux>
 __device__ __forceinline__ void sliced_steps(__amdgpu_buffer_rsrc_t rv, uint32_t q0, uint32_t q1,
                                              f32x4v& a, const uint32_t (&pk)[kMaxRuns],
                                              uint32_t len, const float* s_v, uint32_t lane,
                                              float& sum) {
-  const char* s_b = reinterpret_cast<const char*>(s_v);
   for (uint32_t q = q0; q < q1; ++q) {
     const f32x4v v = a;
     a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
                                        rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kAux));
-#pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-      const uint32_t st = 4u * q + i;
-      const uint32_t off = run_off<R>(st, pk);
-      if constexpr (kGuard) {
-        const bool in = st < len;
-        const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
-        const float pr = v[i] * x;
-        sum = in ? sum + pr : sum;
-      } else {
-        const float x = *reinterpret_cast<const float*>(s_b + off);
-        sum = sum + v[i] * x;
-      }
-    }
+    sliced_quad<R, kGuard>(v, q, pk, len, s_v, sum);
   }
 }
 
-template <uint32_t R, int kAux>
+// A row's stream when its first kQ quads were loaded by sliced_issue (kQ =
+// kPreQuads: issued before the state, so a short row is in flight in full
+// during the workgroup's earlier phases); with kQ = 1 quad q + 1 is loaded
+// while quad q is summed.
+template <uint32_t R, int kAux, int kQ>
 __device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t qa, uint32_t nq,
-                                            f32x4v a, const uint32_t (&pk)[kMaxRuns],
+                                            const SlicedPre& p, const uint32_t (&pk)[kMaxRuns],
                                             uint32_t len, const float* s_v, uint32_t lane) {
   float sum = 0.0f;
-  sliced_steps<R, false, kAux>(rv, 0u, qa, a, pk, len, s_v, lane, sum);
-  sliced_steps<R, true, kAux>(rv, qa, nq, a, pk, len, s_v, lane, sum);
+  f32x4v a = p.a;
+  if constexpr (kQ == 1) {
+    sliced_steps<R, false, kAux>(rv, 0u, qa, a, pk, len, s_v, lane, sum);
+    sliced_steps<R, true, kAux>(rv, qa, nq, a, pk, len, s_v, lane, sum);
+  } else {
+    if (nq > 0u) sliced_quad<R, true>(p.a, 0u, pk, len, s_v, sum);
+#pragma unroll
+    for (uint32_t q = 1; q < (uint32_t)kQ; ++q)
+      if (nq > q) sliced_quad<R, true>(p.ax[q - 1], q, pk, len, s_v, sum);
+    if (nq > (uint32_t)kQ) {
+      a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                         rv, (int)(((uint32_t)kQ * 64u + lane) * 16u), 0, kAux));
+      sliced_steps<R, true, kAux>(rv, (uint32_t)kQ, nq, a, pk, len, s_v, lane, sum);
+    }
+  }
   return sum;
 }
 
@@ -733,46 +770,66 @@ __device__ __forceinline__ float sliced_row(__amdgpu_buffer_rsrc_t rv, uint32_t 
 // products of the stored entries are added in entry order from 0.0, as the
 // full row's; its +-0 entries are left out (see nz_poisoned).  kGuard: steps
 // from the row's stored count cnt on are padding.
+template <uint32_t R, bool kGuard>
+__device__ __forceinline__ void nz_quad(const f32x4v v, uint32_t q, const uint32_t (&pk)[kMaxRuns],
+                                        uint32_t cnt, uint32_t& lo, uint32_t& hi, const float* s_v,
+                                        float& sum) {
+  const char* s_b = reinterpret_cast<const char*>(s_v);
+#pragma unroll
+  for (uint32_t i = 0; i < 4u; ++i) {
+    const bool in_lo = lo != 0u;
+    const uint32_t w = in_lo ? lo : hi;
+    const uint32_t st = (uint32_t)__builtin_ctzg(w, 32) + (in_lo ? 0u : 32u);
+    const uint32_t w1 = w & (w - 1u);
+    lo = in_lo ? w1 : lo;
+    hi = in_lo ? hi : w1;
+    const uint32_t off = run_off<R>(st, pk);
+    if constexpr (kGuard) {
+      const bool in = 4u * q + i < cnt;
+      const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
+      const float pr = v[i] * x;
+      sum = in ? sum + pr : sum;
+    } else {
+      const float x = *reinterpret_cast<const float*>(s_b + off);
+      sum = sum + v[i] * x;
+    }
+  }
+}
+
 template <uint32_t R, bool kGuard, int kAux>
 __device__ __forceinline__ void nz_steps(__amdgpu_buffer_rsrc_t rv, uint32_t q0, uint32_t q1,
                                          f32x4v& a, const uint32_t (&pk)[kMaxRuns], uint32_t cnt,
                                          uint32_t& lo, uint32_t& hi, const float* s_v,
                                          uint32_t lane, float& sum) {
-  const char* s_b = reinterpret_cast<const char*>(s_v);
   for (uint32_t q = q0; q < q1; ++q) {
     const f32x4v v = a;
     a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
                                        rv, (int)(((q + 1u) * 64u + lane) * 16u), 0, kAux));
-#pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-      const bool in_lo = lo != 0u;
-      const uint32_t w = in_lo ? lo : hi;
-      const uint32_t st = (uint32_t)__builtin_ctzg(w, 32) + (in_lo ? 0u : 32u);
-      const uint32_t w1 = w & (w - 1u);
-      lo = in_lo ? w1 : lo;
-      hi = in_lo ? hi : w1;
-      const uint32_t off = run_off<R>(st, pk);
-      if constexpr (kGuard) {
-        const bool in = 4u * q + i < cnt;
-        const float x = *reinterpret_cast<const float*>(s_b + (in ? off : 0u));
-        const float pr = v[i] * x;
-        sum = in ? sum + pr : sum;
-      } else {
-        const float x = *reinterpret_cast<const float*>(s_b + off);
-        sum = sum + v[i] * x;
-      }
-    }
+    nz_quad<R, kGuard>(v, q, pk, cnt, lo, hi, s_v, sum);
   }
 }
 
-template <uint32_t R, int kAux>
+template <uint32_t R, int kAux, int kQ>
 __device__ __forceinline__ float nz_row(__amdgpu_buffer_rsrc_t rv, uint32_t qa, uint32_t nq,
-                                        f32x4v a, const uint32_t (&pk)[kMaxRuns], uint32_t cnt,
-                                        uint32_t lo, uint32_t hi, const float* s_v,
+                                        const SlicedPre& p, const uint32_t (&pk)[kMaxRuns],
+                                        uint32_t cnt, uint32_t lo, uint32_t hi, const float* s_v,
                                         uint32_t lane) {
   float sum = 0.0f;
-  nz_steps<R, false, kAux>(rv, 0u, qa, a, pk, cnt, lo, hi, s_v, lane, sum);
-  nz_steps<R, true, kAux>(rv, qa, nq, a, pk, cnt, lo, hi, s_v, lane, sum);
+  f32x4v a = p.a;
+  if constexpr (kQ == 1) {
+    nz_steps<R, false, kAux>(rv, 0u, qa, a, pk, cnt, lo, hi, s_v, lane, sum);
+    nz_steps<R, true, kAux>(rv, qa, nq, a, pk, cnt, lo, hi, s_v, lane, sum);
+  } else {
+    if (nq > 0u) nz_quad<R, true>(p.a, 0u, pk, cnt, lo, hi, s_v, sum);
+#pragma unroll
+    for (uint32_t q = 1; q < (uint32_t)kQ; ++q)
+      if (nq > q) nz_quad<R, true>(p.ax[q - 1], q, pk, cnt, lo, hi, s_v, sum);
+    if (nq > (uint32_t)kQ) {
+      a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                         rv, (int)(((uint32_t)kQ * 64u + lane) * 16u), 0, kAux));
+      nz_steps<R, true, kAux>(rv, (uint32_t)kQ, nq, a, pk, cnt, lo, hi, s_v, lane, sum);
+    }
+  }
   return sum;
 }
 
@@ -806,12 +863,7 @@ __device__ __forceinline__ bool nz_poisoned(__amdgpu_buffer_rsrc_t rm, const uin
 // issued in two steps around the state's load: sliced_hdr (the wave's slice
 // header, a scalar load) and sliced_issue (the row's runs and first
 // values); sliced_rows then stages the vector and streams the rest.
-struct SlicedPre {
-  __amdgpu_buffer_rsrc_t rv, rm;  // values; masks (kSliceNz waves, else empty)
-  uint32_t nq, nr, nz;
-  uint32_t run[kMaxRuns];
-  f32x4v a;
-};
+
 
 // Row l's mask of stored entries in a kSliceNz wave, low and high words (0
 // elsewhere: the resource is empty there).  Loaded after the state, with the
@@ -833,7 +885,7 @@ __device__ __forceinline__ SliceHdr sliced_hdr(const SysDesc& d, uint32_t tile) 
 // kAux: cache policy of the value loads -- non-temporal (variant 4), or the
 // default, which keeps a layout that fits the GPU's last-level cache there
 // from one SpMV to the next (variant 5).
-template <int kAux>
+template <int kAux, int kQ = 1>
 __device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h, uint32_t row0,
                                              SlicedPre& p) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -852,11 +904,16 @@ __device__ __forceinline__ void sliced_issue(const SysDesc& d, const SliceHdr& h
   for (uint32_t k = 0; k < kMaxRuns; ++k)
     p.run[k] = __builtin_amdgcn_raw_buffer_load_b32(rr, (int)((k * stride + slot) * 4u), 0, 0);
   p.a = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(p.rv, (int)(lane * 16u), 0, kAux));
+  // quads past the slice's nq lie outside the resource and read 0
+#pragma unroll
+  for (int q = 1; q < kQ; ++q)
+    p.ax[q - 1] = __builtin_bit_cast(f32x4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                                 p.rv, (int)(((uint32_t)q * 64u + lane) * 16u), 0, kAux));
 }
 
 // stage() fills s_v (the tile's LDS copy of the vector) while the loads of
 // sliced_issue are in flight.
-template <int kAux, typename Stage>
+template <int kAux, int kQ = 1, typename Stage>
 __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_v, Stage stage) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint2 m = nz_mask<kAux>(p);
@@ -891,14 +948,14 @@ __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_
     float sum;
     switch (p.nr) {
       case 0:
-      case 1: sum = nz_row<1, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      case 2: sum = nz_row<2, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      case 3: sum = nz_row<3, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      case 4: sum = nz_row<4, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      case 5: sum = nz_row<5, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      case 6: sum = nz_row<6, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      case 7: sum = nz_row<7, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane); break;
-      default: sum = nz_row<kMaxRuns, kAux>(p.rv, qz, p.nq, p.a, pk, cnt, lo, hi, s_v, lane);
+      case 1: sum = nz_row<1, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      case 2: sum = nz_row<2, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      case 3: sum = nz_row<3, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      case 4: sum = nz_row<4, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      case 5: sum = nz_row<5, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      case 6: sum = nz_row<6, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      case 7: sum = nz_row<7, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane); break;
+      default: sum = nz_row<kMaxRuns, kAux, kQ>(p.rv, qz, p.nq, p, pk, cnt, lo, hi, s_v, lane);
     }
     // the staging's flags: some staged value is not finite (workgroup-uniform)
     const uint32_t* fl = reinterpret_cast<const uint32_t*>(s_v + kLocalMax);
@@ -909,14 +966,14 @@ __device__ __forceinline__ float sliced_rows(const SlicedPre& p, const float* s_
   const uint32_t qa = min(uni(ml) / 4u, p.nq);
   switch (p.nr) {
     case 0:
-    case 1: return sliced_row<1, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    case 2: return sliced_row<2, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    case 3: return sliced_row<3, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    case 4: return sliced_row<4, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    case 5: return sliced_row<5, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    case 6: return sliced_row<6, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    case 7: return sliced_row<7, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
-    default: return sliced_row<kMaxRuns, kAux>(p.rv, qa, p.nq, p.a, pk, len, s_v, lane);
+    case 1: return sliced_row<1, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    case 2: return sliced_row<2, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    case 3: return sliced_row<3, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    case 4: return sliced_row<4, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    case 5: return sliced_row<5, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    case 6: return sliced_row<6, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    case 7: return sliced_row<7, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
+    default: return sliced_row<kMaxRuns, kAux, kQ>(p.rv, qa, p.nq, p, pk, len, s_v, lane);
   }
 }
 
@@ -1333,7 +1390,9 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   const SliceHdr sh = sliced_hdr(d, tr.tile);
   const int32_t status = uni(st->status), phase = uni(st->phase);
   SlicedPre pre;
-  sliced_issue<kAux>(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
+  // the three-launch SpMV (systems of more than kFetchMax tiles, whose
+  // layouts stream from HBM) issues a row's first kPreQuads quads at once
+  sliced_issue<kAux, kPreQuads>(d, sh, tr.row0 + 64u * uni((uint32_t)(threadIdx.x >> 6)), pre);
   if (status != kRunning) return;
   if (phase == kPhIrResid) {
     spmv_ir_resid_sliced(d, tr, hdr, pre, s_prod);
@@ -1347,7 +1406,7 @@ __device__ __forceinline__ void spmv_tile_sliced(const TileArgs& ta, const TileR
   // the row's own b or p for the dot, issued before the stream
   const uint32_t rowc = threadIdx.x < tr.nrows ? tr.row0 + threadIdx.x : tr.row0;
   const float a = init ? bld(d.b, rowc) : vec_at(rowc);
-  const float q = sliced_rows<kAux>(pre, s_v, [&]() {
+  const float q = sliced_rows<kAux, kPreQuads>(pre, s_v, [&]() {
     if (hdr.goff) stage_groups(d.tgrp, hdr.goff - 1u, hdr.cnt, vec_at, s_v);
     else stage_local(d.tcols, hdr.off, hdr.cnt, vec_at, s_v);
   });

@@ -2121,7 +2121,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
 // The thread's row of A v for the tile, v given by g(column): the sliced
 // layout (its loads issued by the caller), or the tile-local / HBM-gather
 // forms.  Every thread of the workgroup calls it (staging holds barriers).
-template <int V, typename G>
+template <int V, int kQ, typename G>
 __device__ __forceinline__ float spmv_rows(const SysDesc& d, const TileRef& tr, bool sliced,
                                            const TileHdr& hdr, const SlicedPre& pre, G g,
                                            double* s_prod, float* s_v) {
@@ -2134,7 +2134,7 @@ __device__ __forceinline__ float spmv_rows(const SysDesc& d, const TileRef& tr, 
   const uint32_t* __restrict__ tcols = d.tcols;
   if constexpr (V >= 4) {
     if (sliced)
-      return sliced_rows<V == 4 ? kNt : 0>(pre, s_v, [=]() {
+      return sliced_rows<V == 4 ? kNt : 0, kQ>(pre, s_v, [=]() {
         if (h_goff) stage_groups(tgrp, h_goff - 1u, h_cnt, g, s_v);
         else stage_local(tcols, h_off, h_cnt, g, s_v);
       });
@@ -2180,9 +2180,14 @@ __device__ __forceinline__ void fspmv_tile(const TileArgs& ta, const TileRef& tr
   TileHdr hdr{};
   SlicedPre pre;
   if constexpr (SpmvLds<V>::kLocal) hdr = uni_copy(d.thdr[tr.tile]);
+  // a table's FP64-accumulated solves (batches, serving: their partials are
+  // not prefetched) issue a row's first kPreQuads quads at once (256-system
+  // batch 744 -> 776 solves/s); a single system keeps its registers for the
+  // partials' prefetch (50^3: 21.9 against 23.4 us with both)
+  constexpr int kQF = (!kOne && kDot64) ? kPreQuads : 1;
   if constexpr (kSliced) {
     const SliceHdr sh = sliced_hdr(d, tr.tile);
-    sliced_issue<V == 4 ? kNt : 0>(d, sh, tr.row0 + 64u * uni((uint32_t)(t >> 6)), pre);
+    sliced_issue<V == 4 ? kNt : 0, kQF>(d, sh, tr.row0 + 64u * uni((uint32_t)(t >> 6)), pre);
   }
   // the previous update's r.z partials, prefetched with the state (FP64
   // ones only for a single system: in a table their 32 registers cost more
@@ -2297,7 +2302,7 @@ __device__ __forceinline__ void fspmv_tile(const TileArgs& ta, const TileRef& tr
     const __amdgpu_buffer_rsrc_t rx = buf_rsrc(d.x, d.n * 4u);
     auto xg = [&](uint32_t c) -> float { return buf_gather<float>(rx, c); };
     const float a = bld(d.b, rowc);
-    const float q = spmv_rows<V>(d, tr, sliced, hdr, pre, xg, s_prod, s_v);
+    const float q = spmv_rows<V, kQF>(d, tr, sliced, hdr, pre, xg, s_prod, s_v);
     spmv_tail(d, tr, q, true, a, s_prod);
     return;
   }
@@ -2314,7 +2319,7 @@ __device__ __forceinline__ void fspmv_tile(const TileArgs& ta, const TileRef& tr
   };
   const float a = pg(rowc);
   if (own) bst(pcur, row, a);
-  const float q = spmv_rows<V>(d, tr, sliced, hdr, pre, pg, s_prod, s_v);
+  const float q = spmv_rows<V, kQF>(d, tr, sliced, hdr, pre, pg, s_prod, s_v);
   spmv_tail(d, tr, q, false, a, s_prod);
 }
 

@@ -74,7 +74,12 @@ def _systems(ctx, hbj, sides, seed0=10, h=0.0):
     return out
 
 
-def test_batch_fp32_identical_to_alone(ctx, hbj, oracle):
+@pytest.mark.parametrize("fused", ["", "1"])
+def test_batch_fp32_identical_to_alone(ctx, hbj, oracle, monkeypatch, fused):
+    """A batch's table runs the three-launch iteration by default and the
+    two-launch one with HBJ_FUSED=1: both give each system's alone bits."""
+    monkeypatch.setenv("HBJ_FUSED", fused)
+    print(f"batch of 7, HBJ_FUSED={fused!r}", flush=True)
     sides = [4, 9, 5, 12, 7, 4, 10]
     sysl = _systems(ctx, hbj, sides)
     alone = []
@@ -117,7 +122,10 @@ def test_batch_isolates_failures(ctx, hbj):
         assert ra.iterations == r.iterations
 
 
-def test_batch_ir_identical_to_alone(ctx, hbj):
+@pytest.mark.parametrize("fused", ["", "1"])
+def test_batch_ir_identical_to_alone(ctx, hbj, monkeypatch, fused):
+    monkeypatch.setenv("HBJ_FUSED", fused)
+    print(f"refined batch of 5, HBJ_FUSED={fused!r}", flush=True)
     sides = [4, 8, 5, 11, 6]
     sysl = _systems(ctx, hbj, sides, seed0=30, h=1.5)
     p = hbj.IrParams(target_rel_res=1e-10)

@@ -91,12 +91,19 @@ inline uint32_t env_u32(const char* name, uint32_t dflt) {
 // inside hipGraphLaunch (a read of an unmapped address by the runtime's
 // intercepted dispatch path; the same graphs replay correctly without the
 // tool), so the engine launches the kernels directly while one is attached.
-// The two-launch PCG iteration (k_pcg_fspmv + k_pcg_update), the default
-// for tables whose systems have at most kFetchMaxTiles tiles; HBJ_FUSED=0
-// selects the three-launch one (k_pcg_spmv, k_pcg_update, k_pcg_finish)
-// everywhere.  Same arithmetic, same bits.  Read when a system's engine is
-// set up (a later change applies to engines set up after it).
-inline bool fused_mode() { return env_u32("HBJ_FUSED", 1) != 0; }
+// The two-launch PCG iteration (k_pcg_fspmv + k_pcg_update) or the
+// three-launch one (k_pcg_spmv, k_pcg_update, k_pcg_finish): same arithmetic,
+// same bits.  By default a single system of at most kFetchMaxTiles tiles runs
+// two launches (50^3 lattice: 32.9 against 35.2 us per iteration) and a table
+// of several systems three (its finish kernel reduces each system's partials
+// once per four tiles, and its SpMV holds fewer registers: 256-system batch
+// 828 against 777 solves/s, serving 67.5 against 63.8 requests/s);
+// HBJ_FUSED=1 selects two launches for tables too (where their systems
+// allow it), HBJ_FUSED=0 three everywhere.  Read when an engine is set up (a
+// later change applies to engines set up after it).
+inline uint32_t fused_env() { return env_u32("HBJ_FUSED", 2); }
+inline bool fused_mode() { return fused_env() != 0; }    // a single system
+inline bool fused_tables() { return fused_env() == 1; }  // a table of several
 
 inline bool profiler_tool_attached() {
   const char* v = std::getenv("ROCP_TOOL_LIBRARIES");

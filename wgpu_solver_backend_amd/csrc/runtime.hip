@@ -942,8 +942,9 @@ struct hbj_batch {
   hbj_status ensure(int m) {
     if (!ready) {
       CHK(eng.init(ctx->device, (uint32_t)sys.size(), n_tiles ? n_tiles : 1));
-      // the two-launch iteration when every system's partials fit the prefetch
-      bool fused = fused_mode();
+      // the two-launch iteration when every system's partials fit the
+      // prefetch, for a batch of one system or when HBJ_FUSED=1 asks for it
+      bool fused = sys.size() == 1 ? fused_mode() : fused_tables();
       for (const hbj_system* s : sys) fused = fused && s->two_launch();
       eng.fused = fused;
       ready = true;

@@ -469,6 +469,7 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   hbj_status s = e == hipSuccess ? S->eng.init(ctx->device, cfg->max_active, (uint32_t)tiles0) : HBJ_E_DEVICE;
   if (s == HBJ_OK) {
     S->eng.use_graph = false;  // the table changes between chunks
+    S->eng.fused = fused_tables();  // a table of many systems: three launches
     S->eng.n_tiles = 0;
     // slots start zeroed (status 0: final); no tile references a free slot
     const uint32_t cap = S->eng.cap;

@@ -464,12 +464,15 @@ def test_sliced_layout_mixed_groups_bitwise(ctx, hbj, oracle):
     sys_.close()
 
 
-def test_three_launch_form_above_prefetch_limit(ctx, hbj, oracle):
+def test_three_launch_form_above_prefetch_limit(ctx, hbj, oracle, monkeypatch):
     """A system of more than 4,096 tiles runs the three-launch iteration (its
-    r.z partials exceed what a workgroup prefetches); a smaller one the
-    two-launch iteration.  Both give the oracle's bits.  The large system is
-    a chain of 6x6 node blocks, diagonally dominant, so the oracle's solve is
-    short: 183,000 nodes, 1,098,000 rows (4,290 tiles)."""
+    r.z partials exceed what a workgroup prefetches) with the first-level
+    sums of its partials computed once per launch (k_level1); a smaller one
+    the two-launch iteration.  Both give the oracle's bits, and the
+    refinement of the large one (FP64 partials) gives the bits of the same
+    solve with every workgroup reducing every partial (HBJ_L1=0).  The large
+    system is a chain of 6x6 node blocks, diagonally dominant, so the oracle's
+    solve is short: 183,000 nodes, 1,098,000 rows (4,290 tiles)."""
     nodes = 183_000
     n = 6 * nodes
     rng = np.random.default_rng(21)
@@ -521,7 +524,17 @@ def test_three_launch_form_above_prefetch_limit(ctx, hbj, oracle):
     print(f"chain n={n}: {res.iterations} iterations", flush=True)
     assert res.status == st == 0 and res.iterations == it
     assert np.array_equal(bits(x), bits(xo))
+    b64 = b.astype(np.float64)
+    x1, r1 = s.solve_ir(b64)
     s.close()
+    monkeypatch.setenv("HBJ_L1", "0")
+    s3 = hbj.System(ctx, row_ptr, col, val, bs)
+    x3, r3 = s3.solve_ir(b64)
+    print(f"chain refinement: {r1.iterations} inner iterations, {r1.outer_iterations} outer", flush=True)
+    assert r1.status == r3.status == 0
+    assert (r1.iterations, r1.outer_iterations) == (r3.iterations, r3.outer_iterations)
+    assert np.array_equal(x1.view(np.uint64), x3.view(np.uint64))
+    s3.close()
     small = hbj.lattice(8, seed=1)
     s2 = hbj.System(ctx, small["row_ptr"], small["col_idx"], small["values"], small["block_starts"])
     assert s2.info()["launches_per_iteration"] == 2

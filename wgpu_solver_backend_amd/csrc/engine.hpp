@@ -766,6 +766,8 @@ struct hbj_system {
   float *ra = nullptr, *rb = nullptr;
   float* part = nullptr;      // 3 * n_tiles per-tile FP32 partials
   double* part64 = nullptr;  // 5 * n_tiles per-tile FP64 partials
+  float* l1 = nullptr;       // 3 * ceil(n_tiles / 256) first-level sums (kFlagL1)
+  double* l164 = nullptr;
   double* resid2 = nullptr;  // 2 * n_tiles FP64 partials of the residual check
   double *x64 = nullptr, *x0_64 = nullptr, *b64 = nullptr, *r64 = nullptr, *x64b = nullptr;
   float* b32 = nullptr;
@@ -818,6 +820,8 @@ struct hbj_system {
     for (float** v : {&x, &x0, &b, &p, &pn, &q, &z, &ra, &rb}) L.add(v, n_);
     L.add(&part, 3ull * nt);
     L.add(&part64, 5ull * nt);
+    L.add(&l1, 3ull * ((nt + 255) / 256));
+    L.add(&l164, 3ull * ((nt + 255) / 256));
     L.add(&resid2, 2ull * nt);
     L.add(&d_desc1, 1);
     L.add(&tile_blk, nt);
@@ -857,6 +861,8 @@ struct hbj_system {
     for (float** v : {&x, &x0, &b, &p, &pn, &q, &z, &ra, &rb}) L.add(v, n_);
     L.add(&part, 3ull * nt);
     L.add(&part64, 5ull * nt);
+    L.add(&l1, 3ull * ((nt + 255) / 256));
+    L.add(&l164, 3ull * ((nt + 255) / 256));
     L.add(&resid2, 2ull * nt);
     L.add(&d_desc1, 1);
     for (double** v : {&x64, &x0_64, &r64, &x64b}) L.add(v, n_);
@@ -870,7 +876,9 @@ struct hbj_system {
   // workgroups) measured faster (64^3 lattice: 89.5 vs 92.6 us per iteration).
   bool two_launch() const { return fused_mode() && n_tiles <= kFetchMaxTiles; }
 
-  SysDesc desc(bool ir, bool fused) const {
+  // l1: the engine launches k_level1 (kFlagL1; a single system's own engine
+  // in the three-launch iteration past kFetchMaxTiles tiles)
+  SysDesc desc(bool ir, bool fused, bool l1_sums = false) const {
     SysDesc d{};
     d.row_ptr = rp;
     d.col = col;
@@ -901,11 +909,14 @@ struct hbj_system {
     d.sval = sval;
     d.runs = runs;
     d.shdr = shdr;
+    d.l1 = l1;
+    d.l164 = l164;
     d.st = nullptr;
     d.n = n;
     d.nb = nb;
     d.n_tiles = n_tiles;
-    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u) | (fused ? kFlagTwoLaunch : 0u);
+    d.flags = flags | (ir ? (kFlagIr | kFlagDot64) : 0u) | (fused ? kFlagTwoLaunch : 0u) |
+              (l1_sums ? kFlagL1 : 0u);
     return d;
   }
 
@@ -965,7 +976,12 @@ struct hbj_system {
     if (eng_mode != mode) {
       std::vector<uint2> tiles(n_tiles);
       for (uint32_t t = 0; t < n_tiles; ++t) tiles[t] = make_uint2(0u, t);
-      CHK(eng.set_table({desc(mode == 1, eng.fused)}, tiles));
+      // first-level sums once per launch for a single system past the
+      // prefetch limit (its update and finish workgroups would each reduce
+      // every tile partial again)
+      const bool l1_sums = !eng.fused && n_tiles > kFetchMaxTiles && env_u32("HBJ_ONE", 1) != 0 &&
+                           env_u32("HBJ_L1", 1) != 0;
+      CHK(eng.set_table({desc(mode == 1, eng.fused, l1_sums)}, tiles));
       eng_mode = mode;
     }
     return HBJ_OK;

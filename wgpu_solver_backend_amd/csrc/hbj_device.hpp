@@ -117,6 +117,10 @@ constexpr uint32_t kFlagSliced = 16u;   // the sliced SpMV layout (variant 4) ex
 constexpr uint32_t kFlagTwoLaunch = 32u; // the two-launch iteration (k_pcg_fspmv + k_pcg_update):
                                         // the refinement residual folds the correction on the
                                         // fly into x64b, which k_pcg_update copies to x64
+constexpr uint32_t kFlagL1 = 64u;       // a single system of more than 4,096 tiles in the
+                                        // three-launch iteration: k_level1 reduces each chunk
+                                        // of 256 partials of slots 0-2 into l1 / l164 before
+                                        // the kernel that needs their totals
 
 struct alignas(128) SysState {
   // FP32 PCG scalars, the reference's mailbox slots (compute.rs:271-277)
@@ -169,6 +173,8 @@ struct SysDesc {
   const float* sval;         // sliced layout (kFlagSliced): values,
   const uint32_t* runs;      // per-row runs,
   const SliceHdr* shdr;      // and per wave of each tile its offsets and sizes
+  float* l1;                 // kFlagL1: first-level sums of the FP32 partials, slot s
+  double* l164;              // chunk g at [s * ceil(n_tiles / 256) + g]; FP64 ones
   SysState* st;
   uint32_t n, nb, n_tiles, flags;
 };

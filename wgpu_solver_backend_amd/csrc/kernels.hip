@@ -304,6 +304,27 @@ __device__ __forceinline__ T reduce_partials(const T* __restrict__ part, uint32_
   return lds[0];
 }
 
+// The last level of reduce_partials over g first-level sums that k_level1
+// already computed (kFlagL1): the same tree, so the same result.  Run by all
+// 256 threads of a workgroup.
+template <typename T, bool kLean = false>
+__device__ __forceinline__ T reduce_last(const T* __restrict__ l1, uint32_t g, T* lds) {
+  if (g <= 1u) return l1[0];
+  if ((threadIdx.x >> 6) == 0) {
+    const uint32_t lane = threadIdx.x & 63u;
+    T v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = lane + 64u * j;
+      v[j] = i < g ? l1[i] : T(0);
+    }
+    const T r = wave_tree256<T, kLean>(v[0], v[1], v[2], v[3]);
+    if (lane == 0) lds[0] = r;
+  }
+  __syncthreads();
+  return lds[0];
+}
+
 // Values the code knows to be equal in every lane (a system's state, a
 // tile's block range), moved to scalar registers.  The compiler cannot always
 // prove it (a load from memory the kernel also writes, an LDS read), and a
@@ -1487,6 +1508,59 @@ __global__ __launch_bounds__(kTile) void k_pcg_spmv(const TileArgs ta) {
   spmv_tile<V>(ta, tr, s_prod, s_v);
 }
 
+// The total of partial slot `slot` (0: p.Ap or b.b, 1: r.r, 2: r.z) of
+// system d: reduce_partials over its n_tiles partials, or, with kFlagL1, the
+// last level over the first-level sums k_level1 stored (the same bits).
+template <typename T, bool kLean = false>
+__device__ __forceinline__ T slot_total(const SysDesc& d, uint32_t slot, const T* part, T* lds) {
+  const uint32_t nt = d.n_tiles;
+  if (d.flags & kFlagL1) {
+    const uint32_t g = (nt + 255u) / 256u;
+    const T* l1 = std::is_same<T, double>::value ? reinterpret_cast<const T*>(d.l164)
+                                                 : reinterpret_cast<const T*>(d.l1);
+    return reduce_last<T, kLean>(l1 + slot * g, g, lds);
+  }
+  return reduce_partials<T, kLean>(part, nt, lds);
+}
+
+// First level of reduce_partials -- each chunk of 256 partials reduced by
+// the tree of level_pass -- for the slots the next kernel of a kFlagL1
+// system needs, computed once instead of in every one of its workgroups:
+// which 0 (after the SpMV): slot 0 in FP32, and in FP64 as well for a
+// system that accumulates its dots in FP64; which 1 (after the update):
+// slots 1 and 2 in the system's dot type.  Wave w of workgroup b reduces
+// chunk (4 b + w) mod g of job (4 b + w) / g.
+__global__ __launch_bounds__(kTile) void k_level1(const SysDesc d, uint32_t which) {
+  const uint32_t nt = d.n_tiles, g = (nt + 255u) / 256u;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t w = blockIdx.x * 4u + uni((uint32_t)(threadIdx.x >> 6));
+  const bool dot64 = (d.flags & kFlagDot64) != 0u;
+  const uint32_t jobs = which == 0u ? (dot64 ? 2u : 1u) : 2u;
+  if (w >= jobs * g) return;
+  const uint32_t job = w / g, c = w - job * g;
+  const uint32_t slot = which == 0u ? 0u : 1u + job;
+  const bool f64 = which == 0u ? job == 1u : dot64;
+  if (f64) {
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = c * 256u + lane + 64u * j;
+      v[j] = i < nt ? d.part64[slot * nt + i] : 0.0;
+    }
+    const double r = wave_tree256<double>(v[0], v[1], v[2], v[3]);
+    if (lane == 0) d.l164[slot * g + c] = r;
+  } else {
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = c * 256u + lane + 64u * j;
+      v[j] = i < nt ? d.part[slot * nt + i] : 0.0f;
+    }
+    const float r = wave_tree256<float>(v[0], v[1], v[2], v[3]);
+    if (lane == 0) d.l1[slot * g + c] = r;
+  }
+}
+
 // Refinement decision from the FP64 residual partials (k_pcg_spmv, phase
 // kPhIrResid): 0 = converged, 5 = out of outer steps, kRunning = restart the
 // inner solve on r64 * (1/||r64||).  Computed identically by every workgroup
@@ -1578,7 +1652,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
     TP* lds = reinterpret_cast<TP*>(s_ra);
     // lean trees: this kernel's occupancy is set by its register count
     const TP tot =
-        pre ? reduce_fetched<TP, true>(vp, nt, lds) : reduce_partials<TP, true>(part_pap, nt, lds);
+        pre ? reduce_fetched<TP, true>(vp, nt, lds) : slot_total<TP, true>(d, 0u, part_pap, lds);
     if constexpr (kDot64) {
       alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
       if (tr.tile == 0 && t == 0) {
@@ -1594,7 +1668,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
       }
     }
   } else if (!restart) {
-    const float bn = reduce_partials<float, true>(d.part, nt, reinterpret_cast<float*>(s_ra));
+    const float bn = slot_total<float, true>(d, 0u, d.part, reinterpret_cast<float*>(s_ra));
     if (tr.tile == 0 && t == 0) st->b_norm2 = bn;
     if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
   } else {
@@ -1772,7 +1846,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
   if (iter) {
     TP* lds = reinterpret_cast<TP*>(s_ra);
     const TP tot =
-        pre ? reduce_fetched<TP, true>(vp, nt, lds) : reduce_partials<TP, true>(part_pap, nt, lds);
+        pre ? reduce_fetched<TP, true>(vp, nt, lds) : slot_total<TP, true>(d, 0u, part_pap, lds);
     if constexpr (kDot64) {
       alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
       if (tile0 == 0 && t == 0) {
@@ -1788,7 +1862,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
       }
     }
   } else if (!restart) {
-    const float bn = reduce_partials<float, true>(d.part, nt, reinterpret_cast<float*>(s_ra));
+    const float bn = slot_total<float, true>(d, 0u, d.part, reinterpret_cast<float*>(s_ra));
     if (tile0 == 0 && t == 0) st->b_norm2 = bn;
     if (bn == 0.0f) return;  // Ok(0), x untouched (compute.rs:252-254)
   } else {
@@ -2037,7 +2111,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   const bool beta_only = maybe_beta_only && phase == kPhIter;
   if (reduce && beta_only) {
     TP* lb = reinterpret_cast<TP*>(s_b);
-    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
+    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : slot_total<TP>(d, 2u, part + 2u * nt, lb);
     const float rz_old = s_ns.rz_old;
     const float beta = rz_old != 0.0f ? (float)b / rz_old : 0.0f;  // pcg_transition's beta
     load_rows();
@@ -2052,8 +2126,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   if (reduce) {
     TP* la = reinterpret_cast<TP*>(s_a);
     TP* lb = reinterpret_cast<TP*>(s_b);
-    const TP a = pre_a ? reduce_fetched<TP>(va, nt, la) : reduce_partials<TP>(part + nt, nt, la);
-    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : reduce_partials<TP>(part + 2u * nt, nt, lb);
+    const TP a = pre_a ? reduce_fetched<TP>(va, nt, la) : slot_total<TP>(d, 1u, part + nt, la);
+    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : slot_total<TP>(d, 2u, part + 2u * nt, lb);
     rr = (float)a;
     rz = (float)b;
     if constexpr (kDot64) rz64 = b;
@@ -2901,7 +2975,7 @@ __global__ void k_op_update_scalars(float* s, uint32_t i_pap, uint32_t i_rz_new,
 // measured time excludes the dispatch gap before the kernel.
 template <typename... A, typename K>
 static void launch(const LaunchSet& ls, K k, dim3 g, dim3 b, hipStream_t s, A... args) {
-  if (ls.ev_start)
+  if (ls.ev_start || ls.ev_stop)
     hipExtLaunchKernelGGL(k, g, b, 0, s, ls.ev_start, ls.ev_stop, 0, args...);
   else
     hipLaunchKernelGGL(k, g, b, 0, s, args...);
@@ -2937,7 +3011,27 @@ void launch_pcg_spmv(const LaunchSet& ls, hipStream_t s) {
     default: launch_spmv_v<0>(ls, s); break;
   }
 }
-void launch_pcg_update(const LaunchSet& ls, hipStream_t s) {
+// k_level1 ahead of the update (which 0) or the finish (which 1) of a
+// kFlagL1 system; with kernel timing the pair is timed as one step (the
+// start event on k_level1, the stop event on the kernel after it).
+static bool launch_level1(LaunchSet& ls, uint32_t which, hipStream_t s) {
+  if (!ls.one || !(ls.one->flags & kFlagL1)) return false;
+  const SysDesc& d = *ls.one;
+  const uint32_t g = (d.n_tiles + 255u) / 256u;
+  const uint32_t jobs = which == 0u ? ((d.flags & kFlagDot64) ? 2u : 1u) : 2u;
+  const dim3 grid((jobs * g + 3u) / 4u), b(kTile);
+  if (ls.ev_start) {
+    hipExtLaunchKernelGGL(k_level1, grid, b, 0, s, ls.ev_start, nullptr, 0, d, which);
+    ls.ev_start = nullptr;
+  } else {
+    hipLaunchKernelGGL(k_level1, grid, b, 0, s, d, which);
+  }
+  return true;
+}
+
+void launch_pcg_update(const LaunchSet& ls0, hipStream_t s) {
+  LaunchSet ls = ls0;
+  launch_level1(ls, 0u, s);
   const TileArgs a = tile_args(ls);
   const dim3 g(ls.n_tiles), b(kTile);
   if (ls.update_groups && ls.one) {
@@ -2975,7 +3069,9 @@ static void launch_finish_t(const LaunchSet& ls, const TileArgs& a, hipStream_t 
     launch(ls, k_pcg_finish<false, kDot64>, dim3(ls.n_ftiles ? ls.n_ftiles : 1), dim3(kTile), s, ag);
   }
 }
-void launch_pcg_finish(const LaunchSet& ls, hipStream_t s) {
+void launch_pcg_finish(const LaunchSet& ls0, hipStream_t s) {
+  LaunchSet ls = ls0;
+  launch_level1(ls, 1u, s);
   const TileArgs a = tile_args(ls);
   if (ls.dot64) launch_finish_t<true>(ls, a, s);
   else launch_finish_t<false>(ls, a, s);

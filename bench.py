@@ -374,6 +374,8 @@ def run_hbm_only(H, ctx, N=100, iters=200):
             "layout": layout, "stream_bytes": sb, "us": float(avg_us[0]), "achieved": g,
             "frac": g / HBM_PEAK_GBPS, "traffic": t_bytes, "csr_bytes": info["spmv_csr_bytes"],
             "nonzero_only_waves": info["sliced_nz_waves"], "waves": 4 * info["n_tiles"],
+            "iteration_kernel_us": {"spmv": float(avg_us[0]), "update": float(avg_us[1]),
+                                    "finish": float(avg_us[2])},
             "samples": int(cnt[0]), "autotune_us_per_iter": [float(x) for x in tune_us]}
 
 

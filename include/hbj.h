@@ -107,6 +107,21 @@ hbj_status hbj_spmv_tile_tables(uint32_t n, uint32_t nnz, const uint32_t* row_pt
                                 uint16_t* lidx, uint32_t* tcols, uint64_t tcols_cap,
                                 uint64_t* n_tcols);
 
+/* The sliced SpMV layout hbj_system_create would build for this matrix
+ * (host only, nothing on the device): whether the system qualifies for it,
+ * its 64-row waves, how many of them store only their rows' nonzero values
+ * (allow_nz != 0; 0 keeps every entry, as HBJ_SLICED_NZ=0 does), and the
+ * layout's value words (values, the nonzero-only waves' masks and padding)
+ * and run words.  No reference counterpart: the setup of this SpMV's layout,
+ * exposed for tests. */
+typedef struct {
+  uint32_t sliced, waves, nz_waves, reserved0;
+  uint64_t value_words, run_words;
+} hbj_sliced_info;
+hbj_status hbj_sliced_layout_info(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                                  const uint32_t* col_idx, const float* values, int allow_nz,
+                                  hbj_sliced_info* out);
+
 /* ------------------------------------- single-op executors (device) */
 /* Each runs the same device code as the fused solver kernels on host buffers. */
 

@@ -165,6 +165,21 @@ def spmv_tile_tables(row_ptr, col_idx):
 
 # ------------------------------------------------------------- executors
 
+def sliced_layout_info(row_ptr, col_idx, values, allow_nz=True):
+    """The sliced SpMV layout a system of this matrix gets (host only): dict of
+    sliced (0/1), waves, nz_waves (waves storing only their nonzero values),
+    value_words and run_words."""
+    rp = _u32(row_ptr)
+    ci = _u32(col_idx)
+    va = _f32(values)
+    inf = _lib.hbj_sliced_info()
+    ci1, va1 = _nz(ci, np.uint32), _nz(va, np.float32)
+    _check(_lib.lib().hbj_sliced_layout_info(rp.size - 1, ci.size, rp, ci1.ctypes.data,
+                                             va1.ctypes.data, 1 if allow_nz else 0,
+                                             ctypes.byref(inf)))
+    return {f: getattr(inf, f) for f, _ in inf._fields_ if f != "reserved0"}
+
+
 class VecOpsExecutor:
     """VecOpsExecutor (compute/vec_ops_exec.rs:23-239)."""
 

@@ -54,6 +54,11 @@ class hbj_system_info(ctypes.Structure):
                 ("sliced_nz_waves", c_u32)]
 
 
+class hbj_sliced_info(ctypes.Structure):
+    _fields_ = [("sliced", c_u32), ("waves", c_u32), ("nz_waves", c_u32), ("reserved0", c_u32),
+                ("value_words", c_u64), ("run_words", c_u64)]
+
+
 class hbj_compare_stats(ctypes.Structure):
     _fields_ = [("n", c_u32), ("n_worst", c_u32), ("l2_ref", c_f64), ("l2_err", c_f64),
                 ("rel_l2", c_f64), ("rmse", c_f64), ("max_abs_err", c_f32), ("reserved0", c_f32)]
@@ -68,6 +73,7 @@ _SIGS = {
     "hbj_ctx_describe": (c_i32, [c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
     "hbj_lu_factor_inplace_6": (c_i32, [f32p, c_u32]),
     "hbj_build_lu_blocks_6": (c_i32, [c_u32, u32p, c_u32, u32p, c_u32, f32p, c_u32, u32p, c_u32, f32p]),
+    "hbj_sliced_layout_info": (c_i32, [c_u32, c_u32, u32p, c_void_p, c_void_p, c_i32, c_void_p]),
     "hbj_spmv_tile_tables": (c_i32, [c_u32, c_u32, u32p, u32p, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_u64, P(c_u64)]),
     "hbj_op_axpy": (c_i32, [c_void_p, c_u32, c_f32, f32p, f32p]),

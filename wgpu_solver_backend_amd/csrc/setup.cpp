@@ -482,4 +482,32 @@ hbj_status hbj_spmv_tile_tables(uint32_t n, uint32_t nnz, const uint32_t* row_pt
   return HBJ_OK;
 }
 
+hbj_status hbj_sliced_layout_info(uint32_t n, uint32_t nnz, const uint32_t* row_ptr,
+                                  const uint32_t* col_idx, const float* values, int allow_nz,
+                                  hbj_sliced_info* out) {
+  clear_error();
+  const uint32_t bs[2] = {0u, n};
+  const hbj_status v = validate_case(n, nnz, row_ptr, col_idx, bs, n ? 2u : 1u);
+  if (v != HBJ_OK) return v;
+  if (!out || (nnz && !values)) {
+    set_error("hbj_sliced_layout_info: null argument");
+    return HBJ_E_INVALID;
+  }
+  std::vector<uint16_t> lh;
+  std::vector<uint2> tl;
+  std::vector<uint32_t> tc, runs;
+  std::vector<float> sval;
+  std::vector<SliceHdr> shdr;
+  build_tile_local(n, row_ptr, col_idx, lh, tl, tc);
+  std::memset(out, 0, sizeof *out);
+  out->sliced = build_sliced(n, row_ptr, values, lh, tl, allow_nz != 0, sval, runs, shdr) ? 1u : 0u;
+  if (out->sliced) {
+    out->waves = (uint32_t)shdr.size();
+    for (const SliceHdr& h : shdr) out->nz_waves += (h.nr & kSliceNz) ? 1u : 0u;
+    out->value_words = sval.size();
+    out->run_words = runs.size();
+  }
+  return HBJ_OK;
+}
+
 }  // extern "C"

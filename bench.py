@@ -148,6 +148,9 @@ def run_batch(H, ctx, args, steps, warmup):
     elapsed = time.perf_counter() - t0
     value = steps * len(systems) / elapsed
     log(f"batch: {steps} steps of {len(systems)} systems in {elapsed:.3f} s = {value:.1f} solves/s")
+    # per-kernel durations from an untimed solve (the timed ones replay the
+    # graphs without events)
+    kern = batch_kernel_rates(batch, systems, best, params)
     # FP64 verification on the host of every system's last result
     dev_res = [r.rel_res_true_fp64 for r in results]
     host_res = [host_rel_res(L, s.get_x64()) for L, s in zip(Ls, systems)]
@@ -162,6 +165,8 @@ def run_batch(H, ctx, args, steps, warmup):
             _, ra = s.solve_ir(L["rhs"].astype(np.float64), None, params)
             same &= ra.iterations == r.iterations
         alone_sps = len(systems) / (time.perf_counter() - t1)
+    # the same systems refined to 1e-10: batched, and one by one
+    r10 = run_batch_1e10(H, batch, systems, Ls, args)
     ns = [L["n"] for L in Ls]
     out = {
         "value": value, "elapsed": elapsed, "step_ms": step_ms, "latency_ms": lat,
@@ -177,11 +182,94 @@ def run_batch(H, ctx, args, steps, warmup):
             "speedup_vs_per_system": value / alone_sps if alone_sps else None,
             "iterations_identical_alone_vs_batched": same,
             "spmv_variant": best, "autotune_us_per_iter": [float(u) for u in tune_us],
+            "refined_to_1e-10": r10,
         },
+        "kernels": kern,
     }
     batch.close()
     for s in systems:
         s.close()
+    return out
+
+
+def run_batch_1e10(H, batch, systems, Ls, args, steps=2):
+    """The batch refined to a 1e-10 relative residual (FP32 inner PCG,
+    FP64 residuals and dots): solves/s batched (one warmup, `steps` timed),
+    every result verified on the host, and the same refinement one system
+    at a time."""
+    p10 = H.IrParams(max_outer=40, max_inner=20000, inner_rel_tol=args.inner_tol, target_rel_res=1e-10)
+    st, _ = batch.solve_ir(p10)
+    if st != 0:
+        raise RuntimeError(f"batch 1e-10 warmup status {st}: {H._lib.last_error()}")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, res = batch.solve_ir(p10)
+        if st != 0:
+            raise RuntimeError(f"batch 1e-10 status {st}: {H._lib.last_error()}")
+    el = time.perf_counter() - t0
+    value = steps * len(systems) / el
+    host = max(host_rel_res(L, s.get_x64()) for L, s in zip(Ls, systems))
+    out = {"solves_per_s": value, "worst_true_rel_residual_fp64_host": host,
+           "all_verified_1e-10": bool(host <= 1e-10 * 1.001),
+           "inner_iterations_mean": float(np.mean([r.iterations for r in res])),
+           "outer_iterations_max": int(max(r.outer_iterations for r in res)),
+           "latency_ms": {"p50": pct([r.solve_ms for r in res], 50),
+                          "p99": pct([r.solve_ms for r in res], 99)}}
+    if not args.skip_per_system:
+        t1 = time.perf_counter()
+        for L, s in zip(Ls, systems):
+            _, ra = s.solve_ir(L["rhs"].astype(np.float64), None, p10)
+        alone = len(systems) / (time.perf_counter() - t1)
+        out["per_system_path_solves_per_s"] = alone
+        out["speedup_vs_per_system"] = value / alone
+    log(f"batch to 1e-10: {value:.1f} solves/s, worst host residual {host:.2e}"
+        + (f", per-system path {out['per_system_path_solves_per_s']:.1f}" if "per_system_path_solves_per_s" in out else ""))
+    return out
+
+
+def batch_kernel_rates(batch, systems, variant, params, every=4):
+    """HIP-event durations of the batch's three kernels over one untimed
+    solve (every `every`-th launch set) and the bytes each sampled launch
+    streams: a launch set works only on the systems still running (a finished
+    system's workgroups exit before any load), so its algorithmic bytes are
+    those systems' -- the SpMV's layout with the rows' own p and q, the
+    update's 28 n + 144 nb (x rw, p, q, r in and out, z; the 6x6 factors),
+    the finish's 12 n (p rw, z).  Sets that run a system's FP64 refinement
+    residual (3-4 of its ~1,100) are counted as iterations."""
+    per = []
+    for s in systems:
+        info = s.info()
+        sb, _ = spmv_stream_bytes(info, variant)
+        per.append((sb, 28 * info["n"] + 144 * info["n_blocks"], 12 * info["n"]))
+    per = np.array(per, np.float64)
+    batch.set_kernel_timing(every)
+    st, _ = batch.solve_ir(params)
+    batch.set_kernel_timing(0)
+    if st != 0:
+        raise RuntimeError(f"batch timing solve status {st}")
+    rec, sets = batch.kernel_samples()
+    names = ("spmv", "update", "finish")
+    out = {"samples": int(len(rec)), "every": every, "kernel": {}}
+    full_sets = int(sets.min())
+    for k, nm in enumerate(names):
+        t = rec[:, 1 + k]
+        ok = t >= 0
+        if not ok.any():
+            continue
+        live_bytes = np.array([per[sets > int(i), k].sum() for i in rec[ok, 0]])
+        tt = t[ok]
+        full = rec[ok, 0] < full_sets  # every system still running
+        e = {"us_mean": float(tt.mean()), "bytes_mean": float(live_bytes.mean()),
+             "gbps": float(live_bytes.sum() / (tt.sum() * 1e-6) / 1e9)}
+        if full.any():
+            e["all_systems"] = {"samples": int(full.sum()), "us_mean": float(tt[full].mean()),
+                                "bytes": float(per[:, k].sum()),
+                                "gbps": float(per[:, k].sum() / (tt[full].mean() * 1e-6) / 1e9)}
+        out["kernel"][nm] = e
+    sp = out["kernel"]["spmv"]
+    log(f"batch kernels: spmv {sp['us_mean']:.1f} us, {sp['gbps']:.0f} GB/s over the solve; "
+        f"all systems running: {sp.get('all_systems', {}).get('us_mean', 0):.1f} us, "
+        f"{sp.get('all_systems', {}).get('gbps', 0):.0f} GB/s")
     return out
 
 
@@ -489,6 +577,32 @@ def roofline_from_plain(P, N, hbm_only=None):
     return r
 
 
+def roofline_from_batch(B, P, HB):
+    """The headline workload's dominant kernel: the table SpMV of the
+    256-system batch (its layouts, about 0.4 GB, exceed the 256 MB last-level
+    cache; the autotuner's non-temporal variant 4).  achieved = the bytes the
+    sampled launches stream (each launch's running systems: their sliced
+    layouts with the rows' own p and q) / the launches' HIP-event time, over
+    an untimed solve of the same batch; the plain 50^3 point (and its 100^3
+    HBM-only point) as before."""
+    k = B["kernels"]["kernel"]["spmv"]
+    traffic = traffic_from_profiles("batch256")
+    t_bytes = (traffic or {}).get("k_pcg_spmv_bytes")
+    r = {"bound": "hbm", "kernel": "k_pcg_spmv (table of systems)",
+         "workload": "256-system batch, FP64 refinement to 1e-8: the SpMV launches of one solve",
+         "achieved": k["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
+         "traffic": t_bytes, "algorithmic_bytes": k["bytes_mean"], "kernel_us": k["us_mean"],
+         "all_systems_running": k.get("all_systems"),
+         "note": ("achieved = mean algorithmic bytes per launch (the running systems' sliced layouts: "
+                  "values -- only the nonzero ones and an 8-byte mask per row in waves of rows with "
+                  "explicit zeros --, runs, column lists, headers, the rows' own p and q) / mean "
+                  "HIP-event duration, over the sampled launches of an untimed solve; traffic = "
+                  "FETCH_SIZE + WRITE_SIZE per launch with all 256 systems running (rocprofv3, "
+                  "profiles/traffic_batch256.json), to compare with all_systems_running.bytes"),
+         "plain_fp32_50": roofline_from_plain(P, 50, HB)}
+    return r
+
+
 def bench_headline(args):
     import wgpu_solver_backend_amd as H
     ctx = H.GpuContext.create(0)
@@ -536,7 +650,7 @@ def bench_headline(args):
                           "true_rel_residual_fp64": P["res"].rel_res_true_fp64,
                           "spmv": P["spmv"], "iteration": P["iteration"]},
         "stream_copy_gbps": copy_gbps,
-        "roofline": roofline_from_plain(P, 50, HB),
+        "roofline": roofline_from_batch(B, P, HB),
     }
     if not args.no_cpu:
         log("cpu baseline: the batch workload on the host cores")
@@ -606,6 +720,43 @@ def bench_profile(args):
     return None
 
 
+def bench_profile_batch(args):
+    """Counter runs of the table kernels: the 256-system batch in one
+    refinement step of `--profile-iters` inner iterations per system (tolerances
+    that cannot be met), so every launch works on all 256 systems (the two
+    FP64 residual launches aside); the engine's SpMV variant (4, or
+    HBJ_SPMV_VARIANT)."""
+    import wgpu_solver_backend_amd as H
+    ctx = H.GpuContext.create(0)
+    sides = lattice_sides(0, args.batch, 4, 24)
+    Ls = [H.lattice(N, seed=1000 + i) for i, N in enumerate(sides)]
+    systems = [H.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"]) for L in Ls]
+    for L, s in zip(Ls, systems):
+        s.set_rhs64(L["rhs"].astype(np.float64))
+    batch = H.Batch(ctx, systems)
+    copy_bytes = 256 << 20
+    copy_us = ctx.stream_copy_us(copy_bytes, 10)
+    p = H.IrParams(max_outer=1, max_inner=args.profile_iters, inner_rel_tol=1e-30, target_rel_res=1e-300)
+    for _ in range(max(args.steps, 1)):
+        st, res = batch.solve_ir(p)
+        log(f"profile batch run: status {st}, {res[0].iterations} inner iterations per system")
+    variant = int(os.environ.get("HBJ_SPMV_VARIANT", "4"))
+    per = np.zeros(3)
+    for s in systems:
+        info = s.info()
+        per += (spmv_stream_bytes(info, variant)[0], 28 * info["n"] + 144 * info["n_blocks"], 12 * info["n"])
+    print(json.dumps({"profile_run": True, "workload": "batch256", "variant": variant,
+                      "iterations_per_solve": args.profile_iters,
+                      "algorithmic_bytes": {"k_pcg_spmv": per[0], "k_pcg_update_grp": per[1],
+                                            "k_pcg_finish": per[2]},
+                      "copy_bytes": copy_bytes, "copy_us": copy_us}), flush=True)
+    batch.close()
+    for s in systems:
+        s.close()
+    ctx.close()
+    return None
+
+
 def bench_ir(args):
     """Mixed precision at 50^3 with E varied over 3 decades: FP32 PCG inside FP64
     refinement to a 1e-10 relative residual (verified on the host in FP64)."""
@@ -662,7 +813,7 @@ def bench_batch(args):
            "config": {"workload": f"batched: {bsum['systems']} beam lattices, sides log-uniform [4,24], "
                                   f"FP64 refinement to 1e-8 (inner rel tol {args.inner_tol:g})"},
            "latency_ms": {"p50": pct(B["latency_ms"], 50), "p99": pct(B["latency_ms"], 99)},
-           "batch": bsum}
+           "batch": bsum, "kernels": B["kernels"]}
     ctx.close()
     return out
 
@@ -687,7 +838,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=50)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "plain", "ir", "batch", "serve", "profile"])
+                    choices=["headline", "plain", "ir", "batch", "serve", "profile", "profile-batch"])
     ap.add_argument("--alg", default="pcg", choices=["pcg", "one-reduction"],
                     help="plain workload: the reference's PCG (default) or the one-reduction PCG")
     ap.add_argument("--plain-steps", type=int, default=5,
@@ -726,10 +877,11 @@ def main():
         sys.exit(2)
     if args.inner_tol is None:
         args.inner_tol = 1e-4 if args.workload == "ir" else 1e-3
-    if args.profile_iters:
+    if args.profile_iters and args.workload != "profile-batch":
         args.workload = "profile"
     fn = {"headline": bench_headline, "plain": bench_plain, "ir": bench_ir, "batch": bench_batch,
-          "serve": bench_serve, "profile": bench_profile}[args.workload]
+          "serve": bench_serve, "profile": bench_profile,
+          "profile-batch": bench_profile_batch}[args.workload]
     out = fn(args)
     if out is not None:
         print(json.dumps(out), flush=True)

@@ -64,7 +64,9 @@ typedef struct hbj_result {
 typedef struct hbj_ir_params {
   uint32_t max_outer;          /* refinement steps, e.g. 20 */
   uint32_t max_inner;          /* FP32 PCG iterations per step, e.g. 5000 */
-  float inner_rel_tol;         /* FP32 PCG relative tolerance per step, e.g. 1e-4 */
+  float inner_rel_tol;         /* FP32 PCG relative tolerance per step, e.g. 1e-4; a step
+                                  that needs less reduction to reach the target solves to
+                                  0.8 times the reduction it needs instead */
   float reserved0;
   double target_rel_res;       /* stop when ||b - A x||/||b|| (FP64) <= this, e.g. 1e-10 */
 } hbj_ir_params;
@@ -312,6 +314,22 @@ hbj_status hbj_batch_solve_ir(hbj_batch* batch, const hbj_ir_params* params,
  * counterpart. */
 hbj_status hbj_batch_autotune(hbj_batch* batch, uint32_t iters, int ir, uint32_t* best,
                               double* us);
+/* Sample HIP-event timings of the per-iteration kernels during subsequent
+ * solves of this batch: every `every`-th launch set (0 = off); the samples
+ * then launch directly instead of replaying the captured graphs.  No
+ * reference counterpart. */
+hbj_status hbj_batch_set_kernel_timing(hbj_batch* batch, uint32_t every);
+/* The last timed solve's samples: record i is rec[i * (1 + HBJ_ITER_KERNELS)
+ * ...]: the launch set's index within the solve, then each kernel's
+ * duration in microseconds (-1: the iteration form has no such launch).
+ * Only sets in which some system still ran are kept.  *n_out receives the
+ * number of records (at most `cap` are written); sets (count entries, may be
+ * null) receives, per system, the launch sets that found it running: system
+ * i was part of sets [0, sets[i]) of the solve, so the work of a sampled set
+ * follows from the systems with sets[i] above its index.  No reference
+ * counterpart. */
+hbj_status hbj_batch_kernel_samples(hbj_batch* batch, double* rec, uint64_t cap, uint64_t* n_out,
+                                    uint32_t* sets);
 
 /* ------------------------------------------------------------ serving */
 

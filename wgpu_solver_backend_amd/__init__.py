@@ -460,6 +460,22 @@ class Batch:
         _check(_lib.lib().hbj_batch_autotune(self._h, int(iters), int(bool(ir)), ctypes.byref(best), us))
         return int(best.value), us
 
+    def set_kernel_timing(self, every):
+        """hbj_batch_set_kernel_timing: sample every `every`-th launch set (0 = off)."""
+        _check(_lib.lib().hbj_batch_set_kernel_timing(self._h, int(every)))
+
+    def kernel_samples(self):
+        """hbj_batch_kernel_samples: (records, sets) of the last timed solve --
+        records[i] = (launch set, then each kernel's us, -1 where not
+        launched); sets[j] = launch sets that found system j running."""
+        n = ctypes.c_uint64()
+        _check(_lib.lib().hbj_batch_kernel_samples(self._h, None, 0, ctypes.byref(n), None))
+        rec = np.zeros((max(int(n.value), 1), 1 + ITER_KERNELS), np.float64)
+        sets = np.zeros(len(self.systems), np.uint32)
+        _check(_lib.lib().hbj_batch_kernel_samples(self._h, rec.ctypes.data, int(n.value),
+                                                   ctypes.byref(n), sets.ctypes.data))
+        return rec[: int(n.value)], sets
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().hbj_batch_destroy(self._h)

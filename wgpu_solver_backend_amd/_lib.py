@@ -106,6 +106,8 @@ _SIGS = {
     "hbj_batch_solve": (c_i32, [c_void_p, c_u32, c_f32, c_f32, P(hbj_result)]),
     "hbj_batch_solve_ir": (c_i32, [c_void_p, P(hbj_ir_params), P(hbj_result)]),
     "hbj_batch_autotune": (c_i32, [c_void_p, c_u32, ctypes.c_int, P(c_u32), f64p]),
+    "hbj_batch_set_kernel_timing": (c_i32, [c_void_p, c_u32]),
+    "hbj_batch_kernel_samples": (c_i32, [c_void_p, c_void_p, c_u64, P(c_u64), c_void_p]),
     "hbj_server_create": (c_i32, [c_void_p, P(hbj_server_config), P(c_void_p)]),
     "hbj_server_destroy": (None, [c_void_p]),
     "hbj_server_submit": (c_i32, [c_void_p, c_u32, c_u32, u32p, u32p, f32p, u32p, c_u32, f64p,

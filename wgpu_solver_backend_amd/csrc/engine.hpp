@@ -161,6 +161,10 @@ struct Engine {
   static constexpr int kKernels = HBJ_ITER_KERNELS;  // spmv, update, finish
   double t_sum[kKernels] = {0, 0, 0};
   uint64_t t_cnt[kKernels] = {0, 0, 0};
+  // the last timed run: its samples of launch sets in which some system ran,
+  // and each system's launch sets (system i ran in sets [0, sets_last[i]))
+  std::vector<std::array<double, HBJ_ITER_KERNELS + 1>> t_last;
+  std::vector<uint32_t> sets_last;
   // SysState of system i: d_states[i] (parity 0) and d_states[cap + i]
   // (parity 1); chunks hold an even number of iterations, so copy 0 is
   // current whenever the host looks.
@@ -430,12 +434,16 @@ struct Engine {
   // and the chunk queued behind it) exit at once and would bias the means.
   void commit_timing(const SysState* init, const SysState* out, uint32_t n) {
     uint64_t live = 0;
+    sets_last.assign(n, 0u);
     for (uint32_t i = 0; i < n; ++i) {
       const uint64_t s = (uint64_t)(out[i].sets - init[i].sets);
+      sets_last[i] = (uint32_t)s;
       live = s > live ? s : live;
     }
+    t_last.clear();
     for (const auto& smp : t_pending) {
       if ((uint64_t)smp[0] >= live) continue;
+      t_last.push_back(smp);
       for (int k = 0; k < kKernels; ++k) {
         if (smp[k + 1] < 0.0) continue;
         t_sum[k] += smp[k + 1];

@@ -1068,6 +1068,34 @@ hbj_status hbj_batch_solve_ir(hbj_batch* b, const hbj_ir_params* p, hbj_result* 
   return worst;
 }
 
+hbj_status hbj_batch_set_kernel_timing(hbj_batch* b, uint32_t every) {
+  clear_error();
+  if (!b) { set_error("hbj_batch_set_kernel_timing: null batch"); return HBJ_E_INVALID; }
+  HIPCHK(hipSetDevice(b->ctx->device));
+  CHK(b->ensure(b->mode < 0 ? 0 : b->mode));
+  b->eng.timing_every = every;
+  return HBJ_OK;
+}
+
+hbj_status hbj_batch_kernel_samples(hbj_batch* b, double* rec, uint64_t cap, uint64_t* n_out,
+                                    uint32_t* sets) {
+  clear_error();
+  if (!b || !n_out || (cap && !rec)) {
+    set_error("hbj_batch_kernel_samples: bad argument");
+    return HBJ_E_INVALID;
+  }
+  const Engine& E = b->eng;
+  const uint64_t n = b->ready ? (uint64_t)E.t_last.size() : 0u;
+  *n_out = n;
+  constexpr uint32_t kRec = Engine::kKernels + 1;
+  for (uint64_t i = 0; i < n && i < cap; ++i)
+    for (uint32_t k = 0; k < kRec; ++k) rec[i * kRec + k] = E.t_last[i][k];
+  if (sets)
+    for (uint32_t i = 0; i < (uint32_t)b->sys.size(); ++i)
+      sets[i] = b->ready && i < E.sets_last.size() ? E.sets_last[i] : 0u;
+  return HBJ_OK;
+}
+
 hbj_status hbj_batch_autotune(hbj_batch* b, uint32_t iters, int ir, uint32_t* best, double* us) {
   clear_error();
   if (!b || iters == 0) { set_error("hbj_batch_autotune: need a batch and iters > 0"); return HBJ_E_INVALID; }

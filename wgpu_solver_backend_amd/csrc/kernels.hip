@@ -461,6 +461,56 @@ __device__ __forceinline__ double buf_gather<double>(__amdgpu_buffer_rsrc_t r, u
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i << 3), 0, 0));
 }
 
+// A prefetch of up to kLightMax partials for the table kernels that
+// accumulate in FP64 (the [4][4] prefetch would hold 32 registers per slot):
+// thread t loads slot t of chunks 0 and 1 (partials t and 256 + t) with
+// range-checked buffer loads (0 past len; no branch, so nothing waits for
+// them before they are used), before the kernel reads the state.
+// reduce_light then runs reduce_partials' trees over them: each chunk's tree
+// as tree256 (slot t = thread t, the pairs of wave_tree256), and for two
+// chunks the second level over the chunk sums in one wave -- the same bits.
+// For 1 < len <= kLightMax.
+constexpr uint32_t kLightMax = 512;
+__device__ __forceinline__ void light_fetch(const double* src, uint32_t len, bool on, double (&v)[2]) {
+  const __amdgpu_buffer_rsrc_t r = buf_rsrc(src, on ? len * 8u : 0u);
+  v[0] = buf_gather<double>(r, threadIdx.x);
+  v[1] = buf_gather<double>(r, 256u + threadIdx.x);
+}
+
+// reduce_partials over the light_fetch values of one slot (a) or two (a, b:
+// both trees in one pass).  lds_a, lds_b: 256 values each; s_c: 4 values.
+template <bool kTwo>
+__device__ __forceinline__ void reduce_light(const double (&va)[2], const double (&vb)[2],
+                                             uint32_t len, double* lds_a, double* lds_b,
+                                             double* s_c, double& a, double& b) {
+  const uint32_t out_len = (len + 255u) / 256u;  // 1 or 2
+#pragma unroll
+  for (uint32_t k = 0; k < 2u; ++k) {
+    if (k >= out_len) break;
+    double x = va[k], y = kTwo ? vb[k] : 0.0;
+    if constexpr (kTwo) tree256x2<double>(x, y, lds_a, lds_b);
+    else x = tree256<double>(x, lds_a);
+    if (threadIdx.x == 0) {
+      s_c[k] = x;
+      if constexpr (kTwo) s_c[2u + k] = y;
+    }
+  }
+  __syncthreads();
+  if (out_len > 1u) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (wave < (kTwo ? 2u : 1u)) {  // wave 0: slot a, wave 1: slot b
+      const double* c = s_c + 2u * wave;
+      const double w0 = lane < out_len ? c[lane] : 0.0;
+      const double r = wave_tree256<double>(w0, 0.0, 0.0, 0.0);
+      if (lane == 0) s_c[2u * wave] = r;
+    }
+    __syncthreads();
+  }
+  a = s_c[0];
+  if constexpr (kTwo) b = s_c[2];
+}
+
+
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 
@@ -1774,6 +1824,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
   __shared__ float s_lu2[kLuPer * kTile];  // and of odd ones
   __shared__ double s_ra[kTile];
   __shared__ double s_rb[kTile];
+  __shared__ double s_c[4];
   __shared__ uint32_t s_blk[2];
   using TP = typename std::conditional<kDot64, double, float>::type;
   TileRef tr;
@@ -1837,6 +1888,12 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
                               : reinterpret_cast<const TP*>(d.part);
   TP vp[4][4];
   if (pre) level_fetch<TP>(part_pap, nt, vp);
+  // ... except in a table, for a system of at most kLightMax tiles: two per
+  // thread, which cost 4 registers
+  constexpr bool kLightT = !kOne && kDot64;
+  const bool light = kLightT && nt > 1u && nt <= kLightMax;
+  double lvp[2];
+  if constexpr (kLightT) light_fetch(d.part64, nt, light, lvp);
   load(tile0, L, s_lu);
   Ld Ln;  // the next tile's round 1
   const int32_t status = uni(st->status), phase = uni(st->phase);
@@ -1858,8 +1915,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
   double scale = 0.0;
   if (iter) {
     TP* lds = reinterpret_cast<TP*>(s_ra);
-    const TP tot =
-        pre ? reduce_fetched<TP, true>(vp, nt, lds) : slot_total<TP, true>(d, 0u, part_pap, lds);
+    TP tot;
+    if (kLightT && light) {
+      double unused = 0.0;
+      if constexpr (kLightT) reduce_light<false>(lvp, lvp, nt, s_ra, s_rb, s_c, tot, unused);
+    } else {
+      tot = pre ? reduce_fetched<TP, true>(vp, nt, lds) : slot_total<TP, true>(d, 0u, part_pap, lds);
+    }
     if constexpr (kDot64) {
       alpha = tot != 0.0 ? (float)(rz64_old / tot) : 0.0f;
       if (tile0 == 0 && t == 0) {
@@ -2057,6 +2119,7 @@ template <bool kOne, bool kDot64>
 __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   __shared__ double s_a[256];
   __shared__ double s_b[256];
+  __shared__ double s_c[4];
   __shared__ SysState s_ns;
   constexpr uint32_t kWords = sizeof(SysState) / 4;
   static_assert(kWords <= (uint32_t)kTile && sizeof(SysState) % 4 == 0, "state copy by dwords");
@@ -2083,6 +2146,15 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   // than the extra load round; a single system's one round gains from them)
   const bool pre = (kOne || !kDot64) && nt > 1u && nt <= kFetchMax;
   TP va[4][4], vb[4][4];
+  // ... except those of a system of at most kLightMax tiles (every system of
+  // the batches and most served requests): two per thread, before the state
+  constexpr bool kLightT = !kOne && kDot64;
+  const bool light = kLightT && nt > 1u && nt <= kLightMax;
+  double lva[2], lvb[2];
+  if constexpr (kLightT) {
+    light_fetch(d.part64 + nt, nt, light, lva);
+    light_fetch(d.part64 + 2u * nt, nt, light, lvb);
+  }
   // workgroups that will only need beta (below) do not fetch the r.r partials
   const bool maybe_beta_only = !kDot64 && !writer && !(d.flags & kFlagIr);
   const bool pre_a = pre && !maybe_beta_only;
@@ -2144,8 +2216,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   if (reduce) {
     TP* la = reinterpret_cast<TP*>(s_a);
     TP* lb = reinterpret_cast<TP*>(s_b);
-    const TP a = pre_a ? reduce_fetched<TP>(va, nt, la) : slot_total<TP>(d, 1u, part + nt, la);
-    const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : slot_total<TP>(d, 2u, part + 2u * nt, lb);
+    TP a, b;
+    if (kLightT && light) {
+      if constexpr (kLightT) reduce_light<true>(lva, lvb, nt, s_a, s_b, s_c, a, b);
+    } else {
+      a = pre_a ? reduce_fetched<TP>(va, nt, la) : slot_total<TP>(d, 1u, part + nt, la);
+      b = pre ? reduce_fetched<TP>(vb, nt, lb) : slot_total<TP>(d, 2u, part + 2u * nt, lb);
+    }
     rr = (float)a;
     rz = (float)b;
     if constexpr (kDot64) rz64 = b;

@@ -2158,25 +2158,32 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   // workgroups that will only need beta (below) do not fetch the r.r partials
   const bool maybe_beta_only = !kDot64 && !writer && !(d.flags & kFlagIr);
   const bool pre_a = pre && !maybe_beta_only;
-  // the rows' z and p for an iteration's p-update, loaded once beta is
-  // known (rows past the system read row 0).  Issuing them before the state
-  // and the partials measured no faster, and costs 16 VGPRs.
+  // the rows' z and p for an iteration's p-update (range-checked loads: rows
+  // past the system read 0 and are not stored).  A single system's kernel
+  // loads them once beta is known (issuing them before the state measured
+  // no faster there, and costs registers); a table's FP64 kernel issues them
+  // with its partials (kRowsEarly).
   constexpr uint32_t kT = kFinishTiles;
   float zv[kT], pv[kT];
-  uint32_t rw[kT];
+  auto row_of = [&](uint32_t k) -> uint32_t {  // 0xFFFFFFFF: no row
+    const uint32_t tile = tr.tile + k * step;
+    const uint32_t row = tile * (uint32_t)kTile + t;
+    return (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
+  };
   auto load_rows = [&]() {
+    const __amdgpu_buffer_rsrc_t rz = buf_rsrc(d.z, d.n * 4u), rp = buf_rsrc(d.p, d.n * 4u);
 #pragma unroll
     for (uint32_t k = 0; k < kT; ++k) {
-      const uint32_t tile = tr.tile + k * step;
-      const uint32_t row = tile * (uint32_t)kTile + t;
-      rw[k] = (tile < nt && row < d.n) ? row : 0xFFFFFFFFu;
-      const uint32_t rc = rw[k] != 0xFFFFFFFFu ? row : 0u;
-      zv[k] = bld(d.z, rc);
-      pv[k] = bld(d.p, rc);
+      const uint32_t row = (tr.tile + k * step) * (uint32_t)kTile + t;
+      zv[k] = buf_gather<float>(rz, row);
+      pv[k] = buf_gather<float>(rp, row);
     }
   };
   if (pre_a) level_fetch<TP>(part + nt, nt, va);
   if (pre) level_fetch<TP>(part + 2u * nt, nt, vb);
+  // a table's FP64 kernel issues them with its partials (kRowsEarly)
+  constexpr bool kRowsEarly = kLightT;
+  if constexpr (kRowsEarly) load_rows();
   if (t < kWords) reinterpret_cast<uint32_t*>(&s_ns)[t] = reinterpret_cast<const uint32_t*>(in)[t];
   __syncthreads();
   const int32_t status = uni(s_ns.status), phase = uni(s_ns.phase);
@@ -2204,12 +2211,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
     const TP b = pre ? reduce_fetched<TP>(vb, nt, lb) : slot_total<TP>(d, 2u, part + 2u * nt, lb);
     const float rz_old = s_ns.rz_old;
     const float beta = rz_old != 0.0f ? (float)b / rz_old : 0.0f;  // pcg_transition's beta
-    load_rows();
+    if constexpr (!kRowsEarly) load_rows();
 #pragma unroll
     for (uint32_t k = 0; k < kT; ++k) {
-      if (rw[k] == 0xFFFFFFFFu) continue;
+      const uint32_t row = row_of(k);
+      if (row == 0xFFFFFFFFu) continue;
       const float pb = pv[k] * beta;
-      bst(d.p, rw[k], pb + zv[k]);
+      bst(d.p, row, pb + zv[k]);
     }
     return;
   }
@@ -2240,15 +2248,16 @@ __global__ __launch_bounds__(kTile) void k_pcg_finish(const TileArgs ta) {
   }
   if (nstatus != kRunning) return;
   if (nphase == kPhIter) {
-    load_rows();
+    if constexpr (!kRowsEarly) load_rows();
 #pragma unroll
     for (uint32_t k = 0; k < kT; ++k) {
-      if (rw[k] == 0xFFFFFFFFu) continue;
+      const uint32_t row = row_of(k);
+      if (row == 0xFFFFFFFFu) continue;
       if (niter == 0u) {
-        bst(d.p, rw[k], zv[k]);
+        bst(d.p, row, zv[k]);
       } else {
         const float pb = pv[k] * beta;
-        bst(d.p, rw[k], pb + zv[k]);
+        bst(d.p, row, pb + zv[k]);
       }
     }
   } else if (nphase == kPhIrResid) {

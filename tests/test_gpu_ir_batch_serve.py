@@ -104,6 +104,37 @@ def test_batch_fp32_identical_to_alone(ctx, hbj, oracle, monkeypatch, fused):
     batch.close()
 
 
+def test_batch_kernel_samples(ctx, hbj):
+    """hbj_batch_kernel_samples: with every launch set sampled, one record
+    per set in which some system ran (sets numbered from 0 in order), three
+    positive durations each (the three-launch table iteration), and per system
+    the launch sets that found it running: its init set plus one per
+    iteration.  The sampled solve gives the unsampled solve's bits."""
+    sides = [4, 7, 5]
+    sysl = _systems(ctx, hbj, sides)
+    for L, s in sysl:
+        s.set_rhs(L["rhs"])
+    batch = hbj.Batch(ctx, [s for _, s in sysl])
+    st, ref = batch.solve(5000, 1e-6, 0.0)
+    assert st == 0
+    xr = [s.get_x().copy() for _, s in sysl]
+    batch.set_kernel_timing(1)
+    st, res = batch.solve(5000, 1e-6, 0.0)
+    batch.set_kernel_timing(0)
+    assert st == 0
+    rec, sets = batch.kernel_samples()
+    its = [r.iterations for r in res]
+    print(f"samples {len(rec)}, sets {sets.tolist()}, iterations {its}", flush=True)
+    assert [r.iterations for r in ref] == its
+    assert sets.tolist() == [i + 1 for i in its]
+    assert len(rec) == max(sets)
+    assert np.array_equal(rec[:, 0], np.arange(len(rec)))
+    assert np.all(rec[:, 1:] > 0.0)
+    for (L, s), x in zip(sysl, xr):
+        assert np.array_equal(bits(s.get_x()), bits(x))
+    batch.close()
+
+
 def test_batch_isolates_failures(ctx, hbj):
     sysl = _systems(ctx, hbj, [5, 6])
     n = 12

@@ -641,7 +641,7 @@ def bench_headline(args):
                                "showed its FP64-verified solution"},
         "verification": {k: bsum[k] for k in ("worst_true_rel_residual_fp64_device",
                                               "worst_true_rel_residual_fp64_host", "all_verified_1e-8")},
-        "batch": bsum,
+        "batch": dict(bsum, kernels=B["kernels"]),
         "serving": S,
         "plain_fp32_50": {"workload": "plain FP32 block-Jacobi PCG, 50^3 beam lattice (750,000 unknowns, "
                                       "30,352,200 nnz, 6x6 blocks), tol 1e-6 (recurrence), max 5000, x0 = 0",

@@ -178,6 +178,34 @@ def test_batch_ir_identical_to_alone(ctx, hbj, monkeypatch, fused):
     batch.close()
 
 
+def test_batch_ir_partials_paths(ctx, hbj):
+    """A refined batch whose systems take each of the table kernels' ways to
+    their partials: one tile (no reduction), one chunk of at most 256 tiles
+    and two chunks (the prefetch before the state), more than 512 tiles (the
+    reduction after the state) -- each system's bits equal its alone solve's,
+    which never takes the table kernels."""
+    sides = [3, 7, 24, 28]  # 1, 8, 324 and 515 tiles
+    sysl = _systems(ctx, hbj, sides, seed0=50)
+    tiles = [s.info()["n_tiles"] for _, s in sysl]
+    print(f"refined batch, tiles {tiles}", flush=True)
+    assert tiles[0] == 1 and tiles[1] <= 256 < tiles[2] <= 512 < tiles[3]
+    p = hbj.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=1e-3, target_rel_res=1e-8)
+    alone = []
+    for L, s in sysl:
+        x, res = s.solve_ir(L["rhs"].astype(np.float64), None, p)
+        alone.append((x, res))
+        s.set_rhs64(L["rhs"].astype(np.float64))
+    batch = hbj.Batch(ctx, [s for _, s in sysl])
+    st, results = batch.solve_ir(p)
+    assert st == 0
+    for (L, s), (xa, ra), r in zip(sysl, alone, results):
+        assert r.status == 0 and r.iterations == ra.iterations and r.outer_iterations == ra.outer_iterations
+        xb = s.get_x64()
+        assert np.array_equal(bits(xb), bits(xa))
+        assert host_rel_res(L, xb)[0] <= 1e-8 * 1.001
+    batch.close()
+
+
 def test_server_mixed_stream_identical_to_alone(ctx, hbj):
     rng = np.random.default_rng(0)
     sides = [int(round(np.exp(rng.uniform(np.log(4), np.log(12))))) for _ in range(24)]

@@ -1666,7 +1666,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   uint32_t hrow = 0, hslot = 0;
   const bool halo = halo_row(tr.row0, tr.nrows, d.n, hrow, hslot);
   const uint32_t hrowc = halo ? hrow : tr.row0;
-  // round 1
+  // first load round
   const TP* part_pap = kDot64 ? reinterpret_cast<const TP*>(d.part64)
                               : reinterpret_cast<const TP*>(d.part);
   const bool pre = nt > 1u && nt <= kFetchMax;
@@ -1742,7 +1742,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
   __syncthreads();  // s_ra / s_rb are reused below
   const float na = -alpha;
 
-  // round 2, only where the state chooses another r source (b at init;
+  // second load round, only where the state chooses another r source (b at init;
   // r64 * scale = b32 at a refinement restart) or the blocks are general
   float rs = ri_prev, rhs = rh_prev;
   if (restart) {
@@ -1848,7 +1848,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
     float xi, qi, hq, zi_old, ri_prev, rh_prev, pi;
     uint2 tb, blk;
   };
-  // round 1 of a tile: everything that does not depend on the state
+  // first load round of a tile: everything that does not depend on the state
   auto load = [&](uint32_t tile, Ld& L, float* lu_buf) {
     L.tile = tile;
     L.row0 = tile * (uint32_t)kTile;
@@ -1895,7 +1895,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
   double lvp[2];
   if constexpr (kLightT) light_fetch(d.part64, nt, light, lvp);
   load(tile0, L, s_lu);
-  Ld Ln;  // the next tile's round 1
+  Ld Ln;  // the next tile's first load round
   const int32_t status = uni(st->status), phase = uni(st->phase);
   const float rz_old = st->rz_old;
   const double rz64_old = st->rz64_old, target2 = st->target2;
@@ -1952,7 +1952,7 @@ __global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
     if (!restart) return r + (-1.0f) * q;
     return r;
   };
-  // The next tile's round 1 (and, for 6-row blocks, its factor copy into
+  // The next tile's first load round (and, for 6-row blocks, its factor copy into
   // the other LDS buffer) is issued once this tile's factors have arrived,
   // so it is in flight during this tile's block solves and partials
   // (50^3: 10.9 -> 10.5 us per launch; 256-system batch 556 -> 568 solves/s).

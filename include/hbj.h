@@ -294,9 +294,12 @@ hbj_status hbj_system_get_x64(hbj_system* sys, double* x);
 
 /* Many independent systems advanced together: every PCG iteration of every
  * system in one set of kernel launches (graph-replayed), per-system
- * alpha/beta/status on the device.  Systems keep their own resident b/x0
- * (hbj_system_set_rhs / _set_rhs64).  Results are identical to solving each
- * system alone.  No reference counterpart (the reference solves one system). */
+ * alpha/beta/status on the device.  A batch of several systems splits them
+ * into two tables balanced by size, each with its own stream, run side by
+ * side (HBJ_BATCH_TABLES=1 in the environment keeps one table).  Systems
+ * keep their own resident b/x0 (hbj_system_set_rhs / _set_rhs64).  Results
+ * are identical to solving each system alone.  No reference counterpart
+ * (the reference solves one system). */
 hbj_status hbj_batch_create(hbj_ctx* ctx, hbj_system* const* systems, uint32_t count,
                             hbj_batch** out);
 void hbj_batch_destroy(hbj_batch* batch);
@@ -319,6 +322,8 @@ hbj_status hbj_batch_autotune(hbj_batch* batch, uint32_t iters, int ir, uint32_t
  * then launch directly instead of replaying the captured graphs.  No
  * reference counterpart. */
 hbj_status hbj_batch_set_kernel_timing(hbj_batch* batch, uint32_t every);
+/* (While timing is on, the batch runs as one table, so that each sampled
+ * launch has the GPU to itself.) */
 /* The last timed solve's samples: record i is rec[i * (1 + HBJ_ITER_KERNELS)
  * ...]: the launch set's index within the solve, then each kernel's
  * duration in microseconds (-1: the iteration form has no such launch).

@@ -196,11 +196,21 @@ struct Engine {
     return HBJ_OK;
   }
 
+  // A host-to-device copy that completes before the call returns, on the
+  // engine's own (non-blocking) stream: a copy on the legacy stream would be
+  // refused -- and would break the capture -- while another engine of the
+  // process captures a graph on another thread.
+  hbj_status h2d(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return HBJ_OK;
+  }
+
   // descs[i].st is pointed at this engine's state slot i.
   hbj_status set_table(std::vector<SysDesc> descs, const std::vector<uint2>& tiles) {
     for (uint32_t i = 0; i < descs.size(); ++i) descs[i].st = d_states + i;
-    HIPCHK(hipMemcpy(d_descs, descs.data(), sizeof(SysDesc) * descs.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(uint2) * tiles.size(), hipMemcpyHostToDevice));
+    CHK(h2d(d_descs, descs.data(), sizeof(SysDesc) * descs.size()));
+    CHK(h2d(d_tiles, tiles.data(), sizeof(uint2) * tiles.size()));
     CHK(upload_groups(tiles));
     n_sys = (uint32_t)descs.size();
     n_tiles = (uint32_t)tiles.size();
@@ -227,8 +237,7 @@ struct Engine {
   }
   hbj_status upload_groups(const std::vector<uint2>& tiles) {
     const std::vector<uint2> g = groups_of(tiles);
-    if (!g.empty())
-      HIPCHK(hipMemcpy(d_ftiles, g.data(), sizeof(uint2) * g.size(), hipMemcpyHostToDevice));
+    if (!g.empty()) CHK(h2d(d_ftiles, g.data(), sizeof(uint2) * g.size()));
     n_ftiles = (uint32_t)g.size();
     return HBJ_OK;
   }
@@ -244,8 +253,7 @@ struct Engine {
   hbj_status restore_full_table() {
     if (table_full) return HBJ_OK;
     HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy(d_tiles, full_tiles.data(), sizeof(uint2) * full_tiles.size(),
-                     hipMemcpyHostToDevice));
+    CHK(h2d(d_tiles, full_tiles.data(), sizeof(uint2) * full_tiles.size()));
     CHK(upload_groups(full_tiles));
     n_tiles = (uint32_t)full_tiles.size();
     table_full = true;
@@ -267,7 +275,7 @@ struct Engine {
     for (const uint2& t : full_tiles)
       if (latest[t.x].status == kRunning) live.push_back(t);
     if (live.empty()) return HBJ_OK;  // everything finished in the last chunk
-    HIPCHK(hipMemcpy(d_tiles, live.data(), sizeof(uint2) * live.size(), hipMemcpyHostToDevice));
+    CHK(h2d(d_tiles, live.data(), sizeof(uint2) * live.size()));
     CHK(upload_groups(live));
     n_tiles = (uint32_t)live.size();
     table_full = false;

@@ -596,7 +596,9 @@ def roofline_from_batch(B, P, HB):
          "note": ("achieved = mean algorithmic bytes per launch (the running systems' sliced layouts: "
                   "values -- only the nonzero ones and an 8-byte mask per row in waves of rows with "
                   "explicit zeros --, runs, column lists, headers, the rows' own p and q) / mean "
-                  "HIP-event duration, over the sampled launches of an untimed solve; traffic = "
+                  "HIP-event duration, over the sampled launches of an untimed solve (which runs the "
+                  "batch as one table, so each launch has the GPU to itself; the timed solves run "
+                  "two tables side by side); traffic = "
                   "FETCH_SIZE + WRITE_SIZE per launch with all 256 systems running (rocprofv3, "
                   "profiles/traffic_batch256.json), to compare with all_systems_running.bytes"),
          "plain_fp32_50": roofline_from_plain(P, 50, HB)}

@@ -370,7 +370,10 @@ typedef struct hbj_server_stats {
  * device.  A request's system is built on the host (tables, SpMV layout, LU
  * blocks) and copied in one asynchronous upload from pinned staging; a
  * system with the sliced SpMV layout keeps only that layout on the device
- * (no CSR arrays). */
+ * (no CSR arrays).  The resident requests run in two lanes (each its own
+ * table, stream and worker thread, side by side; HBJ_SERVER_LANES=1 in the
+ * environment keeps one); a request joins the lane with fewer resident
+ * tiles. */
 hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_server** out);
 void hbj_server_destroy(hbj_server* srv);
 /* Copies the request; returns a ticket.  b64 in FP64; solved with IR (params).

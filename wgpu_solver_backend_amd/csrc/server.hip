@@ -1,6 +1,6 @@
 // server.hip -- continuous-batching request loop (hbj_server_*).
 //
-// Two host threads share one device:
+// Host threads share one device (an admitter and one worker per lane):
 //   admitter  takes queued requests in arrival order.  For each it first
 //             builds the system's host image (tile tables, the sliced SpMV
 //             layout, the LU blocks, on the host), which fixes the exact
@@ -12,7 +12,11 @@
 //             system with the sliced layout keeps only that layout on the
 //             device (no CSR arrays, no per-entry indices).  This host work
 //             and the copies overlap the iterations running on the GPU;
-//   worker    owns the slot table.  It enqueues chunks of PCG iterations
+//   workers   one per lane (two lanes: two tables on two streams, side by
+//             side, so that each table's launches fill the other's launch
+//             tails and gaps; the admitter puts each request in the lane
+//             with fewer resident tiles).  A worker owns its lane's slot
+//             table.  It enqueues chunks of PCG iterations
 //             (one launch of each kernel covers every resident request) and
 //             always has the next chunk queued before it waits for the
 //             previous one's status poll.  A new request's slot writes wait
@@ -51,6 +55,7 @@ struct Request {
   hbj_system* sys = nullptr;
   hipEvent_t up_ev = nullptr;  // the end of the system's upload
   int slot = -1;
+  int lane = 0;
   bool done = false;
   int32_t status = HBJ_OK;
   std::string error;
@@ -66,7 +71,11 @@ struct hbj_server {
   std::mutex mu;
   std::condition_variable cv_admit, cv_work, cv_done;
   std::deque<Request*> queue;     // submitted, not yet admitted
-  std::deque<Request*> ready;     // resident on the device, not yet in the table
+  static constexpr int kLanes = 2;
+  int n_lanes = 1;
+  std::deque<Request*> ready[kLanes];  // resident on the device, not yet in a lane's table
+  uint64_t lane_tiles[kLanes] = {0, 0};   // tiles of each lane's resident requests
+  uint32_t lane_active[kLanes] = {0, 0};  // requests in each lane's table
   std::unordered_map<uint64_t, Request*> reqs;
   uint64_t next_ticket = 1;
   uint64_t bytes_in_use = 0;      // reserved by admitted requests
@@ -78,8 +87,8 @@ struct hbj_server {
   hbj_server_stats st{};
   bool stop = false;
   bool failed = false;            // the worker stopped on a device failure
-  std::thread admitter, worker;
-  Engine eng;
+  std::thread admitter, worker[kLanes];
+  Engine eng[kLanes];
   // uploads: the admitter's stream, pinned staging buffers (each free once
   // its event has completed) and the buffers they outgrew (freed at the end:
   // hipHostFree waits for all device work)
@@ -233,16 +242,24 @@ void admitter_main(hbj_server* S) {
     std::vector<uint32_t>().swap(r->bs);
     std::vector<float>().swap(r->val);
     std::vector<double>().swap(r->b64);
-    S->ready.push_back(r);
+    int lane = 0;
+    for (int k = 1; k < S->n_lanes; ++k)
+      if (S->lane_tiles[k] < S->lane_tiles[lane]) lane = k;
+    r->lane = lane;
+    S->lane_tiles[lane] += sys->n_tiles;
+    S->ready[lane].push_back(r);
     if (S->bytes_in_use > S->st.peak_device_bytes) S->st.peak_device_bytes = S->bytes_in_use;
     S->cv_work.notify_all();
   }
 }
 
 // Worker: returns a status only on a device failure (all requests then fail).
-hbj_status worker_loop(hbj_server* S) {
+hbj_status worker_loop(hbj_server* S, int lane) {
   HIPCHK(hipSetDevice(S->ctx->device));
-  Engine& E = S->eng;
+  Engine& E = S->eng[lane];
+  std::deque<Request*>& ready = S->ready[lane];
+  // the lanes share the last-level cache
+  const uint64_t cached_limit = kCachedLayoutBytes / (uint64_t)S->n_lanes;
   const uint32_t cap = S->cfg.max_active;
   const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 16;
   // Chunks are numbered in enqueue order.  A request placed in a slot is
@@ -288,6 +305,7 @@ hbj_status worker_loop(hbj_server* S) {
       if (r->up_ev) (void)hipEventDestroy(r->up_ev);  // long complete: the system ran
       r->up_ev = nullptr;
       std::lock_guard<std::mutex> g(S->mu);
+      S->lane_tiles[lane] -= r->sys->n_tiles;
       r->res = res;
       r->res.solve_ms = now_ms() - r->t_submit;
       std::string err;
@@ -325,6 +343,7 @@ hbj_status worker_loop(hbj_server* S) {
     S->resident -= (uint32_t)count;
     S->arena_gives += count;
     S->cv_admit.notify_all();
+    S->cv_work.notify_all();  // the other lanes may be waiting to stop
   };
 
   for (;;) {
@@ -332,12 +351,12 @@ hbj_status worker_loop(hbj_server* S) {
     {
       std::unique_lock<std::mutex> lk(S->mu);
       if (active == 0 && inflight < 0) {
-        S->cv_work.wait(lk, [&] { return !S->ready.empty() || (S->stop && S->queue.empty() && S->resident == 0); });
-        if (S->ready.empty() && S->stop && S->queue.empty() && S->resident == 0) break;
+        S->cv_work.wait(lk, [&] { return !ready.empty() || (S->stop && S->queue.empty() && S->resident == 0); });
+        if (ready.empty() && S->stop && S->queue.empty() && S->resident == 0) break;
       }
-      while (!S->ready.empty()) {
-        incoming.push_back(S->ready.front());
-        S->ready.pop_front();
+      while (!ready.empty()) {
+        incoming.push_back(ready.front());
+        ready.pop_front();
       }
     }
     bool staged = false;
@@ -367,12 +386,14 @@ hbj_status worker_loop(hbj_server* S) {
       CHK(E.stage_tiles(cur, tiles));
       // the resident layouts' bytes pick the SpMV's cache policy (variants 4
       // and 5 are the only ones a system without CSR arrays can run)
-      E.set_spmv_variant(layouts <= kCachedLayoutBytes ? 5u : 4u);
+      E.set_spmv_variant(layouts <= cached_limit ? 5u : 4u);
       table_dirty = false;
       staged = true;
       std::lock_guard<std::mutex> g(S->mu);
-      S->st.active = active;
-      if (active > S->st.peak_active) S->st.peak_active = active;
+      S->lane_active[lane] = active;
+      S->st.active = 0;
+      for (int k = 0; k < S->n_lanes; ++k) S->st.active += S->lane_active[k];
+      if (S->st.active > S->st.peak_active) S->st.peak_active = S->st.active;
       S->st.table_changes += 1;
       S->st.table_ms += now_ms() - t0;
     }
@@ -382,7 +403,7 @@ hbj_status worker_loop(hbj_server* S) {
       CHK(E.enqueue_poll(T, cur));
       enq = cur;
       enq_id = next_chunk++;
-      if (E.spmv_variant == 5u) cached_chunks += 1;
+      if (E.spmv_variant == 5u) cached_chunks = 1;
     } else if (staged) {
       CHK(E.sync_stream());  // nothing follows the staged copies: let them land
     }
@@ -398,15 +419,18 @@ hbj_status worker_loop(hbj_server* S) {
       cur ^= 1;
     }
     std::lock_guard<std::mutex> g(S->mu);
-    S->st.active = active;
+    S->lane_active[lane] = active;
+    S->st.active = 0;
+    for (int k = 0; k < S->n_lanes; ++k) S->st.active += S->lane_active[k];
     if (enq >= 0) S->st.chunks += 1;
-    S->st.chunks_cached_spmv = cached_chunks;
+    S->st.chunks_cached_spmv += cached_chunks;
+    cached_chunks = 0;
   }
   return HBJ_OK;
 }
 
-void worker_main(hbj_server* S) {
-  hbj_status s = worker_loop(S);
+void worker_main(hbj_server* S, int lane) {
+  hbj_status s = worker_loop(S, lane);
   if (s != HBJ_OK) {
     // device failure: fail everything that is still outstanding
     const std::string err = hbj_last_error();
@@ -466,15 +490,22 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   // tile table sized for the budget (a 256-row tile holds at least 72 bytes
   // per row of vectors), at most 2^18 tiles to start; it grows if needed
   const uint64_t tiles0 = std::min<uint64_t>(cfg->hbm_budget_bytes / (72ull * kTile) + 1, 1ull << 18);
-  hbj_status s = e == hipSuccess ? S->eng.init(ctx->device, cfg->max_active, (uint32_t)tiles0) : HBJ_E_DEVICE;
-  if (s == HBJ_OK) {
-    S->eng.use_graph = false;  // the table changes between chunks
-    S->eng.fused = fused_tables();  // a table of many systems: three launches
-    S->eng.n_tiles = 0;
+  {
+    const uint32_t lanes = env_u32("HBJ_SERVER_LANES", 2);
+    S->n_lanes = lanes < 1 ? 1 : (lanes > (uint32_t)hbj_server::kLanes ? hbj_server::kLanes : (int)lanes);
+  }
+  hbj_status s = e == hipSuccess ? HBJ_OK : HBJ_E_DEVICE;
+  for (int k = 0; k < S->n_lanes && s == HBJ_OK; ++k) {
+    Engine& E = S->eng[k];
+    s = E.init(ctx->device, cfg->max_active, (uint32_t)tiles0);
+    if (s != HBJ_OK) break;
+    E.use_graph = false;  // the table changes between chunks
+    E.fused = fused_tables();  // a table of many systems: three launches
+    E.n_tiles = 0;
     // slots start zeroed (status 0: final); no tile references a free slot
-    const uint32_t cap = S->eng.cap;
-    e = hipMemset(S->eng.d_descs, 0, sizeof(SysDesc) * cap);
-    if (e == hipSuccess) e = hipMemset(S->eng.d_states, 0, sizeof(SysState) * 2 * cap);
+    const uint32_t cap = E.cap;
+    e = hipMemset(E.d_descs, 0, sizeof(SysDesc) * cap);
+    if (e == hipSuccess) e = hipMemset(E.d_states, 0, sizeof(SysState) * 2 * cap);
     if (e != hipSuccess) {
       set_error("hbj_server_create: %s", hipGetErrorString(e));
       s = HBJ_E_DEVICE;
@@ -494,13 +525,13 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   }
   if (s != HBJ_OK) {
     release_uploads(S);
-    S->eng.destroy();
+    for (int k = 0; k < hbj_server::kLanes; ++k) S->eng[k].destroy();
     S->arena.destroy();
     delete S;
     return s;
   }
   S->admitter = std::thread(admitter_main, S);
-  S->worker = std::thread(worker_main, S);
+  for (int k = 0; k < S->n_lanes; ++k) S->worker[k] = std::thread(worker_main, S, k);
   *out = S;
   return HBJ_OK;
 }
@@ -518,7 +549,8 @@ void hbj_server_destroy(hbj_server* S) {
     std::lock_guard<std::mutex> g(S->mu);
     S->cv_work.notify_all();
   }
-  if (S->worker.joinable()) S->worker.join();
+  for (int k = 0; k < hbj_server::kLanes; ++k)
+    if (S->worker[k].joinable()) S->worker[k].join();
   (void)hipSetDevice(S->ctx->device);
   for (auto& kv : S->reqs) {
     if (kv.second->sys) hbj_system_destroy(kv.second->sys);
@@ -526,7 +558,7 @@ void hbj_server_destroy(hbj_server* S) {
     delete kv.second;
   }
   release_uploads(S);
-  S->eng.destroy();
+  for (int k = 0; k < hbj_server::kLanes; ++k) S->eng[k].destroy();
   S->arena.destroy();
   delete S;
 }
@@ -628,7 +660,8 @@ hbj_status hbj_server_stats_get(hbj_server* S, hbj_server_stats* out) {
   }
   std::lock_guard<std::mutex> g(S->mu);
   *out = S->st;
-  out->queued = S->queue.size() + S->ready.size();
+  out->queued = S->queue.size();
+  for (int k = 0; k < S->n_lanes; ++k) out->queued += S->ready[k].size();
   out->device_bytes_in_use = S->bytes_in_use;
   return HBJ_OK;
 }

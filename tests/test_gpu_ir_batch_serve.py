@@ -135,6 +135,51 @@ def test_batch_kernel_samples(ctx, hbj):
     batch.close()
 
 
+def test_two_batches_on_two_threads(ctx, hbj):
+    """Two batches of one context solved at once from two host threads (each
+    batch also runs its own two tables on a helper thread): every engine
+    captures and replays its graphs while the others do, and compacts its
+    table with copies on its own stream -- a synchronous copy on the legacy
+    stream is refused while another thread captures.  Results equal the
+    sequential solves bit for bit."""
+    import threading
+    groups = [[4, 9, 6, 11, 5], [10, 4, 8, 7, 12]]
+    batches, sysls = [], []
+    for gi, sides in enumerate(groups):
+        sysl = _systems(ctx, hbj, sides, seed0=70 + 10 * gi)
+        for L, s in sysl:
+            s.set_rhs64(L["rhs"].astype(np.float64))
+        sysls.append(sysl)
+        batches.append(hbj.Batch(ctx, [s for _, s in sysl]))
+    p = hbj.IrParams(max_outer=30, max_inner=20000, inner_rel_tol=1e-3, target_rel_res=1e-8)
+    seq = []
+    for b, sysl in zip(batches, sysls):
+        st, res = b.solve_ir(p)
+        assert st == 0
+        seq.append(([r.iterations for r in res], [s.get_x64().copy() for _, s in sysl]))
+    out = [None, None]
+
+    def work(k):
+        for _ in range(3):
+            out[k] = batches[k].solve_ir(p)
+            if out[k][0] != 0:
+                return
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k in range(2):
+        st, res = out[k]
+        assert st == 0, hbj._lib.last_error()
+        assert [r.iterations for r in res] == seq[k][0]
+        for (L, s), x in zip(sysls[k], seq[k][1]):
+            assert np.array_equal(bits(s.get_x64()), bits(x))
+    for b in batches:
+        b.close()
+
+
 def test_batch_isolates_failures(ctx, hbj):
     sysl = _systems(ctx, hbj, [5, 6])
     n = 12

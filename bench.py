@@ -165,6 +165,14 @@ def run_batch(H, ctx, args, steps, warmup):
             _, ra = s.solve_ir(L["rhs"].astype(np.float64), None, params)
             same &= ra.iterations == r.iterations
         alone_sps = len(systems) / (time.perf_counter() - t1)
+    # all three kernels' algorithmic bytes of one solve over the timed step time
+    # (both tables running side by side)
+    step_s = elapsed / steps
+    kern["effective"] = {"bytes_per_solve": kern["solve_bytes"], "step_ms": step_s * 1e3,
+                         "gbps": kern["solve_bytes"] / step_s / 1e9,
+                         "frac": kern["solve_bytes"] / step_s / 1e9 / HBM_PEAK_GBPS}
+    log(f"batch: {kern['solve_bytes'] / 1e9:.3f} GB of kernel bytes per solve, "
+        f"{kern['effective']['gbps']:.0f} GB/s over the timed steps")
     # the same systems refined to 1e-10: batched, and one by one
     r10 = run_batch_1e10(H, batch, systems, Ls, args)
     ns = [L["n"] for L in Ls]
@@ -266,6 +274,8 @@ def batch_kernel_rates(batch, systems, variant, params, every=4):
                                 "bytes": float(per[:, k].sum()),
                                 "gbps": float(per[:, k].sum() / (tt[full].mean() * 1e-6) / 1e9)}
         out["kernel"][nm] = e
+    # every launch set of the solve: system i took part in sets [0, sets[i])
+    out["solve_bytes"] = float((per.sum(axis=1) * sets.astype(np.float64)).sum())
     sp = out["kernel"]["spmv"]
     log(f"batch kernels: spmv {sp['us_mean']:.1f} us, {sp['gbps']:.0f} GB/s over the solve; "
         f"all systems running: {sp.get('all_systems', {}).get('us_mean', 0):.1f} us, "

@@ -19,8 +19,10 @@
 //             table.  It enqueues chunks of PCG iterations
 //             (one launch of each kernel covers every resident request) and
 //             always has the next chunk queued before it waits for the
-//             previous one's status poll.  A new request's slot writes wait
-//             on its upload event in the engine's stream (not on the host).
+//             previous one's status poll.  A new request joins the table
+//             once its upload event has completed (polled between chunks;
+//             waited on only when the lane has nothing else to run), so
+//             the resident requests' iterations never queue behind an upload.
 //             The SpMV reads the values through the caches (variant 5) while
 //             the resident systems' layouts fit the last-level cache, and
 //             non-temporally (variant 4) otherwise.  Arrivals and retirements change
@@ -346,18 +348,37 @@ hbj_status worker_loop(hbj_server* S, int lane) {
     S->cv_work.notify_all();  // the other lanes may be waiting to stop
   };
 
+  // admitted requests whose upload is still in flight: they join the table
+  // once it has landed, so the lane's iterations never wait for an upload
+  std::vector<Request*> uploading;
   for (;;) {
     std::vector<Request*> incoming;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (active == 0 && inflight < 0) {
+      if (active == 0 && inflight < 0 && uploading.empty()) {
         S->cv_work.wait(lk, [&] { return !ready.empty() || (S->stop && S->queue.empty() && S->resident == 0); });
         if (ready.empty() && S->stop && S->queue.empty() && S->resident == 0) break;
       }
       while (!ready.empty()) {
-        incoming.push_back(ready.front());
+        uploading.push_back(ready.front());
         ready.pop_front();
       }
+    }
+    {
+      size_t keep = 0;
+      for (Request* r : uploading) {
+        hipError_t q = hipEventQuery(r->up_ev);
+        // nothing else to run: wait for the first upload rather than spin
+        if (q == hipErrorNotReady && active == 0 && inflight < 0 && incoming.empty())
+          q = hipEventSynchronize(r->up_ev);
+        if (q == hipSuccess) incoming.push_back(r);
+        else if (q == hipErrorNotReady) uploading[keep++] = r;
+        else {
+          set_error("upload event: %s", hipGetErrorString(q));
+          return HBJ_E_DEVICE;
+        }
+      }
+      uploading.resize(keep);
     }
     bool staged = false;
     if (!incoming.empty() || table_dirty) {
@@ -369,7 +390,8 @@ hbj_status worker_loop(hbj_server* S, int lane) {
         slots[slot] = r;
         since[slot] = next_chunk;
         r->slot = (int)slot;
-        // the slot's writes and every later chunk come after the upload
+        // the upload has landed (its event completed); the stream-ordered
+        // wait only keeps the order explicit
         HIPCHK(hipStreamWaitEvent(E.stream, r->up_ev, 0));
         CHK(E.stage_slot(cur, slot, r->sys->desc(true, E.fused),
                          r->sys->init_state(true, 0, 0, 0, &r->params, false, E.fused)));

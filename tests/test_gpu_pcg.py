@@ -418,6 +418,51 @@ def test_tile_and_block_edges_bitwise(ctx, hbj, oracle, n):
     sys_.close()
 
 
+def test_small_blocks_stage_factors_in_groups(ctx, hbj, oracle):
+    """Tiles meeting more blocks than one LDS staging group holds (48): 300
+    one-row blocks (tile 0 meets 256 blocks, six groups), then sizes 1-6 at
+    random (the full tiles after it meet 72-105 blocks).  Both algorithms, every variant, and
+    the system inside a batch table beside a lattice: bit-identical to the
+    oracle."""
+    n = 1400
+    rp, ci, va, _, b = _banded_spd(n, seed=77)
+    rng = np.random.default_rng(78)
+    starts = list(range(0, 300))
+    while starts[-1] < n:
+        starts.append(min(n, starts[-1] + int(rng.integers(1, 7))))
+    bs = np.array(starts, np.uint32)
+    per_tile = [int(np.searchsorted(bs, min(n, 256 * (k + 1)), "left") - np.searchsorted(bs, 256 * k, "right") + 1)
+                for k in range((n + 255) // 256)]
+    print(f"small blocks: n={n}, {len(bs) - 1} blocks, blocks per tile {per_tile}", flush=True)
+    assert per_tile[0] > 5 * 48 and min(per_tile[1:-1]) > 48
+    x0 = np.zeros(n, F)
+    st, it, xo, _ = oracle_solve(oracle, rp, ci, va, bs, b, x0, 500, 1e-6, 0.0)
+    st_lu, lu = oracle.build_lu_blocks_6(n, rp, ci, va, bs)
+    st_c, it_c, xc = oracle.ccg(rp, ci, va, lu, bs, b, x0, 500, 1e-6, 0.0)
+    assert st == st_c == 0
+    sys_ = hbj.System(ctx, rp, ci, va, bs)
+    for alg, its, xref in ((hbj.ALG_PCG, it, xo), (hbj.ALG_PCG_ONE_REDUCTION, it_c, xc)):
+        sys_.set_algorithm(alg)
+        for v in range(hbj.SPMV_VARIANTS):
+            sys_.set_spmv_variant(v)
+            x, res = sys_.solve(b, x0, 500, 1e-6, 0.0)
+            assert res.iterations == its and np.array_equal(bits(x), bits(xref)), (alg, v)
+    sys_.set_algorithm(hbj.ALG_PCG)
+    sys_.set_rhs(b)
+    L = hbj.lattice(5, seed=5)
+    other = hbj.System(ctx, L["row_ptr"], L["col_idx"], L["values"], L["block_starts"])
+    xl, rl = other.solve(L["rhs"], np.zeros(L["n"], F), 5000, 1e-6, 0.0)
+    other.set_rhs(L["rhs"])
+    batch = hbj.Batch(ctx, [sys_, other])
+    stb, results = batch.solve(500, 1e-6, 0.0)
+    assert stb == 0 and results[0].iterations == it and results[1].iterations == rl.iterations
+    assert np.array_equal(bits(sys_.get_x()), bits(xo))
+    assert np.array_equal(bits(other.get_x()), bits(xl))
+    batch.close()
+    other.close()
+    sys_.close()
+
+
 def _insert_zero(rp, ci, va, r, c):
     """CSR with an explicit zero entry added at (r, c) (columns kept sorted)."""
     e0, e1 = int(rp[r]), int(rp[r + 1])

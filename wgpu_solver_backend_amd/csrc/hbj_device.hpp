@@ -23,8 +23,10 @@ constexpr int kLuStride = 36;
 // the FP64 residual path stages half as many 8-byte products).
 constexpr int kPer = 8;
 constexpr int kWaves = kTile / 64;
-// Blocks whose factors a tile stages in LDS (more than this: read from HBM).
-constexpr int kLuStage = 64;
+// Blocks whose factors a tile stages in LDS at once (a tile meeting more
+// stages them in groups).  A 256-row tile meets at most 44 blocks of 6 rows,
+// so 48 stages a 6-row system's tile in one group with 6.9 KB per buffer.
+constexpr int kLuStage = 48;
 // Tiles of one system per workgroup of k_pcg_finish (its reductions are
 // redundant per workgroup, so fewer workgroups do less of them).
 constexpr uint32_t kFinishTiles = 4;

@@ -1816,8 +1816,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const TileArgs ta) {
 // tile's loads are issued after the previous tile is done (holding two
 // tiles' would cost a wave per SIMD).  kOne: tiles kUT*b + u; a table:
 // entries of the engine's group table (tile0 a multiple of kUT).
+// Waves per SIMD the compiler must fit: the table update with FP64 dots
+// holds 69 registers under a cap of 7 (73 without; 20.6 KB of LDS also
+// allows 7 workgroups per CU), the FP32 instances would spill under it.
+constexpr int update_grp_waves(bool one, bool dot64) { return !one && dot64 ? 7 : 1; }
 template <bool kOne, bool kDot64, uint32_t kUT>
-__global__ __launch_bounds__(kTile) void k_pcg_update_grp(const TileArgs ta) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(update_grp_waves(kOne, kDot64), 8)))
+void k_pcg_update_grp(const TileArgs ta) {
   __shared__ float s_rh[kTile + 2 * kHalo];
   __shared__ float s_z[kTile];
   __shared__ float s_lu[kLuPer * kTile];   // the factors of even tiles of the group

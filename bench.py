@@ -47,8 +47,8 @@ def log(msg):
 
 def host_threads():
     """Host threads for the CPU baselines: the cores this process may run on,
-    capped by OMP_NUM_THREADS (the GPU box grants a job a 16-core share and
-    says so there), and the machine's core count for the record."""
+    capped by OMP_NUM_THREADS (a machine that grants a job a share of its
+    cores says so there), and the machine's core count for the record."""
     try:
         affinity = len(os.sched_getaffinity(0))
     except AttributeError:

@@ -297,9 +297,9 @@ def serve_stream(H, ctx, reqs, rate, seed, params, budget_gb, max_active, submit
     count = len(reqs)
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
     b64s = [L["rhs"].astype(np.float64) for L in reqs]
-    # 32 iterations between status polls: against 8, p50 / p99 at 50 req/s 80 / 297 ms instead of
-    # 87 / 314, and 77 instead of 75.5 req/s completed at 200 req/s offered
-    srv = H.Server(ctx, int(budget_gb * (1 << 30)), max_active=max_active, chunk_iters=32)
+    # 64 iterations between status polls: p50 / p99 at 50 req/s 83 / 276 ms, against 81 / 296 with
+    # 32 and 87 / 314 with 8; 76 req/s completed at 200 req/s offered (75.5 with 8)
+    srv = H.Server(ctx, int(budget_gb * (1 << 30)), max_active=max_active, chunk_iters=64)
     tickets = [None] * count
     t_arr = [0.0] * count
     t_ret = [0.0] * count

@@ -343,7 +343,7 @@ typedef struct hbj_server_config {
                                   the server allocates it at creation and places
                                   each resident system in a block of it */
   uint32_t max_active;         /* most systems advanced together */
-  uint32_t chunk_iters;        /* iterations between host status polls (0: 32) */
+  uint32_t chunk_iters;        /* iterations between host status polls (0: 64) */
 } hbj_server_config;
 
 typedef struct hbj_server_stats {

@@ -173,7 +173,7 @@ def poisson_serve(hbj, ctx, Ls, rate, seed, params, budget, max_active=256):
     from one thread while another waits for them in order; returns per request
     (status, x, result, latency_ms) and the server's stats."""
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(Ls))
-    srv = hbj.Server(ctx, budget, max_active=max_active, chunk_iters=32)
+    srv = hbj.Server(ctx, budget, max_active=max_active, chunk_iters=64)
     tickets = [None] * len(Ls)
     out = [None] * len(Ls)
 

@@ -491,7 +491,7 @@ class Batch:
 class Server:
     """Continuous-batching request loop (hbj_server_*)."""
 
-    def __init__(self, ctx, hbm_budget_bytes, max_active=256, chunk_iters=32):
+    def __init__(self, ctx, hbm_budget_bytes, max_active=256, chunk_iters=64):
         cfg = hbj_server_config(int(hbm_budget_bytes), int(max_active), int(chunk_iters))
         h = ctypes.c_void_p()
         _check(_lib.lib().hbj_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))

@@ -263,7 +263,7 @@ hbj_status worker_loop(hbj_server* S, int lane) {
   // the lanes share the last-level cache
   const uint64_t cached_limit = kCachedLayoutBytes / (uint64_t)S->n_lanes;
   const uint32_t cap = S->cfg.max_active;
-  const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 32;
+  const uint32_t T = S->cfg.chunk_iters ? S->cfg.chunk_iters : 64;
   // Chunks are numbered in enqueue order.  A request placed in a slot is
   // advanced from chunk since[slot] on, so polls of earlier chunks (which
   // show the slot's previous request) are not read for it.
@@ -505,7 +505,7 @@ hbj_status hbj_server_create(hbj_ctx* ctx, const hbj_server_config* cfg, hbj_ser
   auto* S = new hbj_server();
   S->ctx = ctx;
   S->cfg = *cfg;
-  if (S->cfg.chunk_iters == 0) S->cfg.chunk_iters = 32;
+  if (S->cfg.chunk_iters == 0) S->cfg.chunk_iters = 64;
   // even: a system's state copy 0 is current at every chunk boundary
   S->cfg.chunk_iters = S->cfg.chunk_iters < 2 ? 2u : (S->cfg.chunk_iters & ~1u);
   hipError_t e = hipSetDevice(ctx->device);
